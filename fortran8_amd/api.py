@@ -1,0 +1,180 @@
+"""Python host binding of the C-ABI in include/fenep_gpu.h.
+
+``FenepGPU`` mirrors the reference's step interface: the state arrays and
+parameters of fenep.f90's program main (fenep.f90:2424-2446), advanced by
+``step`` (the loop body of fenep.f90:2477-2527) on one MI355X.  The HIP
+library must be built (``fortran8_amd.build``); there is no CPU fallback.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from .params import FpParams, FpDiag
+from .fields import State
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfenep_gpu.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "fenep_gpu.h")
+_LIB = None
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+ERRORS = {1: "FP_E_ARG", 2: "FP_E_HIP", 3: "FP_E_STATE", 4: "FP_E_NAN", 5: "FP_E_NOMEM"}
+
+
+class FenepError(RuntimeError):
+    pass
+
+
+def header_exports():
+    """Function names declared in include/fenep_gpu.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fp_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FenepError("HIP library %s is not built; run fortran8_amd/build.py" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.fp_default_params.argtypes = [C.POINTER(FpParams)]
+        L.fp_create.argtypes = [C.POINTER(FpParams), C.POINTER(_vp)]
+        L.fp_destroy.argtypes = [_vp]
+        L.fp_last_error.restype = C.c_char_p
+        L.fp_last_error.argtypes = [_vp]
+        L.fp_set_state.argtypes = [_vp] + [_dp] * 6 + [C.c_int]
+        L.fp_get_state.argtypes = [_vp] + [_dp] * 6
+        L.fp_get_phi.argtypes = [_vp, _dp]
+        L.fp_set_body_force.argtypes = [_vp, _dp, _dp, _dp]
+        L.fp_set_markers.argtypes = [_vp, C.c_int, C.c_int] + [_dp] * 6
+        L.fp_get_marker_forces.argtypes = [_vp, _dp, _dp, _dp]
+        L.fp_step.argtypes = [_vp, C.c_int, C.c_int]
+        L.fp_diagnostics.argtypes = [_vp, C.POINTER(FpDiag)]
+        L.fp_set_state_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]
+        L.fp_get_state_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fp_enable_timing.argtypes = [_vp, C.c_int]
+        L.fp_kernel_times.argtypes = [_vp, _dp, C.c_int, C.c_char_p, C.c_int]
+        L.fp_seeded_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise FenepError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+def seeded_uniform(seed, n, offset=0):
+    out = np.zeros(n)
+    lib().fp_seeded_uniform(C.c_uint64(seed), C.c_uint64(offset), n, _ptr(out))
+    return out
+
+
+class FenepGPU:
+    """One device context (one HIP stream, device-resident state)."""
+
+    def __init__(self, params):
+        self.p = params.copy()
+        h = _vp()
+        rc = lib().fp_create(C.byref(self.p), C.byref(h))
+        if rc:
+            raise FenepError("fp_create failed: %s" % ERRORS.get(rc, rc))
+        self.h = h
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self._mshape = None
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().fp_last_error(self.h)
+            raise FenepError("%s failed: %s %s" % (what, ERRORS.get(rc, rc), msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, st):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (st.U, st.V, st.W, st.P, st.C)]
+        hist = None if st.hist is None else np.ascontiguousarray(st.hist, dtype=np.float64)
+        self._check(lib().fp_set_state(self.h, *[_ptr(a) for a in arrs], _ptr(hist), int(st.has_history)),
+                    "fp_set_state")
+
+    def get_state(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        g = lambda: np.zeros((nz + 2, ny + 2, nx + 2))
+        U, V, W, P = g(), g(), g(), g()
+        Cc = np.zeros((nz + 2, ny + 2, nx + 2, 6))
+        hist = np.zeros((9, nz, ny, nx))
+        self._check(lib().fp_get_state(self.h, _ptr(U), _ptr(V), _ptr(W), _ptr(P), _ptr(Cc), _ptr(hist)),
+                    "fp_get_state")
+        return State(U, V, W, P, Cc, hist, 1)
+
+    def get_phi(self):
+        a = np.zeros((self.nz + 2, self.ny + 2, self.nx + 2))
+        self._check(lib().fp_get_phi(self.h, _ptr(a)), "fp_get_phi")
+        return a
+
+    def set_body_force(self, f):
+        if f is None:
+            self._check(lib().fp_set_body_force(self.h, None, None, None), "fp_set_body_force")
+            return
+        self._f = [np.ascontiguousarray(a, dtype=np.float64) for a in f]
+        self._check(lib().fp_set_body_force(self.h, *[_ptr(a) for a in self._f]), "fp_set_body_force")
+
+    def set_markers(self, mk):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (mk.xc, mk.yc, mk.zc, mk.uc, mk.vc, mk.wc)]
+        self._check(lib().fp_set_markers(self.h, mk.nc, mk.nl, *[_ptr(a) for a in arrs]), "fp_set_markers")
+        self._mshape = mk.xc.shape
+
+    def marker_forces(self):
+        out = [np.zeros(self._mshape) for _ in range(3)]
+        self._check(lib().fp_get_marker_forces(self.h, *[_ptr(a) for a in out]), "fp_get_marker_forces")
+        return out
+
+    def step(self, n=1, sync=True):
+        self._check(lib().fp_step(self.h, int(n), 1 if sync else 0), "fp_step")
+
+    def diagnostics(self):
+        d = FpDiag()
+        rc = lib().fp_diagnostics(self.h, C.byref(d))
+        if rc and rc != 4:
+            self._check(rc, "fp_diagnostics")
+        return d.as_dict()
+
+    def enable_timing(self, on=True):
+        lib().fp_enable_timing(self.h, 1 if on else 0)
+
+    def kernel_times(self):
+        ms = np.zeros(64)
+        names = C.create_string_buffer(4096)
+        k = lib().fp_kernel_times(self.h, _ptr(ms), 64, names, 4096)
+        nm = names.value.decode().split(",") if k else []
+        return dict(zip(nm, ms[:k].tolist()))
+
+
+def load_case(case):
+    g = FenepGPU(case.params)
+    g.set_state(case.state)
+    if case.force is not None:
+        g.set_body_force(case.force)
+    if case.markers is not None:
+        g.set_markers(case.markers)
+    return g

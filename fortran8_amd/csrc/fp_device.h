@@ -1,0 +1,207 @@
+// Device helpers shared by the FP64 kernels: boundary-rule accessors and the
+// per-cell FENE-P algebra.  Every expression keeps the left-to-right order of
+// the Fortran statement it restates (the file that includes this is compiled
+// without fused multiply-add for the FP64 parity path).
+#pragma once
+#include "fp_internal.h"
+
+namespace fp {
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Value of a scalar field at (i,j,k), |offset| <= 1 outside the interior:
+// periodic copies (PBM, fenep.f90:147-155) in x and z; in y either periodic or
+// the wall rules U_kabe/V_kabe/P_kabe/C_kabe (fenep_2d.f90:1098-1143, NA=0,
+// NB=NY, Uwall=0).
+template <typename T>
+__device__ __forceinline__ T ldb(const T* __restrict__ A, const Grid& g, int i, int j, int k, int kind) {
+  i = wrapi(i, g.nx);
+  k = wrapi(k, g.nz);
+  const long plane = (long)g.nx * g.ny;
+  if (!g.walls) {
+    j = wrapi(j, g.ny);
+    return A[i + (long)g.nx * j + plane * k];
+  }
+  const T* col = A + i + plane * k;
+  if (j >= 0 && j < g.ny) {
+    if (kind == BC_V && j == g.ny - 1) return T(0);
+    return col[(long)g.nx * j];
+  }
+  const int nx = g.nx;
+  if (j < 0) {
+    switch (kind) {
+      case BC_U: case BC_W: return -col[0];
+      case BC_V: return T(0);
+      case BC_P: return col[0];
+      default: return T(2) * col[0] - col[nx];
+    }
+  }
+  const long top = (long)nx * (g.ny - 1);
+  switch (kind) {
+    case BC_U: case BC_W: return T(2) * T(0) - col[top];
+    case BC_V: return T(0);
+    case BC_P: return col[top];
+    default: return T(2) * col[top] - col[top - nx];
+  }
+}
+
+__device__ __forceinline__ long cidx(const Grid& g, int i, int j, int k) {
+  return i + (long)g.nx * (j + (long)g.ny * k);
+}
+
+// ---- FENE-P algebra (FP64) ------------------------------------------------
+// Vieta, fenep.f90:425-444
+__device__ __forceinline__ double vieta_d(const double* a) {
+  double a2 = -((a[0] + a[3]) + a[5]);
+  double a1 = (((((a[0] * a[3]) + (a[3] * a[5])) + (a[5] * a[0])) - (a[4] * a[4])) - (a[2] * a[2])) - (a[1] * a[1]);
+  double a0 = -((((((a[0] * a[3]) * a[5] + (a[1] * a[4]) * a[2]) + (a[1] * a[4]) * a[2]) - (a[0] * a[4]) * a[4]) -
+                  (a[1] * a[1]) * a[5]) - (a[2] * a[3]) * a[2]);
+  double p = a1 - (a2 * a2) / 3.0;
+  double q = (a0 - (a1 * a2) / 3.0) + (2.0 * ((a2 * a2) * a2)) / 27.0;
+  double q2 = q / 2.0, p3 = p / 3.0;
+  double t = q2 * q2 + (p3 * p3) * p3;
+  if (t >= 0.0) return -100.0;
+  double arg = ((3.0 * q) / (2.0 * p)) * sqrt(-(3.0 / p));
+  double ang = acos(arg) / 3.0 + (2.0 * 3.14159265358979323846) / 3.0;
+  return (2.0 * sqrt(-(p / 3.0))) * cos(ang) - a2 / 3.0;
+}
+
+__device__ __forceinline__ double minval2_d(double a, double b) {
+  if (isnan(a)) return b;
+  if (isnan(b)) return a;
+  return (b < a) ? b : a;
+}
+
+// candidate face values of one cell (fenep.f90:179-184): left face (Cp) and
+// right face (Cn) for candidate l in 0..2; l == 3 is the zero slope.
+__device__ __forceinline__ double cand_p(int l, double cm, double c, double cp) {
+  if (l == 0) return (c * 3.0) / 2.0 - cp / 2.0;
+  if (l == 1) return cm / 2.0 + c / 2.0;
+  if (l == 2) return (cm / 4.0 + c) - cp / 4.0;
+  return c;
+}
+__device__ __forceinline__ double cand_n(int l, double cm, double c, double cp) {
+  if (l == 0) return c / 2.0 + cp / 2.0;
+  if (l == 1) return -(cm / 2.0) + (c * 3.0) / 2.0;
+  if (l == 2) return (-(cm / 4.0) + c) + cp / 4.0;
+  return c;
+}
+
+// selection of fenep.f90:186-221 (maxloc over {min eig of candidate l} and 0),
+// returns 0..3 (3 = reference index 4, zero slope)
+__device__ __forceinline__ int select_d(const double* cm, const double* c, const double* cp) {
+  double best = 0.0;
+  int idx = -1;
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    double ap[6], an[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      ap[q] = cand_p(l, cm[q], c[q], cp[q]);
+      an[q] = cand_n(l, cm[q], c[q], cp[q]);
+    }
+    double v = minval2_d(vieta_d(ap), vieta_d(an));
+    if (!isnan(v) && (idx < 0 || v > best)) { idx = l; best = v; }
+  }
+  if (idx < 0 || 0.0 > best) idx = 3;
+  return idx;
+}
+
+// Lyapunov_func, fenep.f90:588-605 (A column-major, already transposed)
+__device__ __forceinline__ void lyap_func_d(const double* A0, const double* b, const double* x, double dt,
+                                            double wi, double lp, double* l) {
+  double f = (lp * lp - 3.0) / (((lp * lp - x[0]) - x[3]) - x[5]);
+  double dg = (1.0 + (f * dt) / wi) / 2.0;
+  double a11 = A0[0] + dg, a22 = A0[4] + dg, a33 = A0[8] + dg;
+  double a21 = A0[1], a31 = A0[2], a12 = A0[3], a32 = A0[5], a13 = A0[6], a23 = A0[7];
+  l[0] = ((((a11 + a11) * x[0] + a12 * x[1]) + a13 * x[2]) + a12 * x[1]) + a13 * x[2] - b[0];
+  l[1] = ((((a21 * x[0] + (a22 + a11) * x[1]) + a23 * x[2]) + a12 * x[3]) + a13 * x[4]) - b[1];
+  l[2] = ((((a31 * x[0] + a32 * x[1]) + (a33 + a11) * x[2]) + a12 * x[4]) + a13 * x[5]) - b[2];
+  l[3] = ((((a21 * x[1] + a21 * x[1]) + (a22 + a22) * x[3]) + a23 * x[4]) + a23 * x[4]) - b[3];
+  l[4] = ((((a21 * x[2] + a31 * x[1]) + a32 * x[3]) + (a33 + a22) * x[4]) + a23 * x[5]) - b[4];
+  l[5] = ((((a31 * x[2] + a32 * x[4]) + a31 * x[2]) + a32 * x[4]) + (a33 + a33) * x[5]) - b[5];
+}
+
+// gauss, fenep.f90:608-643, fully unrolled so that row swaps are selects.
+__device__ __forceinline__ void gauss6_d(double (&Ah)[6][7], double* x) {
+  bool zero = false;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (!zero) {
+      int mi = k;
+      double ma = fabs(Ah[k][k]);
+#pragma unroll
+      for (int r = k + 1; r < 6; r++) {
+        double v = fabs(Ah[r][k]);
+        if (v > ma) { ma = v; mi = r; }
+      }
+      if (ma == 0.0) {
+        zero = true;
+      } else {
+#pragma unroll
+        for (int r = k + 1; r < 6; r++) {
+          if (r == mi) {
+#pragma unroll
+            for (int c = k; c < 7; c++) { double t = Ah[k][c]; Ah[k][c] = Ah[r][c]; Ah[r][c] = t; }
+          }
+        }
+        double piv = Ah[k][k];
+#pragma unroll
+        for (int c = k + 1; c < 7; c++) Ah[k][c] = Ah[k][c] / piv;
+        Ah[k][k] = 1.0;
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+          if (r != k) {
+            double f = Ah[r][k];
+#pragma unroll
+            for (int c = k + 1; c < 7; c++) Ah[r][c] = Ah[r][c] - f * Ah[k][c];
+            Ah[r][k] = 0.0;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; r++) x[r] = zero ? 0.0 : Ah[r][6];
+}
+
+// Newton of Lyapunov (fenep.f90:554-578) with the central-difference Jacobian.
+__device__ __forceinline__ int newton_d(const double* A0, const double* b, double* x, double dt, double wi,
+                                        double lp, double dc, double eps, int itrmax) {
+  int itr;
+  for (itr = 1; itr <= itrmax; itr++) {
+    double Ah[6][7];
+#pragma unroll
+    for (int col = 0; col < 6; col++) {
+      double x1[6], x2[6], l1[6], l2[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) { x1[m] = x[m]; x2[m] = x[m]; }
+      x1[col] = x1[col] + dc;
+      x2[col] = x2[col] - dc;
+      lyap_func_d(A0, b, x1, dt, wi, lp, l1);
+      lyap_func_d(A0, b, x2, dt, wi, lp, l2);
+#pragma unroll
+      for (int row = 0; row < 6; row++) Ah[row][col] = (l1[row] - l2[row]) / (2.0 * dc);
+    }
+    double l[6], r[6];
+    lyap_func_d(A0, b, x, dt, wi, lp, l);
+#pragma unroll
+    for (int m = 0; m < 6; m++) Ah[m][6] = l[m];
+    gauss6_d(Ah, r);
+#pragma unroll
+    for (int m = 0; m < 6; m++) x[m] = x[m] - r[m];
+    double sr = 0.0, sx = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; m++) sr = sr + r[m] * r[m];
+#pragma unroll
+    for (int m = 0; m < 6; m++) sx = sx + x[m] * x[m];
+    if (sqrt(sr / sx) < eps) break;
+  }
+  return itr > itrmax ? itrmax : itr;
+}
+
+__device__ __forceinline__ double peterlin_d(double lp, double c0, double c3, double c5) {
+  return (lp * lp - 3.0) / (((lp * lp - c0) - c3) - c5);
+}
+
+}  // namespace fp

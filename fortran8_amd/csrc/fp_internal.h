@@ -1,0 +1,94 @@
+// Internal definitions shared by the host C-ABI (fp_api.hip) and the kernel
+// translation units.  Device layout: interior cells only, x fastest,
+// idx = i + nx*(j + ny*k) with 0-based i,j,k (reference index i_ref = i+1);
+// ghost values are produced on the fly by the boundary rules in `Grid::ld*`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/fenep_gpu.h"
+
+namespace fp {
+
+enum BcKind { BC_U = 0, BC_V = 1, BC_W = 2, BC_P = 3, BC_C = 4 };
+
+// Grid description passed by value to every kernel.
+struct Grid {
+  int nx, ny, nz, nxh;
+  int walls;
+  long n;           // nx*ny*nz
+  double dx, dy, dz;
+  double dt, re, beta, wi, lp;
+  long sy;          // spectral rows in y (ny or ny+2)
+  int jo;           // spectral row of interior j=0 (0 or 1)
+};
+
+// Device buffers of one context.
+struct Ctx {
+  fp_params p;
+  Grid g;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int has_history = 0;
+  int state_set = 0;
+  int step = 0;
+  int timing = 0;
+  // plain FP64 state
+  double *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr, *Phi = nullptr;
+  double *C[2] = {nullptr, nullptr};  // 6 planes each
+  int cur = 0;
+  double *fC = nullptr;               // Peterlin factor of the current C
+  uint8_t* sel = nullptr;
+  double *A0 = nullptr, *T0 = nullptr, *B0 = nullptr, *B = nullptr;  // 3 planes each
+  double *Q = nullptr;                // 3 planes: predictor RHS / Ua
+  double *Up = nullptr;               // 3 planes: u*
+  double *F = nullptr;                // 3 planes body-force field (nullable)
+  double2* spec = nullptr;            // nxh*sy*nz complex
+  double* thp = nullptr;              // Thomas P scratch (ny+2)*nxh*nz
+  double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  double *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
+  double* red = nullptr;              // reduction scratch
+  int* nitr = nullptr;                // Newton iteration counts (n)
+  // IBM
+  int nc = 0, nl = 0;
+  long nmk = 0;
+  double* mk = nullptr;               // 6 planes: xc,yc,zc,uc,vc,wc
+  double* mf = nullptr;               // 3 planes: Fxc,Fyc,Fzc
+  double* mb = nullptr;               // 3 planes: Ub,Vb,Wb
+  double* ftmp = nullptr;             // 3 ghosted raw spread arrays
+  long gn = 0;
+  int* sp_row[3] = {nullptr, nullptr, nullptr};   // CSR row pointers
+  long* sp_tgt[3] = {nullptr, nullptr, nullptr};  // ghosted target index
+  int* sp_ent[3] = {nullptr, nullptr, nullptr};   // marker*27 + slot
+  int sp_rows[3] = {0, 0, 0};
+  // SOR
+  double sor_itr = -1.0;
+  // timing
+  std::vector<std::pair<std::string, hipEvent_t>> ev;
+  std::vector<std::pair<std::string, double>> ktime;
+  // mixed precision state (allocated only for precision == 1)
+  void* mixed = nullptr;
+};
+
+// ---- launchers (fp_kernels_f64.hip)
+void f64_select(Ctx& c);
+void f64_cstar_newton(Ctx& c);
+void f64_predictor(Ctx& c);
+void f64_solve(Ctx& c, int kind, const double* q, double* x);   // kind 0..2 Helmholtz, 3 Poisson
+void f64_divergence(Ctx& c, const double* up, double* q);
+void f64_march(Ctx& c);
+void f64_ibm(Ctx& c);        // interp + forces + spread + predictor RHS into Q
+void f64_ibm_setup(Ctx& c);  // CSR build (host) after markers are set
+int  f64_sor(Ctx& c, const double* q, double* phi);
+void f64_zero_c(Ctx& c);
+void f64_diag(Ctx& c, fp_diag* d);
+void f64_fill_f(Ctx& c);     // Peterlin factor of the current C
+
+}  // namespace fp
+
+#define FP_HIP_CHECK(expr)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) throw std::string(hipGetErrorString(e_)) + " @ " + #expr; \
+  } while (0)

@@ -1,0 +1,894 @@
+// Plain FP64 path of the FENE-P / IBM SMAC step (the parity path).
+// Compiled with -ffp-contract=off: every expression is evaluated in the
+// order written, which is the order of the reference statement it restates.
+#include "fp_device.h"
+#include <algorithm>
+#include <map>
+#include <cmath>
+
+namespace fp {
+
+static inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
+
+// ============================================================ FENE-P ======
+// CpxCnx/CpyCny/CpzCnz selection (fenep.f90:169-382) for all three axes of
+// one cell; the chosen candidate per axis is packed in 2 bits.
+__global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict__ C, uint8_t* __restrict__ sel) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  uint8_t s = 0;
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    double cm[6], c[6], cp[6];
+    int di = d == 0, dj = d == 1, dk = d == 2;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const double* Cq = C + q * g.n;
+      cm[q] = ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+      c[q] = Cq[m];
+      cp[q] = ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+    }
+    s |= (uint8_t)(select_d(cm, c, cp) << (2 * d));
+  }
+  sel[m] = s;
+}
+
+// face value of cell a (wrapped) along axis d with its own selection:
+// left (Cp) when left=true, right (Cn) otherwise, component q.
+__device__ __forceinline__ void face_vals(const Grid& g, const double* __restrict__ C, const uint8_t* __restrict__ sel,
+                                          int d, int i, int j, int k, bool left, double* out) {
+  i = wrapi(i, g.nx); j = wrapi(j, g.ny); k = wrapi(k, g.nz);
+  long m = cidx(g, i, j, k);
+  int l = (sel[m] >> (2 * d)) & 3;
+  int di = d == 0, dj = d == 1, dk = d == 2;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const double* Cq = C + q * g.n;
+    double c = Cq[m];
+    double cm = l == 3 ? 0.0 : ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+    double cp = l == 3 ? 0.0 : ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+    out[q] = left ? cand_p(l, cm, c, cp) : cand_n(l, cm, c, cp);
+  }
+}
+
+// Cstar (fenep.f90:464-507) + Lyapunov Newton (fenep.f90:518-585) + the
+// Peterlin factor of the new C (f, fenep.f90:510-515, used by polymer_stress).
+__global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __restrict__ C, const uint8_t* __restrict__ sel,
+                                                      const double* __restrict__ U, const double* __restrict__ V,
+                                                      const double* __restrict__ W, double* __restrict__ Cn,
+                                                      double* __restrict__ fC, int* __restrict__ nitr,
+                                                      double dc, double eps, int itrmax) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const double dt = g.dt;
+  double cx[6];
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    double h = d == 0 ? g.dx : (d == 1 ? g.dy : g.dz);
+    double v1, v2;
+    if (d == 0) { v1 = ldb(U, g, i - 1, j, k, BC_U); v2 = U[m]; }
+    else if (d == 1) { v1 = ldb(V, g, i, j - 1, k, BC_V); v2 = ldb(V, g, i, j, k, BC_V); }
+    else { v1 = ldb(W, g, i, j, k - 1, BC_W); v2 = W[m]; }
+    double s1 = ((-v2 + fabs(v2)) * dt) / (2.0 * h);
+    double s2 = 1.0 / 6.0 + ((-v2 - fabs(v2)) * dt) / (2.0 * h);
+    double s3 = 1.0 / 6.0 + ((v1 - fabs(v1)) * dt) / (2.0 * h);
+    double s4 = ((v1 + fabs(v1)) * dt) / (2.0 * h);
+    int di = d == 0, dj = d == 1, dk = d == 2;
+    double pl[6], pr[6], nl[6], nr[6];
+    face_vals(g, C, sel, d, i, j, k, true, pl);                    // Cp(a-1): own left face
+    face_vals(g, C, sel, d, i + di, j + dj, k + dk, true, pr);     // Cp(a): left face of a+1
+    face_vals(g, C, sel, d, i - di, j - dj, k - dk, false, nl);    // Cn(a-1): right face of a-1
+    face_vals(g, C, sel, d, i, j, k, false, nr);                   // Cn(a): own right face
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      double acc = d == 0 ? s3 * pl[q] : cx[q] + s3 * pl[q];
+      acc = acc + s1 * pr[q];
+      acc = acc + s4 * nl[q];
+      acc = acc + s2 * nr[q];
+      cx[q] = acc;
+    }
+  }
+  // Lyapunov
+  double b[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) b[q] = cx[q];
+  b[0] = b[0] + dt / g.wi;
+  b[3] = b[3] + dt / g.wi;
+  b[5] = b[5] + dt / g.wi;
+  const double dx = g.dx, dy = g.dy, dz = g.dz;
+#define UL(a, bb, c) ldb(U, g, a, bb, c, BC_U)
+#define VL(a, bb, c) ldb(V, g, a, bb, c, BC_V)
+#define WL(a, bb, c) ldb(W, g, a, bb, c, BC_W)
+  double a11 = -((dt * (-UL(i - 1, j, k) + UL(i, j, k))) / dx);
+  double a12 = -((dt * (((-VL(i - 1, j - 1, k) + VL(i + 1, j - 1, k)) - VL(i - 1, j, k)) + VL(i + 1, j, k))) / (4.0 * dx));
+  double a13 = -((dt * (((-WL(i - 1, j, k - 1) + WL(i + 1, j, k - 1)) - WL(i - 1, j, k)) + WL(i + 1, j, k))) / (4.0 * dx));
+  double a21 = -((dt * (((-UL(i - 1, j - 1, k) - UL(i, j - 1, k)) + UL(i - 1, j + 1, k)) + UL(i, j + 1, k))) / (4.0 * dy));
+  double a22 = -((dt * (-VL(i, j - 1, k) + VL(i, j, k))) / dy);
+  double a23 = -((dt * (((-WL(i, j - 1, k - 1) + WL(i, j + 1, k - 1)) - WL(i, j - 1, k)) + WL(i, j + 1, k))) / (4.0 * dy));
+  double a31 = -((dt * (((-UL(i - 1, j, k - 1) - UL(i, j, k - 1)) + UL(i - 1, j, k + 1)) + UL(i, j, k + 1))) / (4.0 * dz));
+  double a32 = -((dt * (((-VL(i, j - 1, k - 1) - VL(i, j, k - 1)) + VL(i, j - 1, k + 1)) + VL(i, j, k + 1))) / (4.0 * dz));
+  double a33 = -((dt * (-WL(i, j, k - 1) + WL(i, j, k))) / dz);
+#undef UL
+#undef VL
+#undef WL
+  double A[9] = {a11, a12, a13, a21, a22, a23, a31, a32, a33};  // transpose(A0), column-major
+  double x[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
+  int it = newton_d(A, b, x, dt, g.wi, g.lp, dc, eps, itrmax);
+#pragma unroll
+  for (int q = 0; q < 6; q++) Cn[q * g.n + m] = x[q];
+  fC[m] = peterlin_d(g.lp, x[0], x[3], x[5]);
+  nitr[m] = it;
+}
+
+// ========================================================= predictor ======
+// C0 = f(C) C - E at (i,j,k) with ghosts (polymer_stress, fenep.f90:655-659)
+__device__ __forceinline__ double c0_at(const Grid& g, const double* __restrict__ C, const double* __restrict__ fC,
+                                        int q, int i, int j, int k) {
+  const double E = (q == 0 || q == 3 || q == 5) ? 1.0 : 0.0;
+  if (g.walls && (j < 0 || j >= g.ny)) {
+    double c0 = ldb(C, g, i, j, k, BC_C), c3 = ldb(C + 3 * g.n, g, i, j, k, BC_C), c5 = ldb(C + 5 * g.n, g, i, j, k, BC_C);
+    double f = peterlin_d(g.lp, c0, c3, c5);
+    return f * ldb(C + q * g.n, g, i, j, k, BC_C) - E;
+  }
+  long m = cidx(g, wrapi(i, g.nx), wrapi(j, g.ny), wrapi(k, g.nz));
+  return fC[m] * C[q * g.n + m] - E;
+}
+
+// convection (fenep.f90:694-734), viscous (736-770), polymer_stress
+// (645-691) and the predictor right-hand side of navier (803-826),
+// fft_navier (1466-1511) or ibm_preliminary (1994-2013), with the history
+// updates A0=A, T0=T, B0=B done in place.
+__global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first, int polymer,
+                                                   const double* __restrict__ U, const double* __restrict__ V,
+                                                   const double* __restrict__ W, const double* __restrict__ P,
+                                                   const double* __restrict__ C, const double* __restrict__ fC,
+                                                   double* __restrict__ A0, double* __restrict__ T0, double* __restrict__ B0,
+                                                   double* __restrict__ Bout, const double* __restrict__ F,
+                                                   double bf0, double bf1, double bf2, double* __restrict__ Q) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const double dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
+#define Ug(a, b, c) ldb(U, g, a, b, c, BC_U)
+#define Vg(a, b, c) ldb(V, g, a, b, c, BC_V)
+#define Wg(a, b, c) ldb(W, g, a, b, c, BC_W)
+  double a[3], bv[3], t[3];
+  {
+    double h1 = (Ug(i - 1, j, k) + Ug(i, j, k)) / 2.0, h2 = (Ug(i, j, k) + Ug(i + 1, j, k)) / 2.0;
+    double ax = ((-(h1 * h1) + h2 * h2) / dx
+                 + (-((((Vg(i, j - 1, k) + Vg(i + 1, j - 1, k)) / 2.0) * (Ug(i, j - 1, k) + Ug(i, j, k))) / 2.0)
+                    + (((Vg(i, j, k) + Vg(i + 1, j, k)) / 2.0) * (Ug(i, j, k) + Ug(i, j + 1, k))) / 2.0) / dy)
+                + (-((((Wg(i, j, k - 1) + Wg(i + 1, j, k - 1)) / 2.0) * (Ug(i, j, k - 1) + Ug(i, j, k))) / 2.0)
+                   + (((Wg(i, j, k) + Wg(i + 1, j, k)) / 2.0) * (Ug(i, j, k) + Ug(i, j, k + 1))) / 2.0) / dz;
+    double g1 = (Vg(i, j - 1, k) + Vg(i, j, k)) / 2.0, g2 = (Vg(i, j, k) + Vg(i, j + 1, k)) / 2.0;
+    double ay = ((-((((Ug(i - 1, j, k) + Ug(i - 1, j + 1, k)) / 2.0) * (Vg(i - 1, j, k) + Vg(i, j, k))) / 2.0)
+                  + (((Ug(i, j, k) + Ug(i, j + 1, k)) / 2.0) * (Vg(i, j, k) + Vg(i + 1, j, k))) / 2.0) / dx
+                 + (-(g1 * g1) + g2 * g2) / dy)
+                + (-((((Wg(i, j, k - 1) + Wg(i, j + 1, k - 1)) / 2.0) * (Vg(i, j, k - 1) + Vg(i, j, k))) / 2.0)
+                   + (((Wg(i, j, k) + Wg(i, j + 1, k)) / 2.0) * (Vg(i, j, k) + Vg(i, j, k + 1))) / 2.0) / dz;
+    double e1 = (Wg(i, j, k - 1) + Wg(i, j, k)) / 2.0, e2 = (Wg(i, j, k) + Wg(i, j, k + 1)) / 2.0;
+    double az = ((-((((Ug(i - 1, j, k) + Ug(i - 1, j, k + 1)) / 2.0) * (Wg(i - 1, j, k) + Wg(i, j, k))) / 2.0)
+                  + (((Ug(i, j, k) + Ug(i, j, k + 1)) / 2.0) * (Wg(i, j, k) + Wg(i + 1, j, k))) / 2.0) / dx
+                 + (-((((Vg(i, j - 1, k) + Vg(i, j - 1, k + 1)) / 2.0) * (Wg(i, j - 1, k) + Wg(i, j, k))) / 2.0)
+                    + (((Vg(i, j, k) + Vg(i, j, k + 1)) / 2.0) * (Wg(i, j, k) + Wg(i, j + 1, k))) / 2.0) / dy)
+                + (-(e1 * e1) + e2 * e2) / dz;
+    a[0] = -ax; a[1] = -ay; a[2] = -az;
+  }
+  const double cb = g.beta / g.re;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double* X = c == 0 ? U : (c == 1 ? V : W);
+    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+    double x0 = ldb(X, g, i, j, k, kind);
+    double bb = (((ldb(X, g, i - 1, j, k, kind) - 2.0 * x0) + ldb(X, g, i + 1, j, k, kind)) / (dx * dx)
+                 + ((ldb(X, g, i, j - 1, k, kind) - 2.0 * x0) + ldb(X, g, i, j + 1, k, kind)) / (dy * dy))
+                + ((ldb(X, g, i, j, k - 1, kind) - 2.0 * x0) + ldb(X, g, i, j, k + 1, kind)) / (dz * dz);
+    bv[c] = cb * bb;
+  }
+  if (polymer) {
+#define C0g(q, aa, bb, cc) c0_at(g, C, fC, q, aa, bb, cc)
+    double tx = ((-C0g(0, i, j, k) + C0g(0, i + 1, j, k)) / dx
+                 + (((-C0g(1, i, j - 1, k) - C0g(1, i + 1, j - 1, k)) + C0g(1, i, j + 1, k)) + C0g(1, i + 1, j + 1, k)) / (4.0 * dy))
+                + (((-C0g(2, i, j, k - 1) - C0g(2, i + 1, j, k - 1)) + C0g(2, i, j, k + 1)) + C0g(2, i + 1, j, k + 1)) / (4.0 * dz);
+    double ty = ((((-C0g(1, i - 1, j, k) + C0g(1, i + 1, j, k)) - C0g(1, i - 1, j + 1, k)) + C0g(1, i + 1, j + 1, k)) / (4.0 * dx)
+                 + (-C0g(3, i, j, k) + C0g(3, i, j + 1, k)) / dy)
+                + (((-C0g(4, i, j, k - 1) - C0g(4, i, j + 1, k - 1)) + C0g(4, i, j, k + 1)) + C0g(4, i, j + 1, k + 1)) / (4.0 * dz);
+    double tz = ((((-C0g(2, i - 1, j, k) + C0g(2, i + 1, j, k)) - C0g(2, i - 1, j, k + 1)) + C0g(2, i + 1, j, k + 1)) / (4.0 * dx)
+                 + (((-C0g(4, i, j - 1, k) + C0g(4, i, j + 1, k)) - C0g(4, i, j - 1, k + 1)) + C0g(4, i, j + 1, k + 1)) / (4.0 * dy))
+                + (-C0g(5, i, j, k) + C0g(5, i, j, k + 1)) / dz;
+#undef C0g
+    const double coef = (1.0 - g.beta) / (g.re * g.wi);
+    t[0] = coef * tx; t[1] = coef * ty; t[2] = coef * tz;
+  } else {
+    t[0] = t[1] = t[2] = 0.0;
+  }
+  const double bfs[3] = {bf0, bf1, bf2};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double* X = c == 0 ? U : (c == 1 ? V : W);
+    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+    long o = c * g.n + m;
+    double a0 = first ? a[c] : A0[o];
+    double t0 = first ? t[c] : T0[o];
+    double b0 = first ? bv[c] : B0[o];
+    double u = ldb(X, g, i, j, k, kind);
+    double p1 = c == 0 ? ldb(P, g, i + 1, j, k, BC_P) : (c == 1 ? ldb(P, g, i, j + 1, k, BC_P) : ldb(P, g, i, j, k + 1, BC_P));
+    double h = c == 0 ? dx : (c == 1 ? dy : dz);
+    double dp = (dt * (-P[m] + p1)) / h;
+    double fv = (F ? F[o] : 0.0) + bfs[c];
+    double r;
+    if (method == FP_METHOD_EXPLICIT) {
+      r = (((u - dp) + (dt * (3.0 * (a[c] + bv[c]) - (a0 + b0))) / 2.0) + (dt * (t[c] + t0)) / 2.0) + dt * fv;
+    } else if (method == FP_METHOD_FFT) {
+      r = ((((u - dp) + (dt * (3.0 * a[c] - a0)) / 2.0) + (dt * bv[c]) / 2.0) + (dt * (t[c] + t0)) / 2.0) + dt * fv;
+    } else {
+      r = ((((u - dp) + (dt * (3.0 * a[c] - a0)) / 2.0) + dt * bv[c]) + (dt * (t[c] + t0)) / 2.0) + dt * fv;
+    }
+    Q[o] = r;
+    A0[o] = a[c];
+    T0[o] = t[c];
+    B0[o] = bv[c];
+    if (Bout) Bout[o] = bv[c];
+  }
+#undef Ug
+#undef Vg
+#undef Wg
+}
+
+// ============================================================ FFT ==========
+// Stockham radix-2 autosort (forward e^{-2 pi i jk/n}, unnormalised; the
+// inverse uses the conjugate twiddles) over `nl` lines resident in LDS.
+// x and y are LDS buffers of nl*n complex; line l occupies [l*n, (l+1)*n).
+// Returns the buffer holding the result.
+__device__ double2* stockham_lds(double2* x, double2* y, int n, int logn, int nl, const double2* __restrict__ tw, bool inv) {
+  int len = n, s = 1, lgs = 0;
+  const int half = n >> 1;
+  while (len > 1) {
+    int m = len >> 1;
+    for (int b = threadIdx.x; b < nl * half; b += blockDim.x) {
+      int line = b >> (logn - 1);
+      int t = b & (half - 1);
+      int p = t >> lgs, q = t & (s - 1);
+      double2 w = tw[p * s];
+      double wr = w.x, wi = inv ? -w.y : w.y;
+      const double2* xl = x + line * n;
+      double2* yl = y + line * n;
+      double2 a = xl[q + s * p], bb = xl[q + s * (p + m)];
+      double2 y0, y1;
+      y0.x = a.x + bb.x;
+      y0.y = a.y + bb.y;
+      double dr = a.x - bb.x, di = a.y - bb.y;
+      y1.x = dr * wr - di * wi;
+      y1.y = dr * wi + di * wr;
+      yl[q + s * (2 * p)] = y0;
+      yl[q + s * (2 * p + 1)] = y1;
+    }
+    __syncthreads();
+    double2* tt = x; x = y; y = tt;
+    len = m; s <<= 1; lgs++;
+  }
+  return x;
+}
+
+__device__ __forceinline__ int ilog2(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
+
+// real x-lines -> half spectrum rows (fftr2c along x)
+__global__ __launch_bounds__(256) void k_fft_x_fwd(Grid g, const double* __restrict__ q, double2* __restrict__ spec,
+                                                   const double2* __restrict__ tw, int lines_per_block) {
+  extern __shared__ double2 lds[];
+  const int n = g.nx, nl = lines_per_block;
+  double2* x = lds;
+  double2* y = lds + nl * n;
+  long nlines = (long)g.ny * g.nz;
+  long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+    int l = e / n, i = e % n;
+    long L = l0 + l;
+    double v = L < nlines ? q[L * n + i] : 0.0;
+    x[e] = make_double2(v, 0.0);
+  }
+  __syncthreads();
+  double2* r = stockham_lds(x, y, n, ilog2(n), nl, tw, false);
+  const int nxh = g.nxh;
+  for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
+    int l = e / nxh, i = e % nxh;
+    long L = l0 + l;
+    if (L >= nlines) continue;
+    long j = L % g.ny, k = L / g.ny;
+    spec[i + (long)nxh * ((j + g.jo) + g.sy * k)] = r[l * n + i];
+  }
+}
+
+// half spectrum rows -> real x-lines (fftc2r along x)
+__global__ __launch_bounds__(256) void k_fft_x_inv(Grid g, const double2* __restrict__ spec, double* __restrict__ out,
+                                                   const double2* __restrict__ tw, int lines_per_block) {
+  extern __shared__ double2 lds[];
+  const int n = g.nx, nl = lines_per_block, h = n / 2, nxh = g.nxh;
+  double2* x = lds;
+  double2* y = lds + nl * n;
+  long nlines = (long)g.ny * g.nz;
+  long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
+    int l = e / nxh, i = e % nxh;
+    long L = l0 + l;
+    double2 v = make_double2(0.0, 0.0);
+    if (L < nlines) {
+      long j = L % g.ny, k = L / g.ny;
+      v = spec[i + (long)nxh * ((j + g.jo) + g.sy * k)];
+    }
+    if (i == 0 || i == h) v.y = 0.0;
+    x[l * n + i] = v;
+    if (i > 0 && i < h) x[l * n + (n - i)] = make_double2(v.x, -v.y);
+  }
+  __syncthreads();
+  double2* r = stockham_lds(x, y, n, ilog2(n), nl, tw, true);
+  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+    int l = e / n, i = e % n;
+    long L = l0 + l;
+    if (L < nlines) out[L * n + i] = r[e].x;
+  }
+}
+
+// complex lines along y (axis 1) or z (axis 2) of the spectrum.  mode 0:
+// forward, 1: inverse, 2: forward, divide by the periodic LHS of `kind`, then
+// inverse (fft_solve's divide fused between the z transforms).
+__global__ __launch_bounds__(256) void k_fft_yz(Grid g, double2* __restrict__ spec, const double2* __restrict__ tw,
+                                                int axis, int mode, int kind, int row0, int nrows, int ti,
+                                                const double* __restrict__ kx2, const double* __restrict__ ky2,
+                                                const double* __restrict__ kz2) {
+  extern __shared__ double2 lds[];
+  const int n = axis == 1 ? g.ny : g.nz;
+  const int nxh = g.nxh;
+  const long stride = axis == 1 ? (long)nxh : (long)nxh * g.sy;
+  const int ntile = (nxh + ti - 1) / ti;
+  int tile = blockIdx.x % ntile;
+  long other = blockIdx.x / ntile;  // k for axis 1, row for axis 2
+  int i0 = tile * ti;
+  double2* base;
+  if (axis == 1) base = spec + (long)nxh * (g.jo + g.sy * other);
+  else base = spec + (long)nxh * (row0 + other);
+  double2* x = lds;
+  double2* y = lds + ti * n;
+  // LDS layout: line l (i = i0+l) contiguous
+  for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
+    int l = e % ti, r = e / ti;
+    int i = i0 + l;
+    x[l * n + r] = i < nxh ? base[i + stride * r] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  int logn = ilog2(n);
+  double2* res = stockham_lds(x, y, n, logn, ti, tw, mode == 1);
+  if (mode == 2) {
+    double2* other_buf = res == x ? y : x;
+    const double ntot = (double)((long)g.nx * g.ny * g.nz);
+    const double coef = ((g.dt * g.beta) / g.re) / 2.0;
+    for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
+      int l = e / n, kk = e % n;
+      int i = i0 + l;
+      if (i >= nxh) continue;
+      long jj = other;  // axis 2: row index == j for periodic grids
+      double lhs;
+      if (kind == 3) lhs = -((kx2[i] + ky2[jj]) + kz2[kk]);
+      else lhs = 1.0 + coef * ((kx2[i] + ky2[jj]) + kz2[kk]);
+      double2 s = res[e];
+      double re = s.x / ntot, im = s.y / ntot;
+      if (fabs(lhs) < 1.0e-16) { s.x = 0.0; s.y = 0.0; }
+      else { s.x = re / lhs; s.y = im / lhs; }
+      res[e] = s;
+    }
+    __syncthreads();
+    res = stockham_lds(res, other_buf, n, logn, ti, tw, true);
+  }
+  for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
+    int l = e % ti, r = e / ti;
+    int i = i0 + l;
+    if (i < nxh) base[i + stride * r] = res[l * n + r];
+  }
+}
+
+// tridiagonal line solve along y with the wall rows of fenep_2d.f90:1222-1331
+// (tdma, fenep_2d.f90:1145-1177), one thread per (kx, kz) line.
+__global__ __launch_bounds__(256) void k_thomas(Grid g, double2* __restrict__ spec, double* __restrict__ thp, int kind,
+                                                const double* __restrict__ kx2, const double* __restrict__ kz2) {
+  const long lines = (long)g.nxh * g.nz;
+  long L = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (L >= lines) return;
+  const int nxh = g.nxh, ny = g.ny, size = ny + 2;
+  int i = (int)(L % nxh);
+  long k = L / nxh;
+  double2* col = spec + i + (long)nxh * g.sy * k;  // row r at col[nxh*r]
+  const double nxz = (double)((long)g.nx * g.nz);
+  const double coef = ((g.dt * g.beta) / g.re) / 2.0;
+  const double idy2 = g.dy * g.dy;
+  const double kk = kx2[i] + kz2[k];
+  // row coefficients
+  auto coeffs = [&](int r, double& a, double& b, double& c, bool& rhs) {
+    rhs = false;
+    if (kind == 3) {
+      if (r == 0) { a = 1.0; b = -1.0; c = 0.0; }
+      else if (r == size - 1) { a = 1.0; b = 0.0; c = -1.0; }
+      else { a = -(kk + 2.0 / idy2); b = 1.0 / idy2; c = 1.0 / idy2; rhs = true; }
+    } else if (kind == 1) {
+      if (r == 0) { a = 1.0; b = 0.0; c = 0.0; }
+      else if (r >= ny) { a = 1.0; b = 0.0; c = 0.0; }
+      else { a = 1.0 + coef * (kk + 2.0 / idy2); b = -(coef / idy2); c = -(coef / idy2); rhs = true; }
+    } else {
+      if (r == 0) { a = 1.0; b = 1.0; c = 0.0; }
+      else if (r == size - 1) { a = 1.0; b = 0.0; c = 1.0; }
+      else { a = 1.0 + coef * (kk + 2.0 / idy2); b = -(coef / idy2); c = -(coef / idy2); rhs = true; }
+    }
+  };
+  double a, b, c; bool rhs;
+  coeffs(0, a, b, c, rhs);
+  double Pp = -b / a;
+  double Qr = 0.0 / a, Qi = 0.0 / a;
+  thp[L] = Pp;
+  col[0] = make_double2(Qr, Qi);
+  for (int r = 1; r < size; r++) {
+    coeffs(r, a, b, c, rhs);
+    double dr = 0.0, di = 0.0;
+    if (rhs) { double2 s = col[(long)nxh * r]; dr = s.x / nxz; di = s.y / nxz; }
+    double den = a + c * Pp;
+    double P, nQr, nQi;
+    if (fabs(den) < 1.0e-16) { P = 0.0; nQr = 0.0; nQi = 0.0; }
+    else { P = -b / den; nQr = (dr - c * Qr) / den; nQi = (di - c * Qi) / den; }
+    Pp = P; Qr = nQr; Qi = nQi;
+    thp[L + lines * r] = P;
+    col[(long)nxh * r] = make_double2(Qr, Qi);
+  }
+  double xr = Qr, xi = Qi;
+  for (int r = size - 2; r >= 0; r--) {
+    double P = thp[L + lines * r];
+    double2 q = col[(long)nxh * r];
+    xr = P * xr + q.x;
+    xi = P * xi + q.y;
+    col[(long)nxh * r] = make_double2(xr, xi);
+  }
+}
+
+static int lines_for(int n) { return std::max(1, std::min(16, 2048 / n)); }
+
+void f64_solve(Ctx& c, int kind, const double* q, double* x) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  int lx = lines_for(g.nx);
+  long xlines = (long)g.ny * g.nz;
+  size_t shx = sizeof(double2) * 2 * lx * g.nx;
+  hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, q, c.spec, c.twx, lx);
+  int tiy = std::max(1, std::min(16, 2048 / g.ny)), tiz = std::max(1, std::min(16, 2048 / g.nz));
+  int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
+  size_t shy = sizeof(double2) * 2 * tiy * g.ny, shz = sizeof(double2) * 2 * tiz * g.nz;
+  if (!g.walls) {
+    hipLaunchKernelGGL(k_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, c.spec, c.twy, 1, 0, kind, 0, 0, tiy, c.kx2, c.ky2, c.kz2);
+    hipLaunchKernelGGL(k_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, c.spec, c.twz, 2, 2, kind, 0, g.ny, tiz, c.kx2, c.ky2, c.kz2);
+    hipLaunchKernelGGL(k_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, c.spec, c.twy, 1, 1, kind, 0, 0, tiy, c.kx2, c.ky2, c.kz2);
+  } else {
+    hipLaunchKernelGGL(k_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, c.spec, c.twz, 2, 0, kind, 1, g.ny, tiz, c.kx2, c.ky2, c.kz2);
+    long lines = (long)g.nxh * g.nz;
+    hipLaunchKernelGGL(k_thomas, dim3(nblk(lines, 256)), dim3(256), 0, s, g, c.spec, c.thp, kind, c.kx2, c.kz2);
+    hipLaunchKernelGGL(k_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, c.spec, c.twz, 2, 1, kind, 1, g.ny, tiz, c.kx2, c.ky2, c.kz2);
+  }
+  hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, c.spec, x, c.twx, lx);
+}
+
+// ======================================================= projection =======
+// fft_poisson right-hand side (fenep.f90:1550-1552)
+__global__ __launch_bounds__(256) void k_divergence(Grid g, const double* __restrict__ up, double* __restrict__ q) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+  q[m] = (((-ldb(Uu, g, i - 1, j, k, BC_U) + ldb(Uu, g, i, j, k, BC_U)) / g.dx
+           + (-ldb(Vv, g, i, j - 1, k, BC_V) + ldb(Vv, g, i, j, k, BC_V)) / g.dy)
+          + (-ldb(Ww, g, i, j, k - 1, BC_W) + ldb(Ww, g, i, j, k, BC_W)) / g.dz) / g.dt;
+}
+
+void f64_divergence(Ctx& c, const double* up, double* q) {
+  hipLaunchKernelGGL(k_divergence, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g, up, q);
+}
+
+// march / fft_march (fenep.f90:907-946, 1562-1604)
+__global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const double* __restrict__ up, const double* __restrict__ phi,
+                                               double* __restrict__ U, double* __restrict__ V, double* __restrict__ W,
+                                               double* __restrict__ P) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const double dt = g.dt, dx = g.dx, dy = g.dy, dz = g.dz;
+  const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+#define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)
+  double ph = phi[m];
+  U[m] = ldb(Uu, g, i, j, k, BC_U) - (dt * (-ph + PH(i + 1, j, k))) / dx;
+  V[m] = ldb(Vv, g, i, j, k, BC_V) - (dt * (-ph + PH(i, j + 1, k))) / dy;
+  W[m] = ldb(Ww, g, i, j, k, BC_W) - (dt * (-ph + PH(i, j, k + 1))) / dz;
+  if (explicit_p) {
+    P[m] = P[m] + ph;
+  } else {
+    const double coef = ((dt * g.beta) / g.re) / 2.0;
+    double lap = ((((PH(i - 1, j, k) - 2.0 * ph) + PH(i + 1, j, k)) / (dx * dx)
+                   + ((PH(i, j - 1, k) - 2.0 * ph) + PH(i, j + 1, k)) / (dy * dy))
+                  + ((PH(i, j, k - 1) - 2.0 * ph) + PH(i, j, k + 1)) / (dz * dz));
+    P[m] = (P[m] + ph) - coef * lap;
+  }
+#undef PH
+}
+
+// ============================================================ IBM ==========
+__device__ __forceinline__ double roma_d(double r) {
+  double a = fabs(r);
+  if (a < 1.5 && a >= 0.5) return ((5.0 - 3.0 * a) - sqrt(-(3.0 * ((1.0 - a) * (1.0 - a))) + 1.0)) / 6.0;
+  if (a < 0.5) return (1.0 + sqrt(-(3.0 * (r * r)) + 1.0)) / 3.0;
+  return 0.0;
+}
+
+// delta (fenep.f90:2039-2043) * dX^3 between marker (xc,yc,zc) and the
+// sample of component c at grid index (ii,jj,kk) (1-based, ghosts included).
+__device__ __forceinline__ double ibm_weight(const Grid& g, int c, int ii, int jj, int kk, double xc, double yc, double zc) {
+  double x = ((double)ii - 0.5) * g.dx, y = ((double)jj - 0.5) * g.dy, z = ((double)kk - 0.5) * g.dz;
+  if (c == 0) x = x + 0.5 * g.dx;
+  else if (c == 1) y = y + 0.5 * g.dy;
+  else z = z + 0.5 * g.dz;
+  double d = ((roma_d((x - xc) / g.dx) * roma_d((y - yc) / g.dy)) * roma_d((z - zc) / g.dz)) / ((g.dx * g.dy) * g.dz);
+  return d;
+}
+__device__ __forceinline__ void ibm_origin_d(const Grid& g, int c, double xc, double yc, int& i0, int& j0) {
+  if (c == 0) { i0 = (int)(xc / g.dx - 0.5); j0 = (int)(yc / g.dy); }
+  else if (c == 1) { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy - 0.5); }
+  else { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy); }
+}
+
+// interpolation and marker forces of ibm_Helmholtz (fenep.f90:2076-2119,
+// forces kept as in fenep_mpi.f90:1845-1853)
+__global__ __launch_bounds__(256) void k_ibm_interp(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
+                                                    const double* __restrict__ ua, double* __restrict__ mb,
+                                                    double* __restrict__ mf) {
+  long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (id >= nmk) return;
+  int n = (int)(id / ((long)nc * nl)) + 1;
+  double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
+  const double dx3 = (g.dx * g.dx) * g.dx;
+  for (int c = 0; c < 3; c++) {
+    int i0, j0; ibm_origin_d(g, c, xc, yc, i0, j0);
+    const double* X = ua + c * g.n;
+    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+    double acc = 0.0;
+    for (int kk = n - 1; kk <= n + 1; kk++)
+      for (int jj = j0; jj <= j0 + 2; jj++)
+        for (int ii = i0; ii <= i0 + 2; ii++) {
+          double v = ldb(X, g, ii - 1, jj - 1, kk - 1, kind);
+          acc = acc + (v * ibm_weight(g, c, ii, jj, kk, xc, yc, zc)) * dx3;
+        }
+    mb[c * nmk + id] = acc;
+    double uc = mk[(3 + c) * nmk + id];
+    mf[c * nmk + id] = 0.0 + (uc - acc) / g.dt;
+  }
+}
+
+// spreading (fenep.f90:2125-2162) as a gather over the CSR of contributions
+// of each ghosted target, in the reference's marker loop order.
+__global__ __launch_bounds__(256) void k_ibm_spread(Grid g, int comp, int rows, const int* __restrict__ rowp,
+                                                    const long* __restrict__ tgt, const int* __restrict__ ent,
+                                                    long nmk, int nc, int nl, const double* __restrict__ mk,
+                                                    const double* __restrict__ mf, double* __restrict__ ft) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const long gx = g.nx + 2, gy = g.ny + 2;
+  long t = tgt[r];
+  int ii = (int)(t % gx), jj = (int)((t / gx) % gy), kk = (int)(t / (gx * gy));
+  const double dx3 = (g.dx * g.dx) * g.dx;
+  double acc = 0.0;
+  for (int e = rowp[r]; e < rowp[r + 1]; e++) {
+    long id = ent[e];
+    double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
+    acc = acc + (mf[comp * nmk + id] * ibm_weight(g, comp, ii, jj, kk, xc, yc, zc)) * dx3;
+  }
+  ft[t] = acc;
+}
+
+// ibm_PBM fold (fenep.f90:2195-2205), one axis per launch
+__global__ __launch_bounds__(256) void k_ibm_fold(Grid g, int axis, double* __restrict__ ft) {
+  const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  auto G = [&](long i, long j, long k) { return i + gx * (j + gy * k); };
+  if (axis == 0) {
+    if (t >= gy * gz) return;
+    long j = t % gy, k = t / gy;
+    ft[G(1, j, k)] = ft[G(1, j, k)] + ft[G(g.nx + 1, j, k)];
+    ft[G(g.nx, j, k)] = ft[G(g.nx, j, k)] + ft[G(0, j, k)];
+  } else if (axis == 1) {
+    if (t >= gx * gz) return;
+    long i = t % gx, k = t / gx;
+    ft[G(i, 1, k)] = ft[G(i, 1, k)] + ft[G(i, g.ny + 1, k)];
+    ft[G(i, g.ny, k)] = ft[G(i, g.ny, k)] + ft[G(i, 0, k)];
+  } else {
+    if (t >= gx * gy) return;
+    long i = t % gx, j = t / gx;
+    ft[G(i, j, 1)] = ft[G(i, j, 1)] + ft[G(i, j, g.nz + 1)];
+    ft[G(i, j, g.nz)] = ft[G(i, j, g.nz)] + ft[G(i, j, 0)];
+  }
+}
+
+// ibm_predict right-hand side (fenep.f90:2229): Ua + dt f - dt B/2
+__global__ __launch_bounds__(256) void k_ibm_rhs(Grid g, const double* __restrict__ ua, const double* __restrict__ ft,
+                                                 const double* __restrict__ B, double* __restrict__ q) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= 3 * g.n) return;
+  int c = (int)(m / g.n);
+  long mm = m % g.n;
+  int i = (int)(mm % g.nx), j = (int)((mm / g.nx) % g.ny), k = (int)(mm / ((long)g.nx * g.ny));
+  const long gx = g.nx + 2, gy = g.ny + 2, gn = gx * gy * (g.nz + 2);
+  double f = ft[c * gn + (i + 1) + gx * ((j + 1) + gy * (long)(k + 1))];
+  int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+  double u = ldb(ua + c * g.n, g, i, j, k, kind);
+  q[m] = (u + g.dt * f) - (g.dt * B[m]) / 2.0;
+}
+
+void f64_ibm(Ctx& c) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(k_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, c.nc, c.nl, c.mk, c.Q, c.mb, c.mf);
+  const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
+  for (int comp = 0; comp < 3; comp++) {
+    double* ft = c.ftmp + comp * c.gn;
+    if (c.sp_rows[comp] > 0)
+      hipLaunchKernelGGL(k_ibm_spread, dim3(nblk(c.sp_rows[comp], 256)), dim3(256), 0, s, g, comp, c.sp_rows[comp],
+                         c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.nmk, c.nc, c.nl, c.mk, c.mf, ft);
+    hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gy * gz, 256)), dim3(256), 0, s, g, 0, ft);
+    hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
+    hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
+  }
+  hipLaunchKernelGGL(k_ibm_rhs, dim3(nblk(3 * g.n, 256)), dim3(256), 0, s, g, c.Q, c.ftmp, c.B, c.Up);
+}
+
+// Host-side construction of the spreading CSR: for each ghosted target the
+// contributing (marker, stencil slot) pairs in the order of the reference's
+// loops (markers n, m, l; then k, j, i), extended by every point the ghost
+// folds write to, so that one gather pass per step rewrites all of them.
+void f64_ibm_setup(Ctx& c) {
+  const Grid& g = c.g;
+  std::vector<double> mk(6 * c.nmk);
+  FP_HIP_CHECK(hipMemcpy(mk.data(), c.mk, sizeof(double) * mk.size(), hipMemcpyDeviceToHost));
+  const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2, gn = gx * gy * gz;
+  for (int comp = 0; comp < 3; comp++) {
+    std::map<long, std::vector<int>> rows;
+    for (long id = 0; id < c.nmk; id++) {
+      int n = (int)(id / ((long)c.nc * c.nl)) + 1;
+      double xc = mk[id], yc = mk[c.nmk + id];
+      int i0, j0;
+      if (comp == 0) { i0 = (int)(xc / g.dx - 0.5); j0 = (int)(yc / g.dy); }
+      else if (comp == 1) { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy - 0.5); }
+      else { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy); }
+      int slot = 0;
+      for (int kk = n - 1; kk <= n + 1; kk++)
+        for (int jj = j0; jj <= j0 + 2; jj++)
+          for (int ii = i0; ii <= i0 + 2; ii++, slot++) {
+            if (ii < 0 || ii > g.nx + 1 || jj < 0 || jj > g.ny + 1 || kk < 0 || kk > g.nz + 1)
+              throw std::string("IBM stencil outside the ghosted grid");
+            rows[ii + gx * (jj + gy * (long)kk)].push_back((int)id);
+          }
+    }
+    // fold closure
+    std::vector<char> mark(gn, 0);
+    for (auto& kv : rows) mark[kv.first] = 1;
+    auto G = [&](long i, long j, long k) { return i + gx * (j + gy * k); };
+    for (long k = 0; k < gz; k++) for (long j = 0; j < gy; j++) {
+      mark[G(1, j, k)] |= mark[G(g.nx + 1, j, k)]; mark[G(g.nx, j, k)] |= mark[G(0, j, k)];
+    }
+    for (long k = 0; k < gz; k++) for (long i = 0; i < gx; i++) {
+      mark[G(i, 1, k)] |= mark[G(i, g.ny + 1, k)]; mark[G(i, g.ny, k)] |= mark[G(i, 0, k)];
+    }
+    for (long j = 0; j < gy; j++) for (long i = 0; i < gx; i++) {
+      mark[G(i, j, 1)] |= mark[G(i, j, g.nz + 1)]; mark[G(i, j, g.nz)] |= mark[G(i, j, 0)];
+    }
+    std::vector<int> rowp(1, 0), ent;
+    std::vector<long> tgt;
+    for (long t = 0; t < gn; t++) {
+      if (!mark[t]) continue;
+      auto it = rows.find(t);
+      if (it != rows.end()) for (int id : it->second) ent.push_back(id);
+      tgt.push_back(t);
+      rowp.push_back((int)ent.size());
+    }
+    if (c.sp_row[comp]) { (void)hipFree(c.sp_row[comp]); (void)hipFree(c.sp_tgt[comp]); (void)hipFree(c.sp_ent[comp]); }
+    c.sp_rows[comp] = (int)tgt.size();
+    FP_HIP_CHECK(hipMalloc(&c.sp_row[comp], sizeof(int) * rowp.size()));
+    FP_HIP_CHECK(hipMalloc(&c.sp_tgt[comp], sizeof(long) * std::max<size_t>(1, tgt.size())));
+    FP_HIP_CHECK(hipMalloc(&c.sp_ent[comp], sizeof(int) * std::max<size_t>(1, ent.size())));
+    FP_HIP_CHECK(hipMemcpy(c.sp_row[comp], rowp.data(), sizeof(int) * rowp.size(), hipMemcpyHostToDevice));
+    if (!tgt.empty()) FP_HIP_CHECK(hipMemcpy(c.sp_tgt[comp], tgt.data(), sizeof(long) * tgt.size(), hipMemcpyHostToDevice));
+    if (!ent.empty()) FP_HIP_CHECK(hipMemcpy(c.sp_ent[comp], ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice));
+  }
+}
+
+// ============================================================ SOR ==========
+// Red-black SOR for the pressure-correction Poisson equation (the reference
+// sweeps lexicographically with odd/even i per row, fenep.f90:885-901; the
+// converged solution is the same to the solver tolerance).  Residuals are
+// taken before each update, as E in the reference.
+__global__ __launch_bounds__(256) void k_sor_color(Grid g, int color, double om, const double* __restrict__ q,
+                                                   double* __restrict__ phi, double* __restrict__ e2) {
+  long h = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long half = g.n / 2;
+  double e = 0.0;
+  if (h < half) {
+    // enumerate cells of one colour: (i + j + k) % 2 == color
+    long jk = h / (g.nx / 2);
+    int j = (int)(jk % g.ny), k = (int)(jk / g.ny);
+    int i = (int)(2 * (h % (g.nx / 2))) + ((j + k + color) & 1);
+    long m = cidx(g, i, j, k);
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    double BX = 1.0 / (dx * dx), BY = 1.0 / (dy * dy), BZ = 1.0 / (dz * dz);
+    double B0 = ((((BX + BX) + BY) + BY) + BZ) + BZ;
+    double p0 = phi[m];
+    e = (((((BZ * ldb(phi, g, i, j, k - 1, BC_P) + BY * ldb(phi, g, i, j - 1, k, BC_P)) + BX * ldb(phi, g, i - 1, j, k, BC_P))
+           - B0 * p0) + BX * ldb(phi, g, i + 1, j, k, BC_P)) + BY * ldb(phi, g, i, j + 1, k, BC_P)
+         + BZ * ldb(phi, g, i, j, k + 1, BC_P)) - q[m];
+    phi[m] = p0 + (om * e) / B0;
+  }
+  // block reduction of e^2
+  __shared__ double sh[256];
+  sh[threadIdx.x] = e * e;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(e2, sh[0]);
+}
+
+__global__ __launch_bounds__(256) void k_sumsq(long n, const double* __restrict__ q, double* __restrict__ out) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double v = m < n ? q[m] * q[m] : 0.0;
+  __shared__ double sh[256];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, sh[0]);
+}
+
+__global__ void k_zero_n(double* p, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+int f64_sor(Ctx& c, const double* q, double* phi) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  double om = c.p.sor_omega > 0.0 ? c.p.sor_omega : 2.0 / (1.0 + sin(3.14159265358979323846 * g.dx));
+  const int check = 10;  // residual read back every `check` sweeps
+  double* red = c.red;   // [0] = er0, [1 + s] = residual of sweep s
+  hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, red, 1 + check);
+  hipLaunchKernelGGL(k_sumsq, dim3(nblk(g.n, 256)), dim3(256), 0, s, g.n, q, red);
+  double er0 = 0.0;
+  FP_HIP_CHECK(hipMemcpyAsync(&er0, red, sizeof(double), hipMemcpyDeviceToHost, s));
+  FP_HIP_CHECK(hipStreamSynchronize(s));
+  if (er0 == 0.0) return 0;
+  long half = g.n / 2;
+  int itr = 0;
+  std::vector<double> res(check);
+  while (itr < c.p.sor_itrmax) {
+    int batch = std::min(check, c.p.sor_itrmax - itr);
+    hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, red + 1, check);
+    for (int b = 0; b < batch; b++) {
+      hipLaunchKernelGGL(k_sor_color, dim3(nblk(half, 256)), dim3(256), 0, s, g, 0, om, q, phi, red + 1 + b);
+      hipLaunchKernelGGL(k_sor_color, dim3(nblk(half, 256)), dim3(256), 0, s, g, 1, om, q, phi, red + 1 + b);
+    }
+    FP_HIP_CHECK(hipMemcpyAsync(res.data(), red + 1, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+    for (int b = 0; b < batch; b++) {
+      itr++;
+      if (std::sqrt(res[b] / er0) < c.p.sor_eps) return itr;
+    }
+  }
+  return itr;
+}
+
+// ======================================================= per step =========
+void f64_select(Ctx& c) {
+  hipLaunchKernelGGL(k_select, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g, c.C[c.cur], c.sel);
+}
+
+void f64_cstar_newton(Ctx& c) {
+  hipLaunchKernelGGL(k_cstar_newton, dim3(nblk(c.g.n, 128)), dim3(128), 0, c.stream, c.g, c.C[c.cur], c.sel, c.U, c.V, c.W,
+                     c.C[1 - c.cur], c.fC, c.nitr, c.p.newton_dc, c.p.newton_eps, c.p.newton_itrmax);
+  c.cur = 1 - c.cur;
+}
+
+void f64_predictor(Ctx& c) {
+  const Grid& g = c.g;
+  int polymer = c.p.beta != 1.0;
+  double* Bout = c.p.method == FP_METHOD_IBM ? c.B : nullptr;
+  hipLaunchKernelGGL(k_predictor, dim3(nblk(g.n, 256)), dim3(256), 0, c.stream, g, c.p.method, c.has_history ? 0 : 1,
+                     polymer, c.U, c.V, c.W, c.P, c.C[c.cur], c.fC, c.A0, c.T0, c.B0, Bout, c.F,
+                     c.p.body_force[0], c.p.body_force[1], c.p.body_force[2], c.Q);
+}
+
+void f64_march(Ctx& c) {
+  hipLaunchKernelGGL(k_march, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g,
+                     c.p.method == FP_METHOD_EXPLICIT ? 1 : 0, c.Up, c.Phi, c.U, c.V, c.W, c.P);
+}
+
+void f64_zero_c(Ctx& c) {
+  FP_HIP_CHECK(hipMemsetAsync(c.C[c.cur], 0, sizeof(double) * 6 * c.g.n, c.stream));
+}
+
+// ===================================================== diagnostics ========
+__global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__ U, const double* __restrict__ V,
+                                              const double* __restrict__ W, const double* __restrict__ C,
+                                              const int* __restrict__ nitr, int polymer, double* __restrict__ red) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double s[5] = {0, 0, 0, 0, 0};  // ke, trsum, trsq, nitr, nan
+  double mx[6] = {-1e300, -1e300, -1e300, -1e300, 1e300, 0.0};  // umax vmax wmax trmax -trmin divmax
+  if (m < g.n) {
+    int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+    double u = ldb(U, g, i, j, k, BC_U), v = ldb(V, g, i, j, k, BC_V), w = ldb(W, g, i, j, k, BC_W);
+    double ur = (u + ldb(U, g, i - 1, j, k, BC_U)) / 2.0;
+    double vr = (v + ldb(V, g, i, j - 1, k, BC_V)) / 2.0;
+    double wr = (w + ldb(W, g, i, j, k - 1, BC_W)) / 2.0;
+    s[0] = ur * ur + vr * vr + wr * wr;
+    double tr = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m];
+    s[1] = tr; s[2] = tr * tr;
+    s[3] = polymer ? (double)nitr[m] : 0.0;
+    s[4] = isnan(U[m]) ? 1.0 : 0.0;
+    mx[0] = u; mx[1] = v; mx[2] = w; mx[3] = tr; mx[4] = tr;
+    double dv = ((u - ldb(U, g, i - 1, j, k, BC_U)) / g.dx + (v - ldb(V, g, i, j - 1, k, BC_V)) / g.dy)
+                + (w - ldb(W, g, i, j, k - 1, BC_W)) / g.dz;
+    mx[5] = fabs(dv);
+  }
+  __shared__ double sh[256];
+  for (int a = 0; a < 11; a++) {
+    double v = a < 5 ? s[a] : mx[a - 5];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        double x = sh[threadIdx.x], y = sh[threadIdx.x + o];
+        if (a < 5) sh[threadIdx.x] = x + y;
+        else if (a == 9) sh[threadIdx.x] = fmin(x, y);
+        else sh[threadIdx.x] = fmax(x, y);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) red[a * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+void f64_diag(Ctx& c, fp_diag* d) {
+  const Grid& g = c.g;
+  int nb = nblk(g.n, 256);
+  double* buf;
+  FP_HIP_CHECK(hipMalloc(&buf, sizeof(double) * 11 * nb));
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c.stream, g, c.U, c.V, c.W, c.C[c.cur], c.nitr, c.p.beta != 1.0 ? 1 : 0, buf);
+  std::vector<double> h(11 * (size_t)nb);
+  FP_HIP_CHECK(hipMemcpyAsync(h.data(), buf, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+  (void)hipFree(buf);
+  double s[5] = {0, 0, 0, 0, 0}, mx[6] = {-1e300, -1e300, -1e300, -1e300, 1e300, 0};
+  for (int b = 0; b < nb; b++) {
+    for (int a = 0; a < 5; a++) s[a] += h[a * nb + b];
+    for (int a = 0; a < 6; a++) {
+      double v = h[(5 + a) * nb + b];
+      mx[a] = a == 4 ? std::min(mx[a], v) : std::max(mx[a], v);
+    }
+  }
+  double N = (double)g.n;
+  d->ke = (s[0] / 2.0) / N;
+  d->cfl = std::max(mx[0] * g.dt / g.dx, std::max(mx[1] * g.dt / g.dy, mx[2] * g.dt / g.dz)) * 6.0;
+  d->trc_mean = s[1] / N;
+  d->trc_std = std::sqrt(std::max(0.0, s[2] / N - d->trc_mean * d->trc_mean));
+  d->trc_max = mx[3];
+  d->trc_min = mx[4];
+  d->div_max = mx[5];
+  d->newton_itr_mean = s[3] / N;
+  d->nan_flag = s[4] > 0 ? 1 : 0;
+}
+
+}  // namespace fp

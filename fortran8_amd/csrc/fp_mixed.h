@@ -1,0 +1,17 @@
+// Mixed-precision path: FP32 storage and stencils, FP64 where accuracy
+// demands it (Peterlin factor near the pole, Poisson defect correction,
+// reductions).  Entry points used by the C-ABI host code.
+#pragma once
+#include "fp_internal.h"
+
+namespace fp {
+void mixed_create(Ctx& c);
+void mixed_destroy(Ctx& c);
+void mixed_load_from_f64(Ctx& c);   // FP64 state buffers -> mixed state
+void mixed_store_to_f64(Ctx& c);    // mixed state -> FP64 state buffers
+void mixed_set_force(Ctx& c);
+void mixed_set_markers(Ctx& c);
+const double* mixed_marker_forces(Ctx& c);
+double mixed_last_ndc(Ctx& c);
+void mixed_step(Ctx& c);
+}  // namespace fp

@@ -1,0 +1,160 @@
+/*
+ * fenep_gpu.h -- C-ABI of the MI355X FENE-P / IBM SMAC time step.
+ *
+ * The reference (machapin/fortran8) has no plugin or FFI: the step is a set of
+ * Fortran module subroutines with explicit-shape real(8) dummies, called in a
+ * fixed order from `program main` (fenep.f90:2477-2527).  This header is the
+ * boundary that replaces that loop.  Every entry point takes plain pointers and
+ * sizes, returns 0 on success and a nonzero FP_E* code on error (the message is
+ * then available from fp_last_error).  A context owns all of its device memory
+ * and one HIP stream; contexts are independent, a single context is not
+ * re-entrant.
+ *
+ * Array layouts at the boundary are the reference's (Fortran column-major, x
+ * fastest):
+ *   scalars   U,V,W,P  : (0:NX+1, 0:NY+1, 0:NZ+1)  ghosted, real(8)
+ *   tensor    C        : (6, 0:NX+1, 0:NY+1, 0:NZ+1) components xx,xy,xz,yy,yz,zz
+ *   histories A0,T0,B0 : (1:NX, 1:NY, 1:NZ) per component
+ *   markers            : (NC, NL, NZ) flattened, l (cylinder) fastest
+ * Ghost values passed in are ignored; the library applies the boundary rules
+ * itself (periodic copy `PBM`/`C_PBM`, fenep.f90:147-166, or the y-wall rules
+ * `U_kabe`..`C_kabe`, fenep_2d.f90:1098-1143).
+ */
+#ifndef FENEP_GPU_H
+#define FENEP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define FP_OK            0
+#define FP_E_ARG         1   /* invalid argument / unsupported parameter      */
+#define FP_E_HIP         2   /* HIP runtime error                             */
+#define FP_E_STATE       3   /* call out of order (e.g. step before set_state)*/
+#define FP_E_NAN         4   /* NaN detected in the velocity field            */
+#define FP_E_NOMEM       5
+
+/* method: fenep.f90:17 */
+#define FP_METHOD_EXPLICIT 0  /* navier (AB2) + poisson + march               */
+#define FP_METHOD_FFT      1  /* fft_navier (Crank-Nicolson) + fft_poisson    */
+#define FP_METHOD_IBM      2  /* ibm_preliminary/Helmholtz/predict + poisson  */
+
+#define FP_POISSON_FFT 0      /* spectral (+ tridiagonal line solve with walls) */
+#define FP_POISSON_SOR 1      /* red-black SOR, fenep.f90:848-905               */
+
+#define FP_PREC_F64   0       /* plain FP64, parity path                       */
+#define FP_PREC_MIXED 1       /* FP32 bulk, FP64 defect correction             */
+
+/* Run parameters.  Names follow the reference's module constants
+ * (fenep.f90:4-49); Lp is L itself (f = (Lp^2-3)/(Lp^2-trC), fenep.f90:513). */
+typedef struct fp_params {
+  int32_t nx, ny, nz;          /* NX, NY, NZ (powers of two, >= 4)            */
+  double  lx, ly, lz;          /* domain lengths; dX = lx/NX etc.             */
+  double  dt, re, beta, wi, lp;
+  int32_t method;              /* FP_METHOD_*                                 */
+  int32_t walls_y;             /* 0: periodic in y; 1: no-slip walls at y=0,Ly */
+  int32_t poisson_variant;     /* FP_POISSON_*                                */
+  int32_t precision;           /* FP_PREC_*                                   */
+  int32_t newton_itrmax;       /* Lyapunov Newton, fenep.f90:527 (10000)      */
+  double  newton_eps;          /* fenep.f90:528 (1e-6)                        */
+  double  newton_dc;           /* FD Jacobian step, fenep.f90:526 (1e-5)      */
+  int32_t sor_itrmax;          /* fenep.f90:857 (10000)                       */
+  double  sor_eps;             /* fenep.f90:856 (1e-5)                        */
+  double  sor_omega;           /* <=0: 2/(1+sin(pi*dX)), fenep.f90:855         */
+  double  body_force[3];       /* constant body force (mean pressure gradient)*/
+  int32_t n_dc_max;            /* mixed: max FP64 defect-correction sweeps     */
+  double  dc_tol;              /* mixed: relative defect tolerance             */
+  int32_t reserved[8];
+} fp_params;
+
+/* Diagnostics (log_progress, fenep.f90:2266-2342, plus drag and divergence). */
+typedef struct fp_diag {
+  double ke;          /* kinetic energy on the regular grid / (NX NY NZ)     */
+  double cfl;         /* max(u dt/dX, v dt/dY, w dt/dZ) * 6                  */
+  double trc_min, trc_max, trc_mean, trc_std;
+  double div_max;     /* max |div_h u| over cells                            */
+  double drag;        /* -sum_markers Fxc * dX^3                             */
+  double lift;        /* -sum_markers Fyc * dX^3                             */
+  double newton_itr_mean;
+  double sor_itr;     /* iterations of the last SOR solve (or -1)            */
+  double n_dc;        /* defect-correction sweeps of the last mixed solve    */
+  int32_t nan_flag;   /* nonzero: NaN in U (driver prints 'NaN value')        */
+  int32_t step;       /* steps taken by this context                         */
+} fp_diag;
+
+/* Fill params with the reference defaults (fenep.f90:4-49, method=1, beta=0.9). */
+int fp_default_params(fp_params* p);
+
+/* Replaces the module-level state and plan set-up: fft_init (fenep.f90:1374),
+ * the allocations of program main (fenep.f90:2424-2446). */
+int fp_create(const fp_params* p, void** ctx);
+int fp_destroy(void* ctx);
+const char* fp_last_error(void* ctx);
+
+/* Replaces init/input (fenep.f90:52-145, 1186-1213): copy a full state in.
+ * hist may be NULL; otherwise it holds 9 interior arrays A0x,A0y,A0z,T0x,T0y,
+ * T0z,B0x,B0y,B0z.  has_history=0 reproduces the step-1 rule of fft_navier /
+ * ibm_preliminary / navier (fenep.f90:1445, 1985, 787): the first step copies
+ * the current A,T(,B) into the histories. */
+int fp_set_state(void* ctx, const double* U, const double* V, const double* W,
+                 const double* P, const double* C6, const double* hist,
+                 int has_history);
+/* Copy out (output(), fenep.f90:1215-1233): ghosts filled by the BC rules. */
+int fp_get_state(void* ctx, double* U, double* V, double* W, double* P,
+                 double* C6, double* hist);
+/* Pressure-correction potential Phi, (0:NX+1,0:NY+1,0:NZ+1). */
+int fp_get_phi(void* ctx, double* Phi);
+
+/* Body-force field Fx,Fy,Fz (1:NX,1:NY,1:NZ) (init, fenep.f90:96-120); NULL
+ * pointers clear it.  The constant params.body_force is added on top. */
+int fp_set_body_force(void* ctx, const double* fx, const double* fy,
+                      const double* fz);
+
+/* IBM markers (ibm_init, fenep.f90:1769-1900): n = NC*NL*NZ points with
+ * positions xc,yc,zc and target velocities uc,vc,wc. nc, nl as in the
+ * reference; the marker of plane n sits at zc(n) and interpolates from planes
+ * n-1..n+1. */
+int fp_set_markers(void* ctx, int nc, int nl, const double* xc,
+                   const double* yc, const double* zc, const double* uc,
+                   const double* vc, const double* wc);
+/* Marker forces Fxc,Fyc,Fzc of the last step, (NC,NL,NZ). */
+int fp_get_marker_forces(void* ctx, double* fxc, double* fyc, double* fzc);
+
+/* Advance nsteps time steps (the body of fenep.f90:2477-2527).  sync != 0
+ * waits for completion; otherwise returns after enqueueing on the context's
+ * stream. */
+int fp_step(void* ctx, int nsteps, int sync);
+
+int fp_diagnostics(void* ctx, fp_diag* d);
+
+/* Device-pointer variants for callers that keep the state resident (the
+ * PyTorch wrapper passes tensor.data_ptr()).  Layout: interior only, x
+ * fastest, FP64: U,V,W,P are nx*ny*nz each, C6 is 6 consecutive planes
+ * (structure of arrays).  `stream` is a hipStream_t or NULL for the
+ * context's own stream. */
+int fp_set_state_device(void* ctx, const double* U, const double* V,
+                        const double* W, const double* P, const double* C6,
+                        int has_history, void* stream);
+int fp_get_state_device(void* ctx, double* U, double* V, double* W, double* P,
+                        double* C6, void* stream);
+
+/* Per-kernel timing of the dominant kernels over the last fp_step call with
+ * timing enabled (milliseconds, summed over steps). names is filled with a
+ * comma-separated list; returns the number of entries written to ms. */
+int fp_enable_timing(void* ctx, int on);
+int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
+                    int names_len);
+
+/* Deterministic seeded uniform numbers in [0,1): SplitMix64 of
+ * (seed, offset+i), top 53 bits.  Host-side utility so that the Fortran
+ * driver, the Python wrapper and the tests produce identical inputs. */
+int fp_seeded_uniform(uint64_t seed, uint64_t offset, size_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FENEP_GPU_H */
