@@ -87,11 +87,14 @@ def _spd_perturbed_c(seed, nx, ny, nz, amp=1.0e-3, offset=0):
     c[..., 0] += 1.0
     c[..., 3] += 1.0
     c[..., 5] += 1.0
-    m = np.stack([np.stack([c[..., 0], c[..., 1], c[..., 2]], -1),
-                  np.stack([c[..., 1], c[..., 3], c[..., 4]], -1),
-                  np.stack([c[..., 2], c[..., 4], c[..., 5]], -1)], -2)
-    lam = np.linalg.eigvalsh(m)
-    if lam.min() < 1.0e-6:
+    # Gershgorin: every eigenvalue >= min_r (c_rr - sum_{s!=r} |c_rs|)
+    gersh = np.minimum(np.minimum(c[..., 0] - abs(c[..., 1]) - abs(c[..., 2]),
+                                  c[..., 3] - abs(c[..., 1]) - abs(c[..., 4])),
+                       c[..., 5] - abs(c[..., 2]) - abs(c[..., 4]))
+    if gersh.min() < 1.0e-6:
+        m = np.stack([np.stack([c[..., 0], c[..., 1], c[..., 2]], -1),
+                      np.stack([c[..., 1], c[..., 3], c[..., 4]], -1),
+                      np.stack([c[..., 2], c[..., 4], c[..., 5]], -1)], -2)
         lam_, vec = np.linalg.eigh(m)
         lam_ = np.maximum(lam_, 1.0e-6)
         m = (vec * lam_[..., None, :]) @ np.swapaxes(vec, -1, -2)

@@ -169,7 +169,7 @@ int fp_default_params(fp_params* p) {
   p->method = FP_METHOD_FFT; p->walls_y = 0; p->poisson_variant = FP_POISSON_FFT; p->precision = FP_PREC_F64;
   p->newton_itrmax = 10000; p->newton_eps = 1.0e-6; p->newton_dc = 1.0e-5;
   p->sor_itrmax = 10000; p->sor_eps = 1.0e-5; p->sor_omega = 0.0;
-  p->n_dc_max = 4; p->dc_tol = 1.0e-10;
+  p->n_dc_max = 2; p->dc_tol = 1.0e-10;
   return FP_OK;
 }
 
@@ -182,6 +182,8 @@ int fp_create(const fp_params* p, void** out) {
       p->wi <= 0 || p->lx <= 0 || p->ly <= 0 || p->lz <= 0)
     return FP_E_ARG;
   if (p->walls_y && p->ny < 4) return FP_E_ARG;
+  if (p->precision == FP_PREC_MIXED && (p->method == FP_METHOD_EXPLICIT || p->poisson_variant == FP_POISSON_SOR))
+    return FP_E_ARG;
   Ctx* c = new (std::nothrow) Ctx();
   if (!c) return FP_E_NOMEM;
   try {
@@ -442,6 +444,12 @@ int fp_diagnostics(void* vc, fp_diag* d) {
     }
     d->sor_itr = c->sor_itr;
     d->n_dc = c->mixed ? mixed_last_ndc(*c) : 0.0;
+    if (c->mixed) {
+      double st[3];
+      mixed_stats(*c, st);
+      d->dc_residual = st[0] > 0 ? std::sqrt(st[1] / st[0]) : 0.0;
+      d->sel_guarded = st[2];
+    }
     d->step = c->step;
   } catch (const std::string& e) {
     return fail(c, FP_E_HIP, e);

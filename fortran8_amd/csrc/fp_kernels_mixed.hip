@@ -1,13 +1,1086 @@
-// placeholder, replaced by the mixed-precision kernels
+// Mixed-precision path of the FENE-P / IBM SMAC step.
+//
+// Storage and stencils in FP32 (U, V, W, P, C, histories, FFT spectra, IBM
+// weights); FP64 where the accuracy demands it:
+//   * the pressure-correction Poisson solve is iteratively refined: the
+//     residual b - L_h phi and the potential phi are FP64, each correction is
+//     an FP32 spectral (+ tridiagonal) solve;
+//   * the Peterlin factor f = (L^2-3)/(L^2-trC) when tr C is within 1% of L^2;
+//   * candidate selections of the C+- reconstruction whose FP32 margin is
+//     inside a guard band are re-decided with the FP64 eigenvalue test;
+//   * drag / energy reductions.
+// The Lyapunov Newton uses the analytic Jacobian J = M(f) + (dt/Wi) f' x e^T
+// (e = [1,0,0,1,0,1]) instead of the reference's central differences, which
+// carry no information in FP32 (fenep.f90:558-568 uses dc = 1e-5).
 #include "fp_mixed.h"
+#include "fp_device.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 namespace fp {
-void mixed_create(Ctx&) { throw std::string("mixed precision path not built"); }
-void mixed_destroy(Ctx&) {}
-void mixed_load_from_f64(Ctx&) {}
-void mixed_store_to_f64(Ctx&) {}
-void mixed_set_force(Ctx&) {}
-void mixed_set_markers(Ctx&) {}
-const double* mixed_marker_forces(Ctx& c) { return c.mf; }
-double mixed_last_ndc(Ctx&) { return 0.0; }
-void mixed_step(Ctx&) {}
+
+namespace {
+
+struct Mixed {
+  float *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr;
+  float* C[2] = {nullptr, nullptr};
+  int cur = 0;
+  float* f = nullptr;
+  float *A0 = nullptr, *T0 = nullptr, *B = nullptr;
+  float *Q = nullptr, *Up = nullptr, *F = nullptr;
+  float2* spec = nullptr;
+  float* thp = nullptr;
+  float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  float *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
+  double *phi = nullptr, *b = nullptr;
+  float *r32 = nullptr, *d32 = nullptr;
+  double* red = nullptr;          // [0]=|b|^2, [1]=|r|^2 of the last sweep
+  unsigned* amb = nullptr;        // ambiguous selections re-decided in FP64
+  int* amb_list = nullptr;
+  long amb_cap = 0;
+  // IBM
+  float* wi = nullptr;            // interpolation weights, nmk * 3 * 27
+  int4* org = nullptr;            // per marker and component: i0, j0, n
+  float* uc = nullptr;            // target velocities, 3 planes
+  float* mf = nullptr;            // marker forces, 3 planes
+  double* mf64 = nullptr;
+  float* ftmp = nullptr;          // 3 ghosted raw spread arrays
+  float* sw[3] = {nullptr, nullptr, nullptr};  // CSR entry weights
+  double last_ndc = 0.0;
+};
+
+inline Mixed& MX(Ctx& c) { return *(Mixed*)c.mixed; }
+inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
+
+template <typename T>
+void dalloc(T** p, size_t count) {
+  FP_HIP_CHECK(hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
+  FP_HIP_CHECK(hipMemset(*p, 0, sizeof(T) * (count ? count : 1)));
 }
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+// ================================================================ FENE-P ===
+// Smallest eigenvalue of the symmetric matrix a (xx,xy,xz,yy,yz,zz) by the
+// trigonometric root of Vieta (fenep.f90:425-444), evaluated on the
+// deviatoric part B = a - (tr a/3) I so that p and q do not cancel in FP32.
+// Same sentinel (-100) when the discriminant t >= 0.  Also returns |t| scale
+// via `margin` to flag near-degenerate cases.
+__device__ __forceinline__ float lmin32(const float* a) {
+  const float m = (a[0] + a[3] + a[5]) * (1.0f / 3.0f);
+  const float b11 = a[0] - m, b22 = a[3] - m, b33 = a[5] - m;
+  const float o2 = a[1] * a[1] + a[2] * a[2] + a[4] * a[4];
+  const float p = -(0.5f * (b11 * b11 + b22 * b22 + b33 * b33) + o2);
+  const float det = b11 * (b22 * b33 - a[4] * a[4]) - a[1] * (a[1] * b33 - a[4] * a[2]) + a[2] * (a[1] * a[4] - b22 * a[2]);
+  const float q = -det;
+  const float p3 = p * (1.0f / 3.0f);
+  const float t = 0.25f * q * q + p3 * p3 * p3;
+  if (!(t < 0.0f)) return -100.0f;
+  const float r = sqrtf(-p3);
+  float arg = -q / (2.0f * r * r * r);
+  arg = fminf(1.0f, fmaxf(-1.0f, arg));
+  const float ang = acosf(arg) * (1.0f / 3.0f) + 2.09439510239319549f;
+  return m + 2.0f * r * __cosf(ang);
+}
+
+__device__ __forceinline__ float cand_pf(int l, float cm, float c, float cp) {
+  if (l == 0) return c * 1.5f - cp * 0.5f;
+  if (l == 1) return 0.5f * (cm + c);
+  if (l == 2) return (0.25f * cm + c) - 0.25f * cp;
+  return c;
+}
+__device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
+  if (l == 0) return 0.5f * (c + cp);
+  if (l == 1) return -0.5f * cm + 1.5f * c;
+  if (l == 2) return (c - 0.25f * cm) + 0.25f * cp;
+  return c;
+}
+
+// relative guard band for re-deciding a selection in FP64
+#define SEL_GUARD 1.0e-6f
+
+__global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
+                                                unsigned* __restrict__ amb, int* __restrict__ amb_list, long amb_cap) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  uint8_t s = 0;
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    int di = d == 0, dj = d == 1, dk = d == 2;
+    float cm[6], c[6], cp[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const float* Cq = C + q * g.n;
+      cm[q] = ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+      c[q] = Cq[m];
+      cp[q] = ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+    }
+    float best = 0.0f, second = -1e30f;
+    int idx = 3;
+    float vals[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      float ap[6], an[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, cm[q], c[q], cp[q]); an[q] = cand_nf(l, cm[q], c[q], cp[q]); }
+      vals[l] = fminf(lmin32(ap), lmin32(an));
+    }
+    // maxloc over {v0, v1, v2, 0}: first maximum
+    best = vals[0]; idx = 0;
+    if (vals[1] > best) { best = vals[1]; idx = 1; }
+    if (vals[2] > best) { best = vals[2]; idx = 2; }
+    if (0.0f > best) { best = 0.0f; idx = 3; }
+    // margin to the runner-up
+#pragma unroll
+    for (int l = 0; l < 3; l++) if (l != idx) second = fmaxf(second, vals[l]);
+    if (idx != 3) second = fmaxf(second, 0.0f);
+    float scale = fmaxf(1.0f, fabsf(best));
+    bool sentinel_edge = false;
+#pragma unroll
+    for (int l = 0; l < 3; l++) sentinel_edge |= (vals[l] == -100.0f);
+    if (best - second < SEL_GUARD * scale || sentinel_edge) {
+      unsigned slot = atomicAdd(amb, 1u);
+      if (slot < amb_cap) amb_list[slot] = (int)(m * 3 + d);
+    }
+    s |= (uint8_t)(idx << (2 * d));
+  }
+  sel[m] = s;
+}
+
+// FP64 re-decision of guarded selections with the reference's test
+__global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
+                                                    const unsigned* __restrict__ amb, const int* __restrict__ amb_list,
+                                                    long amb_cap) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long cnt = min((long)*amb, amb_cap);
+  if (t >= cnt) return;
+  int e = amb_list[t];
+  long m = e / 3;
+  int d = e % 3;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int di = d == 0, dj = d == 1, dk = d == 2;
+  double cm[6], c[6], cp[6];
+  for (int q = 0; q < 6; q++) {
+    const float* Cq = C + q * g.n;
+    cm[q] = (double)ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+    c[q] = (double)Cq[m];
+    cp[q] = (double)ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+  }
+  int idx = select_d(cm, c, cp);
+  // the byte holds three axes: update only this axis
+  unsigned* word = (unsigned*)((size_t)(sel + m) & ~(size_t)3);
+  int sh = 8 * (int)((size_t)(sel + m) & 3) + 2 * d;
+  unsigned mask = 3u << sh;
+  unsigned old = *word, assumed;
+  do {
+    assumed = old;
+    unsigned nv = (assumed & ~mask) | ((unsigned)idx << sh);
+    old = atomicCAS(word, assumed, nv);
+  } while (old != assumed);
+}
+
+__device__ __forceinline__ void face_vals_f(const Grid& g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
+                                            int d, int i, int j, int k, bool left, float* out) {
+  i = wrapi(i, g.nx); j = wrapi(j, g.ny); k = wrapi(k, g.nz);
+  long m = cidx(g, i, j, k);
+  int l = (sel[m] >> (2 * d)) & 3;
+  int di = d == 0, dj = d == 1, dk = d == 2;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const float* Cq = C + q * g.n;
+    float c = Cq[m];
+    float cm = l == 3 ? 0.0f : ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+    float cp = l == 3 ? 0.0f : ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+    out[q] = left ? cand_pf(l, cm, c, cp) : cand_nf(l, cm, c, cp);
+  }
+}
+
+// Peterlin factor: FP32 unless tr C is within 1% of L^2 (then FP64)
+__device__ __forceinline__ float peterlin_f(float L2, float s) {
+  if (s > 0.99f * L2) return (float)(((double)L2 - 3.0) / ((double)L2 - (double)s));
+  return (L2 - 3.0f) / (L2 - s);
+}
+__device__ __forceinline__ float dpeterlin_f(float L2, float s) {
+  if (s > 0.99f * L2) {
+    double den = (double)L2 - (double)s;
+    return (float)(((double)L2 - 3.0) / (den * den));
+  }
+  float den = L2 - s;
+  return (L2 - 3.0f) / (den * den);
+}
+
+// Cstar + Lyapunov Newton (analytic Jacobian, FP32) + Peterlin factor of the
+// new C.
+__global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
+                                                      const float* __restrict__ U, const float* __restrict__ V,
+                                                      const float* __restrict__ W, float* __restrict__ Cn,
+                                                      float* __restrict__ fC, int* __restrict__ nitr, float eps,
+                                                      int itrmax) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const float dt = (float)g.dt;
+  float cx[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) cx[q] = 0.0f;
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    float h = (float)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
+    float v1, v2;
+    if (d == 0) { v1 = ldb(U, g, i - 1, j, k, BC_U); v2 = U[m]; }
+    else if (d == 1) { v1 = ldb(V, g, i, j - 1, k, BC_V); v2 = ldb(V, g, i, j, k, BC_V); }
+    else { v1 = ldb(W, g, i, j, k - 1, BC_W); v2 = W[m]; }
+    const float r2h = dt / (2.0f * h);
+    float s1 = (-v2 + fabsf(v2)) * r2h;
+    float s2 = (1.0f / 6.0f) + (-v2 - fabsf(v2)) * r2h;
+    float s3 = (1.0f / 6.0f) + (v1 - fabsf(v1)) * r2h;
+    float s4 = (v1 + fabsf(v1)) * r2h;
+    int di = d == 0, dj = d == 1, dk = d == 2;
+    float pl[6], pr[6], nl[6], nr[6];
+    face_vals_f(g, C, sel, d, i, j, k, true, pl);
+    face_vals_f(g, C, sel, d, i + di, j + dj, k + dk, true, pr);
+    face_vals_f(g, C, sel, d, i - di, j - dj, k - dk, false, nl);
+    face_vals_f(g, C, sel, d, i, j, k, false, nr);
+#pragma unroll
+    for (int q = 0; q < 6; q++) cx[q] += s3 * pl[q] + s1 * pr[q] + s4 * nl[q] + s2 * nr[q];
+  }
+  const float dtw = dt / (float)g.wi;
+  float b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
+  const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz;
+#define UL(a, bb, c) ldb(U, g, a, bb, c, BC_U)
+#define VL(a, bb, c) ldb(V, g, a, bb, c, BC_V)
+#define WL(a, bb, c) ldb(W, g, a, bb, c, BC_W)
+  // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
+  const float a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
+  const float a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (4.0f * dx);
+  const float a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (4.0f * dx);
+  const float a12 = -dt * ((UL(i - 1, j + 1, k) + UL(i, j + 1, k)) - (UL(i - 1, j - 1, k) + UL(i, j - 1, k))) / (4.0f * dy);
+  const float a22 = -dt * (VL(i, j, k) - VL(i, j - 1, k)) / dy;
+  const float a32 = -dt * ((WL(i, j + 1, k - 1) - WL(i, j - 1, k - 1)) + (WL(i, j + 1, k) - WL(i, j - 1, k))) / (4.0f * dy);
+  const float a13 = -dt * ((UL(i - 1, j, k + 1) + UL(i, j, k + 1)) - (UL(i - 1, j, k - 1) + UL(i, j, k - 1))) / (4.0f * dz);
+  const float a23 = -dt * ((VL(i, j - 1, k + 1) + VL(i, j, k + 1)) - (VL(i, j - 1, k - 1) + VL(i, j, k - 1))) / (4.0f * dz);
+  const float a33 = -dt * (WL(i, j, k) - WL(i, j, k - 1)) / dz;
+#undef UL
+#undef VL
+#undef WL
+  const float L2 = (float)(g.lp * g.lp);
+  float x[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
+  int it;
+  for (it = 1; it <= itrmax; it++) {
+    const float s = x[0] + x[3] + x[5];
+    const float f = peterlin_f(L2, s);
+    const float dg = 0.5f * (1.0f + f * dtw);
+    const float A11 = a11 + dg, A22 = a22 + dg, A33 = a33 + dg;
+    // residual l = M x - b (fenep.f90:598-603)
+    float l[6];
+    l[0] = 2.0f * (A11 * x[0] + a12 * x[1] + a13 * x[2]) - b[0];
+    l[1] = a21 * x[0] + (A22 + A11) * x[1] + a23 * x[2] + a12 * x[3] + a13 * x[4] - b[1];
+    l[2] = a31 * x[0] + a32 * x[1] + (A33 + A11) * x[2] + a12 * x[4] + a13 * x[5] - b[2];
+    l[3] = 2.0f * (a21 * x[1] + A22 * x[3] + a23 * x[4]) - b[3];
+    l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (A33 + A22) * x[4] + a23 * x[5] - b[4];
+    l[5] = 2.0f * (a31 * x[2] + a32 * x[4] + A33 * x[5]) - b[5];
+    // Jacobian: linear operator + rank one Peterlin term
+    const float gp = dtw * dpeterlin_f(L2, s);
+    float J[6][6] = {
+        {2.0f * A11, 2.0f * a12, 2.0f * a13, 0.0f, 0.0f, 0.0f},
+        {a21, A22 + A11, a23, a12, a13, 0.0f},
+        {a31, a32, A33 + A11, 0.0f, a12, a13},
+        {0.0f, 2.0f * a21, 0.0f, 2.0f * A22, 2.0f * a23, 0.0f},
+        {0.0f, a31, a21, a32, A33 + A22, a23},
+        {0.0f, 0.0f, 2.0f * a31, 0.0f, 2.0f * a32, 2.0f * A33}};
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      const float gx = gp * x[r];
+      J[r][0] += gx; J[r][3] += gx; J[r][5] += gx;
+    }
+    // Gaussian elimination (diagonally dominant: no pivoting needed)
+#pragma unroll
+    for (int kk = 0; kk < 6; kk++) {
+      const float inv = 1.0f / J[kk][kk];
+#pragma unroll
+      for (int r = kk + 1; r < 6; r++) {
+        const float fct = J[r][kk] * inv;
+#pragma unroll
+        for (int cc = kk + 1; cc < 6; cc++) J[r][cc] -= fct * J[kk][cc];
+        l[r] -= fct * l[kk];
+      }
+    }
+    float rr[6];
+#pragma unroll
+    for (int r = 5; r >= 0; r--) {
+      float acc = l[r];
+#pragma unroll
+      for (int cc = r + 1; cc < 6; cc++) acc -= J[r][cc] * rr[cc];
+      rr[r] = acc / J[r][r];
+    }
+    float sr = 0.0f, sx = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; q++) { x[q] -= rr[q]; sr += rr[q] * rr[q]; sx += x[q] * x[q]; }
+    if (sr < eps * eps * sx) break;
+  }
+  if (it > itrmax) it = itrmax;
+#pragma unroll
+  for (int q = 0; q < 6; q++) Cn[q * g.n + m] = x[q];
+  fC[m] = peterlin_f(L2, x[0] + x[3] + x[5]);
+  nitr[m] = it;
+}
+
+// ============================================================= predictor ===
+__device__ __forceinline__ float c0_atf(const Grid& g, const float* __restrict__ C, const float* __restrict__ fC,
+                                        int q, int i, int j, int k) {
+  const float E = (q == 0 || q == 3 || q == 5) ? 1.0f : 0.0f;
+  if (g.walls && (j < 0 || j >= g.ny)) {
+    float c0 = ldb(C, g, i, j, k, BC_C), c3 = ldb(C + 3 * g.n, g, i, j, k, BC_C), c5 = ldb(C + 5 * g.n, g, i, j, k, BC_C);
+    float f = peterlin_f((float)(g.lp * g.lp), c0 + c3 + c5);
+    return f * ldb(C + q * g.n, g, i, j, k, BC_C) - E;
+  }
+  long m = cidx(g, wrapi(i, g.nx), wrapi(j, g.ny), wrapi(k, g.nz));
+  return fC[m] * C[q * g.n + m] - E;
+}
+
+__global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first, int polymer,
+                                                   const float* __restrict__ U, const float* __restrict__ V,
+                                                   const float* __restrict__ W, const float* __restrict__ P,
+                                                   const float* __restrict__ C, const float* __restrict__ fC,
+                                                   float* __restrict__ A0, float* __restrict__ T0,
+                                                   float* __restrict__ Bout, const float* __restrict__ F,
+                                                   float bf0, float bf1, float bf2, float* __restrict__ Q) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz, dt = (float)g.dt;
+  const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+#define Ug(a, b, c) ldb(U, g, a, b, c, BC_U)
+#define Vg(a, b, c) ldb(V, g, a, b, c, BC_V)
+#define Wg(a, b, c) ldb(W, g, a, b, c, BC_W)
+  float a[3], bv[3], t[3];
+  {
+    // convection (fenep.f90:694-734): -div(u u) with face averages
+    const float uc = Ug(i, j, k), vc = Vg(i, j, k), wc = Wg(i, j, k);
+    float h1 = 0.5f * (Ug(i - 1, j, k) + uc), h2 = 0.5f * (uc + Ug(i + 1, j, k));
+    float ax = (h2 * h2 - h1 * h1) * idx_
+               + 0.25f * ((vc + Vg(i + 1, j, k)) * (uc + Ug(i, j + 1, k)) - (Vg(i, j - 1, k) + Vg(i + 1, j - 1, k)) * (Ug(i, j - 1, k) + uc)) * idy
+               + 0.25f * ((wc + Wg(i + 1, j, k)) * (uc + Ug(i, j, k + 1)) - (Wg(i, j, k - 1) + Wg(i + 1, j, k - 1)) * (Ug(i, j, k - 1) + uc)) * idz;
+    float g1 = 0.5f * (Vg(i, j - 1, k) + vc), g2 = 0.5f * (vc + Vg(i, j + 1, k));
+    float ay = 0.25f * ((uc + Ug(i, j + 1, k)) * (vc + Vg(i + 1, j, k)) - (Ug(i - 1, j, k) + Ug(i - 1, j + 1, k)) * (Vg(i - 1, j, k) + vc)) * idx_
+               + (g2 * g2 - g1 * g1) * idy
+               + 0.25f * ((wc + Wg(i, j + 1, k)) * (vc + Vg(i, j, k + 1)) - (Wg(i, j, k - 1) + Wg(i, j + 1, k - 1)) * (Vg(i, j, k - 1) + vc)) * idz;
+    float e1 = 0.5f * (Wg(i, j, k - 1) + wc), e2 = 0.5f * (wc + Wg(i, j, k + 1));
+    float az = 0.25f * ((uc + Ug(i, j, k + 1)) * (wc + Wg(i + 1, j, k)) - (Ug(i - 1, j, k) + Ug(i - 1, j, k + 1)) * (Wg(i - 1, j, k) + wc)) * idx_
+               + 0.25f * ((vc + Vg(i, j, k + 1)) * (wc + Wg(i, j + 1, k)) - (Vg(i, j - 1, k) + Vg(i, j - 1, k + 1)) * (Wg(i, j - 1, k) + wc)) * idy
+               + (e2 * e2 - e1 * e1) * idz;
+    a[0] = -ax; a[1] = -ay; a[2] = -az;
+    // viscous (fenep.f90:736-770)
+    const float cb = (float)(g.beta / g.re);
+    const float ix2 = idx_ * idx_, iy2 = idy * idy, iz2 = idz * idz;
+    bv[0] = cb * ((Ug(i - 1, j, k) - 2.0f * uc + Ug(i + 1, j, k)) * ix2 + (Ug(i, j - 1, k) - 2.0f * uc + Ug(i, j + 1, k)) * iy2 +
+                  (Ug(i, j, k - 1) - 2.0f * uc + Ug(i, j, k + 1)) * iz2);
+    bv[1] = cb * ((Vg(i - 1, j, k) - 2.0f * vc + Vg(i + 1, j, k)) * ix2 + (Vg(i, j - 1, k) - 2.0f * vc + Vg(i, j + 1, k)) * iy2 +
+                  (Vg(i, j, k - 1) - 2.0f * vc + Vg(i, j, k + 1)) * iz2);
+    bv[2] = cb * ((Wg(i - 1, j, k) - 2.0f * wc + Wg(i + 1, j, k)) * ix2 + (Wg(i, j - 1, k) - 2.0f * wc + Wg(i, j + 1, k)) * iy2 +
+                  (Wg(i, j, k - 1) - 2.0f * wc + Wg(i, j, k + 1)) * iz2);
+  }
+  if (polymer) {
+#define C0g(q, aa, bb, cc) c0_atf(g, C, fC, q, aa, bb, cc)
+    const float tx = (C0g(0, i + 1, j, k) - C0g(0, i, j, k)) * idx_
+                     + 0.25f * ((C0g(1, i, j + 1, k) + C0g(1, i + 1, j + 1, k)) - (C0g(1, i, j - 1, k) + C0g(1, i + 1, j - 1, k))) * idy
+                     + 0.25f * ((C0g(2, i, j, k + 1) + C0g(2, i + 1, j, k + 1)) - (C0g(2, i, j, k - 1) + C0g(2, i + 1, j, k - 1))) * idz;
+    const float ty = 0.25f * ((C0g(1, i + 1, j, k) - C0g(1, i - 1, j, k)) + (C0g(1, i + 1, j + 1, k) - C0g(1, i - 1, j + 1, k))) * idx_
+                     + (C0g(3, i, j + 1, k) - C0g(3, i, j, k)) * idy
+                     + 0.25f * ((C0g(4, i, j, k + 1) + C0g(4, i, j + 1, k + 1)) - (C0g(4, i, j, k - 1) + C0g(4, i, j + 1, k - 1))) * idz;
+    const float tz = 0.25f * ((C0g(2, i + 1, j, k) - C0g(2, i - 1, j, k)) + (C0g(2, i + 1, j, k + 1) - C0g(2, i - 1, j, k + 1))) * idx_
+                     + 0.25f * ((C0g(4, i, j + 1, k) - C0g(4, i, j - 1, k)) + (C0g(4, i, j + 1, k + 1) - C0g(4, i, j - 1, k + 1))) * idy
+                     + (C0g(5, i, j, k + 1) - C0g(5, i, j, k)) * idz;
+#undef C0g
+    const float coef = (float)((1.0 - g.beta) / (g.re * g.wi));
+    t[0] = coef * tx; t[1] = coef * ty; t[2] = coef * tz;
+  } else {
+    t[0] = t[1] = t[2] = 0.0f;
+  }
+  const float bfs[3] = {bf0, bf1, bf2};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float* X = c == 0 ? U : (c == 1 ? V : W);
+    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+    long o = c * g.n + m;
+    float a0 = first ? a[c] : A0[o];
+    float t0 = first ? t[c] : T0[o];
+    float u = ldb(X, g, i, j, k, kind);
+    float p1 = c == 0 ? ldb(P, g, i + 1, j, k, BC_P) : (c == 1 ? ldb(P, g, i, j + 1, k, BC_P) : ldb(P, g, i, j, k + 1, BC_P));
+    float ih = c == 0 ? idx_ : (c == 1 ? idy : idz);
+    float fv = (F ? F[o] : 0.0f) + bfs[c];
+    float bcoef = method == FP_METHOD_FFT ? 0.5f : 1.0f;   // CN half (fft_navier) or full (ibm_preliminary)
+    Q[o] = u - dt * (p1 - P[m]) * ih + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] + 0.5f * dt * (t[c] + t0) + dt * fv;
+    A0[o] = a[c];
+    T0[o] = t[c];
+    if (Bout) Bout[o] = bv[c];
+  }
+#undef Ug
+#undef Vg
+#undef Wg
+}
+
+// ================================================================== FFT ===
+__device__ float2* stockham_f(float2* x, float2* y, int n, int logn, int nl, const float2* __restrict__ tw, bool inv) {
+  int len = n, s = 1, lgs = 0;
+  const int half = n >> 1;
+  while (len > 1) {
+    int m = len >> 1;
+    for (int b = threadIdx.x; b < nl * half; b += blockDim.x) {
+      int line = b >> (logn - 1);
+      int t = b & (half - 1);
+      int p = t >> lgs, q = t & (s - 1);
+      float2 w = tw[p * s];
+      float wr = w.x, wi = inv ? -w.y : w.y;
+      const float2* xl = x + line * n;
+      float2* yl = y + line * n;
+      float2 a = xl[q + s * p], bb = xl[q + s * (p + m)];
+      float dr = a.x - bb.x, di = a.y - bb.y;
+      yl[q + s * (2 * p)] = make_float2(a.x + bb.x, a.y + bb.y);
+      yl[q + s * (2 * p + 1)] = make_float2(dr * wr - di * wi, dr * wi + di * wr);
+    }
+    __syncthreads();
+    float2* tt = x; x = y; y = tt;
+    len = m; s <<= 1; lgs++;
+  }
+  return x;
+}
+
+__device__ __forceinline__ int ilog2f(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
+
+__global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restrict__ q, float2* __restrict__ spec,
+                                                   const float2* __restrict__ tw, int nl) {
+  extern __shared__ float2 ldsf[];
+  const int n = g.nx;
+  float2* x = ldsf;
+  float2* y = ldsf + nl * n;
+  long nlines = (long)g.ny * g.nz;
+  long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+    int l = e / n, i = e % n;
+    long L = l0 + l;
+    x[e] = make_float2(L < nlines ? q[L * n + i] : 0.0f, 0.0f);
+  }
+  __syncthreads();
+  float2* r = stockham_f(x, y, n, ilog2f(n), nl, tw, false);
+  const int nxh = g.nxh;
+  for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
+    int l = e / nxh, i = e % nxh;
+    long L = l0 + l;
+    if (L >= nlines) continue;
+    long j = L % g.ny, k = L / g.ny;
+    spec[i + (long)nxh * ((j + g.jo) + g.sy * k)] = r[l * n + i];
+  }
+}
+
+__global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restrict__ spec, float* __restrict__ out,
+                                                   const float2* __restrict__ tw, int nl) {
+  extern __shared__ float2 ldsf[];
+  const int n = g.nx, h = n / 2, nxh = g.nxh;
+  float2* x = ldsf;
+  float2* y = ldsf + nl * n;
+  long nlines = (long)g.ny * g.nz;
+  long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
+    int l = e / nxh, i = e % nxh;
+    long L = l0 + l;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (L < nlines) {
+      long j = L % g.ny, k = L / g.ny;
+      v = spec[i + (long)nxh * ((j + g.jo) + g.sy * k)];
+    }
+    if (i == 0 || i == h) v.y = 0.0f;
+    x[l * n + i] = v;
+    if (i > 0 && i < h) x[l * n + (n - i)] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  float2* r = stockham_f(x, y, n, ilog2f(n), nl, tw, true);
+  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+    int l = e / n, i = e % n;
+    long L = l0 + l;
+    if (L < nlines) out[L * n + i] = r[e].x;
+  }
+}
+
+__global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spec, const float2* __restrict__ tw, int axis,
+                                                int mode, int kind, int row0, int ti, const float* __restrict__ kx2,
+                                                const float* __restrict__ ky2, const float* __restrict__ kz2) {
+  extern __shared__ float2 ldsf[];
+  const int n = axis == 1 ? g.ny : g.nz;
+  const int nxh = g.nxh;
+  const long stride = axis == 1 ? (long)nxh : (long)nxh * g.sy;
+  const int ntile = (nxh + ti - 1) / ti;
+  int tile = blockIdx.x % ntile;
+  long other = blockIdx.x / ntile;
+  int i0 = tile * ti;
+  float2* base = axis == 1 ? spec + (long)nxh * (g.jo + g.sy * other) : spec + (long)nxh * (row0 + other);
+  float2* x = ldsf;
+  float2* y = ldsf + ti * n;
+  for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
+    int l = e % ti, r = e / ti;
+    int i = i0 + l;
+    x[l * n + r] = i < nxh ? base[i + stride * r] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  int logn = ilog2f(n);
+  float2* res = stockham_f(x, y, n, logn, ti, tw, mode == 1);
+  if (mode == 2) {
+    float2* other_buf = res == x ? y : x;
+    const float intot = (float)(1.0 / ((double)g.nx * g.ny * g.nz));
+    const float coef = (float)(((g.dt * g.beta) / g.re) / 2.0);
+    for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
+      int l = e / n, kk = e % n;
+      int i = i0 + l;
+      if (i >= nxh) continue;
+      long jj = other;
+      float lhs = kind == 3 ? -((kx2[i] + ky2[jj]) + kz2[kk]) : 1.0f + coef * ((kx2[i] + ky2[jj]) + kz2[kk]);
+      float2 s = res[e];
+      float sc = fabsf(lhs) < 1.0e-16f ? 0.0f : intot / lhs;
+      res[e] = make_float2(s.x * sc, s.y * sc);
+    }
+    __syncthreads();
+    res = stockham_f(res, other_buf, n, logn, ti, tw, true);
+  }
+  for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
+    int l = e % ti, r = e / ti;
+    int i = i0 + l;
+    if (i < nxh) base[i + stride * r] = res[l * n + r];
+  }
+}
+
+__global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spec, float* __restrict__ thp, int kind,
+                                                const float* __restrict__ kx2, const float* __restrict__ kz2) {
+  const long lines = (long)g.nxh * g.nz;
+  long L = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (L >= lines) return;
+  const int nxh = g.nxh, ny = g.ny, size = ny + 2;
+  int i = (int)(L % nxh);
+  long k = L / nxh;
+  float2* col = spec + i + (long)nxh * g.sy * k;
+  const float inxz = (float)(1.0 / ((double)g.nx * g.nz));
+  const float coef = (float)(((g.dt * g.beta) / g.re) / 2.0);
+  const float idy2 = (float)(1.0 / (g.dy * g.dy));
+  const float kk = kx2[i] + kz2[k];
+  auto coeffs = [&](int r, float& a, float& b, float& c, bool& rhs) {
+    rhs = false;
+    if (kind == 3) {
+      if (r == 0) { a = 1.0f; b = -1.0f; c = 0.0f; }
+      else if (r == size - 1) { a = 1.0f; b = 0.0f; c = -1.0f; }
+      else { a = -(kk + 2.0f * idy2); b = idy2; c = idy2; rhs = true; }
+    } else if (kind == 1) {
+      if (r == 0 || r >= ny) { a = 1.0f; b = 0.0f; c = 0.0f; }
+      else { a = 1.0f + coef * (kk + 2.0f * idy2); b = -coef * idy2; c = b; rhs = true; }
+    } else {
+      if (r == 0) { a = 1.0f; b = 1.0f; c = 0.0f; }
+      else if (r == size - 1) { a = 1.0f; b = 0.0f; c = 1.0f; }
+      else { a = 1.0f + coef * (kk + 2.0f * idy2); b = -coef * idy2; c = b; rhs = true; }
+    }
+  };
+  float a, b, c; bool rhs;
+  coeffs(0, a, b, c, rhs);
+  float Pp = -b / a, Qr = 0.0f, Qi = 0.0f;
+  thp[L] = Pp;
+  col[0] = make_float2(0.0f, 0.0f);
+  for (int r = 1; r < size; r++) {
+    coeffs(r, a, b, c, rhs);
+    float dr = 0.0f, di = 0.0f;
+    if (rhs) { float2 s = col[(long)nxh * r]; dr = s.x * inxz; di = s.y * inxz; }
+    float den = a + c * Pp;
+    if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
+    else { float id = 1.0f / den; Pp = -b * id; Qr = (dr - c * Qr) * id; Qi = (di - c * Qi) * id; }
+    thp[L + lines * r] = Pp;
+    col[(long)nxh * r] = make_float2(Qr, Qi);
+  }
+  float xr = Qr, xi = Qi;
+  for (int r = size - 2; r >= 0; r--) {
+    float P = thp[L + lines * r];
+    float2 q = col[(long)nxh * r];
+    xr = P * xr + q.x;
+    xi = P * xi + q.y;
+    col[(long)nxh * r] = make_float2(xr, xi);
+  }
+}
+
+static int lines_for32(int n) { return std::max(1, std::min(32, 4096 / n)); }
+
+static void solve32(Ctx& c, int kind, const float* q, float* x) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  int lx = lines_for32(g.nx);
+  long xlines = (long)g.ny * g.nz;
+  size_t shx = sizeof(float2) * 2 * lx * g.nx;
+  hipLaunchKernelGGL(m_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, q, X.spec, X.twx, lx);
+  int tiy = std::max(1, std::min(32, 4096 / g.ny)), tiz = std::max(1, std::min(32, 4096 / g.nz));
+  int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
+  size_t shy = sizeof(float2) * 2 * tiy * g.ny, shz = sizeof(float2) * 2 * tiz * g.nz;
+  if (!g.walls) {
+    hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 0, kind, 0, tiy, X.kx2, X.ky2, X.kz2);
+    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 2, kind, 0, tiz, X.kx2, X.ky2, X.kz2);
+    hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 1, kind, 0, tiy, X.kx2, X.ky2, X.kz2);
+  } else {
+    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 0, kind, 1, tiz, X.kx2, X.ky2, X.kz2);
+    long lines = (long)g.nxh * g.nz;
+    hipLaunchKernelGGL(m_thomas, dim3(nblk(lines, 256)), dim3(256), 0, s, g, X.spec, X.thp, kind, X.kx2, X.kz2);
+    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 1, kind, 1, tiz, X.kx2, X.ky2, X.kz2);
+  }
+  hipLaunchKernelGGL(m_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, X.spec, x, X.twx, lx);
+}
+
+// ============================================================ projection ===
+// b = div_h(u*)/dt in FP64 from the FP32 u*; also |b|^2
+__global__ __launch_bounds__(256) void m_divergence(Grid g, const float* __restrict__ up, double* __restrict__ b,
+                                                    float* __restrict__ b32, double* __restrict__ nrm) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (m < g.n) {
+    int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+    const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+    v = (((double)ldb(Uu, g, i, j, k, BC_U) - (double)ldb(Uu, g, i - 1, j, k, BC_U)) / g.dx +
+         ((double)ldb(Vv, g, i, j, k, BC_V) - (double)ldb(Vv, g, i, j - 1, k, BC_V)) / g.dy +
+         ((double)ldb(Ww, g, i, j, k, BC_W) - (double)ldb(Ww, g, i, j, k - 1, BC_W)) / g.dz) / g.dt;
+    b[m] = v;
+    b32[m] = (float)v;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = v * v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(nrm, sh[0]);
+}
+
+// r = b - L_h phi (FP64, boundary rules of Phi), rounded to FP32 for the
+// correction solve; |r|^2 accumulated
+__global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restrict__ b, const double* __restrict__ phi,
+                                                  float* __restrict__ r32, double* __restrict__ nrm) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (m < g.n) {
+    int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+    double p = phi[m];
+#define PH(a, bb, c) ldb(phi, g, a, bb, c, BC_P)
+    double lap = (PH(i - 1, j, k) - 2.0 * p + PH(i + 1, j, k)) / (g.dx * g.dx) +
+                 (PH(i, j - 1, k) - 2.0 * p + PH(i, j + 1, k)) / (g.dy * g.dy) +
+                 (PH(i, j, k - 1) - 2.0 * p + PH(i, j, k + 1)) / (g.dz * g.dz);
+#undef PH
+    r = b[m] - lap;
+    r32[m] = (float)r;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = r * r;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(nrm, sh[0]);
+}
+
+__global__ __launch_bounds__(256) void m_accum(long n, const float* __restrict__ d, double* __restrict__ phi, int first) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  phi[m] = first ? (double)d[m] : phi[m] + (double)d[m];
+}
+
+// fft_march (fenep.f90:1562-1604) with FP64 potential, FP32 storage
+__global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__ up, const double* __restrict__ phi,
+                                               float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
+                                               float* __restrict__ P) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const double dt = g.dt;
+  const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+#define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)
+  double ph = phi[m];
+  U[m] = (float)((double)ldb(Uu, g, i, j, k, BC_U) - dt * (PH(i + 1, j, k) - ph) / g.dx);
+  V[m] = (float)((double)ldb(Vv, g, i, j, k, BC_V) - dt * (PH(i, j + 1, k) - ph) / g.dy);
+  W[m] = (float)((double)ldb(Ww, g, i, j, k, BC_W) - dt * (PH(i, j, k + 1) - ph) / g.dz);
+  const double coef = ((dt * g.beta) / g.re) / 2.0;
+  double lap = (PH(i - 1, j, k) - 2.0 * ph + PH(i + 1, j, k)) / (g.dx * g.dx) +
+               (PH(i, j - 1, k) - 2.0 * ph + PH(i, j + 1, k)) / (g.dy * g.dy) +
+               (PH(i, j, k - 1) - 2.0 * ph + PH(i, j, k + 1)) / (g.dz * g.dz);
+#undef PH
+  P[m] = (float)(((double)P[m] + ph) - coef * lap);
+}
+
+// ================================================================== IBM ===
+__device__ __forceinline__ double roma64(double r) {
+  double a = fabs(r);
+  if (a < 1.5 && a >= 0.5) return ((5.0 - 3.0 * a) - sqrt(-(3.0 * ((1.0 - a) * (1.0 - a))) + 1.0)) / 6.0;
+  if (a < 0.5) return (1.0 + sqrt(-(3.0 * (r * r)) + 1.0)) / 3.0;
+  return 0.0;
+}
+__device__ __forceinline__ double ibm_w64(const Grid& g, int c, int ii, int jj, int kk, double xc, double yc, double zc) {
+  double x = ((double)ii - 0.5) * g.dx, y = ((double)jj - 0.5) * g.dy, z = ((double)kk - 0.5) * g.dz;
+  if (c == 0) x += 0.5 * g.dx;
+  else if (c == 1) y += 0.5 * g.dy;
+  else z += 0.5 * g.dz;
+  return roma64((x - xc) / g.dx) * roma64((y - yc) / g.dy) * roma64((z - zc) / g.dz) / (g.dx * g.dy * g.dz) *
+         (g.dx * g.dx * g.dx);
+}
+
+__global__ void m_ibm_setup_interp(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
+                                   float* __restrict__ wi, int4* __restrict__ org, float* __restrict__ uc) {
+  long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (id >= nmk) return;
+  int n = (int)(id / ((long)nc * nl)) + 1;
+  double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
+  for (int c = 0; c < 3; c++) {
+    int i0, j0;
+    if (c == 0) { i0 = (int)(xc / g.dx - 0.5); j0 = (int)(yc / g.dy); }
+    else if (c == 1) { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy - 0.5); }
+    else { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy); }
+    org[id * 3 + c] = make_int4(i0, j0, n, 0);
+    int s = 0;
+    for (int kk = n - 1; kk <= n + 1; kk++)
+      for (int jj = j0; jj <= j0 + 2; jj++)
+        for (int ii = i0; ii <= i0 + 2; ii++, s++) wi[(id * 3 + c) * 27 + s] = (float)ibm_w64(g, c, ii, jj, kk, xc, yc, zc);
+    uc[c * nmk + id] = (float)mk[(3 + c) * nmk + id];
+  }
+}
+
+__global__ void m_ibm_setup_spread(Grid g, int comp, int rows, const int* __restrict__ rowp, const long* __restrict__ tgt,
+                                   const int* __restrict__ ent, long nmk, const double* __restrict__ mk,
+                                   float* __restrict__ sw) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const long gx = g.nx + 2, gy = g.ny + 2;
+  long t = tgt[r];
+  int ii = (int)(t % gx), jj = (int)((t / gx) % gy), kk = (int)(t / (gx * gy));
+  for (int e = rowp[r]; e < rowp[r + 1]; e++) {
+    long id = ent[e];
+    sw[e] = (float)ibm_w64(g, comp, ii, jj, kk, mk[id], mk[nmk + id], mk[2 * nmk + id]);
+  }
+}
+
+__global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const float* __restrict__ wi, const int4* __restrict__ org,
+                                                    const float* __restrict__ uc, const float* __restrict__ ua,
+                                                    float* __restrict__ mf) {
+  long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (id >= nmk) return;
+  const float idt = (float)(1.0 / g.dt);
+#pragma unroll 1
+  for (int c = 0; c < 3; c++) {
+    int4 o = org[id * 3 + c];
+    const float* X = ua + c * g.n;
+    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+    const float* w = wi + (id * 3 + c) * 27;
+    float acc = 0.0f;
+    int s = 0;
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+      for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++, s++)
+          acc += w[s] * ldb(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
+    mf[c * nmk + id] = (uc[c * nmk + id] - acc) * idt;
+  }
+}
+
+__global__ __launch_bounds__(256) void m_ibm_spread(int rows, const int* __restrict__ rowp, const long* __restrict__ tgt,
+                                                    const int* __restrict__ ent, const float* __restrict__ sw,
+                                                    const float* __restrict__ mf, float* __restrict__ ft) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float acc = 0.0f;
+  for (int e = rowp[r]; e < rowp[r + 1]; e++) acc += mf[ent[e]] * sw[e];
+  ft[tgt[r]] = acc;
+}
+
+__global__ __launch_bounds__(256) void m_ibm_fold(Grid g, int axis, float* __restrict__ ft) {
+  const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  auto G = [&](long i, long j, long k) { return i + gx * (j + gy * k); };
+  if (axis == 0) {
+    if (t >= gy * gz) return;
+    long j = t % gy, k = t / gy;
+    ft[G(1, j, k)] += ft[G(g.nx + 1, j, k)];
+    ft[G(g.nx, j, k)] += ft[G(0, j, k)];
+  } else if (axis == 1) {
+    if (t >= gx * gz) return;
+    long i = t % gx, k = t / gx;
+    ft[G(i, 1, k)] += ft[G(i, g.ny + 1, k)];
+    ft[G(i, g.ny, k)] += ft[G(i, 0, k)];
+  } else {
+    if (t >= gx * gy) return;
+    long i = t % gx, j = t / gx;
+    ft[G(i, j, 1)] += ft[G(i, j, g.nz + 1)];
+    ft[G(i, j, g.nz)] += ft[G(i, j, 0)];
+  }
+}
+
+__global__ __launch_bounds__(256) void m_ibm_rhs(Grid g, const float* __restrict__ ua, const float* __restrict__ ft,
+                                                 const float* __restrict__ B, float* __restrict__ q) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= 3 * g.n) return;
+  int c = (int)(m / g.n);
+  long mm = m % g.n;
+  int i = (int)(mm % g.nx), j = (int)((mm / g.nx) % g.ny), k = (int)(mm / ((long)g.nx * g.ny));
+  const long gx = g.nx + 2, gy = g.ny + 2, gn = gx * gy * (g.nz + 2);
+  float f = ft[c * gn + (i + 1) + gx * ((j + 1) + gy * (long)(k + 1))];
+  int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+  const float dt = (float)g.dt;
+  q[m] = ldb(ua + c * g.n, g, i, j, k, kind) + dt * f - 0.5f * dt * B[m];
+}
+
+// ========================================================== conversions ===
+__global__ void m_to32(long n, const double* __restrict__ a, float* __restrict__ b) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m < n) b[m] = (float)a[m];
+}
+__global__ void m_to64(long n, const float* __restrict__ a, double* __restrict__ b) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m < n) b[m] = (double)a[m];
+}
+__global__ void m_zero_d(int n, double* p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+// ================================================================= host ===
+void mixed_create(Ctx& c) {
+  if (c.p.method == FP_METHOD_EXPLICIT || c.p.poisson_variant == FP_POISSON_SOR)
+    throw std::string("mixed precision supports the FFT and IBM methods with the spectral Poisson solve");
+  Mixed* X = new Mixed();
+  c.mixed = X;
+  const Grid& g = c.g;
+  const long n = g.n;
+  dalloc(&X->U, n); dalloc(&X->V, n); dalloc(&X->W, n); dalloc(&X->P, n);
+  dalloc(&X->C[0], 6 * n); dalloc(&X->C[1], 6 * n); dalloc(&X->f, n);
+  dalloc(&X->A0, 3 * n); dalloc(&X->T0, 3 * n); dalloc(&X->B, 3 * n);
+  dalloc(&X->Q, 3 * n); dalloc(&X->Up, 3 * n);
+  dalloc(&X->spec, (size_t)g.nxh * g.sy * g.nz);
+  if (g.walls) dalloc(&X->thp, (size_t)(g.ny + 2) * g.nxh * g.nz);
+  dalloc(&X->phi, n); dalloc(&X->b, n); dalloc(&X->r32, n); dalloc(&X->d32, n);
+  dalloc(&X->red, 8);
+  X->amb_cap = std::max<long>(1024, 3 * n / 8);
+  dalloc(&X->amb, 1); dalloc(&X->amb_list, X->amb_cap);
+  // FP32 twiddles and wavenumbers from the FP64 host tables
+  auto tw32 = [&](const double2* d, int nn, float2** out) {
+    std::vector<double2> h(nn / 2);
+    FP_HIP_CHECK(hipMemcpy(h.data(), d, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
+    std::vector<float2> f(h.size());
+    for (size_t a = 0; a < h.size(); a++) f[a] = make_float2((float)h[a].x, (float)h[a].y);
+    dalloc(out, f.size());
+    FP_HIP_CHECK(hipMemcpy(*out, f.data(), sizeof(float2) * f.size(), hipMemcpyHostToDevice));
+  };
+  auto k32 = [&](const double* d, int nn, float** out) {
+    std::vector<double> h(nn);
+    FP_HIP_CHECK(hipMemcpy(h.data(), d, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    std::vector<float> f(nn);
+    for (int a = 0; a < nn; a++) f[a] = (float)h[a];
+    dalloc(out, nn);
+    FP_HIP_CHECK(hipMemcpy(*out, f.data(), sizeof(float) * nn, hipMemcpyHostToDevice));
+  };
+  tw32(c.twx, g.nx, &X->twx); tw32(c.twy, g.ny, &X->twy); tw32(c.twz, g.nz, &X->twz);
+  k32(c.kx2, g.nxh, &X->kx2); k32(c.ky2, g.ny, &X->ky2); k32(c.kz2, g.nz, &X->kz2);
+}
+
+void mixed_destroy(Ctx& c) {
+  Mixed* X = (Mixed*)c.mixed;
+  if (!X) return;
+  dfree(X->U); dfree(X->V); dfree(X->W); dfree(X->P); dfree(X->C[0]); dfree(X->C[1]); dfree(X->f);
+  dfree(X->A0); dfree(X->T0); dfree(X->B); dfree(X->Q); dfree(X->Up); dfree(X->F);
+  dfree(X->spec); dfree(X->thp); dfree(X->twx); dfree(X->twy); dfree(X->twz);
+  dfree(X->kx2); dfree(X->ky2); dfree(X->kz2);
+  dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->amb); dfree(X->amb_list);
+  dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
+  for (int k = 0; k < 3; k++) dfree(X->sw[k]);
+  delete X;
+  c.mixed = nullptr;
+}
+
+void mixed_load_from_f64(Ctx& c) {
+  Mixed& X = MX(c);
+  const long n = c.g.n;
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.U, X.U);
+  hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.V, X.V);
+  hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.W, X.W);
+  hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.P, X.P);
+  X.cur = 0;
+  hipLaunchKernelGGL(m_to32, dim3(nblk(6 * n, 256)), dim3(256), 0, s, 6 * n, c.C[c.cur], X.C[0]);
+  hipLaunchKernelGGL(m_to32, dim3(nblk(3 * n, 256)), dim3(256), 0, s, 3 * n, c.A0, X.A0);
+  hipLaunchKernelGGL(m_to32, dim3(nblk(3 * n, 256)), dim3(256), 0, s, 3 * n, c.T0, X.T0);
+  FP_HIP_CHECK(hipMemsetAsync(X.phi, 0, sizeof(double) * n, s));
+  FP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mixed_store_to_f64(Ctx& c) {
+  Mixed& X = MX(c);
+  const long n = c.g.n;
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.U, c.U);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.V, c.V);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.W, c.W);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.P, c.P);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(6 * n, 256)), dim3(256), 0, s, 6 * n, X.C[X.cur], c.C[c.cur]);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(3 * n, 256)), dim3(256), 0, s, 3 * n, X.A0, c.A0);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(3 * n, 256)), dim3(256), 0, s, 3 * n, X.T0, c.T0);
+  FP_HIP_CHECK(hipMemcpyAsync(c.Phi, X.phi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  if (c.nmk) hipLaunchKernelGGL(m_to64, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s, 3 * c.nmk, X.mf, X.mf64);
+  FP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mixed_set_force(Ctx& c) {
+  Mixed& X = MX(c);
+  dfree(X.F);
+  if (!c.F) return;
+  dalloc(&X.F, 3 * c.g.n);
+  hipLaunchKernelGGL(m_to32, dim3(nblk(3 * c.g.n, 256)), dim3(256), 0, c.stream, 3 * c.g.n, c.F, X.F);
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void mixed_set_markers(Ctx& c) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  dfree(X.wi); dfree(X.org); dfree(X.uc); dfree(X.mf); dfree(X.mf64); dfree(X.ftmp);
+  dalloc(&X.wi, (size_t)c.nmk * 81); dalloc(&X.org, (size_t)c.nmk * 3); dalloc(&X.uc, 3 * c.nmk);
+  dalloc(&X.mf, 3 * c.nmk); dalloc(&X.mf64, 3 * c.nmk); dalloc(&X.ftmp, 3 * c.gn);
+  hipLaunchKernelGGL(m_ibm_setup_interp, dim3(nblk(c.nmk, 128)), dim3(128), 0, c.stream, g, c.nmk, c.nc, c.nl, c.mk, X.wi,
+                     X.org, X.uc);
+  for (int k = 0; k < 3; k++) {
+    dfree(X.sw[k]);
+    std::vector<int> rowp(c.sp_rows[k] + 1);
+    FP_HIP_CHECK(hipMemcpy(rowp.data(), c.sp_row[k], sizeof(int) * rowp.size(), hipMemcpyDeviceToHost));
+    dalloc(&X.sw[k], std::max(1, rowp.back()));
+    if (c.sp_rows[k])
+      hipLaunchKernelGGL(m_ibm_setup_spread, dim3(nblk(c.sp_rows[k], 128)), dim3(128), 0, c.stream, g, k, c.sp_rows[k],
+                         c.sp_row[k], c.sp_tgt[k], c.sp_ent[k], c.nmk, c.mk, X.sw[k]);
+  }
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+const double* mixed_marker_forces(Ctx& c) {
+  Mixed& X = MX(c);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, c.stream, 3 * c.nmk, X.mf, X.mf64);
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return X.mf64;
+}
+
+double mixed_last_ndc(Ctx& c) { return (double)std::max(1, c.p.n_dc_max); }
+
+// statistics of the last step: [0] |b|^2, [1] |r|^2 before the last
+// correction, [2] guarded selections re-decided in FP64
+int mixed_stats(Ctx& c, double* out3) {
+  Mixed& X = MX(c);
+  double h[8];
+  FP_HIP_CHECK(hipMemcpy(h, X.red, sizeof(h), hipMemcpyDeviceToHost));
+  unsigned amb = 0;
+  FP_HIP_CHECK(hipMemcpy(&amb, X.amb, sizeof(unsigned), hipMemcpyDeviceToHost));
+  out3[0] = h[0]; out3[1] = h[1]; out3[2] = (double)amb;
+  return 0;
+}
+
+namespace {
+struct MTimer {
+  Ctx& c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  MTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
+    if (!c.timing) return;
+    (void)hipEventCreate(&a); (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, c.stream);
+  }
+  ~MTimer() {
+    if (!c.timing) return;
+    (void)hipEventRecord(b, c.stream);
+    c.ev.push_back({std::string(name) + "#a", a});
+    c.ev.push_back({std::string(name) + "#b", b});
+  }
+};
+}  // namespace
+
+void mixed_step(Ctx& c) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const long n = g.n;
+  const int polymer = c.p.beta != 1.0;
+  if (!polymer) {
+    FP_HIP_CHECK(hipMemsetAsync(X.C[X.cur], 0, sizeof(float) * 6 * n, s));
+  } else {
+    {
+      MTimer t(c, "fenep_select");
+      FP_HIP_CHECK(hipMemsetAsync(X.amb, 0, sizeof(unsigned), s));
+      hipLaunchKernelGGL(m_select, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list, X.amb_cap);
+    }
+    {
+      MTimer t(c, "fenep_select_fp64_fix");
+      hipLaunchKernelGGL(m_select_fix, dim3(nblk(X.amb_cap, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.amb,
+                         X.amb_list, X.amb_cap);
+    }
+    {
+      MTimer t(c, "fenep_cstar_newton");
+      int itmax = std::min(c.p.newton_itrmax, 16);
+      hipLaunchKernelGGL(m_cstar_newton, dim3(nblk(n, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
+                         X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
+      X.cur = 1 - X.cur;
+    }
+  }
+  {
+    MTimer t(c, "predictor");
+    float* Bout = c.p.method == FP_METHOD_IBM ? X.B : nullptr;
+    hipLaunchKernelGGL(m_predictor, dim3(nblk(n, 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
+                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Bout, X.F, (float)c.p.body_force[0],
+                       (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
+  }
+  if (c.p.method == FP_METHOD_FFT) {
+    MTimer t(c, "helmholtz");
+    for (int k = 0; k < 3; k++) solve32(c, k, X.Q + k * n, X.Up + k * n);
+  } else {
+    {
+      MTimer t(c, "ibm");
+      hipLaunchKernelGGL(m_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
+      const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
+      for (int k = 0; k < 3; k++) {
+        float* ft = X.ftmp + k * c.gn;
+        if (c.sp_rows[k])
+          hipLaunchKernelGGL(m_ibm_spread, dim3(nblk(c.sp_rows[k], 256)), dim3(256), 0, s, c.sp_rows[k], c.sp_row[k],
+                             c.sp_tgt[k], c.sp_ent[k], X.sw[k], X.mf + k * c.nmk, ft);
+        hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gy * gz, 256)), dim3(256), 0, s, g, 0, ft);
+        hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
+        hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
+      }
+      hipLaunchKernelGGL(m_ibm_rhs, dim3(nblk(3 * n, 256)), dim3(256), 0, s, g, X.Q, X.ftmp, X.B, X.Up);
+    }
+    MTimer t(c, "helmholtz");
+    for (int k = 0; k < 3; k++) solve32(c, k, X.Up + k * n, X.Up + k * n);
+  }
+  {
+    MTimer t(c, "poisson");
+    hipLaunchKernelGGL(m_zero_d, dim3(1), dim3(64), 0, s, 8, X.red);
+    hipLaunchKernelGGL(m_divergence, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.b, X.r32, X.red);
+    const int ndc = std::max(1, c.p.n_dc_max);
+    for (int it = 0; it < ndc; it++) {
+      if (it > 0) {
+        hipLaunchKernelGGL(m_zero_d, dim3(1), dim3(64), 0, s, 1, X.red + 1);
+        hipLaunchKernelGGL(m_residual, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.b, X.phi, X.r32, X.red + 1);
+      }
+      solve32(c, 3, X.r32, X.d32);
+      hipLaunchKernelGGL(m_accum, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.d32, X.phi, it == 0 ? 1 : 0);
+    }
+  }
+  {
+    MTimer t(c, "march");
+    hipLaunchKernelGGL(m_march, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.phi, X.U, X.V, X.W, X.P);
+  }
+  c.has_history = 1;
+  c.step++;
+}
+
+}  // namespace fp

@@ -48,7 +48,7 @@ class FpDiag(C.Structure):
         ("trc_mean", C.c_double), ("trc_std", C.c_double),
         ("div_max", C.c_double), ("drag", C.c_double), ("lift", C.c_double),
         ("newton_itr_mean", C.c_double), ("sor_itr", C.c_double),
-        ("n_dc", C.c_double),
+        ("n_dc", C.c_double), ("dc_residual", C.c_double), ("sel_guarded", C.c_double),
         ("nan_flag", C.c_int32), ("step", C.c_int32),
     ]
 
@@ -67,7 +67,7 @@ def default_params(**kw):
     p.newton_itrmax, p.newton_eps, p.newton_dc = 10000, 1.0e-6, 1.0e-5
     p.sor_itrmax, p.sor_eps, p.sor_omega = 10000, 1.0e-5, 0.0
     p.body_force[0] = p.body_force[1] = p.body_force[2] = 0.0
-    p.n_dc_max, p.dc_tol = 4, 1.0e-10
+    p.n_dc_max, p.dc_tol = 2, 1.0e-10
     for k, v in kw.items():
         if k == "body_force":
             for i in range(3):

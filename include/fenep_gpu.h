@@ -82,6 +82,8 @@ typedef struct fp_diag {
   double newton_itr_mean;
   double sor_itr;     /* iterations of the last SOR solve (or -1)            */
   double n_dc;        /* defect-correction sweeps of the last mixed solve    */
+  double dc_residual; /* mixed: |b - L phi| / |b| before the last correction */
+  double sel_guarded; /* mixed: C+- selections re-decided in FP64, last step */
   int32_t nan_flag;   /* nonzero: NaN in U (driver prints 'NaN value')        */
   int32_t step;       /* steps taken by this context                         */
 } fp_diag;

@@ -1191,3 +1191,22 @@ int orc_sor(void* vo, const double* q, double* phi) {
   fill_ghosts(o, phi, BC_P);
   return sor_solve(o, q, phi);
 }
+
+/* ibm_Helmholtz alone on a given ghosted velocity (u*,v*,w*): returns the
+ * spread force (interior), marker forces and interpolated marker velocities
+ * (for the adjointness / conservation tests). */
+int orc_ibm_apply(void* vo, const double* up3, double* fint3, double* fm3, double* ub3) {
+  orc_t* o = (orc_t*)vo;
+  if (!o->nmk) return FP_E_STATE;
+  for (int c = 0; c < 3; c++) memcpy(o->Up[c], up3 + (long)c * o->gn, sizeof(double) * o->gn);
+  for (long id = 0; id < o->nmk; id++) { o->fxc[id] = 0.0; o->fyc[id] = 0.0; o->fzc[id] = 0.0; }
+  ibm_helmholtz(o);
+  double* Fm[3] = {o->fxc, o->fyc, o->fzc};
+  double* Bm[3] = {o->ubm, o->vbm, o->wbm};
+  for (int c = 0; c < 3; c++) {
+    memcpy(fint3 + (long)c * o->n, o->fint[c], sizeof(double) * o->n);
+    memcpy(fm3 + (long)c * o->nmk, Fm[c], sizeof(double) * o->nmk);
+    memcpy(ub3 + (long)c * o->nmk, Bm[c], sizeof(double) * o->nmk);
+  }
+  return 0;
+}

@@ -46,6 +46,9 @@ int   orc_solve(void* o, int kind, const double* q, double* x);
 /* SOR Poisson (fenep.f90:848-905) on the context's grid; returns iterations */
 int   orc_sor(void* o, const double* q, double* phi_ghosted);
 
+int    orc_ibm_apply(void* o, const double* up3_ghosted, double* fint3,
+                     double* fm3, double* ub3);
+
 double orc_vieta(const double* a6);
 int    orc_select(const double* cm1, const double* c0, const double* cp1,
                   double* cp_out, double* cn_out);

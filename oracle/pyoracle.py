@@ -47,6 +47,7 @@ def lib():
         L.orc_eval_operator.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_sor.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_ibm_apply.argtypes = [C.c_void_p] + [_dp] * 4
         L.orc_vieta.restype = C.c_double
         L.orc_vieta.argtypes = [_dp]
         L.orc_select.argtypes = [_dp] * 5
@@ -122,6 +123,18 @@ class Oracle:
         out = [np.zeros(self._mshape) for _ in range(3)]
         lib().orc_get_marker_forces(self.h, *[ptr(a) for a in out])
         return out
+
+    def ibm_apply(self, up3):
+        """ibm_Helmholtz on a ghosted (3, nz+2, ny+2, nx+2) velocity; returns
+        (spread force (3, nz, ny, nx), marker forces, marker velocities)."""
+        up3 = np.ascontiguousarray(up3, dtype=np.float64)
+        fint = np.zeros((3, self.nz, self.ny, self.nx))
+        fm = np.zeros((3,) + self._mshape)
+        ub = np.zeros((3,) + self._mshape)
+        rc = lib().orc_ibm_apply(self.h, ptr(up3), ptr(fint), ptr(fm), ptr(ub))
+        if rc:
+            raise RuntimeError("ibm_apply rc=%d" % rc)
+        return fint, fm, ub
 
     def step(self, n=1):
         rc = lib().orc_step(self.h, int(n))

@@ -1,0 +1,70 @@
+"""C-ABI library checks that need no GPU: it loads, exports every entry
+point include/fenep_gpu.h declares, validates parameters before touching
+the device, and produces the seeded input stream the host code uses."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fortran8_amd import api, fields
+from fortran8_amd.params import FpParams, default_params
+
+
+def test_library_exports_every_declared_entry_point():
+    names = api.header_exports()
+    assert {"fp_create", "fp_step", "fp_last_error", "fp_seeded_uniform"} <= set(names) and len(names) >= 17
+    L = api.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", api.LIB_PATH], capture_output=True, text=True).stdout
+    for n in names:
+        assert (" T " + n) in out, n
+
+
+def test_library_is_built_for_gfx950():
+    out = subprocess.run(["/opt/rocm/lib/llvm/bin/llvm-objdump", "--offloading", api.LIB_PATH],
+                         capture_output=True, text=True)
+    assert "gfx950" in (out.stdout + out.stderr)
+
+
+def test_default_params_match_reference_constants():
+    p = FpParams()
+    assert api.lib().fp_default_params(C.byref(p)) == 0
+    assert p.newton_eps == 1e-6 and p.newton_itrmax == 10000 and p.newton_dc == 1e-5   # fenep.f90:526-528
+    assert p.sor_eps == 1e-5 and p.sor_itrmax == 10000                                  # fenep.f90:856-857
+    q = default_params()
+    for name in ("nx", "ny", "nz", "newton_eps", "newton_dc", "sor_eps", "method"):
+        assert getattr(p, name) == getattr(q, name)
+
+
+@pytest.mark.parametrize("bad", [dict(nx=12), dict(ny=2), dict(dt=-1.0), dict(re=0.0), dict(method=5),
+                                 dict(precision=3), dict(nz=2048)])
+def test_create_rejects_invalid_parameters_without_a_device(bad):
+    p = default_params(**bad)
+    h = C.c_void_p()
+    assert api.lib().fp_create(C.byref(p), C.byref(h)) == 1
+    assert not h.value
+
+
+def test_null_handles_are_safe():
+    L = api.lib()
+    assert L.fp_destroy(None) == 0
+    assert L.fp_last_error(None) == b"null context"
+    assert L.fp_step(None, 1, 1) == 1
+
+
+def test_seeded_uniform_stream_matches_host_implementation():
+    a = api.seeded_uniform(12345, 1000, offset=77)
+    b = fields.seeded_uniform(12345, 1000, offset=77)
+    assert np.array_equal(a, b)
+    assert a.min() >= 0.0 and a.max() < 1.0
+    assert abs(a.mean() - 0.5) < 0.05
+
+
+def test_oracle_is_not_linked_by_the_product():
+    out = subprocess.run(["ldd", api.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in out
+    syms = subprocess.run(["nm", "-D", api.LIB_PATH], capture_output=True, text=True).stdout
+    assert "orc_" not in syms

@@ -1,0 +1,185 @@
+"""Parity of the HIP step (through the C-ABI) with the FP64 CPU oracle.
+
+The plain FP64 path evaluates every expression in the oracle's order without
+fused multiply-add, so the bar is bit-exact equality of U, V, W, P and C
+after several steps; the SOR variant (red-black on the GPU, lexicographic in
+the reference) is compared at the solver tolerance."""
+import numpy as np
+import pytest
+
+from fortran8_amd import api, configs
+from fortran8_amd.fields import interior, interior_c, rel_l2, cylinder_markers
+from fortran8_amd.params import METHOD_EXPLICIT, METHOD_FFT, METHOD_IBM, POISSON_SOR, default_params
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U", "V", "W", "P")
+
+
+def run_both(case, steps):
+    print("parity case", case.name, case.params.nx, case.params.ny, case.params.nz, "steps", steps, flush=True)
+    o = po.load_case(case)
+    g = api.load_case(case)
+    o.step(steps)
+    g.step(steps)
+    return o, g
+
+
+def assert_bitwise(o, g):
+    so, sg = o.get_state(), g.get_state()
+    for name in FIELDS:
+        a, b = interior(getattr(sg, name)), interior(getattr(so, name))
+        assert np.array_equal(a, b), (name, rel_l2(a, b), np.max(np.abs(a - b)))
+    a, b = interior_c(sg.C), interior_c(so.C)
+    assert np.array_equal(a, b), ("C", rel_l2(a, b), np.max(np.abs(a - b)))
+    assert np.array_equal(sg.hist, so.hist)
+
+
+def assert_close(o, g, tol):
+    so, sg = o.get_state(), g.get_state()
+    for name in FIELDS[:3]:
+        assert rel_l2(interior(getattr(sg, name)), interior(getattr(so, name))) < tol, name
+    assert rel_l2(interior_c(sg.C), interior_c(so.C)) < tol
+
+
+def test_taylor_green_newtonian_bitwise():
+    o, g = run_both(configs.taylor_green(n=16), 10)
+    assert_bitwise(o, g)
+
+
+def test_kolmogorov_fenep_bitwise():
+    o, g = run_both(configs.kolmogorov(n=32), 10)
+    assert_bitwise(o, g)
+
+
+def test_anisotropic_noncubic_grid_bitwise():
+    case = configs.kolmogorov(n=32, nz=8)
+    case.params.ny = 16
+    case = configs.Case(case.name, case.params, _resize(case, 32, 16, 8), force=None)
+    o, g = run_both(case, 5)
+    assert_bitwise(o, g)
+
+
+def _resize(case, nx, ny, nz):
+    from fortran8_amd.configs import _spd_perturbed_c, _ghost_c, _noise_velocity
+    from fortran8_amd.fields import State, pad_periodic, ghosted_zeros
+    p = case.params
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.lx, p.ly, p.lz = 2.0, 1.0, 1.0
+    u, v, w = _noise_velocity(9, nx, ny, nz, 1e-2, 2.0 / nx, 1.0 / ny, 1.0 / nz, False, 0)
+    c = _spd_perturbed_c(9, nx, ny, nz, amp=0.05, offset=99999)
+    return State(pad_periodic(u), pad_periodic(v), pad_periodic(w), ghosted_zeros(nx, ny, nz), _ghost_c(c))
+
+
+def test_channel_cylinder_fft_line_solve_bitwise():
+    o, g = run_both(configs.channel_cylinder(nx=64, ny=32, nz=16), 5)
+    assert_bitwise(o, g)
+    fo, fg = o.marker_forces(), g.marker_forces()
+    for a, b in zip(fo, fg):
+        assert np.array_equal(a, b)
+
+
+def test_channel_cylinder_sor_variant_at_solver_tolerance():
+    case = configs.channel_cylinder(nx=32, ny=16, nz=8, poisson_variant=POISSON_SOR)
+    case.params.sor_eps = 1e-10
+    case.params.sor_itrmax = 20000
+    o, g = run_both(case, 3)
+    assert_close(o, g, 1e-8)
+    assert g.diagnostics()["sor_itr"] > 0
+
+
+def test_throughput_layout_small_bitwise():
+    o, g = run_both(configs.throughput(nx=64, ny=64, nz=16, ncyl=2), 4)
+    assert_bitwise(o, g)
+
+
+def test_periodic_ibm_adjoint_config_bitwise():
+    o, g = run_both(configs.adjoint_case(nx=32, ny=32, nz=16), 4)
+    assert_bitwise(o, g)
+
+
+def test_explicit_method_sor_at_solver_tolerance():
+    case = configs.kolmogorov(n=16)
+    case.params.method = METHOD_EXPLICIT
+    case.params.poisson_variant = POISSON_SOR
+    case.params.sor_eps = 1e-10
+    case.params.sor_itrmax = 20000
+    o, g = run_both(case, 3)
+    assert_close(o, g, 1e-8)
+
+
+def test_restart_through_the_boundary_is_exact():
+    case = configs.kolmogorov(n=16)
+    g1 = api.load_case(case)
+    g1.step(6)
+    g2 = api.load_case(case)
+    g2.step(3)
+    st = g2.get_state()
+    g3 = api.FenepGPU(case.params)
+    g3.set_state(st)
+    g3.set_body_force(case.force)
+    g3.step(3)
+    a, b = g1.get_state(), g3.get_state()
+    for name in FIELDS:
+        assert np.array_equal(getattr(a, name), getattr(b, name))
+    assert np.array_equal(a.C, b.C)
+
+
+def test_diagnostics_agree_with_oracle():
+    o, g = run_both(configs.channel_cylinder(nx=32, ny=16, nz=8), 3)
+    do, dg = o.diagnostics(), g.diagnostics()
+    for k in ("ke", "trc_min", "trc_max", "trc_mean", "drag", "lift"):
+        assert dg[k] == pytest.approx(do[k], rel=1e-12, abs=1e-14), k
+    assert dg["div_max"] < 1e-10 and dg["nan_flag"] == 0
+    assert dg["newton_itr_mean"] == pytest.approx(do["newton_itr_mean"], rel=1e-12)
+
+
+def test_full_size_throughput_case_two_steps_bitwise():
+    case = configs.throughput()
+    o, g = run_both(case, 2)
+    assert_bitwise(o, g)
+
+
+def test_full_size_throughput_properties_over_twenty_steps():
+    case = configs.throughput()
+    g = api.load_case(case)
+    g.step(20)
+    d = g.diagnostics()
+    assert d["nan_flag"] == 0
+    assert d["div_max"] < 1e-10
+    assert 2.9 < d["trc_min"] and d["trc_max"] < 400.0
+    assert d["cfl"] < 1.0
+
+
+def test_nan_is_reported():
+    case = configs.taylor_green(n=8)
+    case.state.U[3, 3, 3] = np.nan
+    g = api.load_case(case)
+    g.step(1)
+    assert g.diagnostics()["nan_flag"] == 1
+
+
+def test_error_paths():
+    p = default_params(nx=8, ny=8, nz=8)
+    g = api.FenepGPU(p)
+    with pytest.raises(api.FenepError, match="FP_E_STATE"):
+        g.step(1)
+    case = configs.channel_cylinder(nx=32, ny=16, nz=8)
+    g = api.FenepGPU(case.params)
+    g.set_state(case.state)
+    with pytest.raises(api.FenepError, match="FP_E_STATE"):
+        g.step(1)   # IBM method without markers
+    bad = cylinder_markers([(1.0, 0.05)], 0.5, 0.0625, 0.0625, 8)
+    with pytest.raises(api.FenepError, match="FP_E_ARG"):
+        g.set_markers(bad)
+
+
+def test_device_pointer_boundary_round_trip():
+    import os
+    import subprocess
+    import sys
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_devptr_check.py")
+    r = subprocess.run([sys.executable, script], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "round trip ok" in r.stdout
