@@ -21,6 +21,9 @@ template <typename T>
 void dalloc(T** p, size_t count) {
   FP_HIP_CHECK(hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
   FP_HIP_CHECK(hipMemset(*p, 0, sizeof(T) * (count ? count : 1)));
+  // the zero fill runs on the null stream; kernels that initialise the
+  // buffer next run on the context's non-blocking stream
+  FP_HIP_CHECK(hipDeviceSynchronize());
 }
 template <typename T>
 void dfree(T*& p) {
@@ -169,7 +172,7 @@ int fp_default_params(fp_params* p) {
   p->method = FP_METHOD_FFT; p->walls_y = 0; p->poisson_variant = FP_POISSON_FFT; p->precision = FP_PREC_F64;
   p->newton_itrmax = 10000; p->newton_eps = 1.0e-6; p->newton_dc = 1.0e-5;
   p->sor_itrmax = 10000; p->sor_eps = 1.0e-5; p->sor_omega = 0.0;
-  p->n_dc_max = 2; p->dc_tol = 1.0e-10;
+  p->n_dc_max = 2; p->dc_tol = 1.0e-6;
   return FP_OK;
 }
 
@@ -206,6 +209,7 @@ int fp_create(const fp_params* p, void** out) {
     dalloc(&c->spec, (size_t)g.nxh * g.sy * g.nz);
     if (g.walls) dalloc(&c->thp, (size_t)(g.ny + 2) * g.nxh * g.nz);
     dalloc(&c->red, 64);
+    dalloc(&c->part, (size_t)(g.n / 256 + 1024));
     auto up2 = [&](double2** d, const std::vector<double2>& h) {
       dalloc(d, h.size());
       FP_HIP_CHECK(hipMemcpy(*d, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
@@ -239,7 +243,7 @@ int fp_destroy(void* vc) {
   dfree(c->U); dfree(c->V); dfree(c->W); dfree(c->P); dfree(c->Phi);
   dfree(c->C[0]); dfree(c->C[1]); dfree(c->fC); dfree(c->sel); dfree(c->nitr);
   dfree(c->A0); dfree(c->T0); dfree(c->B0); dfree(c->B); dfree(c->Q); dfree(c->Up); dfree(c->F);
-  dfree(c->spec); dfree(c->thp); dfree(c->red);
+  dfree(c->spec); dfree(c->thp); dfree(c->red); dfree(c->part);
   dfree(c->twx); dfree(c->twy); dfree(c->twz); dfree(c->kx2); dfree(c->ky2); dfree(c->kz2);
   dfree(c->mk); dfree(c->mf); dfree(c->mb); dfree(c->ftmp);
   for (int k = 0; k < 3; k++) { dfree(c->sp_row[k]); dfree(c->sp_tgt[k]); dfree(c->sp_ent[k]); }

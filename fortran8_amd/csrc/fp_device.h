@@ -204,4 +204,31 @@ __device__ __forceinline__ double peterlin_d(double lp, double c0, double c3, do
   return (lp * lp - 3.0) / (((lp * lp - c0) - c3) - c5);
 }
 
+// Deterministic two-level reductions: each block stores its partial sum,
+// one block adds the partials (no contended atomics on a single address).
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// out[0] = sum(part[0..n)) (accumulate = 0) or out[0] += sum (accumulate = 1)
+static __global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ part, int n, double* __restrict__ out,
+                                                       int accumulate) {
+  __shared__ double sh[1024];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) v += part[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = accumulate ? out[0] + sh[0] : sh[0];
+}
+
 }  // namespace fp

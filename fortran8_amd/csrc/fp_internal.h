@@ -49,6 +49,7 @@ struct Ctx {
   double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
   double *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
   double* red = nullptr;              // reduction scratch
+  double* part = nullptr;             // per-block partial sums
   int* nitr = nullptr;                // Newton iteration counts (n)
   // IBM
   int nc = 0, nl = 0;

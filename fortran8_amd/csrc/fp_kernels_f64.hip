@@ -729,28 +729,18 @@ __global__ __launch_bounds__(256) void k_sor_color(Grid g, int color, double om,
          + BZ * ldb(phi, g, i, j, k + 1, BC_P)) - q[m];
     phi[m] = p0 + (om * e) / B0;
   }
-  // block reduction of e^2
+  // block partial of e^2 (the second colour adds to the first's partial)
   __shared__ double sh[256];
-  sh[threadIdx.x] = e * e;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(e2, sh[0]);
+  double s = block_sum256(e * e, sh);
+  if (threadIdx.x == 0) e2[blockIdx.x] = color == 0 ? s : e2[blockIdx.x] + s;
 }
 
-__global__ __launch_bounds__(256) void k_sumsq(long n, const double* __restrict__ q, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sumsq(long n, const double* __restrict__ q, double* __restrict__ part) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double v = m < n ? q[m] * q[m] : 0.0;
   __shared__ double sh[256];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(out, sh[0]);
+  double s = block_sum256(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 __global__ void k_zero_n(double* p, int n) {
@@ -764,21 +754,23 @@ int f64_sor(Ctx& c, const double* q, double* phi) {
   double om = c.p.sor_omega > 0.0 ? c.p.sor_omega : 2.0 / (1.0 + sin(3.14159265358979323846 * g.dx));
   const int check = 10;  // residual read back every `check` sweeps
   double* red = c.red;   // [0] = er0, [1 + s] = residual of sweep s
-  hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, red, 1 + check);
-  hipLaunchKernelGGL(k_sumsq, dim3(nblk(g.n, 256)), dim3(256), 0, s, g.n, q, red);
+  double* part = c.part; // per-block partial sums
+  hipLaunchKernelGGL(k_sumsq, dim3(nblk(g.n, 256)), dim3(256), 0, s, g.n, q, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part, nblk(g.n, 256), red, 0);
   double er0 = 0.0;
   FP_HIP_CHECK(hipMemcpyAsync(&er0, red, sizeof(double), hipMemcpyDeviceToHost, s));
   FP_HIP_CHECK(hipStreamSynchronize(s));
   if (er0 == 0.0) return 0;
   long half = g.n / 2;
+  const int nbh = nblk(half, 256);
   int itr = 0;
   std::vector<double> res(check);
   while (itr < c.p.sor_itrmax) {
     int batch = std::min(check, c.p.sor_itrmax - itr);
-    hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, red + 1, check);
     for (int b = 0; b < batch; b++) {
-      hipLaunchKernelGGL(k_sor_color, dim3(nblk(half, 256)), dim3(256), 0, s, g, 0, om, q, phi, red + 1 + b);
-      hipLaunchKernelGGL(k_sor_color, dim3(nblk(half, 256)), dim3(256), 0, s, g, 1, om, q, phi, red + 1 + b);
+      hipLaunchKernelGGL(k_sor_color, dim3(nbh), dim3(256), 0, s, g, 0, om, q, phi, part);
+      hipLaunchKernelGGL(k_sor_color, dim3(nbh), dim3(256), 0, s, g, 1, om, q, phi, part);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part, nbh, red + 1 + b, 0);
     }
     FP_HIP_CHECK(hipMemcpyAsync(res.data(), red + 1, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
     FP_HIP_CHECK(hipStreamSynchronize(s));

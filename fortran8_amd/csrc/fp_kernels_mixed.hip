@@ -36,6 +36,10 @@ struct Mixed {
   double *phi = nullptr, *b = nullptr;
   float *r32 = nullptr, *d32 = nullptr;
   double* red = nullptr;          // [0]=|b|^2, [1]=|r|^2 of the last sweep
+  double* part = nullptr;         // per-block partial sums
+  int* skip = nullptr;            // defect correction converged: skip the sweep
+  float *R = nullptr, *X2 = nullptr;  // Helmholtz right-hand side / iterate
+  int helm_iters = 0;             // fixed-point sweeps (0: spectral solve)
   unsigned* amb = nullptr;        // ambiguous selections re-decided in FP64
   int* amb_list = nullptr;
   long amb_cap = 0;
@@ -57,6 +61,9 @@ template <typename T>
 void dalloc(T** p, size_t count) {
   FP_HIP_CHECK(hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
   FP_HIP_CHECK(hipMemset(*p, 0, sizeof(T) * (count ? count : 1)));
+  // the zero fill runs on the null stream; kernels that initialise the
+  // buffer next run on the context's non-blocking stream
+  FP_HIP_CHECK(hipDeviceSynchronize());
 }
 template <typename T>
 void dfree(T*& p) {
@@ -458,7 +465,8 @@ __device__ float2* stockham_f(float2* x, float2* y, int n, int logn, int nl, con
 __device__ __forceinline__ int ilog2f(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
 
 __global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restrict__ q, float2* __restrict__ spec,
-                                                   const float2* __restrict__ tw, int nl) {
+                                                   const float2* __restrict__ tw, int nl, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
   const int n = g.nx;
   float2* x = ldsf;
@@ -483,7 +491,8 @@ __global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restri
 }
 
 __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restrict__ spec, float* __restrict__ out,
-                                                   const float2* __restrict__ tw, int nl) {
+                                                   const float2* __restrict__ tw, int nl, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
   const int n = g.nx, h = n / 2, nxh = g.nxh;
   float2* x = ldsf;
@@ -513,7 +522,9 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
 
 __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spec, const float2* __restrict__ tw, int axis,
                                                 int mode, int kind, int row0, int ti, const float* __restrict__ kx2,
-                                                const float* __restrict__ ky2, const float* __restrict__ kz2) {
+                                                const float* __restrict__ ky2, const float* __restrict__ kz2,
+                                                const int* __restrict__ skip) {
+  if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
   const int n = axis == 1 ? g.ny : g.nz;
   const int nxh = g.nxh;
@@ -558,7 +569,9 @@ __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spe
 }
 
 __global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spec, float* __restrict__ thp, int kind,
-                                                const float* __restrict__ kx2, const float* __restrict__ kz2) {
+                                                const float* __restrict__ kx2, const float* __restrict__ kz2,
+                                                const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const long lines = (long)g.nxh * g.nz;
   long L = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (L >= lines) return;
@@ -612,28 +625,28 @@ __global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spe
 
 static int lines_for32(int n) { return std::max(1, std::min(32, 4096 / n)); }
 
-static void solve32(Ctx& c, int kind, const float* q, float* x) {
+static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip = nullptr) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
   hipStream_t s = c.stream;
   int lx = lines_for32(g.nx);
   long xlines = (long)g.ny * g.nz;
   size_t shx = sizeof(float2) * 2 * lx * g.nx;
-  hipLaunchKernelGGL(m_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, q, X.spec, X.twx, lx);
+  hipLaunchKernelGGL(m_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, q, X.spec, X.twx, lx, skip);
   int tiy = std::max(1, std::min(32, 4096 / g.ny)), tiz = std::max(1, std::min(32, 4096 / g.nz));
   int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
   size_t shy = sizeof(float2) * 2 * tiy * g.ny, shz = sizeof(float2) * 2 * tiz * g.nz;
   if (!g.walls) {
-    hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 0, kind, 0, tiy, X.kx2, X.ky2, X.kz2);
-    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 2, kind, 0, tiz, X.kx2, X.ky2, X.kz2);
-    hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 1, kind, 0, tiy, X.kx2, X.ky2, X.kz2);
+    hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 0, kind, 0, tiy, X.kx2, X.ky2, X.kz2, skip);
+    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 2, kind, 0, tiz, X.kx2, X.ky2, X.kz2, skip);
+    hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 1, kind, 0, tiy, X.kx2, X.ky2, X.kz2, skip);
   } else {
-    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 0, kind, 1, tiz, X.kx2, X.ky2, X.kz2);
+    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 0, kind, 1, tiz, X.kx2, X.ky2, X.kz2, skip);
     long lines = (long)g.nxh * g.nz;
-    hipLaunchKernelGGL(m_thomas, dim3(nblk(lines, 256)), dim3(256), 0, s, g, X.spec, X.thp, kind, X.kx2, X.kz2);
-    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 1, kind, 1, tiz, X.kx2, X.ky2, X.kz2);
+    hipLaunchKernelGGL(m_thomas, dim3(nblk(lines, 256)), dim3(256), 0, s, g, X.spec, X.thp, kind, X.kx2, X.kz2, skip);
+    hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 1, kind, 1, tiz, X.kx2, X.ky2, X.kz2, skip);
   }
-  hipLaunchKernelGGL(m_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, X.spec, x, X.twx, lx);
+  hipLaunchKernelGGL(m_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, X.spec, x, X.twx, lx, skip);
 }
 
 // ============================================================ projection ===
@@ -652,13 +665,8 @@ __global__ __launch_bounds__(256) void m_divergence(Grid g, const float* __restr
     b32[m] = (float)v;
   }
   __shared__ double sh[256];
-  sh[threadIdx.x] = v * v;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(nrm, sh[0]);
+  double sum = block_sum256(v * v, sh);
+  if (threadIdx.x == 0) nrm[blockIdx.x] = sum;
 }
 
 // r = b - L_h phi (FP64, boundary rules of Phi), rounded to FP32 for the
@@ -679,19 +687,66 @@ __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restri
     r32[m] = (float)r;
   }
   __shared__ double sh[256];
-  sh[threadIdx.x] = r * r;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(nrm, sh[0]);
+  double sum = block_sum256(r * r, sh);
+  if (threadIdx.x == 0) nrm[blockIdx.x] = sum;
 }
 
-__global__ __launch_bounds__(256) void m_accum(long n, const float* __restrict__ d, double* __restrict__ phi, int first) {
+// skip the correction sweep once |r| <= tol |b|
+__global__ void m_dc_decide(const double* __restrict__ red, double tol, int* __restrict__ skip) {
+  if (threadIdx.x == 0) skip[0] = red[1] <= tol * tol * red[0] ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void m_accum(long n, const float* __restrict__ d, double* __restrict__ phi, int first,
+                                               const int* __restrict__ skip) {
+  if (skip && *skip) return;
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= n) return;
   phi[m] = first ? (double)d[m] : phi[m] + (double)d[m];
+}
+
+// One fixed-point sweep of the Crank-Nicolson Helmholtz problem
+// [1 - eps lap_h] x = q for the three velocity components at once:
+// x_out = q + eps lap_h x_in (x_in = q on the first sweep).  With
+// eps * |lap_h| << 1 the error contracts by eps*lambda_max per sweep.
+__global__ __launch_bounds__(256) void m_helm_sweep(Grid g, float eps, const float* __restrict__ q,
+                                                    const float* __restrict__ xin, float* __restrict__ xout) {
+  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  const float ix2 = (float)(1.0 / (g.dx * g.dx)), iy2 = (float)(1.0 / (g.dy * g.dy)), iz2 = (float)(1.0 / (g.dz * g.dz));
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float* X = xin + c * g.n;
+    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+    float x0 = ldb(X, g, i, j, k, kind);
+    float lap = (ldb(X, g, i - 1, j, k, kind) - 2.0f * x0 + ldb(X, g, i + 1, j, k, kind)) * ix2 +
+                (ldb(X, g, i, j - 1, k, kind) - 2.0f * x0 + ldb(X, g, i, j + 1, k, kind)) * iy2 +
+                (ldb(X, g, i, j, k - 1, kind) - 2.0f * x0 + ldb(X, g, i, j, k + 1, kind)) * iz2;
+    float v = q[c * g.n + m] + eps * lap;
+    if (c == 1 && g.walls && j == g.ny - 1) v = 0.0f;   // wall face of V
+    xout[c * g.n + m] = v;
+  }
+}
+
+// Helmholtz solves of the three predictor components: fixed-point sweeps when
+// eps*lambda_max is small, the spectral (+ line) solve otherwise.  q and the
+// result x are 3-plane buffers; x may not alias q.
+static void helmholtz32(Ctx& c, const float* q, float* x) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  const long n = g.n;
+  if (X.helm_iters <= 0) {
+    for (int k = 0; k < 3; k++) solve32(c, k, q + k * n, x + k * n);
+    return;
+  }
+  const float eps = (float)(((g.dt * g.beta) / g.re) / 2.0);
+  const float* in = q;
+  for (int it = 0; it < X.helm_iters; it++) {
+    // the last sweep writes x; earlier ones alternate so that it does
+    float* out = ((X.helm_iters - 1 - it) % 2 == 0) ? x : X.X2;
+    hipLaunchKernelGGL(m_helm_sweep, dim3(nblk(n, 256)), dim3(256), 0, c.stream, g, eps, q, in, out);
+    in = out;
+  }
 }
 
 // fft_march (fenep.f90:1562-1604) with FP64 potential, FP32 storage
@@ -867,6 +922,18 @@ void mixed_create(Ctx& c) {
   if (g.walls) dalloc(&X->thp, (size_t)(g.ny + 2) * g.nxh * g.nz);
   dalloc(&X->phi, n); dalloc(&X->b, n); dalloc(&X->r32, n); dalloc(&X->d32, n);
   dalloc(&X->red, 8);
+  dalloc(&X->part, (size_t)(n / 256 + 1024));
+  dalloc(&X->skip, 1);
+  dalloc(&X->R, 3 * n); dalloc(&X->X2, 3 * n);
+  {
+    // fixed-point Helmholtz sweeps: contraction eps*lambda_max per sweep,
+    // enough sweeps for a 1e-8 relative error, spectral solve if > 6
+    const double eps = ((g.dt * g.beta) / g.re) / 2.0;
+    const double lam = 4.0 / (g.dx * g.dx) + 4.0 / (g.dy * g.dy) + 4.0 / (g.dz * g.dz);
+    const double rho = eps * lam;
+    int m = rho > 0.0 ? (int)std::ceil(std::log(1.0e-8) / std::log(rho)) : 1;
+    X->helm_iters = (rho < 0.25 && m <= 6) ? std::max(1, m) : 0;
+  }
   X->amb_cap = std::max<long>(1024, 3 * n / 8);
   dalloc(&X->amb, 1); dalloc(&X->amb_list, X->amb_cap);
   // FP32 twiddles and wavenumbers from the FP64 host tables
@@ -897,7 +964,8 @@ void mixed_destroy(Ctx& c) {
   dfree(X->A0); dfree(X->T0); dfree(X->B); dfree(X->Q); dfree(X->Up); dfree(X->F);
   dfree(X->spec); dfree(X->thp); dfree(X->twx); dfree(X->twy); dfree(X->twz);
   dfree(X->kx2); dfree(X->ky2); dfree(X->kz2);
-  dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->amb); dfree(X->amb_list);
+  dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->part); dfree(X->skip);
+  dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
   delete X;
@@ -1041,7 +1109,7 @@ void mixed_step(Ctx& c) {
   }
   if (c.p.method == FP_METHOD_FFT) {
     MTimer t(c, "helmholtz");
-    for (int k = 0; k < 3; k++) solve32(c, k, X.Q + k * n, X.Up + k * n);
+    helmholtz32(c, X.Q, X.Up);
   } else {
     {
       MTimer t(c, "ibm");
@@ -1056,23 +1124,30 @@ void mixed_step(Ctx& c) {
         hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
         hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
       }
-      hipLaunchKernelGGL(m_ibm_rhs, dim3(nblk(3 * n, 256)), dim3(256), 0, s, g, X.Q, X.ftmp, X.B, X.Up);
+      hipLaunchKernelGGL(m_ibm_rhs, dim3(nblk(3 * n, 256)), dim3(256), 0, s, g, X.Q, X.ftmp, X.B, X.R);
     }
     MTimer t(c, "helmholtz");
-    for (int k = 0; k < 3; k++) solve32(c, k, X.Up + k * n, X.Up + k * n);
+    helmholtz32(c, X.R, X.Up);
   }
   {
     MTimer t(c, "poisson");
+    const int nb = nblk(n, 256);
     hipLaunchKernelGGL(m_zero_d, dim3(1), dim3(64), 0, s, 8, X.red);
-    hipLaunchKernelGGL(m_divergence, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.b, X.r32, X.red);
+    hipLaunchKernelGGL(m_divergence, dim3(nb), dim3(256), 0, s, g, X.Up, X.b, X.r32, X.part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, X.part, nb, X.red, 0);
     const int ndc = std::max(1, c.p.n_dc_max);
     for (int it = 0; it < ndc; it++) {
+      const int* skip = nullptr;
       if (it > 0) {
-        hipLaunchKernelGGL(m_zero_d, dim3(1), dim3(64), 0, s, 1, X.red + 1);
-        hipLaunchKernelGGL(m_residual, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.b, X.phi, X.r32, X.red + 1);
+        // FP64 defect: r = b - L_h phi; the sweep is skipped on device once
+        // |r| <= dc_tol |b|
+        hipLaunchKernelGGL(m_residual, dim3(nb), dim3(256), 0, s, g, X.b, X.phi, X.r32, X.part);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, X.part, nb, X.red + 1, 0);
+        hipLaunchKernelGGL(m_dc_decide, dim3(1), dim3(64), 0, s, X.red, c.p.dc_tol, X.skip);
+        skip = X.skip;
       }
-      solve32(c, 3, X.r32, X.d32);
-      hipLaunchKernelGGL(m_accum, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.d32, X.phi, it == 0 ? 1 : 0);
+      solve32(c, 3, X.r32, X.d32, skip);
+      hipLaunchKernelGGL(m_accum, dim3(nb), dim3(256), 0, s, n, X.d32, X.phi, it == 0 ? 1 : 0, skip);
     }
   }
   {
