@@ -74,25 +74,46 @@ void dfree(T*& p) {
 }  // namespace
 
 // ================================================================ FENE-P ===
+// Wrapped offset of cell a+d along an axis of n cells and stride s (|d| <= 2).
+__device__ __forceinline__ long woff(int a, int d, int n, long s) {
+  int t = a + d;
+  t = t < 0 ? t + n : (t >= n ? t - n : t);
+  return (long)(t - a) * s;
+}
+
+// arccos by Abramowitz & Stegun 4.4.46 (|error| <= 2e-8 on [0,1]), reflected
+// for x < 0
+__device__ __forceinline__ float acos_as(float x) {
+  const float ax = fabsf(x);
+  float p = -0.0012624911f;
+  p = fmaf(p, ax, 0.0066700901f);
+  p = fmaf(p, ax, -0.0170881256f);
+  p = fmaf(p, ax, 0.0308918810f);
+  p = fmaf(p, ax, -0.0501743046f);
+  p = fmaf(p, ax, 0.0889789874f);
+  p = fmaf(p, ax, -0.2145988016f);
+  p = fmaf(p, ax, 1.5707963050f);
+  const float r = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ax)) * p;
+  return x < 0.0f ? 3.14159265358979f - r : r;
+}
+
 // Smallest eigenvalue of the symmetric matrix a (xx,xy,xz,yy,yz,zz) by the
 // trigonometric root of Vieta (fenep.f90:425-444), evaluated on the
 // deviatoric part B = a - (tr a/3) I so that p and q do not cancel in FP32.
-// Same sentinel (-100) when the discriminant t >= 0.  Also returns |t| scale
-// via `margin` to flag near-degenerate cases.
+// Same sentinel (-100) when the discriminant t >= 0.
 __device__ __forceinline__ float lmin32(const float* a) {
   const float m = (a[0] + a[3] + a[5]) * (1.0f / 3.0f);
   const float b11 = a[0] - m, b22 = a[3] - m, b33 = a[5] - m;
   const float o2 = a[1] * a[1] + a[2] * a[2] + a[4] * a[4];
   const float p = -(0.5f * (b11 * b11 + b22 * b22 + b33 * b33) + o2);
   const float det = b11 * (b22 * b33 - a[4] * a[4]) - a[1] * (a[1] * b33 - a[4] * a[2]) + a[2] * (a[1] * a[4] - b22 * a[2]);
-  const float q = -det;
   const float p3 = p * (1.0f / 3.0f);
-  const float t = 0.25f * q * q + p3 * p3 * p3;
+  const float t = 0.25f * det * det + p3 * p3 * p3;
   if (!(t < 0.0f)) return -100.0f;
-  const float r = sqrtf(-p3);
-  float arg = -q / (2.0f * r * r * r);
+  const float r = __builtin_amdgcn_sqrtf(-p3);
+  float arg = 0.5f * det * __builtin_amdgcn_rcpf(r * r * r);   // -q/(2 r^3) with q = -det
   arg = fminf(1.0f, fmaxf(-1.0f, arg));
-  const float ang = acosf(arg) * (1.0f / 3.0f) + 2.09439510239319549f;
+  const float ang = acos_as(arg) * (1.0f / 3.0f) + 2.09439510239319549f;
   return m + 2.0f * r * __cosf(ang);
 }
 
@@ -109,6 +130,37 @@ __device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
   return c;
 }
 
+// C at cells a-r..a+r along axis d (r = 1 or 2) for the 6 components; wall
+// ghosts by linear extrapolation (C_kabe); points beyond one ghost layer
+// only feed faces whose upwind weight is exactly zero and get the centre.
+template <int R>
+__device__ __forceinline__ void load_line(const float* __restrict__ C, const Grid& g, long m, int i, int j, int k, int d,
+                                          float (&v)[2 * R + 1][6]) {
+  if (d == 1 && g.walls && (j < R || j > g.ny - 1 - R)) {
+#pragma unroll
+    for (int o = -R; o <= R; o++) {
+      const int jj = j + o;
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        const float* Cq = C + q * g.n;
+        v[o + R][q] = (jj >= -1 && jj <= g.ny) ? ldb(Cq, g, i, jj, k, BC_C) : Cq[m];
+      }
+    }
+    return;
+  }
+  long off[2 * R + 1];
+#pragma unroll
+  for (int o = -R; o <= R; o++) {
+    if (d == 0) off[o + R] = woff(i, o, g.nx, 1);
+    else if (d == 1) off[o + R] = woff(j, o, g.ny, g.nx);
+    else off[o + R] = woff(k, o, g.nz, (long)g.nx * g.ny);
+  }
+#pragma unroll
+  for (int o = 0; o < 2 * R + 1; o++)
+#pragma unroll
+    for (int q = 0; q < 6; q++) v[o][q] = C[q * g.n + m + off[o]];
+}
+
 // relative guard band for re-deciding a selection in FP64
 #define SEL_GUARD 1.0e-6f
 
@@ -120,39 +172,27 @@ __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict_
   uint8_t s = 0;
 #pragma unroll 1
   for (int d = 0; d < 3; d++) {
-    int di = d == 0, dj = d == 1, dk = d == 2;
-    float cm[6], c[6], cp[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-      const float* Cq = C + q * g.n;
-      cm[q] = ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
-      c[q] = Cq[m];
-      cp[q] = ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
-    }
-    float best = 0.0f, second = -1e30f;
-    int idx = 3;
+    float v[3][6];
+    load_line<1>(C, g, m, i, j, k, d, v);
     float vals[3];
 #pragma unroll
     for (int l = 0; l < 3; l++) {
       float ap[6], an[6];
 #pragma unroll
-      for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, cm[q], c[q], cp[q]); an[q] = cand_nf(l, cm[q], c[q], cp[q]); }
+      for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, v[0][q], v[1][q], v[2][q]); an[q] = cand_nf(l, v[0][q], v[1][q], v[2][q]); }
       vals[l] = fminf(lmin32(ap), lmin32(an));
     }
     // maxloc over {v0, v1, v2, 0}: first maximum
-    best = vals[0]; idx = 0;
+    float best = vals[0];
+    int idx = 0;
     if (vals[1] > best) { best = vals[1]; idx = 1; }
     if (vals[2] > best) { best = vals[2]; idx = 2; }
     if (0.0f > best) { best = 0.0f; idx = 3; }
-    // margin to the runner-up
+    float second = idx == 3 ? -1e30f : 0.0f;
 #pragma unroll
     for (int l = 0; l < 3; l++) if (l != idx) second = fmaxf(second, vals[l]);
-    if (idx != 3) second = fmaxf(second, 0.0f);
-    float scale = fmaxf(1.0f, fabsf(best));
-    bool sentinel_edge = false;
-#pragma unroll
-    for (int l = 0; l < 3; l++) sentinel_edge |= (vals[l] == -100.0f);
-    if (best - second < SEL_GUARD * scale || sentinel_edge) {
+    const bool sentinel = vals[0] == -100.0f || vals[1] == -100.0f || vals[2] == -100.0f;
+    if (best - second < SEL_GUARD * fmaxf(1.0f, fabsf(best)) || sentinel) {
       unsigned slot = atomicAdd(amb, 1u);
       if (slot < amb_cap) amb_list[slot] = (int)(m * 3 + d);
     }
@@ -193,22 +233,6 @@ __global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restr
   } while (old != assumed);
 }
 
-__device__ __forceinline__ void face_vals_f(const Grid& g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
-                                            int d, int i, int j, int k, bool left, float* out) {
-  i = wrapi(i, g.nx); j = wrapi(j, g.ny); k = wrapi(k, g.nz);
-  long m = cidx(g, i, j, k);
-  int l = (sel[m] >> (2 * d)) & 3;
-  int di = d == 0, dj = d == 1, dk = d == 2;
-#pragma unroll
-  for (int q = 0; q < 6; q++) {
-    const float* Cq = C + q * g.n;
-    float c = Cq[m];
-    float cm = l == 3 ? 0.0f : ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
-    float cp = l == 3 ? 0.0f : ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
-    out[q] = left ? cand_pf(l, cm, c, cp) : cand_nf(l, cm, c, cp);
-  }
-}
-
 // Peterlin factor: FP32 unless tr C is within 1% of L^2 (then FP64)
 __device__ __forceinline__ float peterlin_f(float L2, float s) {
   if (s > 0.99f * L2) return (float)(((double)L2 - 3.0) / ((double)L2 - (double)s));
@@ -223,6 +247,23 @@ __device__ __forceinline__ float dpeterlin_f(float L2, float s) {
   return (L2 - 3.0f) / (den * den);
 }
 
+// wrapped neighbour offsets of one cell (periodic in every axis; callers use
+// the boundary-rule accessor for wall rows)
+struct Nbr {
+  long x[3], y[3], z[3];
+};
+__device__ __forceinline__ Nbr nbr(const Grid& g, int i, int j, int k) {
+  Nbr o;
+  const long sy = g.nx, sz = (long)g.nx * g.ny;
+#pragma unroll
+  for (int d = -1; d <= 1; d++) {
+    o.x[d + 1] = woff(i, d, g.nx, 1);
+    o.y[d + 1] = woff(j, d, g.ny, sy);
+    o.z[d + 1] = woff(k, d, g.nz, sz);
+  }
+  return o;
+}
+
 // Cstar + Lyapunov Newton (analytic Jacobian, FP32) + Peterlin factor of the
 // new C.
 __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
@@ -234,49 +275,75 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
   if (m >= g.n) return;
   int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
   const float dt = (float)g.dt;
+  const bool fast = !g.walls || (j > 0 && j < g.ny - 1);
+  const Nbr o = nbr(g, i, j, k);
   float cx[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) cx[q] = 0.0f;
 #pragma unroll 1
   for (int d = 0; d < 3; d++) {
-    float h = (float)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
+    const float h = (float)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
     float v1, v2;
-    if (d == 0) { v1 = ldb(U, g, i - 1, j, k, BC_U); v2 = U[m]; }
-    else if (d == 1) { v1 = ldb(V, g, i, j - 1, k, BC_V); v2 = ldb(V, g, i, j, k, BC_V); }
-    else { v1 = ldb(W, g, i, j, k - 1, BC_W); v2 = W[m]; }
+    if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
+    else if (d == 1) { v1 = fast ? V[m + o.y[0]] : ldb(V, g, i, j - 1, k, BC_V); v2 = fast ? V[m] : ldb(V, g, i, j, k, BC_V); }
+    else { v1 = W[m + o.z[0]]; v2 = W[m]; }
     const float r2h = dt / (2.0f * h);
-    float s1 = (-v2 + fabsf(v2)) * r2h;
-    float s2 = (1.0f / 6.0f) + (-v2 - fabsf(v2)) * r2h;
-    float s3 = (1.0f / 6.0f) + (v1 - fabsf(v1)) * r2h;
-    float s4 = (v1 + fabsf(v1)) * r2h;
-    int di = d == 0, dj = d == 1, dk = d == 2;
-    float pl[6], pr[6], nl[6], nr[6];
-    face_vals_f(g, C, sel, d, i, j, k, true, pl);
-    face_vals_f(g, C, sel, d, i + di, j + dj, k + dk, true, pr);
-    face_vals_f(g, C, sel, d, i - di, j - dj, k - dk, false, nl);
-    face_vals_f(g, C, sel, d, i, j, k, false, nr);
+    const float s1 = (-v2 + fabsf(v2)) * r2h;
+    const float s2 = (1.0f / 6.0f) + (-v2 - fabsf(v2)) * r2h;
+    const float s3 = (1.0f / 6.0f) + (v1 - fabsf(v1)) * r2h;
+    const float s4 = (v1 + fabsf(v1)) * r2h;
+    const long om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
+    const long op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
+    const int lm = (sel[m + om] >> (2 * d)) & 3, l0 = (sel[m] >> (2 * d)) & 3, lp = (sel[m + op] >> (2 * d)) & 3;
+    float v[5][6];
+    load_line<2>(C, g, m, i, j, k, d, v);
 #pragma unroll
-    for (int q = 0; q < 6; q++) cx[q] += s3 * pl[q] + s1 * pr[q] + s4 * nl[q] + s2 * nr[q];
+    for (int q = 0; q < 6; q++) {
+      const float pl = cand_pf(l0, v[1][q], v[2][q], v[3][q]);   // Cp(a-1): own left face
+      const float nr = cand_nf(l0, v[1][q], v[2][q], v[3][q]);   // Cn(a):   own right face
+      const float pr = cand_pf(lp, v[2][q], v[3][q], v[4][q]);   // Cp(a):   left face of a+1
+      const float nl = cand_nf(lm, v[0][q], v[1][q], v[2][q]);   // Cn(a-1): right face of a-1
+      cx[q] += s3 * pl + s1 * pr + s4 * nl + s2 * nr;
+    }
   }
   const float dtw = dt / (float)g.wi;
   float b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
   const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz;
+  // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
+  float a11, a12, a13, a21, a22, a23, a31, a32, a33;
+  if (fast) {
+#define UF(a, bb, c) U[m + o.x[a] + o.y[bb] + o.z[c]]
+#define VF(a, bb, c) V[m + o.x[a] + o.y[bb] + o.z[c]]
+#define WF(a, bb, c) W[m + o.x[a] + o.y[bb] + o.z[c]]
+    a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) / dx;
+    a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) / (4.0f * dx);
+    a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) / (4.0f * dx);
+    a12 = -dt * ((UF(0, 2, 1) + UF(1, 2, 1)) - (UF(0, 0, 1) + UF(1, 0, 1))) / (4.0f * dy);
+    a22 = -dt * (VF(1, 1, 1) - VF(1, 0, 1)) / dy;
+    a32 = -dt * ((WF(1, 2, 0) - WF(1, 0, 0)) + (WF(1, 2, 1) - WF(1, 0, 1))) / (4.0f * dy);
+    a13 = -dt * ((UF(0, 1, 2) + UF(1, 1, 2)) - (UF(0, 1, 0) + UF(1, 1, 0))) / (4.0f * dz);
+    a23 = -dt * ((VF(1, 0, 2) + VF(1, 1, 2)) - (VF(1, 0, 0) + VF(1, 1, 0))) / (4.0f * dz);
+    a33 = -dt * (WF(1, 1, 1) - WF(1, 1, 0)) / dz;
+#undef UF
+#undef VF
+#undef WF
+  } else {
 #define UL(a, bb, c) ldb(U, g, a, bb, c, BC_U)
 #define VL(a, bb, c) ldb(V, g, a, bb, c, BC_V)
 #define WL(a, bb, c) ldb(W, g, a, bb, c, BC_W)
-  // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
-  const float a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
-  const float a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (4.0f * dx);
-  const float a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (4.0f * dx);
-  const float a12 = -dt * ((UL(i - 1, j + 1, k) + UL(i, j + 1, k)) - (UL(i - 1, j - 1, k) + UL(i, j - 1, k))) / (4.0f * dy);
-  const float a22 = -dt * (VL(i, j, k) - VL(i, j - 1, k)) / dy;
-  const float a32 = -dt * ((WL(i, j + 1, k - 1) - WL(i, j - 1, k - 1)) + (WL(i, j + 1, k) - WL(i, j - 1, k))) / (4.0f * dy);
-  const float a13 = -dt * ((UL(i - 1, j, k + 1) + UL(i, j, k + 1)) - (UL(i - 1, j, k - 1) + UL(i, j, k - 1))) / (4.0f * dz);
-  const float a23 = -dt * ((VL(i, j - 1, k + 1) + VL(i, j, k + 1)) - (VL(i, j - 1, k - 1) + VL(i, j, k - 1))) / (4.0f * dz);
-  const float a33 = -dt * (WL(i, j, k) - WL(i, j, k - 1)) / dz;
+    a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
+    a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (4.0f * dx);
+    a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (4.0f * dx);
+    a12 = -dt * ((UL(i - 1, j + 1, k) + UL(i, j + 1, k)) - (UL(i - 1, j - 1, k) + UL(i, j - 1, k))) / (4.0f * dy);
+    a22 = -dt * (VL(i, j, k) - VL(i, j - 1, k)) / dy;
+    a32 = -dt * ((WL(i, j + 1, k - 1) - WL(i, j - 1, k - 1)) + (WL(i, j + 1, k) - WL(i, j - 1, k))) / (4.0f * dy);
+    a13 = -dt * ((UL(i - 1, j, k + 1) + UL(i, j, k + 1)) - (UL(i - 1, j, k - 1) + UL(i, j, k - 1))) / (4.0f * dz);
+    a23 = -dt * ((VL(i, j - 1, k + 1) + VL(i, j, k + 1)) - (VL(i, j - 1, k - 1) + VL(i, j, k - 1))) / (4.0f * dz);
+    a33 = -dt * (WL(i, j, k) - WL(i, j, k - 1)) / dz;
 #undef UL
 #undef VL
 #undef WL
+  }
   const float L2 = (float)(g.lp * g.lp);
   float x[6];
 #pragma unroll
@@ -312,7 +379,7 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
     // Gaussian elimination (diagonally dominant: no pivoting needed)
 #pragma unroll
     for (int kk = 0; kk < 6; kk++) {
-      const float inv = 1.0f / J[kk][kk];
+      const float inv = __builtin_amdgcn_rcpf(J[kk][kk]);
 #pragma unroll
       for (int r = kk + 1; r < 6; r++) {
         const float fct = J[r][kk] * inv;
@@ -320,6 +387,7 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
         for (int cc = kk + 1; cc < 6; cc++) J[r][cc] -= fct * J[kk][cc];
         l[r] -= fct * l[kk];
       }
+      J[kk][kk] = inv;
     }
     float rr[6];
 #pragma unroll
@@ -327,7 +395,7 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
       float acc = l[r];
 #pragma unroll
       for (int cc = r + 1; cc < 6; cc++) acc -= J[r][cc] * rr[cc];
-      rr[r] = acc / J[r][r];
+      rr[r] = acc * J[r][r];
     }
     float sr = 0.0f, sx = 0.0f;
 #pragma unroll
@@ -354,6 +422,94 @@ __device__ __forceinline__ float c0_atf(const Grid& g, const float* __restrict__
   return fC[m] * C[q * g.n + m] - E;
 }
 
+// predictor body; FAST reads neighbours through wrapped offsets (no wall
+// row involved), otherwise through the boundary-rule accessor
+template <bool FAST>
+__device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int j, int k, int method, int first, int polymer,
+                                               const float* __restrict__ U, const float* __restrict__ V,
+                                               const float* __restrict__ W, const float* __restrict__ P,
+                                               const float* __restrict__ C, const float* __restrict__ fC,
+                                               float* __restrict__ A0, float* __restrict__ T0, float* __restrict__ Bout,
+                                               const float* __restrict__ F, float bf0, float bf1, float bf2,
+                                               float* __restrict__ Q) {
+  const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz, dt = (float)g.dt;
+  const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+  Nbr o;
+  if (FAST) o = nbr(g, i, j, k);
+  // (a, b, c) in {-1, 0, 1}
+#define NB(A, a, b, c, kind) (FAST ? A[m + o.x[(a) + 1] + o.y[(b) + 1] + o.z[(c) + 1]] : ldb(A, g, i + (a), j + (b), k + (c), kind))
+#define Ug(a, b, c) NB(U, a, b, c, BC_U)
+#define Vg(a, b, c) NB(V, a, b, c, BC_V)
+#define Wg(a, b, c) NB(W, a, b, c, BC_W)
+#define Pg(a, b, c) NB(P, a, b, c, BC_P)
+  float a[3], bv[3], t[3];
+  const float uc = Ug(0, 0, 0), vc = Vg(0, 0, 0), wc = Wg(0, 0, 0);
+  {
+    // convection (fenep.f90:694-734): -div(u u) with face averages
+    float h1 = 0.5f * (Ug(-1, 0, 0) + uc), h2 = 0.5f * (uc + Ug(1, 0, 0));
+    float ax = (h2 * h2 - h1 * h1) * idx_
+               + 0.25f * ((vc + Vg(1, 0, 0)) * (uc + Ug(0, 1, 0)) - (Vg(0, -1, 0) + Vg(1, -1, 0)) * (Ug(0, -1, 0) + uc)) * idy
+               + 0.25f * ((wc + Wg(1, 0, 0)) * (uc + Ug(0, 0, 1)) - (Wg(0, 0, -1) + Wg(1, 0, -1)) * (Ug(0, 0, -1) + uc)) * idz;
+    float g1 = 0.5f * (Vg(0, -1, 0) + vc), g2 = 0.5f * (vc + Vg(0, 1, 0));
+    float ay = 0.25f * ((uc + Ug(0, 1, 0)) * (vc + Vg(1, 0, 0)) - (Ug(-1, 0, 0) + Ug(-1, 1, 0)) * (Vg(-1, 0, 0) + vc)) * idx_
+               + (g2 * g2 - g1 * g1) * idy
+               + 0.25f * ((wc + Wg(0, 1, 0)) * (vc + Vg(0, 0, 1)) - (Wg(0, 0, -1) + Wg(0, 1, -1)) * (Vg(0, 0, -1) + vc)) * idz;
+    float e1 = 0.5f * (Wg(0, 0, -1) + wc), e2 = 0.5f * (wc + Wg(0, 0, 1));
+    float az = 0.25f * ((uc + Ug(0, 0, 1)) * (wc + Wg(1, 0, 0)) - (Ug(-1, 0, 0) + Ug(-1, 0, 1)) * (Wg(-1, 0, 0) + wc)) * idx_
+               + 0.25f * ((vc + Vg(0, 0, 1)) * (wc + Wg(0, 1, 0)) - (Vg(0, -1, 0) + Vg(0, -1, 1)) * (Wg(0, -1, 0) + wc)) * idy
+               + (e2 * e2 - e1 * e1) * idz;
+    a[0] = -ax; a[1] = -ay; a[2] = -az;
+    // viscous (fenep.f90:736-770)
+    const float cb = (float)(g.beta / g.re);
+    const float ix2 = idx_ * idx_, iy2 = idy * idy, iz2 = idz * idz;
+    bv[0] = cb * ((Ug(-1, 0, 0) - 2.0f * uc + Ug(1, 0, 0)) * ix2 + (Ug(0, -1, 0) - 2.0f * uc + Ug(0, 1, 0)) * iy2 +
+                  (Ug(0, 0, -1) - 2.0f * uc + Ug(0, 0, 1)) * iz2);
+    bv[1] = cb * ((Vg(-1, 0, 0) - 2.0f * vc + Vg(1, 0, 0)) * ix2 + (Vg(0, -1, 0) - 2.0f * vc + Vg(0, 1, 0)) * iy2 +
+                  (Vg(0, 0, -1) - 2.0f * vc + Vg(0, 0, 1)) * iz2);
+    bv[2] = cb * ((Wg(-1, 0, 0) - 2.0f * wc + Wg(1, 0, 0)) * ix2 + (Wg(0, -1, 0) - 2.0f * wc + Wg(0, 1, 0)) * iy2 +
+                  (Wg(0, 0, -1) - 2.0f * wc + Wg(0, 0, 1)) * iz2);
+  }
+  if (polymer) {
+#define C0g(q, aa, bb, cc) (FAST ? (fC[m + o.x[(aa) + 1] + o.y[(bb) + 1] + o.z[(cc) + 1]] * C[(q) * g.n + m + o.x[(aa) + 1] + o.y[(bb) + 1] + o.z[(cc) + 1]] - (((q) == 0 || (q) == 3 || (q) == 5) ? 1.0f : 0.0f)) : c0_atf(g, C, fC, q, i + (aa), j + (bb), k + (cc)))
+    const float tx = (C0g(0, 1, 0, 0) - C0g(0, 0, 0, 0)) * idx_
+                     + 0.25f * ((C0g(1, 0, 1, 0) + C0g(1, 1, 1, 0)) - (C0g(1, 0, -1, 0) + C0g(1, 1, -1, 0))) * idy
+                     + 0.25f * ((C0g(2, 0, 0, 1) + C0g(2, 1, 0, 1)) - (C0g(2, 0, 0, -1) + C0g(2, 1, 0, -1))) * idz;
+    const float ty = 0.25f * ((C0g(1, 1, 0, 0) - C0g(1, -1, 0, 0)) + (C0g(1, 1, 1, 0) - C0g(1, -1, 1, 0))) * idx_
+                     + (C0g(3, 0, 1, 0) - C0g(3, 0, 0, 0)) * idy
+                     + 0.25f * ((C0g(4, 0, 0, 1) + C0g(4, 0, 1, 1)) - (C0g(4, 0, 0, -1) + C0g(4, 0, 1, -1))) * idz;
+    const float tz = 0.25f * ((C0g(2, 1, 0, 0) - C0g(2, -1, 0, 0)) + (C0g(2, 1, 0, 1) - C0g(2, -1, 0, 1))) * idx_
+                     + 0.25f * ((C0g(4, 0, 1, 0) - C0g(4, 0, -1, 0)) + (C0g(4, 0, 1, 1) - C0g(4, 0, -1, 1))) * idy
+                     + (C0g(5, 0, 0, 1) - C0g(5, 0, 0, 0)) * idz;
+#undef C0g
+    const float coef = (float)((1.0 - g.beta) / (g.re * g.wi));
+    t[0] = coef * tx; t[1] = coef * ty; t[2] = coef * tz;
+  } else {
+    t[0] = t[1] = t[2] = 0.0f;
+  }
+  const float bfs[3] = {bf0, bf1, bf2};
+  const float p0 = P[m];
+  const float pn[3] = {Pg(1, 0, 0), Pg(0, 1, 0), Pg(0, 0, 1)};
+  const float uu[3] = {uc, vc, wc};
+  const float ih[3] = {idx_, idy, idz};
+  const float bcoef = method == FP_METHOD_FFT ? 0.5f : 1.0f;   // CN half (fft_navier) or full (ibm_preliminary)
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    long oo = c * g.n + m;
+    float a0 = first ? a[c] : A0[oo];
+    float t0 = first ? t[c] : T0[oo];
+    float fv = (F ? F[oo] : 0.0f) + bfs[c];
+    Q[oo] = uu[c] - dt * (pn[c] - p0) * ih[c] + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] + 0.5f * dt * (t[c] + t0) + dt * fv;
+    A0[oo] = a[c];
+    T0[oo] = t[c];
+    if (Bout) Bout[oo] = bv[c];
+  }
+#undef NB
+#undef Ug
+#undef Vg
+#undef Wg
+#undef Pg
+}
+
 __global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first, int polymer,
                                                    const float* __restrict__ U, const float* __restrict__ V,
                                                    const float* __restrict__ W, const float* __restrict__ P,
@@ -364,76 +520,10 @@ __global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
   int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
-  const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz, dt = (float)g.dt;
-  const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
-#define Ug(a, b, c) ldb(U, g, a, b, c, BC_U)
-#define Vg(a, b, c) ldb(V, g, a, b, c, BC_V)
-#define Wg(a, b, c) ldb(W, g, a, b, c, BC_W)
-  float a[3], bv[3], t[3];
-  {
-    // convection (fenep.f90:694-734): -div(u u) with face averages
-    const float uc = Ug(i, j, k), vc = Vg(i, j, k), wc = Wg(i, j, k);
-    float h1 = 0.5f * (Ug(i - 1, j, k) + uc), h2 = 0.5f * (uc + Ug(i + 1, j, k));
-    float ax = (h2 * h2 - h1 * h1) * idx_
-               + 0.25f * ((vc + Vg(i + 1, j, k)) * (uc + Ug(i, j + 1, k)) - (Vg(i, j - 1, k) + Vg(i + 1, j - 1, k)) * (Ug(i, j - 1, k) + uc)) * idy
-               + 0.25f * ((wc + Wg(i + 1, j, k)) * (uc + Ug(i, j, k + 1)) - (Wg(i, j, k - 1) + Wg(i + 1, j, k - 1)) * (Ug(i, j, k - 1) + uc)) * idz;
-    float g1 = 0.5f * (Vg(i, j - 1, k) + vc), g2 = 0.5f * (vc + Vg(i, j + 1, k));
-    float ay = 0.25f * ((uc + Ug(i, j + 1, k)) * (vc + Vg(i + 1, j, k)) - (Ug(i - 1, j, k) + Ug(i - 1, j + 1, k)) * (Vg(i - 1, j, k) + vc)) * idx_
-               + (g2 * g2 - g1 * g1) * idy
-               + 0.25f * ((wc + Wg(i, j + 1, k)) * (vc + Vg(i, j, k + 1)) - (Wg(i, j, k - 1) + Wg(i, j + 1, k - 1)) * (Vg(i, j, k - 1) + vc)) * idz;
-    float e1 = 0.5f * (Wg(i, j, k - 1) + wc), e2 = 0.5f * (wc + Wg(i, j, k + 1));
-    float az = 0.25f * ((uc + Ug(i, j, k + 1)) * (wc + Wg(i + 1, j, k)) - (Ug(i - 1, j, k) + Ug(i - 1, j, k + 1)) * (Wg(i - 1, j, k) + wc)) * idx_
-               + 0.25f * ((vc + Vg(i, j, k + 1)) * (wc + Wg(i, j + 1, k)) - (Vg(i, j - 1, k) + Vg(i, j - 1, k + 1)) * (Wg(i, j - 1, k) + wc)) * idy
-               + (e2 * e2 - e1 * e1) * idz;
-    a[0] = -ax; a[1] = -ay; a[2] = -az;
-    // viscous (fenep.f90:736-770)
-    const float cb = (float)(g.beta / g.re);
-    const float ix2 = idx_ * idx_, iy2 = idy * idy, iz2 = idz * idz;
-    bv[0] = cb * ((Ug(i - 1, j, k) - 2.0f * uc + Ug(i + 1, j, k)) * ix2 + (Ug(i, j - 1, k) - 2.0f * uc + Ug(i, j + 1, k)) * iy2 +
-                  (Ug(i, j, k - 1) - 2.0f * uc + Ug(i, j, k + 1)) * iz2);
-    bv[1] = cb * ((Vg(i - 1, j, k) - 2.0f * vc + Vg(i + 1, j, k)) * ix2 + (Vg(i, j - 1, k) - 2.0f * vc + Vg(i, j + 1, k)) * iy2 +
-                  (Vg(i, j, k - 1) - 2.0f * vc + Vg(i, j, k + 1)) * iz2);
-    bv[2] = cb * ((Wg(i - 1, j, k) - 2.0f * wc + Wg(i + 1, j, k)) * ix2 + (Wg(i, j - 1, k) - 2.0f * wc + Wg(i, j + 1, k)) * iy2 +
-                  (Wg(i, j, k - 1) - 2.0f * wc + Wg(i, j, k + 1)) * iz2);
-  }
-  if (polymer) {
-#define C0g(q, aa, bb, cc) c0_atf(g, C, fC, q, aa, bb, cc)
-    const float tx = (C0g(0, i + 1, j, k) - C0g(0, i, j, k)) * idx_
-                     + 0.25f * ((C0g(1, i, j + 1, k) + C0g(1, i + 1, j + 1, k)) - (C0g(1, i, j - 1, k) + C0g(1, i + 1, j - 1, k))) * idy
-                     + 0.25f * ((C0g(2, i, j, k + 1) + C0g(2, i + 1, j, k + 1)) - (C0g(2, i, j, k - 1) + C0g(2, i + 1, j, k - 1))) * idz;
-    const float ty = 0.25f * ((C0g(1, i + 1, j, k) - C0g(1, i - 1, j, k)) + (C0g(1, i + 1, j + 1, k) - C0g(1, i - 1, j + 1, k))) * idx_
-                     + (C0g(3, i, j + 1, k) - C0g(3, i, j, k)) * idy
-                     + 0.25f * ((C0g(4, i, j, k + 1) + C0g(4, i, j + 1, k + 1)) - (C0g(4, i, j, k - 1) + C0g(4, i, j + 1, k - 1))) * idz;
-    const float tz = 0.25f * ((C0g(2, i + 1, j, k) - C0g(2, i - 1, j, k)) + (C0g(2, i + 1, j, k + 1) - C0g(2, i - 1, j, k + 1))) * idx_
-                     + 0.25f * ((C0g(4, i, j + 1, k) - C0g(4, i, j - 1, k)) + (C0g(4, i, j + 1, k + 1) - C0g(4, i, j - 1, k + 1))) * idy
-                     + (C0g(5, i, j, k + 1) - C0g(5, i, j, k)) * idz;
-#undef C0g
-    const float coef = (float)((1.0 - g.beta) / (g.re * g.wi));
-    t[0] = coef * tx; t[1] = coef * ty; t[2] = coef * tz;
-  } else {
-    t[0] = t[1] = t[2] = 0.0f;
-  }
-  const float bfs[3] = {bf0, bf1, bf2};
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const float* X = c == 0 ? U : (c == 1 ? V : W);
-    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
-    long o = c * g.n + m;
-    float a0 = first ? a[c] : A0[o];
-    float t0 = first ? t[c] : T0[o];
-    float u = ldb(X, g, i, j, k, kind);
-    float p1 = c == 0 ? ldb(P, g, i + 1, j, k, BC_P) : (c == 1 ? ldb(P, g, i, j + 1, k, BC_P) : ldb(P, g, i, j, k + 1, BC_P));
-    float ih = c == 0 ? idx_ : (c == 1 ? idy : idz);
-    float fv = (F ? F[o] : 0.0f) + bfs[c];
-    float bcoef = method == FP_METHOD_FFT ? 0.5f : 1.0f;   // CN half (fft_navier) or full (ibm_preliminary)
-    Q[o] = u - dt * (p1 - P[m]) * ih + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] + 0.5f * dt * (t[c] + t0) + dt * fv;
-    A0[o] = a[c];
-    T0[o] = t[c];
-    if (Bout) Bout[o] = bv[c];
-  }
-#undef Ug
-#undef Vg
-#undef Wg
+  if (!g.walls || (j > 0 && j < g.ny - 1))
+    predictor_body<true>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Bout, F, bf0, bf1, bf2, Q);
+  else
+    predictor_body<false>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Bout, F, bf0, bf1, bf2, Q);
 }
 
 // ================================================================== FFT ===
