@@ -115,7 +115,8 @@ def main():
             "achieved": round(step_bytes / wall / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
         },
-        "diag": {k: d[k] for k in ("ke", "cfl", "trc_min", "trc_max", "drag", "newton_itr_mean", "nan_flag")},
+        "diag": {k: d[k] for k in ("ke", "cfl", "trc_min", "trc_max", "drag", "div_max", "newton_itr_mean", "n_dc",
+                                    "dc_residual", "sel_guarded", "nan_flag")},
     }
     if not a.no_cpu:
         cv, cores, sec = cpu_baseline(configs.throughput(precision=0), a.cpu_steps)

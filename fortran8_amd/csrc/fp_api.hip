@@ -193,6 +193,8 @@ int fp_create(const fp_params* p, void** out) {
     c->p = *p;
     Grid& g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nxh = p->nx / 2 + 1;
+    g.lnx = 0; while ((1 << g.lnx) < g.nx) g.lnx++;
+    g.lny = 0; while ((1 << g.lny) < g.ny) g.lny++;
     g.walls = p->walls_y ? 1 : 0;
     g.n = (long)p->nx * p->ny * p->nz;
     g.dx = p->lx / p->nx; g.dy = p->ly / p->ny; g.dz = p->lz / p->nz;

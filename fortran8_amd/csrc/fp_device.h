@@ -45,6 +45,14 @@ __device__ __forceinline__ T ldb(const T* __restrict__ A, const Grid& g, int i, 
   }
 }
 
+// (i, j, k) of interior cell m (all dimensions are powers of two)
+__device__ __forceinline__ void cell_ijk(const Grid& g, long m, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)m;
+  i = (int)(u & (unsigned)(g.nx - 1));
+  j = (int)((u >> g.lnx) & (unsigned)(g.ny - 1));
+  k = (int)(u >> (g.lnx + g.lny));
+}
+
 __device__ __forceinline__ long cidx(const Grid& g, int i, int j, int k) {
   return i + (long)g.nx * (j + (long)g.ny * k);
 }
@@ -60,10 +68,12 @@ __device__ __forceinline__ double vieta_d(const double* a) {
   double q = (a0 - (a1 * a2) / 3.0) + (2.0 * ((a2 * a2) * a2)) / 27.0;
   double q2 = q / 2.0, p3 = p / 3.0;
   double t = q2 * q2 + (p3 * p3) * p3;
-  if (t >= 0.0) return -100.0;
+  // evaluated without a branch (keeps the six tests of an axis in one
+  // scheduling region); the t >= 0 sentinel is selected afterwards
   double arg = ((3.0 * q) / (2.0 * p)) * sqrt(-(3.0 / p));
   double ang = acos(arg) / 3.0 + (2.0 * 3.14159265358979323846) / 3.0;
-  return (2.0 * sqrt(-(p / 3.0))) * cos(ang) - a2 / 3.0;
+  double x = (2.0 * sqrt(-(p / 3.0))) * cos(ang) - a2 / 3.0;
+  return t >= 0.0 ? -100.0 : x;
 }
 
 __device__ __forceinline__ double minval2_d(double a, double b) {

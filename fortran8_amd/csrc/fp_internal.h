@@ -16,6 +16,7 @@ enum BcKind { BC_U = 0, BC_V = 1, BC_W = 2, BC_P = 3, BC_C = 4 };
 // Grid description passed by value to every kernel.
 struct Grid {
   int nx, ny, nz, nxh;
+  int lnx, lny;     // log2(nx), log2(ny): cell index -> (i, j, k) by shifts
   int walls;
   long n;           // nx*ny*nz
   double dx, dy, dz;

@@ -16,7 +16,8 @@ static inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
 __global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict__ C, uint8_t* __restrict__ sel) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   uint8_t s = 0;
 #pragma unroll 1
   for (int d = 0; d < 3; d++) {
@@ -61,7 +62,8 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
                                                       double dc, double eps, int itrmax) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const double dt = g.dt;
   double cx[6];
 #pragma unroll 1
@@ -151,7 +153,8 @@ __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first
                                                    double bf0, double bf1, double bf2, double* __restrict__ Q) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const double dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
 #define Ug(a, b, c) ldb(U, g, a, b, c, BC_U)
 #define Vg(a, b, c) ldb(V, g, a, b, c, BC_V)
@@ -480,7 +483,8 @@ void f64_solve(Ctx& c, int kind, const double* q, double* x) {
 __global__ __launch_bounds__(256) void k_divergence(Grid g, const double* __restrict__ up, double* __restrict__ q) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
   q[m] = (((-ldb(Uu, g, i - 1, j, k, BC_U) + ldb(Uu, g, i, j, k, BC_U)) / g.dx
            + (-ldb(Vv, g, i, j - 1, k, BC_V) + ldb(Vv, g, i, j, k, BC_V)) / g.dy)
@@ -497,7 +501,8 @@ __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const dou
                                                double* __restrict__ P) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const double dt = g.dt, dx = g.dx, dy = g.dy, dz = g.dz;
   const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
 #define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)
@@ -619,7 +624,8 @@ __global__ __launch_bounds__(256) void k_ibm_rhs(Grid g, const double* __restric
   if (m >= 3 * g.n) return;
   int c = (int)(m / g.n);
   long mm = m % g.n;
-  int i = (int)(mm % g.nx), j = (int)((mm / g.nx) % g.ny), k = (int)(mm / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, mm, i, j, k);
   const long gx = g.nx + 2, gy = g.ny + 2, gn = gx * gy * (g.nz + 2);
   double f = ft[c * gn + (i + 1) + gx * ((j + 1) + gy * (long)(k + 1))];
   int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
@@ -819,7 +825,8 @@ __global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__
   double s[5] = {0, 0, 0, 0, 0};  // ke, trsum, trsq, nitr, nan
   double mx[6] = {-1e300, -1e300, -1e300, -1e300, 1e300, 0.0};  // umax vmax wmax trmax -trmin divmax
   if (m < g.n) {
-    int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+    int i, j, k;
+  cell_ijk(g, m, i, j, k);
     double u = ldb(U, g, i, j, k, BC_U), v = ldb(V, g, i, j, k, BC_V), w = ldb(W, g, i, j, k, BC_W);
     double ur = (u + ldb(U, g, i - 1, j, k, BC_U)) / 2.0;
     double vr = (v + ldb(V, g, i, j - 1, k, BC_V)) / 2.0;

@@ -16,6 +16,7 @@
 #include "fp_device.h"
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 namespace fp {
@@ -51,6 +52,13 @@ struct Mixed {
   double* mf64 = nullptr;
   float* ftmp = nullptr;          // 3 ghosted raw spread arrays
   float* sw[3] = {nullptr, nullptr, nullptr};  // CSR entry weights
+  // spreading folded onto interior targets (all three components)
+  int* fr_row = nullptr;
+  long* fr_tgt = nullptr;         // c*n + interior index
+  int* fr_ent = nullptr;          // c*nmk + marker
+  float* fr_w = nullptr;
+  int fr_rows = 0;
+  float* fint = nullptr;          // 3 planes, interior
   double last_ndc = 0.0;
 };
 
@@ -109,12 +117,13 @@ __device__ __forceinline__ float lmin32(const float* a) {
   const float det = b11 * (b22 * b33 - a[4] * a[4]) - a[1] * (a[1] * b33 - a[4] * a[2]) + a[2] * (a[1] * a[4] - b22 * a[2]);
   const float p3 = p * (1.0f / 3.0f);
   const float t = 0.25f * det * det + p3 * p3 * p3;
-  if (!(t < 0.0f)) return -100.0f;
-  const float r = __builtin_amdgcn_sqrtf(-p3);
+  // branch-free: the sentinel is selected after the trig path
+  const float r = __builtin_amdgcn_sqrtf(fmaxf(-p3, 1.0e-30f));
   float arg = 0.5f * det * __builtin_amdgcn_rcpf(r * r * r);   // -q/(2 r^3) with q = -det
   arg = fminf(1.0f, fmaxf(-1.0f, arg));
   const float ang = acos_as(arg) * (1.0f / 3.0f) + 2.09439510239319549f;
-  return m + 2.0f * r * __cosf(ang);
+  const float lam = m + 2.0f * r * __cosf(ang);
+  return t < 0.0f ? lam : -100.0f;
 }
 
 __device__ __forceinline__ float cand_pf(int l, float cm, float c, float cp) {
@@ -168,7 +177,8 @@ __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict_
                                                 unsigned* __restrict__ amb, int* __restrict__ amb_list, long amb_cap) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   uint8_t s = 0;
 #pragma unroll 1
   for (int d = 0; d < 3; d++) {
@@ -192,9 +202,19 @@ __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict_
 #pragma unroll
     for (int l = 0; l < 3; l++) if (l != idx) second = fmaxf(second, vals[l]);
     const bool sentinel = vals[0] == -100.0f || vals[1] == -100.0f || vals[2] == -100.0f;
-    if (best - second < SEL_GUARD * fmaxf(1.0f, fabsf(best)) || sentinel) {
-      unsigned slot = atomicAdd(amb, 1u);
-      if (slot < amb_cap) amb_list[slot] = (int)(m * 3 + d);
+    const bool guarded = best - second < SEL_GUARD * fmaxf(1.0f, fabsf(best)) || sentinel;
+    // one atomic per wavefront: the first guarded lane reserves slots for all
+    const unsigned long long ball = __ballot(guarded);
+    if (ball) {
+      const int lane = __lane_id();
+      const int leader = __ffsll((long long)ball) - 1;
+      unsigned base = 0;
+      if (lane == leader) base = atomicAdd(amb, (unsigned)__popcll(ball));
+      base = __shfl(base, leader);
+      if (guarded) {
+        const unsigned slot = base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
+        if (slot < amb_cap) amb_list[slot] = (int)(m * 3 + d);
+      }
     }
     s |= (uint8_t)(idx << (2 * d));
   }
@@ -211,7 +231,8 @@ __global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restr
   int e = amb_list[t];
   long m = e / 3;
   int d = e % 3;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   int di = d == 0, dj = d == 1, dk = d == 2;
   double cm[6], c[6], cp[6];
   for (int q = 0; q < 6; q++) {
@@ -273,7 +294,8 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
                                                       int itrmax) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const float dt = (float)g.dt;
   const bool fast = !g.walls || (j > 0 && j < g.ny - 1);
   const Nbr o = nbr(g, i, j, k);
@@ -519,7 +541,8 @@ __global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first
                                                    float bf0, float bf1, float bf2, float* __restrict__ Q) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   if (!g.walls || (j > 0 && j < g.ny - 1))
     predictor_body<true>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Bout, F, bf0, bf1, bf2, Q);
   else
@@ -746,7 +769,8 @@ __global__ __launch_bounds__(256) void m_divergence(Grid g, const float* __restr
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double v = 0.0;
   if (m < g.n) {
-    int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+    int i, j, k;
+  cell_ijk(g, m, i, j, k);
     const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
     v = (((double)ldb(Uu, g, i, j, k, BC_U) - (double)ldb(Uu, g, i - 1, j, k, BC_U)) / g.dx +
          ((double)ldb(Vv, g, i, j, k, BC_V) - (double)ldb(Vv, g, i, j - 1, k, BC_V)) / g.dy +
@@ -766,7 +790,8 @@ __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restri
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double r = 0.0;
   if (m < g.n) {
-    int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+    int i, j, k;
+  cell_ijk(g, m, i, j, k);
     double p = phi[m];
 #define PH(a, bb, c) ldb(phi, g, a, bb, c, BC_P)
     double lap = (PH(i - 1, j, k) - 2.0 * p + PH(i + 1, j, k)) / (g.dx * g.dx) +
@@ -802,7 +827,8 @@ __global__ __launch_bounds__(256) void m_helm_sweep(Grid g, float eps, const flo
                                                     const float* __restrict__ xin, float* __restrict__ xout) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const float ix2 = (float)(1.0 / (g.dx * g.dx)), iy2 = (float)(1.0 / (g.dy * g.dy)), iz2 = (float)(1.0 / (g.dz * g.dz));
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -845,7 +871,8 @@ __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__
                                                float* __restrict__ P) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
-  int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((long)g.nx * g.ny));
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
   const double dt = g.dt;
   const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
 #define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)
@@ -936,50 +963,30 @@ __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const floa
   }
 }
 
-__global__ __launch_bounds__(256) void m_ibm_spread(int rows, const int* __restrict__ rowp, const long* __restrict__ tgt,
-                                                    const int* __restrict__ ent, const float* __restrict__ sw,
-                                                    const float* __restrict__ mf, float* __restrict__ ft) {
+// spreading gathered onto interior targets with the ghost images folded in
+// (ibm_PBM, fenep.f90:2195-2205), FP32 accumulation
+__global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* __restrict__ rowp, const long* __restrict__ tgt,
+                                                           const int* __restrict__ ent, const float* __restrict__ w,
+                                                           const float* __restrict__ mf, float* __restrict__ fint) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   float acc = 0.0f;
-  for (int e = rowp[r]; e < rowp[r + 1]; e++) acc += mf[ent[e]] * sw[e];
-  ft[tgt[r]] = acc;
+  for (int e = rowp[r]; e < rowp[r + 1]; e++) acc += mf[ent[e]] * w[e];
+  fint[tgt[r]] = acc;
 }
 
-__global__ __launch_bounds__(256) void m_ibm_fold(Grid g, int axis, float* __restrict__ ft) {
-  const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  auto G = [&](long i, long j, long k) { return i + gx * (j + gy * k); };
-  if (axis == 0) {
-    if (t >= gy * gz) return;
-    long j = t % gy, k = t / gy;
-    ft[G(1, j, k)] += ft[G(g.nx + 1, j, k)];
-    ft[G(g.nx, j, k)] += ft[G(0, j, k)];
-  } else if (axis == 1) {
-    if (t >= gx * gz) return;
-    long i = t % gx, k = t / gx;
-    ft[G(i, 1, k)] += ft[G(i, g.ny + 1, k)];
-    ft[G(i, g.ny, k)] += ft[G(i, 0, k)];
-  } else {
-    if (t >= gx * gy) return;
-    long i = t % gx, j = t / gx;
-    ft[G(i, j, 1)] += ft[G(i, j, g.nz + 1)];
-    ft[G(i, j, g.nz)] += ft[G(i, j, 0)];
-  }
-}
-
-__global__ __launch_bounds__(256) void m_ibm_rhs(Grid g, const float* __restrict__ ua, const float* __restrict__ ft,
+// ibm_predict right-hand side (fenep.f90:2229): Ua + dt f - dt B/2
+__global__ __launch_bounds__(256) void m_ibm_rhs(Grid g, const float* __restrict__ ua, const float* __restrict__ fint,
                                                  const float* __restrict__ B, float* __restrict__ q) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= 3 * g.n) return;
   int c = (int)(m / g.n);
   long mm = m % g.n;
-  int i = (int)(mm % g.nx), j = (int)((mm / g.nx) % g.ny), k = (int)(mm / ((long)g.nx * g.ny));
-  const long gx = g.nx + 2, gy = g.ny + 2, gn = gx * gy * (g.nz + 2);
-  float f = ft[c * gn + (i + 1) + gx * ((j + 1) + gy * (long)(k + 1))];
+  int i, j, k;
+  cell_ijk(g, mm, i, j, k);
   int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
   const float dt = (float)g.dt;
-  q[m] = ldb(ua + c * g.n, g, i, j, k, kind) + dt * f - 0.5f * dt * B[m];
+  q[m] = ldb(ua + c * g.n, g, i, j, k, kind) + dt * fint[m] - 0.5f * dt * B[m];
 }
 
 // ========================================================== conversions ===
@@ -1058,6 +1065,7 @@ void mixed_destroy(Ctx& c) {
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
+  dfree(X->fr_row); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w); dfree(X->fint);
   delete X;
   c.mixed = nullptr;
 }
@@ -1108,7 +1116,7 @@ void mixed_set_markers(Ctx& c) {
   const Grid& g = c.g;
   dfree(X.wi); dfree(X.org); dfree(X.uc); dfree(X.mf); dfree(X.mf64); dfree(X.ftmp);
   dalloc(&X.wi, (size_t)c.nmk * 81); dalloc(&X.org, (size_t)c.nmk * 3); dalloc(&X.uc, 3 * c.nmk);
-  dalloc(&X.mf, 3 * c.nmk); dalloc(&X.mf64, 3 * c.nmk); dalloc(&X.ftmp, 3 * c.gn);
+  dalloc(&X.mf, 3 * c.nmk); dalloc(&X.mf64, 3 * c.nmk);
   hipLaunchKernelGGL(m_ibm_setup_interp, dim3(nblk(c.nmk, 128)), dim3(128), 0, c.stream, g, c.nmk, c.nc, c.nl, c.mk, X.wi,
                      X.org, X.uc);
   for (int k = 0; k < 3; k++) {
@@ -1121,6 +1129,51 @@ void mixed_set_markers(Ctx& c) {
                          c.sp_row[k], c.sp_tgt[k], c.sp_ent[k], c.nmk, c.mk, X.sw[k]);
   }
   FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+  // fold the ghosted targets onto their periodic interior images
+  const long gx = g.nx + 2, gy = g.ny + 2;
+  std::map<long, std::vector<std::pair<int, float>>> rows;
+  for (int k = 0; k < 3; k++) {
+    const int nr = c.sp_rows[k];
+    if (!nr) continue;
+    std::vector<int> rowp(nr + 1);
+    std::vector<long> tgt(nr);
+    FP_HIP_CHECK(hipMemcpy(rowp.data(), c.sp_row[k], sizeof(int) * rowp.size(), hipMemcpyDeviceToHost));
+    FP_HIP_CHECK(hipMemcpy(tgt.data(), c.sp_tgt[k], sizeof(long) * nr, hipMemcpyDeviceToHost));
+    std::vector<int> ent(std::max(1, rowp.back()));
+    std::vector<float> w(std::max(1, rowp.back()));
+    if (rowp.back()) {
+      FP_HIP_CHECK(hipMemcpy(ent.data(), c.sp_ent[k], sizeof(int) * rowp.back(), hipMemcpyDeviceToHost));
+      FP_HIP_CHECK(hipMemcpy(w.data(), X.sw[k], sizeof(float) * rowp.back(), hipMemcpyDeviceToHost));
+    }
+    for (int r = 0; r < nr; r++) {
+      if (rowp[r] == rowp[r + 1]) continue;
+      long t = tgt[r];
+      int ii = (int)(t % gx) - 1, jj = (int)((t / gx) % gy) - 1, kk = (int)(t / (gx * gy)) - 1;
+      ii = (ii + g.nx) % g.nx; jj = (jj + g.ny) % g.ny; kk = (kk + g.nz) % g.nz;
+      long lin = (long)k * g.n + ii + (long)g.nx * (jj + (long)g.ny * kk);
+      auto& dst = rows[lin];
+      for (int e = rowp[r]; e < rowp[r + 1]; e++) dst.push_back({(int)(k * c.nmk + ent[e]), w[e]});
+    }
+  }
+  std::vector<int> frow(1, 0), fent;
+  std::vector<long> ftgt;
+  std::vector<float> fw;
+  for (auto& kv : rows) {
+    ftgt.push_back(kv.first);
+    for (auto& pr : kv.second) { fent.push_back(pr.first); fw.push_back(pr.second); }
+    frow.push_back((int)fent.size());
+  }
+  dfree(X.fr_row); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w); dfree(X.fint);
+  X.fr_rows = (int)ftgt.size();
+  dalloc(&X.fr_row, frow.size()); dalloc(&X.fr_tgt, std::max<size_t>(1, ftgt.size()));
+  dalloc(&X.fr_ent, std::max<size_t>(1, fent.size())); dalloc(&X.fr_w, std::max<size_t>(1, fw.size()));
+  dalloc(&X.fint, 3 * g.n);
+  FP_HIP_CHECK(hipMemcpy(X.fr_row, frow.data(), sizeof(int) * frow.size(), hipMemcpyHostToDevice));
+  if (!ftgt.empty()) {
+    FP_HIP_CHECK(hipMemcpy(X.fr_tgt, ftgt.data(), sizeof(long) * ftgt.size(), hipMemcpyHostToDevice));
+    FP_HIP_CHECK(hipMemcpy(X.fr_ent, fent.data(), sizeof(int) * fent.size(), hipMemcpyHostToDevice));
+    FP_HIP_CHECK(hipMemcpy(X.fr_w, fw.data(), sizeof(float) * fw.size(), hipMemcpyHostToDevice));
+  }
 }
 
 const double* mixed_marker_forces(Ctx& c) {
@@ -1204,17 +1257,10 @@ void mixed_step(Ctx& c) {
     {
       MTimer t(c, "ibm");
       hipLaunchKernelGGL(m_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
-      const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
-      for (int k = 0; k < 3; k++) {
-        float* ft = X.ftmp + k * c.gn;
-        if (c.sp_rows[k])
-          hipLaunchKernelGGL(m_ibm_spread, dim3(nblk(c.sp_rows[k], 256)), dim3(256), 0, s, c.sp_rows[k], c.sp_row[k],
-                             c.sp_tgt[k], c.sp_ent[k], X.sw[k], X.mf + k * c.nmk, ft);
-        hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gy * gz, 256)), dim3(256), 0, s, g, 0, ft);
-        hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
-        hipLaunchKernelGGL(m_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
-      }
-      hipLaunchKernelGGL(m_ibm_rhs, dim3(nblk(3 * n, 256)), dim3(256), 0, s, g, X.Q, X.ftmp, X.B, X.R);
+      if (X.fr_rows)
+        hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
+                           X.fr_tgt, X.fr_ent, X.fr_w, X.mf, X.fint);
+      hipLaunchKernelGGL(m_ibm_rhs, dim3(nblk(3 * n, 256)), dim3(256), 0, s, g, X.Q, X.fint, X.B, X.R);
     }
     MTimer t(c, "helmholtz");
     helmholtz32(c, X.R, X.Up);

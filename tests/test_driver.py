@@ -1,0 +1,90 @@
+"""The Fortran host driver (driver/fenep_gpu, ROCm flang over iso_c_binding):
+argument rejection on the CPU; on the GPU a Restart.d round trip against the
+oracle and the output formats the reference's Python scripts read."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fortran8_amd import configs
+from fortran8_amd.fields import interior, interior_c, rel_l2
+from oracle import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "driver", "fenep_gpu")
+
+
+def run_driver(tmp_path, nml):
+    path = os.path.join(str(tmp_path), "run.nml")
+    with open(path, "w") as f:
+        f.write("&run\n" + nml + "\n/\n")
+    return subprocess.run([DRIVER, path], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+
+
+def test_driver_is_built_and_rejects_invalid_parameters(tmp_path):
+    assert os.path.exists(DRIVER)
+    r = run_driver(tmp_path, "NX=12, NY=8, NZ=8, Nstep=1")
+    assert r.returncode != 0
+    assert "fp_create failed, code 1" in r.stdout
+
+
+def write_restart(path, st):
+    n = st.U.shape
+    U, V, W, P = (interior(a) for a in (st.U, st.V, st.W, st.P))
+    Cc = interior_c(st.C)
+    h = st.hist
+    cols = [U, V, W, P] + [Cc[..., q] for q in range(6)] + [h[q] for q in range(6)]
+    data = np.stack([c.ravel() for c in cols], axis=1)
+    np.savetxt(path, data, fmt="%.17e")
+
+
+def read_restart(path, nz, ny, nx):
+    d = np.loadtxt(path).reshape(nz, ny, nx, 16)
+    return d
+
+
+@pytest.mark.gpu
+def test_driver_restart_round_trip_matches_oracle(tmp_path):
+    case = configs.kolmogorov(n=16)
+    case.force = None
+    o = po.load_case(case)
+    o.step(2)
+    st = o.get_state()
+    os.makedirs(os.path.join(str(tmp_path), "run"))
+    write_restart(os.path.join(str(tmp_path), "run", "Restart.d"), st)
+    p = case.params
+    nml = ("NX=16, NY=16, NZ=16, Lx=%r, Ly=%r, Lz=%r, dt=%r, Re=%r, beta=%r, Wi=%r, Lp=%r, method=1, "
+           "Nstep=5, Lstep=5, Ostep=5, Gstep=5, koide_step=5, input_type=1, flow_type=0, dir='./run/'"
+           % (p.lx, p.ly, p.lz, p.dt, p.re, p.beta, p.wi, p.lp))
+    r = run_driver(tmp_path, nml)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "step:       5" in r.stdout and "K_energy" in r.stdout
+    o.step(5)
+    so = o.get_state()
+    d = read_restart(os.path.join(str(tmp_path), "run", "Restart.d"), 16, 16, 16)
+    assert np.array_equal(d[..., 0], interior(so.U))
+    assert np.array_equal(d[..., 3], interior(so.P))
+    assert np.array_equal(d[..., 4:10], interior_c(so.C))
+    for q in range(6):
+        assert np.array_equal(d[..., 10 + q], so.hist[q])
+    # output formats read by the reference's post-processing scripts
+    m = np.fromfile(os.path.join(str(tmp_path), "run", "Map", "z_00000005.bin"), dtype=np.float64).reshape(-1, 18)
+    assert m.shape == (16 * 16, 18)
+    assert np.allclose(m[:, 11], m[:, 12] + m[:, 15] + m[:, 17])          # trC column
+    k3 = np.loadtxt(os.path.join(str(tmp_path), "run", "koide3.d"))
+    assert k3.reshape(-1, 2)[0, 0] == 5
+    cc = interior_c(so.C)[8 - 1, 8 - 1, 8 - 1]
+    assert k3.reshape(-1, 2)[0, 1] == pytest.approx(cc[0] + cc[3] + cc[5], rel=1e-14)
+
+
+@pytest.mark.gpu
+def test_driver_ibm_channel_runs_and_logs_drag(tmp_path):
+    nml = ("NX=64, NY=32, NZ=16, Lx=4.0, Ly=2.0, Lz=1.0, dt=0.002, Re=200, beta=0.8, Wi=1, Lp=10, method=2, "
+           "walls_y=1, precision=1, body_force=0.015,0,0, ncyl=1, DC=0.5, xcyl(1)=1.0, ycyl(1)=1.0, "
+           "Nstep=20, Lstep=10, koide_step=0, dir='./chan/'")
+    r = run_driver(tmp_path, nml)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert r.stdout.count("drag:") == 2 and "Mcell-steps/s" in r.stdout
