@@ -5,7 +5,12 @@ Poisson (the throughput configuration).  A "step" is one full time step of
 the state resident in HBM.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--precision mixed|f64]
-                  [--no-cpu] [--small]
+                  [--no-cpu] [--cpu-steps S] [--small]
+
+The per-kernel durations come from HIP events recorded on the context's own
+stream around each phase, inside the timed region.  HBM traffic per launch
+of the dominant kernel is the rocprofv3 FETCH_SIZE/WRITE_SIZE reading kept in
+profiles/traffic.txt (already scaled by the copy-kernel calibration).
 """
 import argparse
 import json
@@ -13,18 +18,26 @@ import os
 import sys
 import time
 
-import numpy as np
-
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 from fortran8_amd import configs, api  # noqa: E402
 
 # Algorithmic HBM words per cell-step of the canonical pass list (FP64 plain:
-# 80 words = 640 B).  Pass P1 (FENE-P: C6 + UVW read, C6 write) is 15 words.
+# 80 words = 640 B).
 STEP_WORDS = 80
-FENEP_WORDS = 15
 HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per cell of the single-kernel phases: (words read +
+# written at the storage width, extra bytes).  select: C 6 read, the
+# selection byte written.  cstar_newton: C 6 + U,V,W read, selection byte
+# read, C 6 + Peterlin f written, Newton count (int32) written.  predictor
+# (IBM form): U,V,W,P, C 6, f, A0 3, T0 3 read; Q 3, A0 3, T0 3, B 3 written.
+KERNEL_WORDS = {
+    "fenep_select": (6, 1),
+    "fenep_cstar_newton": (16, 5),
+    "predictor": (29, 0),
+}
 
 
 def mixed_step_bytes(n_dc):
@@ -34,18 +47,31 @@ def mixed_step_bytes(n_dc):
 def run_gpu(case, steps, warmup):
     g = api.load_case(case)
     g.step(warmup, sync=True)
+    # timed region: K steps with the phase events on the context's stream
+    g.enable_timing(True)
     t0 = time.perf_counter()
     g.step(steps, sync=True)
     t1 = time.perf_counter()
-    wall = (t1 - t0) / steps
-    # phase timing with HIP events on the context's stream (separate pass)
-    g.enable_timing(True)
-    g.step(steps, sync=True)
     kt = {k: v / steps for k, v in g.kernel_times().items()}
     g.enable_timing(False)
+    wall = (t1 - t0) / steps
     d = g.diagnostics()
     g.close()
     return wall, kt, d
+
+
+def read_traffic(precision):
+    """Bytes per launch of each kernel from profiles/traffic.txt (lines
+    `traffic <phase> <precision> <bytes>`), or {} if absent."""
+    out = {}
+    path = os.path.join(HERE, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic" and f[2] == precision:
+            out[f[1]] = float(f[3])
+    return out
 
 
 def cpu_baseline(case, steps=1):
@@ -65,9 +91,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["mixed", "f64"], default="f64")
+    ap.add_argument("--precision", choices=["mixed", "f64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="64x32x32 smoke size")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -82,9 +108,18 @@ def main():
     wall, kt, d = run_gpu(case, a.steps, a.warmup)
     cells = case.cells
     value = cells / wall / 1e6
-    fen_ms = kt.get("fenep_cstar_newton", 0.0) + kt.get("fenep_select", 0.0)
-    fen_bytes = FENEP_WORDS * (8 if prec == 0 else 4) * cells
+    width = 8 if prec == 0 else 4
+    # dominant single-kernel phase and its algorithmic bytes per launch
+    dom = max(KERNEL_WORDS, key=lambda k: kt.get(k, 0.0))
+    words, extra = KERNEL_WORDS[dom]
+    dom_bytes = (words * width + extra) * cells
+    dom_ms = kt.get(dom, 0.0)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
+    traffic = read_traffic(a.precision).get(dom)
     step_bytes = STEP_WORDS * 8 * cells if prec == 0 else mixed_step_bytes(d.get("n_dc", 0)) * cells
+    kernel_names = {"fenep_select": "m_select" if prec else "k_select",
+                    "fenep_cstar_newton": "m_cstar_newton" if prec else "k_cstar_newton",
+                    "predictor": "m_predictor" if prec else "k_predictor"}
     out = {
         "metric": "Mcell-steps/s 256x128x128 FENE-P+IBM",
         "value": round(value, 3),
@@ -104,11 +139,13 @@ def main():
         "phases_ms": {k: round(v, 4) for k, v in kt.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": "fenep (select + cstar/newton)",
-            "achieved": round(fen_bytes / (fen_ms * 1e-3) / 1e9, 2) if fen_ms > 0 else None,
+            "kernel": kernel_names[dom],
+            "algorithmic_bytes_per_cell": words * width + extra,
+            "launch_ms": round(dom_ms, 4),
+            "achieved": round(achieved, 2) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(fen_bytes / (fen_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if fen_ms > 0 else None,
-            "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": traffic,
         },
         "step_roofline": {
             "algorithmic_bytes_per_cell_step": step_bytes / cells,
@@ -122,7 +159,7 @@ def main():
         cv, cores, sec = cpu_baseline(configs.throughput(precision=0), a.cpu_steps)
         out["cpu_baseline"] = {"value": round(cv, 4), "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
                                "sample": "%d step(s) of the full 256x128x128 case on the FP64 CPU restatement "
-                                         "(OpenMP), %.1f s" % (a.cpu_steps, sec)}
+                                         "(OpenMP, %d threads), %.1f s" % (a.cpu_steps, cores, sec)}
     print(json.dumps(out), flush=True)
 
 

@@ -204,10 +204,19 @@ program fenep_gpu
     chunk = min(Nstep - step, next_event(step) - step)
     call check(fp_step(ctx, chunk, 1), ctx, 'fp_step')
     step = step + chunk
-    if (Lstep > 0 .and. mod(step, Lstep) == 0) call log_progress(step)
-    if (koide_step > 0 .and. mod(step, koide_step) == 0) call write_koide(step)
-    if (mod(step, Gstep) == 0) call get_data_binary(step)
-    if (mod(step, Ostep) == 0) call write_restart()
+    ! Fortran does not short-circuit .and.: test the period before mod()
+    if (Lstep > 0) then
+      if (mod(step, Lstep) == 0) call log_progress(step)
+    endif
+    if (koide_step > 0) then
+      if (mod(step, koide_step) == 0) call write_koide(step)
+    endif
+    if (Gstep > 0) then
+      if (mod(step, Gstep) == 0) call get_data_binary(step)
+    endif
+    if (Ostep > 0) then
+      if (mod(step, Ostep) == 0) call write_restart()
+    endif
   enddo
   call system_clock(t1)
   rc = fp_diagnostics(ctx, dg)

@@ -58,6 +58,9 @@ def lib():
         L.fp_enable_timing.argtypes = [_vp, C.c_int]
         L.fp_kernel_times.argtypes = [_vp, _dp, C.c_int, C.c_char_p, C.c_int]
         L.fp_seeded_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, _dp]
+        L.fp_set_physics.argtypes = [_vp, C.c_double, C.c_double, C.c_double]
+        L.fp_adj_record.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
+        L.fp_adj_backward.argtypes = [_vp, _dp, _vp, _vp, _dp, _vp]
         _LIB = L
     return _LIB
 
@@ -158,6 +161,27 @@ class FenepGPU:
         if rc and rc != 4:
             self._check(rc, "fp_diagnostics")
         return d.as_dict()
+
+    # ---- sensitivities (FP64 path, periodic grids)
+    def set_physics(self, wi, beta, lp):
+        self._check(lib().fp_set_physics(self.h, float(wi), float(beta), float(lp)), "fp_set_physics")
+
+    def adj_record(self, nsteps, every, replay=False):
+        """Advance nsteps steps while recording; returns the drag of each step."""
+        drag = np.zeros(nsteps)
+        self._check(lib().fp_adj_record(self.h, int(nsteps), int(every), 1 if replay else 0, _ptr(drag)),
+                    "fp_adj_record")
+        return drag
+
+    def adj_backward(self, weights=None, bar_final=0, bar_initial=0, bar_force=0):
+        """Reverse sweep of the last recording.  weights: host array over the
+        recorded steps; bar_final / bar_initial / bar_force: device pointers
+        (int, 0 = none).  Returns (dJ/dWi, dJ/dbeta, dJ/dL^2)."""
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        out = np.zeros(3)
+        self._check(lib().fp_adj_backward(self.h, _ptr(w), bar_final or None, bar_initial or None, _ptr(out),
+                                          bar_force or None), "fp_adj_backward")
+        return out
 
     def enable_timing(self, on=True):
         lib().fp_enable_timing(self.h, 1 if on else 0)

@@ -21,6 +21,7 @@ UNITS = [
     ("fp_kernels_f64.hip", ["-ffp-contract=off"]),
     ("fp_kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("fp_api.hip", ["-ffp-contract=off"]),
+    ("fp_adjoint.hip", ["-ffp-contract=fast"]),
 ]
 
 
@@ -30,7 +31,7 @@ def hipcc():
             return c
 
 
-def build(verbose=False, jobs=3):
+def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     procs = []
     objs = []

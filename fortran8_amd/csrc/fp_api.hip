@@ -129,12 +129,19 @@ void flush_timers(Ctx& c) {
   c.ev.clear();
 }
 
-void step_f64(Ctx& c) {
+}  // namespace
+
+void fp::step_f64(Ctx& c) {
   const Grid& g = c.g;
   if (c.p.beta == 1.0) {
     f64_zero_c(c);
   } else {
-    { Timer t(c, "fenep_select"); f64_select(c); }
+    if (c.sel_replay) {
+      FP_HIP_CHECK(hipMemcpyAsync(c.sel, c.sel_replay, g.n, hipMemcpyDeviceToDevice, c.stream));
+    } else {
+      Timer t(c, "fenep_select");
+      f64_select(c);
+    }
     { Timer t(c, "fenep_cstar_newton"); f64_cstar_newton(c); }
   }
   { Timer t(c, "predictor"); f64_predictor(c); }
@@ -158,8 +165,6 @@ void step_f64(Ctx& c) {
   c.has_history = 1;
   c.step++;
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -242,6 +247,7 @@ int fp_destroy(void* vc) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->ev) (void)hipEventDestroy(kv.second);
   if (c->mixed) mixed_destroy(*c);
+  adj_free(*c);
   dfree(c->U); dfree(c->V); dfree(c->W); dfree(c->P); dfree(c->Phi);
   dfree(c->C[0]); dfree(c->C[1]); dfree(c->fC); dfree(c->sel); dfree(c->nitr);
   dfree(c->A0); dfree(c->T0); dfree(c->B0); dfree(c->B); dfree(c->Q); dfree(c->Up); dfree(c->F);
@@ -541,6 +547,49 @@ int fp_seeded_uniform(uint64_t seed, uint64_t offset, size_t n, double* out) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z = z ^ (z >> 31);
     out[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  }
+  return FP_OK;
+}
+
+// Errors of the recording/adjoint layer arrive as "E_ARG:..." or
+// "E_STATE:..." (argument / call order) or as HIP errors.
+static int adj_fail(Ctx* c, const std::string& e) {
+  if (e.rfind("E_ARG:", 0) == 0) return fail(c, FP_E_ARG, e.substr(6));
+  if (e.rfind("E_STATE:", 0) == 0) return fail(c, FP_E_STATE, e.substr(8));
+  return fail(c, FP_E_HIP, e);
+}
+
+int fp_set_physics(void* vc, double wi, double beta, double lp) {
+  Ctx* c = (Ctx*)vc;
+  if (!c) return FP_E_ARG;
+  if (c->mixed) return fail(c, FP_E_ARG, "fp_set_physics runs on the FP64 path");
+  if (!(wi > 0.0) || !(beta > 0.0 && beta <= 1.0) || !(lp * lp > 3.0))
+    return fail(c, FP_E_ARG, "need Wi > 0, 0 < beta <= 1 and Lp^2 > 3");
+  c->p.wi = wi; c->p.beta = beta; c->p.lp = lp;
+  c->g.wi = wi; c->g.beta = beta; c->g.lp = lp;
+  return FP_OK;
+}
+
+int fp_adj_record(void* vc, int nsteps, int every, int replay, double* drag_out) {
+  Ctx* c = (Ctx*)vc;
+  if (!c) return FP_E_ARG;
+  if (!c->state_set) return fail(c, FP_E_STATE, "fp_adj_record before fp_set_state");
+  try {
+    adj_record(*c, nsteps, every, replay, drag_out);
+  } catch (const std::string& e) {
+    return adj_fail(c, e);
+  }
+  return FP_OK;
+}
+
+int fp_adj_backward(void* vc, const double* weights, const double* bar_final, double* bar_initial,
+                    double* bar_params, double* bar_force) {
+  Ctx* c = (Ctx*)vc;
+  if (!c) return FP_E_ARG;
+  try {
+    adj_backward(*c, weights, bar_final, bar_initial, bar_params, bar_force);
+  } catch (const std::string& e) {
+    return adj_fail(c, e);
   }
   return FP_OK;
 }

@@ -45,6 +45,28 @@ __device__ __forceinline__ T ldb(const T* __restrict__ A, const Grid& g, int i, 
   }
 }
 
+// Workgroups of a launch are handed to the 8 XCDs of the chip in turn, and
+// each XCD has its own L2.  Stencil kernels renumber their workgroups so that
+// each XCD walks one contiguous range of cells: the neighbouring y rows and z
+// planes a stencil reads are then fetched into one L2 instead of into all
+// eight.  The renumbering is a bijection for any number of workgroups.
+__device__ __forceinline__ long xcd_block() {
+  const long nb = gridDim.x, b = blockIdx.x;
+  const long q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+__device__ __forceinline__ long xcd_cell() { return xcd_block() * (long)blockDim.x + threadIdx.x; }
+
+// Zero n 32-bit words.  A kernel rather than a memset, so that a caller may
+// capture the step in a HIP graph.
+static __global__ __launch_bounds__(256) void k_zero_words(long n, unsigned* __restrict__ p) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0u;
+}
+static inline void zero_words(void* p, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (unsigned*)p);
+}
+
 // (i, j, k) of interior cell m (all dimensions are powers of two)
 __device__ __forceinline__ void cell_ijk(const Grid& g, long m, int& i, int& j, int& k) {
   const unsigned u = (unsigned)m;
@@ -212,6 +234,31 @@ __device__ __forceinline__ int newton_d(const double* A0, const double* b, doubl
 
 __device__ __forceinline__ double peterlin_d(double lp, double c0, double c3, double c5) {
   return (lp * lp - 3.0) / (((lp * lp - c0) - c3) - c5);
+}
+
+// ---- IBM (FP64) -------------------------------------------------------------
+// 3-point kernel of Roma, Peskin & Berger (delta/weight, fenep.f90:2039-2055)
+__device__ __forceinline__ double roma_d(double r) {
+  double a = fabs(r);
+  if (a < 1.5 && a >= 0.5) return ((5.0 - 3.0 * a) - sqrt(-(3.0 * ((1.0 - a) * (1.0 - a))) + 1.0)) / 6.0;
+  if (a < 0.5) return (1.0 + sqrt(-(3.0 * (r * r)) + 1.0)) / 3.0;
+  return 0.0;
+}
+
+// delta (fenep.f90:2039-2043) * dX^3 between marker (xc,yc,zc) and the
+// sample of component c at grid index (ii,jj,kk) (1-based, ghosts included).
+__device__ __forceinline__ double ibm_weight(const Grid& g, int c, int ii, int jj, int kk, double xc, double yc, double zc) {
+  double x = ((double)ii - 0.5) * g.dx, y = ((double)jj - 0.5) * g.dy, z = ((double)kk - 0.5) * g.dz;
+  if (c == 0) x = x + 0.5 * g.dx;
+  else if (c == 1) y = y + 0.5 * g.dy;
+  else z = z + 0.5 * g.dz;
+  double d = ((roma_d((x - xc) / g.dx) * roma_d((y - yc) / g.dy)) * roma_d((z - zc) / g.dz)) / ((g.dx * g.dy) * g.dz);
+  return d;
+}
+__device__ __forceinline__ void ibm_origin_d(const Grid& g, int c, double xc, double yc, int& i0, int& j0) {
+  if (c == 0) { i0 = (int)(xc / g.dx - 0.5); j0 = (int)(yc / g.dy); }
+  else if (c == 1) { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy - 0.5); }
+  else { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy); }
 }
 
 // Deterministic two-level reductions: each block stores its partial sum,

@@ -71,6 +71,11 @@ struct Ctx {
   std::vector<std::pair<std::string, double>> ktime;
   // mixed precision state (allocated only for precision == 1)
   void* mixed = nullptr;
+  // selections to use instead of computing them (frozen-branch replays of
+  // a recorded run); nullptr: computed by the step
+  const uint8_t* sel_replay = nullptr;
+  // recording and adjoint state (fp_adjoint.hip)
+  void* adj = nullptr;
 };
 
 // ---- launchers (fp_kernels_f64.hip)
@@ -86,6 +91,14 @@ int  f64_sor(Ctx& c, const double* q, double* phi);
 void f64_zero_c(Ctx& c);
 void f64_diag(Ctx& c, fp_diag* d);
 void f64_fill_f(Ctx& c);     // Peterlin factor of the current C
+void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp);  // spread + ghost folds
+void step_f64(Ctx& c);       // one FP64 time step (fp_api.hip)
+
+// ---- recording and discrete adjoint (fp_adjoint.hip)
+void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out);
+void adj_backward(Ctx& c, const double* weights, const double* bar_final, double* bar_initial, double* bar_params,
+                  double* bar_force);
+void adj_free(Ctx& c);
 
 }  // namespace fp
 

@@ -14,7 +14,7 @@ static inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
 // CpxCnx/CpyCny/CpzCnz selection (fenep.f90:169-382) for all three axes of
 // one cell; the chosen candidate per axis is packed in 2 bits.
 __global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict__ C, uint8_t* __restrict__ sel) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
                                                       const double* __restrict__ W, double* __restrict__ Cn,
                                                       double* __restrict__ fC, int* __restrict__ nitr,
                                                       double dc, double eps, int itrmax) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first
                                                    double* __restrict__ A0, double* __restrict__ T0, double* __restrict__ B0,
                                                    double* __restrict__ Bout, const double* __restrict__ F,
                                                    double bf0, double bf1, double bf2, double* __restrict__ Q) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -481,7 +481,7 @@ void f64_solve(Ctx& c, int kind, const double* q, double* x) {
 // ======================================================= projection =======
 // fft_poisson right-hand side (fenep.f90:1550-1552)
 __global__ __launch_bounds__(256) void k_divergence(Grid g, const double* __restrict__ up, double* __restrict__ q) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -499,7 +499,7 @@ void f64_divergence(Ctx& c, const double* up, double* q) {
 __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const double* __restrict__ up, const double* __restrict__ phi,
                                                double* __restrict__ U, double* __restrict__ V, double* __restrict__ W,
                                                double* __restrict__ P) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -523,29 +523,6 @@ __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const dou
 }
 
 // ============================================================ IBM ==========
-__device__ __forceinline__ double roma_d(double r) {
-  double a = fabs(r);
-  if (a < 1.5 && a >= 0.5) return ((5.0 - 3.0 * a) - sqrt(-(3.0 * ((1.0 - a) * (1.0 - a))) + 1.0)) / 6.0;
-  if (a < 0.5) return (1.0 + sqrt(-(3.0 * (r * r)) + 1.0)) / 3.0;
-  return 0.0;
-}
-
-// delta (fenep.f90:2039-2043) * dX^3 between marker (xc,yc,zc) and the
-// sample of component c at grid index (ii,jj,kk) (1-based, ghosts included).
-__device__ __forceinline__ double ibm_weight(const Grid& g, int c, int ii, int jj, int kk, double xc, double yc, double zc) {
-  double x = ((double)ii - 0.5) * g.dx, y = ((double)jj - 0.5) * g.dy, z = ((double)kk - 0.5) * g.dz;
-  if (c == 0) x = x + 0.5 * g.dx;
-  else if (c == 1) y = y + 0.5 * g.dy;
-  else z = z + 0.5 * g.dz;
-  double d = ((roma_d((x - xc) / g.dx) * roma_d((y - yc) / g.dy)) * roma_d((z - zc) / g.dz)) / ((g.dx * g.dy) * g.dz);
-  return d;
-}
-__device__ __forceinline__ void ibm_origin_d(const Grid& g, int c, double xc, double yc, int& i0, int& j0) {
-  if (c == 0) { i0 = (int)(xc / g.dx - 0.5); j0 = (int)(yc / g.dy); }
-  else if (c == 1) { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy - 0.5); }
-  else { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy); }
-}
-
 // interpolation and marker forces of ibm_Helmholtz (fenep.f90:2076-2119,
 // forces kept as in fenep_mpi.f90:1845-1853)
 __global__ __launch_bounds__(256) void k_ibm_interp(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
@@ -633,20 +610,29 @@ __global__ __launch_bounds__(256) void k_ibm_rhs(Grid g, const double* __restric
   q[m] = (u + g.dt * f) - (g.dt * B[m]) / 2.0;
 }
 
-void f64_ibm(Ctx& c) {
+// spread of the marker values mfield (3 x nmk) onto the ghosted arrays ft
+// (3 x gn) followed by the ghost folds; rows of the CSR that no marker
+// reaches keep their previous (zero) value.
+void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp) {
   const Grid& g = c.g;
   hipStream_t s = c.stream;
-  hipLaunchKernelGGL(k_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, c.nc, c.nl, c.mk, c.Q, c.mb, c.mf);
   const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
   for (int comp = 0; comp < 3; comp++) {
-    double* ft = c.ftmp + comp * c.gn;
+    double* ft = ftmp + comp * c.gn;
     if (c.sp_rows[comp] > 0)
       hipLaunchKernelGGL(k_ibm_spread, dim3(nblk(c.sp_rows[comp], 256)), dim3(256), 0, s, g, comp, c.sp_rows[comp],
-                         c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.nmk, c.nc, c.nl, c.mk, c.mf, ft);
+                         c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.nmk, c.nc, c.nl, c.mk, mfield, ft);
     hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gy * gz, 256)), dim3(256), 0, s, g, 0, ft);
     hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
     hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
   }
+}
+
+void f64_ibm(Ctx& c) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(k_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, c.nc, c.nl, c.mk, c.Q, c.mb, c.mf);
+  f64_spread_markers(c, c.mf, c.ftmp);
   hipLaunchKernelGGL(k_ibm_rhs, dim3(nblk(3 * g.n, 256)), dim3(256), 0, s, g, c.Q, c.ftmp, c.B, c.Up);
 }
 
@@ -814,7 +800,7 @@ void f64_march(Ctx& c) {
 }
 
 void f64_zero_c(Ctx& c) {
-  FP_HIP_CHECK(hipMemsetAsync(c.C[c.cur], 0, sizeof(double) * 6 * c.g.n, c.stream));
+  zero_words(c.C[c.cur], 12 * c.g.n, c.stream);
 }
 
 // ===================================================== diagnostics ========

@@ -175,7 +175,7 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
 
 __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                 unsigned* __restrict__ amb, int* __restrict__ amb_list, long amb_cap) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
                                                       const float* __restrict__ W, float* __restrict__ Cn,
                                                       float* __restrict__ fC, int* __restrict__ nitr, float eps,
                                                       int itrmax) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first
                                                    float* __restrict__ A0, float* __restrict__ T0,
                                                    float* __restrict__ Bout, const float* __restrict__ F,
                                                    float bf0, float bf1, float bf2, float* __restrict__ Q) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -766,7 +766,7 @@ static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip 
 // b = div_h(u*)/dt in FP64 from the FP32 u*; also |b|^2
 __global__ __launch_bounds__(256) void m_divergence(Grid g, const float* __restrict__ up, double* __restrict__ b,
                                                     float* __restrict__ b32, double* __restrict__ nrm) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   double v = 0.0;
   if (m < g.n) {
     int i, j, k;
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(256) void m_divergence(Grid g, const float* __restr
 // correction solve; |r|^2 accumulated
 __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restrict__ b, const double* __restrict__ phi,
                                                   float* __restrict__ r32, double* __restrict__ nrm) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   double r = 0.0;
   if (m < g.n) {
     int i, j, k;
@@ -825,7 +825,7 @@ __global__ __launch_bounds__(256) void m_accum(long n, const float* __restrict__
 // eps * |lap_h| << 1 the error contracts by eps*lambda_max per sweep.
 __global__ __launch_bounds__(256) void m_helm_sweep(Grid g, float eps, const float* __restrict__ q,
                                                     const float* __restrict__ xin, float* __restrict__ xout) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -869,7 +869,7 @@ static void helmholtz32(Ctx& c, const float* q, float* x) {
 __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__ up, const double* __restrict__ phi,
                                                float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
                                                float* __restrict__ P) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -1223,11 +1223,11 @@ void mixed_step(Ctx& c) {
   const long n = g.n;
   const int polymer = c.p.beta != 1.0;
   if (!polymer) {
-    FP_HIP_CHECK(hipMemsetAsync(X.C[X.cur], 0, sizeof(float) * 6 * n, s));
+    zero_words(X.C[X.cur], 6 * n, s);
   } else {
+    zero_words(X.amb, 1, s);
     {
       MTimer t(c, "fenep_select");
-      FP_HIP_CHECK(hipMemsetAsync(X.amb, 0, sizeof(unsigned), s));
       hipLaunchKernelGGL(m_select, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list, X.amb_cap);
     }
     {

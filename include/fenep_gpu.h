@@ -151,6 +151,34 @@ int fp_enable_timing(void* ctx, int on);
 int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
                     int names_len);
 
+/* ---- Sensitivities (FP64 path, periodic grids, FFT Poisson, methods 1 and
+ * 2).  The reference has no derivative code: these entry points serve the
+ * discrete adjoint of the step it advances (fenep.f90:2477-2527) with respect
+ * to Wi, beta, L^2 = Lp^2 (fenep.f90:23-26 module constants) and the body-force
+ * field F of init (fenep.f90:96-120). */
+
+/* Change Wi, beta and Lp of an FP64 context between runs. */
+int fp_set_physics(void* ctx, double wi, double beta, double lp);
+
+/* Advance nsteps steps from the current state while recording: the state is
+ * checkpointed every `every` steps, the C+- selections of every step and the
+ * drag D_s = -sum_markers Fxc dX^3 of every step (IBM method; 0 otherwise)
+ * are kept; drag_out (host, nsteps, may be NULL) receives D_s.  replay != 0
+ * reuses the selections of the previous recording instead of computing them
+ * (the same discrete branch; for finite-difference checks). */
+int fp_adj_record(void* ctx, int nsteps, int every, int replay, double* drag_out);
+
+/* Reverse sweep over the last recording for the objective
+ *   J = sum_s weights[s] D_s + <bar_final, state after the last step>.
+ * weights: host, nsteps (NULL = 0).  bar_final / bar_initial: device arrays
+ * of 16 planes of NX*NY*NZ doubles (interior, x fastest): U V W P C(6) A0(3)
+ * T0(3); NULL = zero / not wanted.  bar_params (host, 3): dJ/dWi, dJ/dbeta,
+ * dJ/d(L^2).  bar_force (device, 3 planes): dJ/dF.  The selections are those
+ * of the recording (frozen branch).  The context's state is left at the
+ * state after the recording. */
+int fp_adj_backward(void* ctx, const double* weights, const double* bar_final,
+                    double* bar_initial, double* bar_params, double* bar_force);
+
 /* Deterministic seeded uniform numbers in [0,1): SplitMix64 of
  * (seed, offset+i), top 53 bits.  Host-side utility so that the Fortran
  * driver, the Python wrapper and the tests produce identical inputs. */

@@ -1,0 +1,195 @@
+"""Discrete-adjoint checks, run in a fresh interpreter (torch first, so that
+the C-ABI library binds to the HIP runtime torch loaded).
+
+  python tests/_adjoint_check.py vjp     one-step state VJP vs central FD
+  python tests/_adjoint_check.py params  12-step parameter/force gradients vs FD
+  python tests/_adjoint_check.py cfg5    64x64x32, 400 steps, checkpoints every
+                                         20, drag averaged over the last 200
+                                         steps; torch.autograd path vs FD
+
+Finite differences replay the selections of the recorded run (the same
+discrete C+- branch the adjoint differentiates).  Each check prints one JSON
+line with the relative errors.
+"""
+import json
+import math
+import os
+import sys
+import time
+
+import torch
+
+torch.cuda.init()
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import numpy as np  # noqa: E402
+
+from fortran8_amd import api, configs  # noqa: E402
+from fortran8_amd.adjoint import DragRun, drag_objective  # noqa: E402
+from fortran8_amd.fields import State, interior, interior_c, pad_periodic  # noqa: E402
+
+DEV = torch.device("cuda", 0)
+
+
+def flat_state(st):
+    """(16, nz, ny, nx): U V W P C(6) A0(3) T0(3)"""
+    planes = [interior(st.U), interior(st.V), interior(st.W), interior(st.P)]
+    c = interior_c(st.C)
+    planes += [c[..., q] for q in range(6)]
+    planes += [st.hist[q] for q in range(6)]
+    return np.stack(planes)
+
+
+def make_state(x, b0):
+    """ghosted State from (16, nz, ny, nx) and the B0 history (3, nz, ny, nx)"""
+    C = np.stack([pad_periodic(x[4 + q]) for q in range(6)], axis=-1)
+    hist = np.concatenate([x[10:16], b0]).copy()
+    return State(pad_periodic(x[0]), pad_periodic(x[1]), pad_periodic(x[2]), pad_periodic(x[3]), C, hist, 1)
+
+
+def small_case():
+    case = configs.adjoint_case(nx=32, ny=32, nz=16)
+    g = api.load_case(case)
+    g.step(4)   # developed histories and flow
+    st = g.get_state()
+    return case, g, st
+
+
+def check_vjp():
+    case, g, st = small_case()
+    x0 = flat_state(st)
+    b0 = st.hist[6:9].copy()
+    rng = np.random.default_rng(7)
+    scale = np.array([max(np.std(p), 1e-3) for p in x0])[:, None, None, None]
+    w = rng.standard_normal(x0.shape) * scale
+    vbar = rng.standard_normal(x0.shape)
+    wd = 0.7
+    # base recording and adjoint
+    g.set_state(make_state(x0, b0))
+    g.adj_record(1, 1)
+    vb = torch.from_numpy(vbar.reshape(-1)).to(DEV)
+    bi = torch.zeros(vb.numel(), dtype=torch.float64, device=DEV)
+    torch.cuda.synchronize()
+    g.adj_backward(weights=[wd], bar_final=vb.data_ptr(), bar_initial=bi.data_ptr())
+    adj = float(np.dot(bi.cpu().numpy(), w.reshape(-1)))
+
+    def J(x):
+        g.set_state(make_state(x, b0))
+        d = g.adj_record(1, 1, replay=True)
+        return float(np.sum(vbar * flat_state(g.get_state()))) + wd * d[0]
+
+    out = {"check": "vjp"}
+    for h in (1e-5, 1e-6):
+        fd = (J(x0 + h * w) - J(x0 - h * w)) / (2 * h)
+        out["fd_h%g" % h] = fd
+        out["rel_h%g" % h] = abs(fd - adj) / abs(adj)
+    out["adjoint"] = adj
+    # per-block attribution: only the velocity planes / only C / only histories
+    for name, sl in (("uvwp", slice(0, 4)), ("c", slice(4, 10)), ("hist", slice(10, 16))):
+        wm = np.zeros_like(w)
+        wm[sl] = w[sl]
+        a = float(np.dot(bi.cpu().numpy(), wm.reshape(-1)))
+        h = 1e-6
+        fd = (J(x0 + h * wm) - J(x0 - h * wm)) / (2 * h)
+        out["rel_" + name] = abs(fd - a) / max(abs(a), 1e-300)
+    print(json.dumps(out), flush=True)
+
+
+def check_params():
+    case, g, st = small_case()
+    x0 = flat_state(st)
+    b0 = st.hist[6:9].copy()
+    nsteps, every = 12, 5
+    rng = np.random.default_rng(11)
+    wts = rng.uniform(0.5, 1.5, nsteps)
+    p = case.params
+    fx = np.stack([np.asarray(a) for a in case.force])
+    dF = rng.standard_normal(fx.shape) * 0.05
+
+    def run(wi, beta, L2, force, replay):
+        g.set_physics(wi, beta, math.sqrt(L2))
+        g.set_state(make_state(x0, b0))
+        g.set_body_force([force[0].copy(), force[1].copy(), force[2].copy()])
+        d = g.adj_record(nsteps, every, replay=replay)
+        return float(np.dot(wts, d))
+
+    base = (p.wi, p.beta, p.lp ** 2)
+    J0 = run(*base, fx, False)
+    fb = torch.zeros(3 * fx[0].size, dtype=torch.float64, device=DEV)
+    torch.cuda.synchronize()
+    grads = g.adj_backward(weights=wts, bar_force=fb.data_ptr())
+    dirF = float(np.dot(fb.cpu().numpy(), dF.reshape(-1)))
+    out = {"check": "params", "J": J0}
+    names = ("wi", "beta", "L2")
+    for q in range(3):
+        h = 1e-6 * base[q]
+        hi = list(base); lo = list(base)
+        hi[q] += h; lo[q] -= h
+        fd = (run(*hi, fx, True) - run(*lo, fx, True)) / (2 * h)
+        out["d" + names[q]] = grads[q]
+        out["fd_" + names[q]] = fd
+        out["rel_" + names[q]] = abs(fd - grads[q]) / abs(grads[q])
+    h = 1e-6
+    fd = (run(*base, fx + h * dF, True) - run(*base, fx - h * dF, True)) / (2 * h)
+    out["dF_dir"] = dirF
+    out["fd_F_dir"] = fd
+    out["rel_F"] = abs(fd - dirF) / abs(dirF)
+    print(json.dumps(out), flush=True)
+
+
+def check_cfg5():
+    case = configs.adjoint_case()          # 64x64x32, Lz = pi
+    nsteps, every, last = 400, 20, 200
+    p = case.params
+    fx = np.stack([np.asarray(a) for a in case.force])
+    run = DragRun(case, nsteps, every, last)
+    wi = torch.tensor(p.wi, dtype=torch.float64, requires_grad=True)
+    beta = torch.tensor(p.beta, dtype=torch.float64, requires_grad=True)
+    L2 = torch.tensor(p.lp ** 2, dtype=torch.float64, requires_grad=True)
+    F = torch.from_numpy(fx).to(DEV).requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = drag_objective(run, wi, beta, L2, F)
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    # plain forward of the same steps (no recording) for the cost ratio
+    g = run.g
+    g.set_physics(p.wi, p.beta, p.lp)
+    g.set_state(case.state)
+    g.set_body_force([fx[0].copy(), fx[1].copy(), fx[2].copy()])
+    t3 = time.perf_counter()
+    g.step(nsteps)
+    t4 = time.perf_counter()
+    out = {"check": "cfg5", "J": float(J), "t_record_s": t1 - t0, "t_backward_s": t2 - t1,
+           "t_forward_s": t4 - t3, "ratio_backward_forward": (t2 - t1) / (t4 - t3),
+           "ratio_record_forward": (t1 - t0) / (t4 - t3)}
+    run.replay = True
+    rng = np.random.default_rng(5)
+    dF = rng.standard_normal(fx.shape)
+    base = [p.wi, p.beta, p.lp ** 2]
+    grads = [float(wi.grad), float(beta.grad), float(L2.grad)]
+    for q, name in enumerate(("wi", "beta", "L2")):
+        for hrel in (1e-6, 1e-4):
+            h = hrel * base[q]
+            hi = list(base); lo = list(base)
+            hi[q] += h; lo[q] -= h
+            fd = (run.forward(*hi, fx)[0] - run.forward(*lo, fx)[0]) / (2 * h)
+            out["fd_%s_h%g" % (name, hrel)] = fd
+            out["rel_%s_h%g" % (name, hrel)] = abs(fd - grads[q]) / abs(grads[q])
+        out["d" + name] = grads[q]
+    dirF = float(torch.sum(F.grad.cpu() * torch.from_numpy(dF)))
+    h = 1e-6
+    fd = (run.forward(*base, fx + h * dF)[0] - run.forward(*base, fx - h * dF)[0]) / (2 * h)
+    out["dF_dir"] = dirF
+    out["fd_F_dir"] = fd
+    out["rel_F"] = abs(fd - dirF) / abs(dirF)
+    run.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1] if len(sys.argv) > 1 else "vjp"
+    print("adjoint check:", which, flush=True)
+    {"vjp": check_vjp, "params": check_params, "cfg5": check_cfg5}[which]()
