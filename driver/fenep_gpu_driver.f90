@@ -9,6 +9,8 @@
 !   Map/{z,y,x}_*.bin  18-column float64 planes (get_data_binary, fenep.f90:1108-1183)
 !   Restart.d       U V W P C(1:6) Ax Ay Az Tx Ty Tz per cell (output/input,
 !                   fenep.f90:1186-1233), written with 17 significant digits
+!   Log/time_same.d 13-column time series of all_time_dif every Tstep steps
+!                   (fenep_mpi.f90:2200-2348, read by _energy.py)
 !
 ! Usage: fenep_gpu [namelist-file]      (default: fenep.nml)
 module fenep_gpu_api
@@ -45,6 +47,11 @@ module fenep_gpu_api
       import :: c_int, c_ptr, fp_params
       type(fp_params), intent(in) :: p
       type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function fp_time_stats(ctx, out) bind(c, name='fp_time_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(13)
     end function
     integer(c_int) function fp_destroy(ctx) bind(c, name='fp_destroy')
       import :: c_int, c_ptr
@@ -131,6 +138,7 @@ program fenep_gpu
   real(8) :: dt = 0.001d0, Re = 2.0d0, beta = 0.9d0, Wi = 2.0d0, Lp = 10.0d0
   integer :: method = 1, walls_y = 0, poisson_variant = 0, precision = 0
   integer :: Nstep = 100, Lstep = 10, Gstep = 100000000, Ostep = 100000000, koide_step = 20
+  integer :: Tstep = 0
   integer :: flow_type = 0, input_type = 0
   integer(8) :: seed = 1
   real(8) :: force_amp = 0.1d0, body_force(3) = 0.0d0
@@ -140,7 +148,7 @@ program fenep_gpu
   real(8) :: newton_eps = 1.0d-6, sor_eps = 1.0d-5
   character(256) :: dir = './'
   namelist /run/ NX, NY, NZ, Lx, Ly, Lz, dt, Re, beta, Wi, Lp, method, walls_y, poisson_variant, &
-                 precision, Nstep, Lstep, Gstep, Ostep, koide_step, flow_type, input_type, seed, &
+                 precision, Nstep, Lstep, Gstep, Ostep, koide_step, Tstep, flow_type, input_type, seed, &
                  force_amp, body_force, ncyl, DC, xcyl, ycyl, newton_itrmax, sor_itrmax, n_dc_max, &
                  newton_eps, sor_eps, dir
 
@@ -217,6 +225,9 @@ program fenep_gpu
     if (Ostep > 0) then
       if (mod(step, Ostep) == 0) call write_restart()
     endif
+    if (Tstep > 0) then
+      if (mod(step, Tstep) == 0) call write_time_stats()
+    endif
   enddo
   call system_clock(t1)
   rc = fp_diagnostics(ctx, dg)
@@ -235,6 +246,7 @@ contains
     if (koide_step > 0) e = min(e, (s/koide_step + 1)*koide_step)
     if (Gstep > 0) e = min(e, (s/Gstep + 1)*Gstep)
     if (Ostep > 0) e = min(e, (s/Ostep + 1)*Ostep)
+    if (Tstep > 0) e = min(e, (s/Tstep + 1)*Tstep)
     next_event = e
   end function next_event
 
@@ -341,6 +353,18 @@ contains
     write(iu, *) s, C(1, NX/2, NY/2, NZ/2) + C(4, NX/2, NY/2, NZ/2) + C(6, NX/2, NY/2, NZ/2)
     close(iu)
   end subroutine write_koide
+
+  subroutine write_time_stats()
+    ! one record of Log/time_same.d (all_time_dif, fenep_mpi.f90:2342-2345),
+    ! statistics reduced on the device
+    integer :: iu
+    real(c_double) :: ts(13)
+    call check(fp_time_stats(ctx, ts), ctx, 'fp_time_stats')
+    call execute_command_line('mkdir -p '//trim(dir)//'Log')
+    open(newunit=iu, file=trim(dir)//'Log/time_same.d', position='append')
+    write(iu, *) nint(ts(1)), ts(2), ts(3), ts(4), ts(5), ts(6), ts(7), ts(8), ts(9), ts(10), ts(11), ts(12), ts(13)
+    close(iu)
+  end subroutine write_time_stats
 
   subroutine plane_row(iu, i, j, k)
     ! one 18-column record of get_data_binary (fenep.f90:1121-1146)

@@ -59,6 +59,7 @@ def lib():
         L.fp_kernel_times.argtypes = [_vp, _dp, C.c_int, C.c_char_p, C.c_int]
         L.fp_seeded_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, _dp]
         L.fp_set_physics.argtypes = [_vp, C.c_double, C.c_double, C.c_double]
+        L.fp_time_stats.argtypes = [_vp, _dp]
         L.fp_adj_record.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
         L.fp_adj_backward.argtypes = [_vp, _dp, _vp, _vp, _dp, _vp]
         _LIB = L
@@ -161,6 +162,12 @@ class FenepGPU:
         if rc and rc != 4:
             self._check(rc, "fp_diagnostics")
         return d.as_dict()
+
+    def time_stats(self):
+        """all_time_dif statistics (13 values, fenep_mpi.f90:2343-2344 order)."""
+        out = np.zeros(13)
+        self._check(lib().fp_time_stats(self.h, _ptr(out)), "fp_time_stats")
+        return out
 
     # ---- sensitivities (FP64 path, periodic grids)
     def set_physics(self, wi, beta, lp):

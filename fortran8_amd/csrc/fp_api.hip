@@ -551,6 +551,19 @@ int fp_seeded_uniform(uint64_t seed, uint64_t offset, size_t n, double* out) {
   return FP_OK;
 }
 
+int fp_time_stats(void* vc, double* out) {
+  Ctx* c = (Ctx*)vc;
+  if (!c || !out) return FP_E_ARG;
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  try {
+    if (c->mixed) mixed_store_to_f64(*c);
+    f64_time_stats(*c, out);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
 // Errors of the recording/adjoint layer arrive as "E_ARG:..." or
 // "E_STATE:..." (argument / call order) or as HIP errors.
 static int adj_fail(Ctx* c, const std::string& e) {

@@ -395,62 +395,87 @@ __global__ __launch_bounds__(256) void k_fft_yz(Grid g, double2* __restrict__ sp
 
 // tridiagonal line solve along y with the wall rows of fenep_2d.f90:1222-1331
 // (tdma, fenep_2d.f90:1145-1177), one thread per (kx, kz) line.
-__global__ __launch_bounds__(256) void k_thomas(Grid g, double2* __restrict__ spec, double* __restrict__ thp, int kind,
+// Tridiagonal line solve along y of each (kx, kz) mode (tdma with the wall
+// rows of fft_*_kabe, fenep_2d.f90:1145-1177, 1222-1331).  One block of 256
+// threads solves `tl` consecutive kx lines of one kz plane: the rows are
+// staged through LDS with coalesced loads, one thread per line runs the
+// recurrence from LDS, and the rows are stored back.  The arithmetic is the
+// Thomas recurrence of the reference, in its order.
+__global__ __launch_bounds__(256) void k_thomas(Grid g, double2* __restrict__ spec, int kind, int tl,
                                                 const double* __restrict__ kx2, const double* __restrict__ kz2) {
-  const long lines = (long)g.nxh * g.nz;
-  long L = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (L >= lines) return;
+  extern __shared__ double2 tq[];                 // [size][tl] right-hand sides / Q
   const int nxh = g.nxh, ny = g.ny, size = ny + 2;
-  int i = (int)(L % nxh);
-  long k = L / nxh;
-  double2* col = spec + i + (long)nxh * g.sy * k;  // row r at col[nxh*r]
-  const double nxz = (double)((long)g.nx * g.nz);
-  const double coef = ((g.dt * g.beta) / g.re) / 2.0;
-  const double idy2 = g.dy * g.dy;
-  const double kk = kx2[i] + kz2[k];
-  // row coefficients
-  auto coeffs = [&](int r, double& a, double& b, double& c, bool& rhs) {
-    rhs = false;
-    if (kind == 3) {
-      if (r == 0) { a = 1.0; b = -1.0; c = 0.0; }
-      else if (r == size - 1) { a = 1.0; b = 0.0; c = -1.0; }
-      else { a = -(kk + 2.0 / idy2); b = 1.0 / idy2; c = 1.0 / idy2; rhs = true; }
-    } else if (kind == 1) {
-      if (r == 0) { a = 1.0; b = 0.0; c = 0.0; }
-      else if (r >= ny) { a = 1.0; b = 0.0; c = 0.0; }
-      else { a = 1.0 + coef * (kk + 2.0 / idy2); b = -(coef / idy2); c = -(coef / idy2); rhs = true; }
-    } else {
-      if (r == 0) { a = 1.0; b = 1.0; c = 0.0; }
-      else if (r == size - 1) { a = 1.0; b = 0.0; c = 1.0; }
-      else { a = 1.0 + coef * (kk + 2.0 / idy2); b = -(coef / idy2); c = -(coef / idy2); rhs = true; }
+  double* tp = (double*)(tq + (long)size * tl);   // [size][tl] P
+  const int ngrp = (nxh + tl - 1) / tl;
+  const int i0 = (int)(blockIdx.x % ngrp) * tl;
+  const long k = blockIdx.x / ngrp;
+  const int nl = min(tl, nxh - i0);
+  double2* base = spec + (long)nxh * g.sy * k + i0;  // row r of line l at base[nxh*r + l]
+  for (int e = threadIdx.x; e < size * tl; e += blockDim.x) {
+    const int r = e / tl, l = e - r * tl;
+    if (l < nl) tq[e] = base[(long)nxh * r + l];
+  }
+  __syncthreads();
+  if (threadIdx.x < nl) {
+    const int l = threadIdx.x, i = i0 + l;
+    const double nxz = (double)((long)g.nx * g.nz);
+    const double coef = ((g.dt * g.beta) / g.re) / 2.0;
+    const double idy2 = g.dy * g.dy;
+    const double kk = kx2[i] + kz2[k];
+    auto coeffs = [&](int r, double& a, double& b, double& c, bool& rhs) {
+      rhs = false;
+      if (kind == 3) {
+        if (r == 0) { a = 1.0; b = -1.0; c = 0.0; }
+        else if (r == size - 1) { a = 1.0; b = 0.0; c = -1.0; }
+        else { a = -(kk + 2.0 / idy2); b = 1.0 / idy2; c = 1.0 / idy2; rhs = true; }
+      } else if (kind == 1) {
+        if (r == 0) { a = 1.0; b = 0.0; c = 0.0; }
+        else if (r >= ny) { a = 1.0; b = 0.0; c = 0.0; }
+        else { a = 1.0 + coef * (kk + 2.0 / idy2); b = -(coef / idy2); c = -(coef / idy2); rhs = true; }
+      } else {
+        if (r == 0) { a = 1.0; b = 1.0; c = 0.0; }
+        else if (r == size - 1) { a = 1.0; b = 0.0; c = 1.0; }
+        else { a = 1.0 + coef * (kk + 2.0 / idy2); b = -(coef / idy2); c = -(coef / idy2); rhs = true; }
+      }
+    };
+    double a, b, c; bool rhs;
+    coeffs(0, a, b, c, rhs);
+    double Pp = -b / a;
+    double Qr = 0.0 / a, Qi = 0.0 / a;
+    tp[l] = Pp;
+    tq[l] = make_double2(Qr, Qi);
+    for (int r = 1; r < size; r++) {
+      coeffs(r, a, b, c, rhs);
+      double dr = 0.0, di = 0.0;
+      if (rhs) { const double2 sv = tq[r * tl + l]; dr = sv.x / nxz; di = sv.y / nxz; }
+      const double den = a + c * Pp;
+      double P, nQr, nQi;
+      if (fabs(den) < 1.0e-16) { P = 0.0; nQr = 0.0; nQi = 0.0; }
+      else { P = -b / den; nQr = (dr - c * Qr) / den; nQi = (di - c * Qi) / den; }
+      Pp = P; Qr = nQr; Qi = nQi;
+      tp[r * tl + l] = P;
+      tq[r * tl + l] = make_double2(Qr, Qi);
     }
-  };
-  double a, b, c; bool rhs;
-  coeffs(0, a, b, c, rhs);
-  double Pp = -b / a;
-  double Qr = 0.0 / a, Qi = 0.0 / a;
-  thp[L] = Pp;
-  col[0] = make_double2(Qr, Qi);
-  for (int r = 1; r < size; r++) {
-    coeffs(r, a, b, c, rhs);
-    double dr = 0.0, di = 0.0;
-    if (rhs) { double2 s = col[(long)nxh * r]; dr = s.x / nxz; di = s.y / nxz; }
-    double den = a + c * Pp;
-    double P, nQr, nQi;
-    if (fabs(den) < 1.0e-16) { P = 0.0; nQr = 0.0; nQi = 0.0; }
-    else { P = -b / den; nQr = (dr - c * Qr) / den; nQi = (di - c * Qi) / den; }
-    Pp = P; Qr = nQr; Qi = nQi;
-    thp[L + lines * r] = P;
-    col[(long)nxh * r] = make_double2(Qr, Qi);
+    double xr = Qr, xi = Qi;
+    for (int r = size - 2; r >= 0; r--) {
+      const double P = tp[r * tl + l];
+      const double2 q = tq[r * tl + l];
+      xr = P * xr + q.x;
+      xi = P * xi + q.y;
+      tq[r * tl + l] = make_double2(xr, xi);
+    }
   }
-  double xr = Qr, xi = Qi;
-  for (int r = size - 2; r >= 0; r--) {
-    double P = thp[L + lines * r];
-    double2 q = col[(long)nxh * r];
-    xr = P * xr + q.x;
-    xi = P * xi + q.y;
-    col[(long)nxh * r] = make_double2(xr, xi);
+  __syncthreads();
+  for (int e = threadIdx.x; e < size * tl; e += blockDim.x) {
+    const int r = e / tl, l = e - r * tl;
+    if (l < nl) base[(long)nxh * r + l] = tq[e];
   }
+}
+
+// lines per block of the LDS line solve: 64 KiB of LDS at most
+int thomas_lines(int ny, int bytes_per_row) {
+  const int per_line = (ny + 2) * bytes_per_row;
+  return std::max(1, std::min(32, 65536 / per_line));
 }
 
 static int lines_for(int n) { return std::max(1, std::min(16, 2048 / n)); }
@@ -471,8 +496,10 @@ void f64_solve(Ctx& c, int kind, const double* q, double* x) {
     hipLaunchKernelGGL(k_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, c.spec, c.twy, 1, 1, kind, 0, 0, tiy, c.kx2, c.ky2, c.kz2);
   } else {
     hipLaunchKernelGGL(k_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, c.spec, c.twz, 2, 0, kind, 1, g.ny, tiz, c.kx2, c.ky2, c.kz2);
-    long lines = (long)g.nxh * g.nz;
-    hipLaunchKernelGGL(k_thomas, dim3(nblk(lines, 256)), dim3(256), 0, s, g, c.spec, c.thp, kind, c.kx2, c.kz2);
+    const int tl = thomas_lines(g.ny, (int)(sizeof(double2) + sizeof(double)));
+    const int ngrp = (g.nxh + tl - 1) / tl;
+    hipLaunchKernelGGL(k_thomas, dim3(ngrp * g.nz), dim3(256), (size_t)(g.ny + 2) * tl * (sizeof(double2) + sizeof(double)),
+                       s, g, c.spec, kind, tl, c.kx2, c.kz2);
     hipLaunchKernelGGL(k_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, c.spec, c.twz, 2, 1, kind, 1, g.ny, tiz, c.kx2, c.ky2, c.kz2);
   }
   hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, c.spec, x, c.twx, lx);
@@ -801,6 +828,119 @@ void f64_march(Ctx& c) {
 
 void f64_zero_c(Ctx& c) {
   zero_words(c.C[c.cur], 12 * c.g.n, c.stream);
+}
+
+// =============================================== time-series statistics ===
+// Partial sums of all_time_dif (fenep_mpi.f90:2200-2348, no subtracted mean):
+// 0-2 regular-grid velocity squares, 3-5 squared staggered gradients, 6 S:S
+// with S = D + D^T of the 4-point velocity gradient, 7 |u|^2 on the staggered
+// grid, 8 log f(C), 9 tr C.
+constexpr int TS_N = 10;
+__global__ __launch_bounds__(256) void k_time_stats(Grid g, const double* __restrict__ U, const double* __restrict__ V,
+                                                    const double* __restrict__ W, const double* __restrict__ C,
+                                                    double* __restrict__ part, int nb) {
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double s[TS_N];
+  for (int q = 0; q < TS_N; q++) s[q] = 0.0;
+  if (m < g.n) {
+    int i, j, k;
+    cell_ijk(g, m, i, j, k);
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+#define UG(a, b, c) ldb(U, g, i + (a), j + (b), k + (c), BC_U)
+#define VG(a, b, c) ldb(V, g, i + (a), j + (b), k + (c), BC_V)
+#define WG(a, b, c) ldb(W, g, i + (a), j + (b), k + (c), BC_W)
+    const double u0 = UG(0, 0, 0), v0 = VG(0, 0, 0), w0 = WG(0, 0, 0);
+    const double ur = (u0 + UG(-1, 0, 0)) / 2.0, vr = (v0 + VG(0, -1, 0)) / 2.0, wr = (w0 + WG(0, 0, -1)) / 2.0;
+    s[0] = ur * ur; s[1] = vr * vr; s[2] = wr * wr;
+    double D[3][3];
+    D[0][0] = (u0 - UG(-1, 0, 0)) / dx;
+    D[0][1] = (UG(0, 1, 0) - UG(0, -1, 0) + UG(-1, 1, 0) - UG(-1, -1, 0)) / (4 * dy);
+    D[0][2] = (UG(0, 0, 1) - UG(0, 0, -1) + UG(-1, 0, 1) - UG(-1, 0, -1)) / (4 * dz);
+    D[1][0] = (VG(1, 0, 0) - VG(-1, 0, 0) + VG(1, -1, 0) - VG(-1, -1, 0)) / (4 * dx);
+    D[1][1] = (v0 - VG(0, -1, 0)) / dy;
+    D[1][2] = (VG(0, 0, 1) - VG(0, 0, -1) + VG(0, -1, 1) - VG(0, -1, -1)) / (4 * dz);
+    D[2][0] = (WG(1, 0, 0) - WG(-1, 0, 0) + WG(1, 0, -1) - WG(-1, 0, -1)) / (4 * dx);
+    D[2][1] = (WG(0, 1, 0) - WG(0, -1, 0) + WG(0, 1, -1) - WG(0, -1, -1)) / (4 * dy);
+    D[2][2] = (w0 - WG(0, 0, -1)) / dz;
+#undef UG
+#undef VG
+#undef WG
+    s[3] = D[0][0] * D[0][0]; s[4] = D[1][1] * D[1][1]; s[5] = D[2][2] * D[2][2];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) { const double sab = D[a][b] + D[b][a]; s[6] += sab * sab; }
+    s[7] = U[m] * U[m] + V[m] * V[m] + W[m] * W[m];
+    const double t = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m];
+    const double L2 = g.lp * g.lp;
+    s[8] = log((L2 - 3.0) / (L2 - t));
+    s[9] = t;
+  }
+  __shared__ double sh[256];
+  for (int q = 0; q < TS_N; q++) {
+    const double v = block_sum256(s[q], sh);
+    if (threadIdx.x == 0) part[(long)q * nb + blockIdx.x] = v;
+    __syncthreads();
+  }
+}
+
+// sum of (tr C - mean)^2, per block
+__global__ __launch_bounds__(256) void k_trc_var(Grid g, const double* __restrict__ C, const double* __restrict__ mean,
+                                                 double* __restrict__ part) {
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (m < g.n) {
+    const double t = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m] - mean[0];
+    v = t * t;
+  }
+  __shared__ double sh[256];
+  const double s = block_sum256(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+void f64_time_stats(Ctx& c, double* out) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const int nb = nblk(g.n, 256);
+  double* part = nullptr;
+  double* red = nullptr;
+  FP_HIP_CHECK(hipMalloc((void**)&part, sizeof(double) * (size_t)(TS_N + 1) * nb));
+  FP_HIP_CHECK(hipMalloc((void**)&red, sizeof(double) * (TS_N + 2)));
+  double r[TS_N + 2];
+  try {
+    hipLaunchKernelGGL(k_time_stats, dim3(nb), dim3(256), 0, s, g, c.U, c.V, c.W, c.C[c.cur], part, nb);
+    for (int q = 0; q < TS_N; q++)
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part + (long)q * nb, nb, red + q, 0);
+    FP_HIP_CHECK(hipMemcpyAsync(r, red, sizeof(double) * TS_N, hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+    const double mean = r[9] / (double)g.n;
+    FP_HIP_CHECK(hipMemcpyAsync(red + TS_N, &mean, sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_trc_var, dim3(nb), dim3(256), 0, s, g, c.C[c.cur], red + TS_N, part + (long)TS_N * nb);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part + (long)TS_N * nb, nb, red + TS_N + 1, 0);
+    FP_HIP_CHECK(hipMemcpyAsync(r + TS_N + 1, red + TS_N + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(part); (void)hipFree(red);
+    throw;
+  }
+  (void)hipFree(part);
+  (void)hipFree(red);
+  const double N = (double)g.n, nu = 1.0 / c.p.re, L2 = c.p.lp * c.p.lp;
+  const double urms = r[0] / N, vrms = r[1] / N, wrms = r[2] / N;
+  const double all_rms = (urms + vrms + wrms) / 3.0, all_grad = (r[3] / N + r[4] / N + r[5] / N) / 3.0;
+  const double eps = nu / 2.0 * r[6] / N;
+  const double lambda = std::sqrt(all_rms / all_grad);
+  out[0] = (double)c.step;
+  out[1] = c.step * c.p.dt;
+  out[2] = (urms + vrms + wrms) / 2.0;
+  out[3] = r[7] / (2.0 * N);
+  out[4] = lambda;
+  out[5] = std::sqrt(all_rms) * lambda / nu;
+  out[6] = eps;
+  out[7] = std::pow(nu * nu * nu / eps, 0.25);
+  out[8] = std::sqrt(nu / eps);
+  out[9] = std::pow(nu * eps, 0.25);
+  out[10] = 0.5 * (1.0 - c.p.beta) / c.p.re / c.p.wi * (L2 - 3.0) * r[8] / N;
+  out[11] = r[9] / N;
+  out[12] = std::sqrt(r[TS_N + 1] / N);
 }
 
 // ===================================================== diagnostics ========

@@ -58,12 +58,13 @@ struct Mixed {
   int* fr_ent = nullptr;          // c*nmk + marker
   float* fr_w = nullptr;
   int fr_rows = 0;
-  float* fint = nullptr;          // 3 planes, interior
   double last_ndc = 0.0;
 };
 
 inline Mixed& MX(Ctx& c) { return *(Mixed*)c.mixed; }
 inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
+// cells in the 2R rows next to the y-walls (the GEN = true instances)
+inline long wall_rows(const Grid& g, int R) { return (long)g.nx * 2 * R * g.nz; }
 
 template <typename T>
 void dalloc(T** p, size_t count) {
@@ -142,10 +143,10 @@ __device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
 // C at cells a-r..a+r along axis d (r = 1 or 2) for the 6 components; wall
 // ghosts by linear extrapolation (C_kabe); points beyond one ghost layer
 // only feed faces whose upwind weight is exactly zero and get the centre.
-template <int R>
+template <int R, bool GEN>
 __device__ __forceinline__ void load_line(const float* __restrict__ C, const Grid& g, long m, int i, int j, int k, int d,
                                           float (&v)[2 * R + 1][6]) {
-  if (d == 1 && g.walls && (j < R || j > g.ny - 1 - R)) {
+  if (GEN && d == 1 && g.walls && (j < R || j > g.ny - 1 - R)) {
 #pragma unroll
     for (int o = -R; o <= R; o++) {
       const int jj = j + o;
@@ -170,20 +171,45 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
     for (int q = 0; q < 6; q++) v[o][q] = C[q * g.n + m + off[o]];
 }
 
+// Stencil kernels come in two instances.  GEN = false covers the cells whose
+// stencils of radius R stay clear of the y-wall ghosts (every cell of a
+// periodic grid) and reads neighbours through wrapped offsets only;
+// GEN = true covers the 2R rows next to the walls through the boundary-rule
+// accessor.  Separate kernels keep the bulk kernel's register budget, and so
+// its occupancy, free of the wall path.
+template <bool GEN, int R>
+__device__ __forceinline__ bool row_cell(const Grid& g, long& m, int& i, int& j, int& k) {
+  if (!GEN) {
+    m = xcd_cell();
+    if (m >= g.n) return false;
+    cell_ijk(g, m, i, j, k);
+    return !g.walls || (j >= R && j < g.ny - R);
+  }
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long per = (long)g.nx * 2 * R;
+  if (t >= per * g.nz) return false;
+  i = (int)(t % g.nx);
+  const int r = (int)((t / g.nx) % (2 * R));
+  k = (int)(t / per);
+  j = r < R ? r : g.ny - 2 * R + r;
+  m = cidx(g, i, j, k);
+  return true;
+}
+
 // relative guard band for re-deciding a selection in FP64
 #define SEL_GUARD 1.0e-6f
 
+template <bool GEN>
 __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                 unsigned* __restrict__ amb, int* __restrict__ amb_list, long amb_cap) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  if (!row_cell<GEN, 1>(g, m, i, j, k)) return;
   uint8_t s = 0;
 #pragma unroll 1
   for (int d = 0; d < 3; d++) {
     float v[3][6];
-    load_line<1>(C, g, m, i, j, k, d, v);
+    load_line<1, GEN>(C, g, m, i, j, k, d, v);
     float vals[3];
 #pragma unroll
     for (int l = 0; l < 3; l++) {
@@ -287,17 +313,17 @@ __device__ __forceinline__ Nbr nbr(const Grid& g, int i, int j, int k) {
 
 // Cstar + Lyapunov Newton (analytic Jacobian, FP32) + Peterlin factor of the
 // new C.
+template <bool GEN>
 __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
                                                       const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, float* __restrict__ Cn,
                                                       float* __restrict__ fC, int* __restrict__ nitr, float eps,
                                                       int itrmax) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  if (!row_cell<GEN, 2>(g, m, i, j, k)) return;
   const float dt = (float)g.dt;
-  const bool fast = !g.walls || (j > 0 && j < g.ny - 1);
+  const bool fast = !GEN || !g.walls || (j > 0 && j < g.ny - 1);
   const Nbr o = nbr(g, i, j, k);
   float cx[6];
 #pragma unroll
@@ -318,7 +344,7 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
     const long op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
     const int lm = (sel[m + om] >> (2 * d)) & 3, l0 = (sel[m] >> (2 * d)) & 3, lp = (sel[m + op] >> (2 * d)) & 3;
     float v[5][6];
-    load_line<2>(C, g, m, i, j, k, d, v);
+    load_line<2, GEN>(C, g, m, i, j, k, d, v);
 #pragma unroll
     for (int q = 0; q < 6; q++) {
       const float pl = cand_pf(l0, v[1][q], v[2][q], v[3][q]);   // Cp(a-1): own left face
@@ -451,7 +477,7 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
                                                const float* __restrict__ U, const float* __restrict__ V,
                                                const float* __restrict__ W, const float* __restrict__ P,
                                                const float* __restrict__ C, const float* __restrict__ fC,
-                                               float* __restrict__ A0, float* __restrict__ T0, float* __restrict__ Bout,
+                                               float* __restrict__ A0, float* __restrict__ T0, float* __restrict__ Rp,
                                                const float* __restrict__ F, float bf0, float bf1, float bf2,
                                                float* __restrict__ Q) {
   const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz, dt = (float)g.dt;
@@ -520,10 +546,18 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
     float a0 = first ? a[c] : A0[oo];
     float t0 = first ? t[c] : T0[oo];
     float fv = (F ? F[oo] : 0.0f) + bfs[c];
-    Q[oo] = uu[c] - dt * (pn[c] - p0) * ih[c] + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] + 0.5f * dt * (t[c] + t0) + dt * fv;
+    const float qv = uu[c] - dt * (pn[c] - p0) * ih[c] + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] +
+                     0.5f * dt * (t[c] + t0) + dt * fv;
+    Q[oo] = qv;
     A0[oo] = a[c];
     T0[oo] = t[c];
-    if (Bout) Bout[oo] = bv[c];
+    if (Rp) {
+      // ibm_predict right-hand side without the forcing (fenep.f90:2229):
+      // Ua - dt B/2, with the wall face of V read as 0 (V_kabe); the spread
+      // adds dt f at the cells the markers reach
+      const float ua = (!FAST && c == 1 && g.walls && j == g.ny - 1) ? 0.0f : qv;
+      Rp[oo] = ua - 0.5f * dt * bv[c];
+    }
   }
 #undef NB
 #undef Ug
@@ -532,21 +566,18 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
 #undef Pg
 }
 
+template <bool GEN>
 __global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first, int polymer,
                                                    const float* __restrict__ U, const float* __restrict__ V,
                                                    const float* __restrict__ W, const float* __restrict__ P,
                                                    const float* __restrict__ C, const float* __restrict__ fC,
                                                    float* __restrict__ A0, float* __restrict__ T0,
-                                                   float* __restrict__ Bout, const float* __restrict__ F,
+                                                   float* __restrict__ Rp, const float* __restrict__ F,
                                                    float bf0, float bf1, float bf2, float* __restrict__ Q) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
-  if (!g.walls || (j > 0 && j < g.ny - 1))
-    predictor_body<true>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Bout, F, bf0, bf1, bf2, Q);
-  else
-    predictor_body<false>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Bout, F, bf0, bf1, bf2, Q);
+  if (!row_cell<GEN, 1>(g, m, i, j, k)) return;
+  predictor_body<!GEN>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Rp, F, bf0, bf1, bf2, Q);
 }
 
 // ================================================================== FFT ===
@@ -681,58 +712,75 @@ __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spe
   }
 }
 
-__global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spec, float* __restrict__ thp, int kind,
+// Line solve along y of each (kx, kz) mode (FP32), staged through LDS like
+// the FP64 k_thomas: coalesced row loads, one thread per line for the
+// recurrence, coalesced stores.
+__global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spec, int kind, int tl,
                                                 const float* __restrict__ kx2, const float* __restrict__ kz2,
                                                 const int* __restrict__ skip) {
   if (skip && *skip) return;
-  const long lines = (long)g.nxh * g.nz;
-  long L = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (L >= lines) return;
+  extern __shared__ float2 sq[];                 // [size][tl]
   const int nxh = g.nxh, ny = g.ny, size = ny + 2;
-  int i = (int)(L % nxh);
-  long k = L / nxh;
-  float2* col = spec + i + (long)nxh * g.sy * k;
-  const float inxz = (float)(1.0 / ((double)g.nx * g.nz));
-  const float coef = (float)(((g.dt * g.beta) / g.re) / 2.0);
-  const float idy2 = (float)(1.0 / (g.dy * g.dy));
-  const float kk = kx2[i] + kz2[k];
-  auto coeffs = [&](int r, float& a, float& b, float& c, bool& rhs) {
-    rhs = false;
-    if (kind == 3) {
-      if (r == 0) { a = 1.0f; b = -1.0f; c = 0.0f; }
-      else if (r == size - 1) { a = 1.0f; b = 0.0f; c = -1.0f; }
-      else { a = -(kk + 2.0f * idy2); b = idy2; c = idy2; rhs = true; }
-    } else if (kind == 1) {
-      if (r == 0 || r >= ny) { a = 1.0f; b = 0.0f; c = 0.0f; }
-      else { a = 1.0f + coef * (kk + 2.0f * idy2); b = -coef * idy2; c = b; rhs = true; }
-    } else {
-      if (r == 0) { a = 1.0f; b = 1.0f; c = 0.0f; }
-      else if (r == size - 1) { a = 1.0f; b = 0.0f; c = 1.0f; }
-      else { a = 1.0f + coef * (kk + 2.0f * idy2); b = -coef * idy2; c = b; rhs = true; }
-    }
-  };
-  float a, b, c; bool rhs;
-  coeffs(0, a, b, c, rhs);
-  float Pp = -b / a, Qr = 0.0f, Qi = 0.0f;
-  thp[L] = Pp;
-  col[0] = make_float2(0.0f, 0.0f);
-  for (int r = 1; r < size; r++) {
-    coeffs(r, a, b, c, rhs);
-    float dr = 0.0f, di = 0.0f;
-    if (rhs) { float2 s = col[(long)nxh * r]; dr = s.x * inxz; di = s.y * inxz; }
-    float den = a + c * Pp;
-    if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
-    else { float id = 1.0f / den; Pp = -b * id; Qr = (dr - c * Qr) * id; Qi = (di - c * Qi) * id; }
-    thp[L + lines * r] = Pp;
-    col[(long)nxh * r] = make_float2(Qr, Qi);
+  float* sp = (float*)(sq + (long)size * tl);    // [size][tl]
+  const int ngrp = (nxh + tl - 1) / tl;
+  const int i0 = (int)(blockIdx.x % ngrp) * tl;
+  const long k = blockIdx.x / ngrp;
+  const int nl = min(tl, nxh - i0);
+  float2* base = spec + (long)nxh * g.sy * k + i0;
+  for (int e = threadIdx.x; e < size * tl; e += blockDim.x) {
+    const int r = e / tl, l = e - r * tl;
+    if (l < nl) sq[e] = base[(long)nxh * r + l];
   }
-  float xr = Qr, xi = Qi;
-  for (int r = size - 2; r >= 0; r--) {
-    float P = thp[L + lines * r];
-    float2 q = col[(long)nxh * r];
-    xr = P * xr + q.x;
-    xi = P * xi + q.y;
-    col[(long)nxh * r] = make_float2(xr, xi);
+  __syncthreads();
+  if (threadIdx.x < nl) {
+    const int l = threadIdx.x, i = i0 + l;
+    const float inxz = (float)(1.0 / ((double)g.nx * g.nz));
+    const float coef = (float)(((g.dt * g.beta) / g.re) / 2.0);
+    const float idy2 = (float)(1.0 / (g.dy * g.dy));
+    const float kk = kx2[i] + kz2[k];
+    auto coeffs = [&](int r, float& a, float& b, float& c, bool& rhs) {
+      rhs = false;
+      if (kind == 3) {
+        if (r == 0) { a = 1.0f; b = -1.0f; c = 0.0f; }
+        else if (r == size - 1) { a = 1.0f; b = 0.0f; c = -1.0f; }
+        else { a = -(kk + 2.0f * idy2); b = idy2; c = idy2; rhs = true; }
+      } else if (kind == 1) {
+        if (r == 0 || r >= ny) { a = 1.0f; b = 0.0f; c = 0.0f; }
+        else { a = 1.0f + coef * (kk + 2.0f * idy2); b = -coef * idy2; c = b; rhs = true; }
+      } else {
+        if (r == 0) { a = 1.0f; b = 1.0f; c = 0.0f; }
+        else if (r == size - 1) { a = 1.0f; b = 0.0f; c = 1.0f; }
+        else { a = 1.0f + coef * (kk + 2.0f * idy2); b = -coef * idy2; c = b; rhs = true; }
+      }
+    };
+    float a, b, c; bool rhs;
+    coeffs(0, a, b, c, rhs);
+    float Pp = -b / a, Qr = 0.0f, Qi = 0.0f;
+    sp[l] = Pp;
+    sq[l] = make_float2(0.0f, 0.0f);
+    for (int r = 1; r < size; r++) {
+      coeffs(r, a, b, c, rhs);
+      float dr = 0.0f, di = 0.0f;
+      if (rhs) { const float2 sv = sq[r * tl + l]; dr = sv.x * inxz; di = sv.y * inxz; }
+      const float den = a + c * Pp;
+      if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
+      else { const float id = 1.0f / den; Pp = -b * id; Qr = (dr - c * Qr) * id; Qi = (di - c * Qi) * id; }
+      sp[r * tl + l] = Pp;
+      sq[r * tl + l] = make_float2(Qr, Qi);
+    }
+    float xr = Qr, xi = Qi;
+    for (int r = size - 2; r >= 0; r--) {
+      const float P = sp[r * tl + l];
+      const float2 q = sq[r * tl + l];
+      xr = P * xr + q.x;
+      xi = P * xi + q.y;
+      sq[r * tl + l] = make_float2(xr, xi);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < size * tl; e += blockDim.x) {
+    const int r = e / tl, l = e - r * tl;
+    if (l < nl) base[(long)nxh * r + l] = sq[e];
   }
 }
 
@@ -755,8 +803,10 @@ static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip 
     hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 1, kind, 0, tiy, X.kx2, X.ky2, X.kz2, skip);
   } else {
     hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 0, kind, 1, tiz, X.kx2, X.ky2, X.kz2, skip);
-    long lines = (long)g.nxh * g.nz;
-    hipLaunchKernelGGL(m_thomas, dim3(nblk(lines, 256)), dim3(256), 0, s, g, X.spec, X.thp, kind, X.kx2, X.kz2, skip);
+    const int tl = thomas_lines(g.ny, (int)(sizeof(float2) + sizeof(float)));
+    const int ngrp = (g.nxh + tl - 1) / tl;
+    hipLaunchKernelGGL(m_thomas, dim3(ngrp * g.nz), dim3(256), (size_t)(g.ny + 2) * tl * (sizeof(float2) + sizeof(float)),
+                       s, g, X.spec, kind, tl, X.kx2, X.kz2, skip);
     hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 1, kind, 1, tiz, X.kx2, X.ky2, X.kz2, skip);
   }
   hipLaunchKernelGGL(m_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, X.spec, x, X.twx, lx, skip);
@@ -967,26 +1017,13 @@ __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const floa
 // (ibm_PBM, fenep.f90:2195-2205), FP32 accumulation
 __global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* __restrict__ rowp, const long* __restrict__ tgt,
                                                            const int* __restrict__ ent, const float* __restrict__ w,
-                                                           const float* __restrict__ mf, float* __restrict__ fint) {
+                                                           const float* __restrict__ mf, float dt,
+                                                           float* __restrict__ rhs) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   float acc = 0.0f;
   for (int e = rowp[r]; e < rowp[r + 1]; e++) acc += mf[ent[e]] * w[e];
-  fint[tgt[r]] = acc;
-}
-
-// ibm_predict right-hand side (fenep.f90:2229): Ua + dt f - dt B/2
-__global__ __launch_bounds__(256) void m_ibm_rhs(Grid g, const float* __restrict__ ua, const float* __restrict__ fint,
-                                                 const float* __restrict__ B, float* __restrict__ q) {
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (m >= 3 * g.n) return;
-  int c = (int)(m / g.n);
-  long mm = m % g.n;
-  int i, j, k;
-  cell_ijk(g, mm, i, j, k);
-  int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
-  const float dt = (float)g.dt;
-  q[m] = ldb(ua + c * g.n, g, i, j, k, kind) + dt * fint[m] - 0.5f * dt * B[m];
+  rhs[tgt[r]] += dt * acc;   // each target appears in one row only
 }
 
 // ========================================================== conversions ===
@@ -1065,7 +1102,7 @@ void mixed_destroy(Ctx& c) {
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
-  dfree(X->fr_row); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w); dfree(X->fint);
+  dfree(X->fr_row); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
   delete X;
   c.mixed = nullptr;
 }
@@ -1163,11 +1200,10 @@ void mixed_set_markers(Ctx& c) {
     for (auto& pr : kv.second) { fent.push_back(pr.first); fw.push_back(pr.second); }
     frow.push_back((int)fent.size());
   }
-  dfree(X.fr_row); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w); dfree(X.fint);
+  dfree(X.fr_row); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w);
   X.fr_rows = (int)ftgt.size();
   dalloc(&X.fr_row, frow.size()); dalloc(&X.fr_tgt, std::max<size_t>(1, ftgt.size()));
   dalloc(&X.fr_ent, std::max<size_t>(1, fent.size())); dalloc(&X.fr_w, std::max<size_t>(1, fw.size()));
-  dalloc(&X.fint, 3 * g.n);
   FP_HIP_CHECK(hipMemcpy(X.fr_row, frow.data(), sizeof(int) * frow.size(), hipMemcpyHostToDevice));
   if (!ftgt.empty()) {
     FP_HIP_CHECK(hipMemcpy(X.fr_tgt, ftgt.data(), sizeof(long) * ftgt.size(), hipMemcpyHostToDevice));
@@ -1228,7 +1264,11 @@ void mixed_step(Ctx& c) {
     zero_words(X.amb, 1, s);
     {
       MTimer t(c, "fenep_select");
-      hipLaunchKernelGGL(m_select, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list, X.amb_cap);
+      hipLaunchKernelGGL(m_select<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list,
+                         X.amb_cap);
+      if (g.walls)
+        hipLaunchKernelGGL(m_select<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb,
+                           X.amb_list, X.amb_cap);
     }
     {
       MTimer t(c, "fenep_select_fp64_fix");
@@ -1238,16 +1278,23 @@ void mixed_step(Ctx& c) {
     {
       MTimer t(c, "fenep_cstar_newton");
       int itmax = std::min(c.p.newton_itrmax, 16);
-      hipLaunchKernelGGL(m_cstar_newton, dim3(nblk(n, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
+      hipLaunchKernelGGL(m_cstar_newton<false>, dim3(nblk(n, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
+                         X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
+      if (g.walls)
+        hipLaunchKernelGGL(m_cstar_newton<true>, dim3(nblk(wall_rows(g, 2), 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
                          X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
       X.cur = 1 - X.cur;
     }
   }
   {
     MTimer t(c, "predictor");
-    float* Bout = c.p.method == FP_METHOD_IBM ? X.B : nullptr;
-    hipLaunchKernelGGL(m_predictor, dim3(nblk(n, 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
-                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Bout, X.F, (float)c.p.body_force[0],
+    float* Rp = c.p.method == FP_METHOD_IBM ? X.R : nullptr;
+    hipLaunchKernelGGL(m_predictor<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
+                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
+                       (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
+    if (g.walls)
+      hipLaunchKernelGGL(m_predictor<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
+                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   }
   if (c.p.method == FP_METHOD_FFT) {
@@ -1259,8 +1306,7 @@ void mixed_step(Ctx& c) {
       hipLaunchKernelGGL(m_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
       if (X.fr_rows)
         hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
-                           X.fr_tgt, X.fr_ent, X.fr_w, X.mf, X.fint);
-      hipLaunchKernelGGL(m_ibm_rhs, dim3(nblk(3 * n, 256)), dim3(256), 0, s, g, X.Q, X.fint, X.B, X.R);
+                           X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
     }
     MTimer t(c, "helmholtz");
     helmholtz32(c, X.R, X.Up);

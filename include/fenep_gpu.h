@@ -151,6 +151,14 @@ int fp_enable_timing(void* ctx, int on);
 int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
                     int names_len);
 
+/* Time-series statistics of all_time_dif (fenep_mpi.f90:2200-2348) of the
+ * current state, without a subtracted mean (the Log/time_same.d record) and
+ * with unit reference scales (U_C = L_C = 1, nu = 1/Re).  out[13] = step,
+ * step*dt, K, K_staggered, lambda, Re_lambda, epsilon, eta, tau_eta, v_eta,
+ * Ep, trC mean, trC std (the column order of fenep_mpi.f90:2343-2344).
+ * Reductions in FP64 on the device. */
+int fp_time_stats(void* ctx, double* out);
+
 /* ---- Sensitivities (FP64 path, periodic grids, FFT Poisson, methods 1 and
  * 2).  The reference has no derivative code: these entry points serve the
  * discrete adjoint of the step it advances (fenep.f90:2477-2527) with respect

@@ -1161,6 +1161,80 @@ int orc_diagnostics(void* vo, fp_diag* d) {
   return 0;
 }
 
+/* Time-series statistics of all_time_dif (fenep_mpi.f90:2200-2348) without
+ * a subtracted mean (the 'same' file) and with unit reference scales
+ * (U_C = L_C = 1, nu = 1/Re): out = step, step*dt, K, K_staggered, lambda,
+ * Re_lambda, epsilon, eta, tau_eta, v_eta, Ep, trC mean, trC std. */
+int orc_time_stats(void* vo, double* out) {
+  orc_t* o = (orc_t*)vo;
+  const double dx = o->dx, dy = o->dy, dz = o->dz;
+  double ur2 = 0, vr2 = 0, wr2 = 0, ug2 = 0, vg2 = 0, wg2 = 0, s2 = 0, kst = 0, ep = 0, tr = 0;
+  const double L2 = o->p.lp * o->p.lp;
+  for (int k = 1; k <= o->nz; k++)
+    for (int j = 1; j <= o->ny; j++)
+      for (int i = 1; i <= o->nx; i++) {
+        const double* U = o->U;
+        const double* V = o->V;
+        const double* W = o->W;
+        long g = GI(o, i, j, k);
+        double ur = (U[g] + U[GI(o, i - 1, j, k)]) / 2.0;
+        double vr = (V[g] + V[GI(o, i, j - 1, k)]) / 2.0;
+        double wr = (W[g] + W[GI(o, i, j, k - 1)]) / 2.0;
+        ur2 += ur * ur; vr2 += vr * vr; wr2 += wr * wr;
+        double ug = (U[g] - U[GI(o, i - 1, j, k)]) / dx;
+        double vg = (V[g] - V[GI(o, i, j - 1, k)]) / dy;
+        double wg = (W[g] - W[GI(o, i, j, k - 1)]) / dz;
+        ug2 += ug * ug; vg2 += vg * vg; wg2 += wg * wg;
+        double D[3][3];
+        D[0][0] = (U[g] - U[GI(o, i - 1, j, k)]) / dx;
+        D[0][1] = (U[GI(o, i, j + 1, k)] - U[GI(o, i, j - 1, k)] + U[GI(o, i - 1, j + 1, k)] - U[GI(o, i - 1, j - 1, k)]) / (4 * dy);
+        D[0][2] = (U[GI(o, i, j, k + 1)] - U[GI(o, i, j, k - 1)] + U[GI(o, i - 1, j, k + 1)] - U[GI(o, i - 1, j, k - 1)]) / (4 * dz);
+        D[1][0] = (V[GI(o, i + 1, j, k)] - V[GI(o, i - 1, j, k)] + V[GI(o, i + 1, j - 1, k)] - V[GI(o, i - 1, j - 1, k)]) / (4 * dx);
+        D[1][1] = (V[g] - V[GI(o, i, j - 1, k)]) / dy;
+        D[1][2] = (V[GI(o, i, j, k + 1)] - V[GI(o, i, j, k - 1)] + V[GI(o, i, j - 1, k + 1)] - V[GI(o, i, j - 1, k - 1)]) / (4 * dz);
+        D[2][0] = (W[GI(o, i + 1, j, k)] - W[GI(o, i - 1, j, k)] + W[GI(o, i + 1, j, k - 1)] - W[GI(o, i - 1, j, k - 1)]) / (4 * dx);
+        D[2][1] = (W[GI(o, i, j + 1, k)] - W[GI(o, i, j - 1, k)] + W[GI(o, i, j + 1, k - 1)] - W[GI(o, i, j - 1, k - 1)]) / (4 * dy);
+        D[2][2] = (W[g] - W[GI(o, i, j, k - 1)]) / dz;
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) {
+            double sab = D[a][b] + D[b][a];
+            s2 += sab * sab;
+          }
+        kst += U[g] * U[g] + V[g] * V[g] + W[g] * W[g];
+        double t = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
+        tr += t;
+        ep += log((L2 - 3.0) / (L2 - t));
+      }
+  const double N = (double)o->n, nu = 1.0 / o->p.re;
+  const double urms = ur2 / N, vrms = vr2 / N, wrms = wr2 / N;
+  const double ugr = ug2 / N, vgr = vg2 / N, wgr = wg2 / N;
+  const double all_rms = (urms + vrms + wrms) / 3.0, all_grad = (ugr + vgr + wgr) / 3.0;
+  const double eps = nu / 2.0 * s2 / N;
+  const double lambda = sqrt(all_rms / all_grad);
+  double mean = tr / N, var = 0;
+  for (int k = 1; k <= o->nz; k++)
+    for (int j = 1; j <= o->ny; j++)
+      for (int i = 1; i <= o->nx; i++) {
+        long g = GI(o, i, j, k);
+        double t = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
+        var += (t - mean) * (t - mean);
+      }
+  out[0] = (double)o->step;
+  out[1] = o->step * o->p.dt;
+  out[2] = (urms + vrms + wrms) / 2.0;
+  out[3] = kst / (2.0 * N);
+  out[4] = lambda;
+  out[5] = sqrt(all_rms) * lambda / nu;
+  out[6] = eps;
+  out[7] = pow(nu * nu * nu / eps, 0.25);
+  out[8] = sqrt(nu / eps);
+  out[9] = pow(nu * eps, 0.25);
+  out[10] = 0.5 * (1.0 - o->p.beta) / o->p.re / o->p.wi * (L2 - 3.0) * ep / N;
+  out[11] = mean;
+  out[12] = sqrt(var / N);
+  return 0;
+}
+
 int orc_selection_counts(void* vo, double* c4) {
   orc_t* o = (orc_t*)vo;
   for (int l = 0; l < 4; l++) c4[l] = o->sel_count[l];

@@ -49,6 +49,7 @@ int   orc_sor(void* o, const double* q, double* phi_ghosted);
 int    orc_ibm_apply(void* o, const double* up3_ghosted, double* fint3,
                      double* fm3, double* ub3);
 
+int orc_time_stats(void* o, double* out13);
 double orc_vieta(const double* a6);
 int    orc_select(const double* cm1, const double* c0, const double* cp1,
                   double* cp_out, double* cn_out);

@@ -43,6 +43,7 @@ def lib():
         L.orc_get_marker_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_diagnostics.argtypes = [C.c_void_p, C.POINTER(FpDiag)]
+        L.orc_time_stats.argtypes = [C.c_void_p, _dp]
         L.orc_selection_counts.argtypes = [C.c_void_p, _dp]
         L.orc_eval_operator.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -145,6 +146,12 @@ class Oracle:
         d = FpDiag()
         lib().orc_diagnostics(self.h, C.byref(d))
         return d.as_dict()
+
+    def time_stats(self):
+        """all_time_dif statistics (13 values, fenep_mpi.f90:2342-2344 order)."""
+        out = np.zeros(13)
+        lib().orc_time_stats(self.h, ptr(out))
+        return out
 
     def selection_counts(self):
         c = np.zeros(4)

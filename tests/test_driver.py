@@ -83,8 +83,14 @@ def test_driver_restart_round_trip_matches_oracle(tmp_path):
 def test_driver_ibm_channel_runs_and_logs_drag(tmp_path):
     nml = ("NX=64, NY=32, NZ=16, Lx=4.0, Ly=2.0, Lz=1.0, dt=0.002, Re=200, beta=0.8, Wi=1, Lp=10, method=2, "
            "walls_y=1, precision=1, body_force=0.015,0,0, ncyl=1, DC=0.5, xcyl(1)=1.0, ycyl(1)=1.0, "
-           "Nstep=20, Lstep=10, koide_step=0, dir='./chan/'")
+           "Nstep=20, Lstep=10, koide_step=0, Tstep=5, dir='./chan/'")
     r = run_driver(tmp_path, nml)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0
     assert r.stdout.count("drag:") == 2 and "Mcell-steps/s" in r.stdout
+    # Log/time_same.d as _energy.py reads it: 13 columns, one row per Tstep
+    ts = np.loadtxt(os.path.join(str(tmp_path), "chan", "Log", "time_same.d"))
+    assert ts.shape == (4, 13)
+    assert np.array_equal(ts[:, 0], [5, 10, 15, 20])
+    assert np.allclose(ts[:, 1], ts[:, 0] * 0.002)
+    assert np.all(np.isfinite(ts)) and np.all(ts[:, 3] > 0.5)   # K_staggered of a mean flow U ~ 1

@@ -183,3 +183,16 @@ def test_device_pointer_boundary_round_trip():
     r = subprocess.run([sys.executable, script], capture_output=True, text=True, timeout=240)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0 and "round trip ok" in r.stdout
+
+
+def test_time_stats_match_oracle():
+    """all_time_dif statistics reduced on the device equal the oracle's."""
+    for case in (configs.kolmogorov(n=32), configs.channel_cylinder(nx=64, ny=32, nz=16)):
+        print("time stats", case.name, flush=True)
+        o, g = po.load_case(case), api.load_case(case)
+        o.step(3)
+        g.step(3)
+        so, sg = o.time_stats(), g.time_stats()
+        np.testing.assert_allclose(sg, so, rtol=1e-12, atol=1e-300)
+        g.close()
+        o.close()
