@@ -112,7 +112,13 @@ def main():
     # dominant single-kernel phase and its algorithmic bytes per launch
     dom = max(KERNEL_WORDS, key=lambda k: kt.get(k, 0.0))
     words, extra = KERNEL_WORDS[dom]
-    dom_bytes = (words * width + extra) * cells
+    # the mixed bulk select kernel covers the cells off the wall rows (the
+    # two wall rows run in their own instance, timed separately)
+    p_ = case.params
+    units = cells
+    if prec == 1 and dom == "fenep_select" and p_.walls_y:
+        units = p_.nx * (p_.ny - 2) * p_.nz
+    dom_bytes = (words * width + extra) * units
     dom_ms = kt.get(dom, 0.0)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     traffic = read_traffic(a.precision).get(dom)
@@ -141,6 +147,7 @@ def main():
             "bound": "hbm",
             "kernel": kernel_names[dom],
             "algorithmic_bytes_per_cell": words * width + extra,
+            "cells_per_launch": units,
             "launch_ms": round(dom_ms, 4),
             "achieved": round(achieved, 2) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",

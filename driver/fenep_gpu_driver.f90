@@ -11,6 +11,8 @@
 !                   fenep.f90:1186-1233), written with 17 significant digits
 !   Log/time_same.d 13-column time series of all_time_dif every Tstep steps
 !                   (fenep_mpi.f90:2200-2348, read by _energy.py)
+!   Log/energy_same_<step>.d  shell spectrum averaged over the Tstep records
+!                   of each Gstep period (fft_energy_dif, fenep_mpi.f90:2566-2665)
 !
 ! Usage: fenep_gpu [namelist-file]      (default: fenep.nml)
 module fenep_gpu_api
@@ -52,6 +54,12 @@ module fenep_gpu_api
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(13)
+    end function
+    integer(c_int) function fp_energy_spectrum(ctx, e, nbins) bind(c, name='fp_energy_spectrum')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: e(*)
+      integer(c_int), value :: nbins
     end function
     integer(c_int) function fp_destroy(ctx) bind(c, name='fp_destroy')
       import :: c_int, c_ptr
@@ -163,6 +171,8 @@ program fenep_gpu
   real(8) :: dX, dY, dZ
   integer(8) :: t0, t1, trate
   integer(c_int) :: has_hist
+  real(8), allocatable :: Energy(:), Etmp(:)
+  integer :: n_energy = 0
 
   nml_file = 'fenep.nml'
   if (command_argument_count() >= 1) call get_command_argument(1, nml_file)
@@ -226,7 +236,13 @@ program fenep_gpu
       if (mod(step, Ostep) == 0) call write_restart()
     endif
     if (Tstep > 0) then
-      if (mod(step, Tstep) == 0) call write_time_stats()
+      if (mod(step, Tstep) == 0) then
+        call write_time_stats()
+        if (walls_y == 0) call add_energy()
+      endif
+    endif
+    if (Gstep > 0 .and. Tstep > 0) then
+      if (mod(step, Gstep) == 0 .and. n_energy > 0) call write_energy(step)
     endif
   enddo
   call system_clock(t1)
@@ -362,9 +378,39 @@ contains
     call check(fp_time_stats(ctx, ts), ctx, 'fp_time_stats')
     call execute_command_line('mkdir -p '//trim(dir)//'Log')
     open(newunit=iu, file=trim(dir)//'Log/time_same.d', position='append')
-    write(iu, *) nint(ts(1)), ts(2), ts(3), ts(4), ts(5), ts(6), ts(7), ts(8), ts(9), ts(10), ts(11), ts(12), ts(13)
+    ! one line per record (list-directed output may wrap long records)
+    write(iu, '(i10, 12es25.16e3)') nint(ts(1)), ts(2:13)
     close(iu)
   end subroutine write_time_stats
+
+  subroutine add_energy()
+    ! shell spectrum of the current state, accumulated between Gstep writes
+    ! (fft_energy_dif, fenep_mpi.f90:2566-2665)
+    if (.not. allocated(Energy)) then
+      allocate(Energy(0:NX-1), Etmp(0:NX-1))
+      Energy = 0.0d0
+    endif
+    call check(fp_energy_spectrum(ctx, Etmp, int(NX, c_int)), ctx, 'fp_energy_spectrum')
+    Energy = Energy + Etmp
+    n_energy = n_energy + 1
+  end subroutine add_energy
+
+  subroutine write_energy(s)
+    ! Log/energy_same_<step>.d: the spectrum averaged over the accumulated
+    ! records, rows k = 0..NX-1
+    integer, intent(in) :: s
+    integer :: iu, i
+    character(8) :: str
+    write(str, '(I8.8)') s
+    call execute_command_line('mkdir -p '//trim(dir)//'Log')
+    open(newunit=iu, file=trim(dir)//'Log/energy_same_'//str//'.d', status='replace')
+    do i = 0, NX - 1
+      write(iu, '(i8, es25.16e3)') i, Energy(i)/dble(n_energy)
+    enddo
+    close(iu)
+    Energy = 0.0d0
+    n_energy = 0
+  end subroutine write_energy
 
   subroutine plane_row(iu, i, j, k)
     ! one 18-column record of get_data_binary (fenep.f90:1121-1146)

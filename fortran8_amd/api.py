@@ -60,6 +60,7 @@ def lib():
         L.fp_seeded_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, _dp]
         L.fp_set_physics.argtypes = [_vp, C.c_double, C.c_double, C.c_double]
         L.fp_time_stats.argtypes = [_vp, _dp]
+        L.fp_energy_spectrum.argtypes = [_vp, _dp, C.c_int]
         L.fp_adj_record.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
         L.fp_adj_backward.argtypes = [_vp, _dp, _vp, _vp, _dp, _vp]
         _LIB = L
@@ -167,6 +168,13 @@ class FenepGPU:
         """all_time_dif statistics (13 values, fenep_mpi.f90:2343-2344 order)."""
         out = np.zeros(13)
         self._check(lib().fp_time_stats(self.h, _ptr(out)), "fp_time_stats")
+        return out
+
+    def energy_spectrum(self, nbins=None):
+        """Shell spectrum E(k), k = 0..nbins-1 (default NX)."""
+        nb = self.nx if nbins is None else int(nbins)
+        out = np.zeros(nb)
+        self._check(lib().fp_energy_spectrum(self.h, _ptr(out), nb), "fp_energy_spectrum")
         return out
 
     # ---- sensitivities (FP64 path, periodic grids)

@@ -564,6 +564,20 @@ int fp_time_stats(void* vc, double* out) {
   return FP_OK;
 }
 
+int fp_energy_spectrum(void* vc, double* E, int nbins) {
+  Ctx* c = (Ctx*)vc;
+  if (!c || !E || nbins < 1) return FP_E_ARG;
+  if (c->g.walls) return fail(c, FP_E_ARG, "the shell spectrum needs a periodic grid");
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  try {
+    if (c->mixed) mixed_store_to_f64(*c);
+    f64_energy_spectrum(*c, E, nbins);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
 // Errors of the recording/adjoint layer arrive as "E_ARG:..." or
 // "E_STATE:..." (argument / call order) or as HIP errors.
 static int adj_fail(Ctx* c, const std::string& e) {

@@ -91,6 +91,7 @@ int  f64_sor(Ctx& c, const double* q, double* phi);
 void f64_zero_c(Ctx& c);
 void f64_diag(Ctx& c, fp_diag* d);
 void f64_time_stats(Ctx& c, double* out13);  // all_time_dif statistics
+void f64_energy_spectrum(Ctx& c, double* E, int nbins);  // shell spectrum (periodic)
 void f64_fill_f(Ctx& c);     // Peterlin factor of the current C
 void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp);  // spread + ghost folds
 int thomas_lines(int ny, int bytes_per_row);  // lines per block of the LDS line solve

@@ -444,10 +444,15 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double2* __restrict__ sp
     double Qr = 0.0 / a, Qi = 0.0 / a;
     tp[l] = Pp;
     tq[l] = make_double2(Qr, Qi);
+    // the next row's right-hand side is read before the current row's
+    // results are stored (the recurrence then waits on arithmetic only)
+    double2 nxt = tq[tl + l];
     for (int r = 1; r < size; r++) {
       coeffs(r, a, b, c, rhs);
+      const double2 sv = nxt;
+      if (r + 1 < size) nxt = tq[(r + 1) * tl + l];
       double dr = 0.0, di = 0.0;
-      if (rhs) { const double2 sv = tq[r * tl + l]; dr = sv.x / nxz; di = sv.y / nxz; }
+      if (rhs) { dr = sv.x / nxz; di = sv.y / nxz; }
       const double den = a + c * Pp;
       double P, nQr, nQi;
       if (fabs(den) < 1.0e-16) { P = 0.0; nQr = 0.0; nQi = 0.0; }
@@ -457,9 +462,12 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double2* __restrict__ sp
       tq[r * tl + l] = make_double2(Qr, Qi);
     }
     double xr = Qr, xi = Qi;
+    double Pn = tp[(size - 2) * tl + l];
+    double2 qn = tq[(size - 2) * tl + l];
     for (int r = size - 2; r >= 0; r--) {
-      const double P = tp[r * tl + l];
-      const double2 q = tq[r * tl + l];
+      const double P = Pn;
+      const double2 q = qn;
+      if (r > 0) { Pn = tp[(r - 1) * tl + l]; qn = tq[(r - 1) * tl + l]; }
       xr = P * xr + q.x;
       xi = P * xi + q.y;
       tq[r * tl + l] = make_double2(xr, xi);
@@ -941,6 +949,80 @@ void f64_time_stats(Ctx& c, double* out) {
   out[10] = 0.5 * (1.0 - c.p.beta) / c.p.re / c.p.wi * (L2 - 3.0) * r[8] / N;
   out[11] = r[9] / N;
   out[12] = std::sqrt(r[TS_N + 1] / N);
+}
+
+// ===================================================== energy spectrum =====
+// fft_energy_dif (fenep_mpi.f90:2566-2665) without a subtracted mean: the
+// regular-grid velocity, its forward transform / N, |u^|^2 summed over the
+// three components into integer shells nint(|k|), half weight on the kx = 0
+// and Nyquist planes of the half spectrum.
+__global__ __launch_bounds__(256) void k_reg_velocity(Grid g, const double* __restrict__ U, const double* __restrict__ V,
+                                                      const double* __restrict__ W, double* __restrict__ out) {
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= g.n) return;
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
+  out[m] = (U[m] + ldb(U, g, i - 1, j, k, BC_U)) / 2.0;
+  out[g.n + m] = (V[m] + ldb(V, g, i, j - 1, k, BC_V)) / 2.0;
+  out[2 * g.n + m] = (W[m] + ldb(W, g, i, j, k - 1, BC_W)) / 2.0;
+}
+
+__global__ __launch_bounds__(256) void k_spec_shells(Grid g, const double2* __restrict__ spec, int nbins,
+                                                     double* __restrict__ E) {
+  extern __shared__ double bins[];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) bins[b] = 0.0;
+  __syncthreads();
+  const long total = (long)g.nxh * g.ny * g.nz;
+  const double N = (double)g.n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % g.nxh);
+    const int j = (int)((e / g.nxh) % g.ny);
+    const int k = (int)(e / ((long)g.nxh * g.ny));
+    const double kx = (double)i;
+    const double ky = (double)(j <= g.ny / 2 ? j : j - g.ny);
+    const double kz = (double)(k <= g.nz / 2 ? k : k - g.nz);
+    const int idx = (int)floor(sqrt(kx * kx + ky * ky + kz * kz) + 0.5);
+    const double2 s = spec[e];
+    const double a = (s.x / N) * (s.x / N) + (s.y / N) * (s.y / N);
+    const double w = (i == 0 || i == g.nxh - 1) ? 0.5 : 1.0;
+    if (idx < nbins) atomicAdd(&bins[idx], a * w);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (bins[b] != 0.0) atomicAdd(&E[b], bins[b]);
+}
+
+void f64_energy_spectrum(Ctx& c, double* E, int nbins) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  double* dE = nullptr;
+  FP_HIP_CHECK(hipMalloc((void**)&dE, sizeof(double) * nbins));
+  try {
+    zero_words(dE, 2L * nbins, s);
+    hipLaunchKernelGGL(k_reg_velocity, dim3(nblk(g.n, 256)), dim3(256), 0, s, g, c.U, c.V, c.W, c.Q);
+    const int lx = lines_for(g.nx);
+    const long xlines = (long)g.ny * g.nz;
+    const size_t shx = sizeof(double2) * 2 * lx * g.nx;
+    const int tiy = std::max(1, std::min(16, 2048 / g.ny)), tiz = std::max(1, std::min(16, 2048 / g.nz));
+    const int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
+    const size_t shy = sizeof(double2) * 2 * tiy * g.ny, shz = sizeof(double2) * 2 * tiz * g.nz;
+    const long total = (long)g.nxh * g.ny * g.nz;
+    const int sb = (int)std::min<long>(1024, (total + 255) / 256);
+    for (int comp = 0; comp < 3; comp++) {
+      hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, c.Q + comp * g.n, c.spec, c.twx, lx);
+      hipLaunchKernelGGL(k_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, c.spec, c.twy, 1, 0, 3, 0, 0, tiy, c.kx2, c.ky2,
+                         c.kz2);
+      hipLaunchKernelGGL(k_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, c.spec, c.twz, 2, 0, 3, 0, g.ny, tiz, c.kx2,
+                         c.ky2, c.kz2);
+      hipLaunchKernelGGL(k_spec_shells, dim3(sb), dim3(256), sizeof(double) * nbins, s, g, c.spec, nbins, dE);
+    }
+    FP_HIP_CHECK(hipMemcpyAsync(E, dE, sizeof(double) * nbins, hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(dE);
+    throw;
+  }
+  (void)hipFree(dE);
 }
 
 // ===================================================== diagnostics ========

@@ -53,7 +53,8 @@ struct Mixed {
   float* ftmp = nullptr;          // 3 ghosted raw spread arrays
   float* sw[3] = {nullptr, nullptr, nullptr};  // CSR entry weights
   // spreading folded onto interior targets (all three components)
-  int* fr_row = nullptr;
+  int* fr_row = nullptr;          // sliced ELL: offset of each 64-row slice
+  int* fr_len = nullptr;          // entries per row of each slice (padded)
   long* fr_tgt = nullptr;         // c*n + interior index
   int* fr_ent = nullptr;          // c*nmk + marker
   float* fr_w = nullptr;
@@ -177,12 +178,30 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
 // GEN = true covers the 2R rows next to the walls through the boundary-rule
 // accessor.  Separate kernels keep the bulk kernel's register budget, and so
 // its occupancy, free of the wall path.
+//
+// The bulk instance walks the cells in band order: x fastest, then the 16
+// rows of a y-band, then z, then the bands.  With the XCD-contiguous block
+// numbering each XCD sweeps whole bands plane by plane, so the planes k-2..k+2
+// a stencil reads stay in that XCD's L2 (a plane-major walk keeps five full
+// planes of every field live, more than one L2 holds).
+__device__ __forceinline__ void band_cell(const Grid& g, long L, long& m, int& i, int& j, int& k) {
+  const int lb = g.lny < 4 ? g.lny : 4;
+  i = (int)(L & (g.nx - 1));
+  long t = L >> g.lnx;
+  const int jj = (int)(t & ((1 << lb) - 1));
+  t >>= lb;
+  const int lnz = __ffs(g.nz) - 1;
+  k = (int)(t & (g.nz - 1));
+  j = ((int)(t >> lnz) << lb) + jj;
+  m = cidx(g, i, j, k);
+}
+
 template <bool GEN, int R>
 __device__ __forceinline__ bool row_cell(const Grid& g, long& m, int& i, int& j, int& k) {
   if (!GEN) {
-    m = xcd_cell();
-    if (m >= g.n) return false;
-    cell_ijk(g, m, i, j, k);
+    const long L = xcd_cell();
+    if (L >= g.n) return false;
+    band_cell(g, L, m, i, j, k);
     return !g.walls || (j >= R && j < g.ny - R);
   }
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -206,7 +225,7 @@ __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict_
   int i, j, k;
   if (!row_cell<GEN, 1>(g, m, i, j, k)) return;
   uint8_t s = 0;
-#pragma unroll 1
+#pragma unroll
   for (int d = 0; d < 3; d++) {
     float v[3][6];
     load_line<1, GEN>(C, g, m, i, j, k, d, v);
@@ -581,21 +600,26 @@ __global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first
 }
 
 // ================================================================== FFT ===
-__device__ float2* stockham_f(float2* x, float2* y, int n, int logn, int nl, const float2* __restrict__ tw, bool inv) {
+// Radix-2 Stockham autosort (forward e^{-2 pi i jk/n}, unnormalised; the
+// inverse uses the conjugate twiddles) over nl lines held in LDS with line
+// pitch `pitch` (>= n, padded where lines are staged column-wise so that the
+// staging is free of bank conflicts).  tws: the n/2 twiddles (cos, -sin)(2 pi
+// j/n), already in LDS.  Returns the buffer holding the result.
+__device__ float2* stockham_f(float2* x, float2* y, int n, int logn, int nl, int pitch, const float2* tws, bool inv) {
   int len = n, s = 1, lgs = 0;
   const int half = n >> 1;
   while (len > 1) {
-    int m = len >> 1;
+    const int m = len >> 1;
     for (int b = threadIdx.x; b < nl * half; b += blockDim.x) {
-      int line = b >> (logn - 1);
-      int t = b & (half - 1);
-      int p = t >> lgs, q = t & (s - 1);
-      float2 w = tw[p * s];
-      float wr = w.x, wi = inv ? -w.y : w.y;
-      const float2* xl = x + line * n;
-      float2* yl = y + line * n;
-      float2 a = xl[q + s * p], bb = xl[q + s * (p + m)];
-      float dr = a.x - bb.x, di = a.y - bb.y;
+      const int line = b >> (logn - 1);
+      const int t = b & (half - 1);
+      const int p = t >> lgs, q = t & (s - 1);
+      const float2 w = tws[p * s];
+      const float wr = w.x, wi = inv ? -w.y : w.y;
+      const float2* xl = x + line * pitch;
+      float2* yl = y + line * pitch;
+      const float2 a = xl[q + s * p], bb = xl[q + s * (p + m)];
+      const float dr = a.x - bb.x, di = a.y - bb.y;
       yl[q + s * (2 * p)] = make_float2(a.x + bb.x, a.y + bb.y);
       yl[q + s * (2 * p + 1)] = make_float2(dr * wr - di * wi, dr * wi + di * wr);
     }
@@ -608,62 +632,84 @@ __device__ float2* stockham_f(float2* x, float2* y, int n, int logn, int nl, con
 
 __device__ __forceinline__ int ilog2f(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
 
+__device__ __forceinline__ void load_twiddles(float2* tws, const float2* __restrict__ tw, int n) {
+  for (int e = threadIdx.x; e < n / 2; e += blockDim.x) tws[e] = tw[e];
+}
+
+// Real x-lines -> half-spectrum rows, two real lines per complex transform:
+// z = a + i b, A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i.
+// nl (even) real lines per block.
 __global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restrict__ q, float2* __restrict__ spec,
                                                    const float2* __restrict__ tw, int nl, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
-  const int n = g.nx;
+  const int n = g.nx, nc = nl / 2, nxh = g.nxh;
   float2* x = ldsf;
-  float2* y = ldsf + nl * n;
-  long nlines = (long)g.ny * g.nz;
-  long l0 = (long)blockIdx.x * nl;
-  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
-    int l = e / n, i = e % n;
-    long L = l0 + l;
-    x[e] = make_float2(L < nlines ? q[L * n + i] : 0.0f, 0.0f);
+  float2* y = ldsf + nc * n;
+  float2* tws = ldsf + 2 * nc * n;
+  load_twiddles(tws, tw, n);
+  const long nlines = (long)g.ny * g.nz;
+  const long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
+    const int c = e / n, i = e - c * n;
+    const long La = l0 + 2 * c, Lb = La + 1;
+    x[e] = make_float2(La < nlines ? q[La * n + i] : 0.0f, Lb < nlines ? q[Lb * n + i] : 0.0f);
   }
   __syncthreads();
-  float2* r = stockham_f(x, y, n, ilog2f(n), nl, tw, false);
-  const int nxh = g.nxh;
-  for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
-    int l = e / nxh, i = e % nxh;
-    long L = l0 + l;
-    if (L >= nlines) continue;
-    long j = L % g.ny, k = L / g.ny;
-    spec[i + (long)nxh * ((j + g.jo) + g.sy * k)] = r[l * n + i];
+  const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, false);
+  for (int e = threadIdx.x; e < nc * nxh; e += blockDim.x) {
+    const int c = e / nxh, k = e - c * nxh;
+    const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
+    const long La = l0 + 2 * c, Lb = La + 1;
+    if (La < nlines) {
+      const long j = La % g.ny, kk = La / g.ny;
+      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
+    }
+    if (Lb < nlines) {
+      const long j = Lb % g.ny, kk = Lb / g.ny;
+      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
+    }
   }
 }
 
+// Half-spectrum rows -> real x-lines (unnormalised), two lines per complex
+// transform: Z = A + i B over the full Hermitian spectra; a = Re z, b = Im z.
 __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restrict__ spec, float* __restrict__ out,
                                                    const float2* __restrict__ tw, int nl, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
-  const int n = g.nx, h = n / 2, nxh = g.nxh;
+  const int n = g.nx, h = n / 2, nxh = g.nxh, nc = nl / 2;
   float2* x = ldsf;
-  float2* y = ldsf + nl * n;
-  long nlines = (long)g.ny * g.nz;
-  long l0 = (long)blockIdx.x * nl;
-  for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
-    int l = e / nxh, i = e % nxh;
-    long L = l0 + l;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (L < nlines) {
-      long j = L % g.ny, k = L / g.ny;
-      v = spec[i + (long)nxh * ((j + g.jo) + g.sy * k)];
-    }
-    if (i == 0 || i == h) v.y = 0.0f;
-    x[l * n + i] = v;
-    if (i > 0 && i < h) x[l * n + (n - i)] = make_float2(v.x, -v.y);
+  float2* y = ldsf + nc * n;
+  float2* tws = ldsf + 2 * nc * n;
+  load_twiddles(tws, tw, n);
+  const long nlines = (long)g.ny * g.nz;
+  const long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
+    const int c = e / n, i = e - c * n;
+    const long La = l0 + 2 * c, Lb = La + 1;
+    const int src = i <= h ? i : n - i;
+    float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
+    if (La < nlines) A = spec[src + (long)nxh * ((La % g.ny + g.jo) + g.sy * (La / g.ny))];
+    if (Lb < nlines) B = spec[src + (long)nxh * ((Lb % g.ny + g.jo) + g.sy * (Lb / g.ny))];
+    if (src == 0 || src == h) { A.y = 0.0f; B.y = 0.0f; }
+    if (i > h) { A.y = -A.y; B.y = -B.y; }
+    x[e] = make_float2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  float2* r = stockham_f(x, y, n, ilog2f(n), nl, tw, true);
-  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
-    int l = e / n, i = e % n;
-    long L = l0 + l;
-    if (L < nlines) out[L * n + i] = r[e].x;
+  const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, true);
+  for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
+    const int c = e / n, i = e - c * n;
+    const long La = l0 + 2 * c, Lb = La + 1;
+    const float2 z = r[e];
+    if (La < nlines) out[La * n + i] = z.x;
+    if (Lb < nlines) out[Lb * n + i] = z.y;
   }
 }
 
+// y or z transforms of ti consecutive kx columns; lines are staged
+// column-wise into LDS with pitch n + 1.  mode 0 forward, 1 inverse, 2
+// forward + spectral divide (Helmholtz or Poisson symbol) + inverse.
 __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spec, const float2* __restrict__ tw, int axis,
                                                 int mode, int kind, int row0, int ti, const float* __restrict__ kx2,
                                                 const float* __restrict__ ky2, const float* __restrict__ kz2,
@@ -671,44 +717,47 @@ __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spe
   if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
   const int n = axis == 1 ? g.ny : g.nz;
+  const int pitch = n + 1;
   const int nxh = g.nxh;
   const long stride = axis == 1 ? (long)nxh : (long)nxh * g.sy;
   const int ntile = (nxh + ti - 1) / ti;
-  int tile = blockIdx.x % ntile;
-  long other = blockIdx.x / ntile;
-  int i0 = tile * ti;
+  const int tile = blockIdx.x % ntile;
+  const long other = blockIdx.x / ntile;
+  const int i0 = tile * ti;
   float2* base = axis == 1 ? spec + (long)nxh * (g.jo + g.sy * other) : spec + (long)nxh * (row0 + other);
   float2* x = ldsf;
-  float2* y = ldsf + ti * n;
+  float2* y = ldsf + ti * pitch;
+  float2* tws = ldsf + 2 * ti * pitch;
+  load_twiddles(tws, tw, n);
   for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
-    int l = e % ti, r = e / ti;
-    int i = i0 + l;
-    x[l * n + r] = i < nxh ? base[i + stride * r] : make_float2(0.0f, 0.0f);
+    const int l = e % ti, r = e / ti;
+    const int i = i0 + l;
+    x[l * pitch + r] = i < nxh ? base[i + stride * r] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
-  int logn = ilog2f(n);
-  float2* res = stockham_f(x, y, n, logn, ti, tw, mode == 1);
+  const int logn = ilog2f(n);
+  float2* res = stockham_f(x, y, n, logn, ti, pitch, tws, mode == 1);
   if (mode == 2) {
     float2* other_buf = res == x ? y : x;
     const float intot = (float)(1.0 / ((double)g.nx * g.ny * g.nz));
     const float coef = (float)(((g.dt * g.beta) / g.re) / 2.0);
     for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
-      int l = e / n, kk = e % n;
-      int i = i0 + l;
+      const int l = e / n, kk = e - l * n;
+      const int i = i0 + l;
       if (i >= nxh) continue;
-      long jj = other;
-      float lhs = kind == 3 ? -((kx2[i] + ky2[jj]) + kz2[kk]) : 1.0f + coef * ((kx2[i] + ky2[jj]) + kz2[kk]);
-      float2 s = res[e];
-      float sc = fabsf(lhs) < 1.0e-16f ? 0.0f : intot / lhs;
-      res[e] = make_float2(s.x * sc, s.y * sc);
+      const long jj = other;
+      const float lhs = kind == 3 ? -((kx2[i] + ky2[jj]) + kz2[kk]) : 1.0f + coef * ((kx2[i] + ky2[jj]) + kz2[kk]);
+      const float2 sv = res[l * pitch + kk];
+      const float sc = fabsf(lhs) < 1.0e-16f ? 0.0f : intot / lhs;
+      res[l * pitch + kk] = make_float2(sv.x * sc, sv.y * sc);
     }
     __syncthreads();
-    res = stockham_f(res, other_buf, n, logn, ti, tw, true);
+    res = stockham_f(res, other_buf, n, logn, ti, pitch, tws, true);
   }
   for (int e = threadIdx.x; e < ti * n; e += blockDim.x) {
-    int l = e % ti, r = e / ti;
-    int i = i0 + l;
-    if (i < nxh) base[i + stride * r] = res[l * n + r];
+    const int l = e % ti, r = e / ti;
+    const int i = i0 + l;
+    if (i < nxh) base[i + stride * r] = res[l * pitch + r];
   }
 }
 
@@ -758,20 +807,28 @@ __global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spe
     float Pp = -b / a, Qr = 0.0f, Qi = 0.0f;
     sp[l] = Pp;
     sq[l] = make_float2(0.0f, 0.0f);
+    // prefetch the next row before storing the current one: the recurrence
+    // then waits on arithmetic only
+    float2 nxt = sq[tl + l];
     for (int r = 1; r < size; r++) {
       coeffs(r, a, b, c, rhs);
+      const float2 sv = nxt;
+      if (r + 1 < size) nxt = sq[(r + 1) * tl + l];
       float dr = 0.0f, di = 0.0f;
-      if (rhs) { const float2 sv = sq[r * tl + l]; dr = sv.x * inxz; di = sv.y * inxz; }
+      if (rhs) { dr = sv.x * inxz; di = sv.y * inxz; }
       const float den = a + c * Pp;
       if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
-      else { const float id = 1.0f / den; Pp = -b * id; Qr = (dr - c * Qr) * id; Qi = (di - c * Qi) * id; }
+      else { const float id = __builtin_amdgcn_rcpf(den); Pp = -b * id; Qr = (dr - c * Qr) * id; Qi = (di - c * Qi) * id; }
       sp[r * tl + l] = Pp;
       sq[r * tl + l] = make_float2(Qr, Qi);
     }
     float xr = Qr, xi = Qi;
+    float Pn = sp[(size - 2) * tl + l];
+    float2 qn = sq[(size - 2) * tl + l];
     for (int r = size - 2; r >= 0; r--) {
-      const float P = sp[r * tl + l];
-      const float2 q = sq[r * tl + l];
+      const float P = Pn;
+      const float2 q = qn;
+      if (r > 0) { Pn = sp[(r - 1) * tl + l]; qn = sq[(r - 1) * tl + l]; }
       xr = P * xr + q.x;
       xi = P * xi + q.y;
       sq[r * tl + l] = make_float2(xr, xi);
@@ -784,7 +841,11 @@ __global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spe
   }
 }
 
-static int lines_for32(int n) { return std::max(1, std::min(32, 4096 / n)); }
+// real x-lines per block (even: the x transforms pack two real lines per
+// complex line); LDS 8 B * n * lines + n/2 twiddles
+static int lines_for32(int n) { return std::max(2, std::min(16, 4096 / n)) & ~1; }
+// kx columns per block of the y/z transforms (pitch n + 1)
+static int cols_for32(int n) { return std::max(1, std::min(16, 3968 / (n + 1))); }
 
 static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip = nullptr) {
   Mixed& X = MX(c);
@@ -792,11 +853,12 @@ static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip 
   hipStream_t s = c.stream;
   int lx = lines_for32(g.nx);
   long xlines = (long)g.ny * g.nz;
-  size_t shx = sizeof(float2) * 2 * lx * g.nx;
+  size_t shx = sizeof(float2) * ((size_t)lx * g.nx + g.nx / 2);
   hipLaunchKernelGGL(m_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, q, X.spec, X.twx, lx, skip);
-  int tiy = std::max(1, std::min(32, 4096 / g.ny)), tiz = std::max(1, std::min(32, 4096 / g.nz));
+  int tiy = cols_for32(g.ny), tiz = cols_for32(g.nz);
   int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
-  size_t shy = sizeof(float2) * 2 * tiy * g.ny, shz = sizeof(float2) * 2 * tiz * g.nz;
+  size_t shy = sizeof(float2) * (2 * (size_t)tiy * (g.ny + 1) + g.ny / 2);
+  size_t shz = sizeof(float2) * (2 * (size_t)tiz * (g.nz + 1) + g.nz / 2);
   if (!g.walls) {
     hipLaunchKernelGGL(m_fft_yz, dim3(nty * g.nz), dim3(256), shy, s, g, X.spec, X.twy, 1, 0, kind, 0, tiy, X.kx2, X.ky2, X.kz2, skip);
     hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 2, kind, 0, tiz, X.kx2, X.ky2, X.kz2, skip);
@@ -965,11 +1027,13 @@ __global__ void m_ibm_setup_interp(Grid g, long nmk, int nc, int nl, const doubl
     if (c == 0) { i0 = (int)(xc / g.dx - 0.5); j0 = (int)(yc / g.dy); }
     else if (c == 1) { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy - 0.5); }
     else { i0 = (int)(xc / g.dx); j0 = (int)(yc / g.dy); }
-    org[id * 3 + c] = make_int4(i0, j0, n, 0);
+    org[c * nmk + id] = make_int4(i0, j0, n, 0);
     int s = 0;
+    // slot-major: weight s of (component c, marker id) at (c*27 + s)*nmk + id
     for (int kk = n - 1; kk <= n + 1; kk++)
       for (int jj = j0; jj <= j0 + 2; jj++)
-        for (int ii = i0; ii <= i0 + 2; ii++, s++) wi[(id * 3 + c) * 27 + s] = (float)ibm_w64(g, c, ii, jj, kk, xc, yc, zc);
+        for (int ii = i0; ii <= i0 + 2; ii++, s++)
+          wi[(long)(c * 27 + s) * nmk + id] = (float)ibm_w64(g, c, ii, jj, kk, xc, yc, zc);
     uc[c * nmk + id] = (float)mk[(3 + c) * nmk + id];
   }
 }
@@ -988,41 +1052,51 @@ __global__ void m_ibm_setup_spread(Grid g, int comp, int rows, const int* __rest
   }
 }
 
+// interpolation of Ua at the markers and the direct forcing Fc = (Uc - Ub)/dt
+// (ibm_Helmholtz, fenep.f90:2076-2119), one thread per (component, marker)
 __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const float* __restrict__ wi, const int4* __restrict__ org,
                                                     const float* __restrict__ uc, const float* __restrict__ ua,
                                                     float* __restrict__ mf) {
-  long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (id >= nmk) return;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 3 * nmk) return;
+  const int c = (int)(t / nmk);
+  const long id = t - c * nmk;
   const float idt = (float)(1.0 / g.dt);
-#pragma unroll 1
-  for (int c = 0; c < 3; c++) {
-    int4 o = org[id * 3 + c];
-    const float* X = ua + c * g.n;
-    int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
-    const float* w = wi + (id * 3 + c) * 27;
-    float acc = 0.0f;
-    int s = 0;
+  const int4 o = org[t];
+  const float* X = ua + c * g.n;
+  const int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+  const float* w = wi + (long)c * 27 * nmk + id;
+  float acc = 0.0f;
+  int s = 0;
 #pragma unroll
-    for (int kk = 0; kk < 3; kk++)
+  for (int kk = 0; kk < 3; kk++)
 #pragma unroll
-      for (int jj = 0; jj < 3; jj++)
+    for (int jj = 0; jj < 3; jj++)
 #pragma unroll
-        for (int ii = 0; ii < 3; ii++, s++)
-          acc += w[s] * ldb(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
-    mf[c * nmk + id] = (uc[c * nmk + id] - acc) * idt;
-  }
+      for (int ii = 0; ii < 3; ii++, s++)
+        acc += w[(long)s * nmk] * ldb(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
+  mf[t] = (uc[t] - acc) * idt;
 }
 
 // spreading gathered onto interior targets with the ghost images folded in
 // (ibm_PBM, fenep.f90:2195-2205), FP32 accumulation
-__global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* __restrict__ rowp, const long* __restrict__ tgt,
+__global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* __restrict__ slice_off,
+                                                           const int* __restrict__ slice_len, const long* __restrict__ tgt,
                                                            const int* __restrict__ ent, const float* __restrict__ w,
                                                            const float* __restrict__ mf, float dt,
                                                            float* __restrict__ rhs) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  // sliced ELL: the rows of a 64-row slice store their entries slot-major
+  // (entry k of lane l at off + 64 k + l), padded with zero weights, so each
+  // wave reads its entries with contiguous loads
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
+  const int sl = r >> 6, lane = r & 63;
+  const int off = slice_off[sl], len = slice_len[sl];
   float acc = 0.0f;
-  for (int e = rowp[r]; e < rowp[r + 1]; e++) acc += mf[ent[e]] * w[e];
+  for (int k = 0; k < len; k++) {
+    const int e = off + 64 * k + lane;
+    acc += mf[ent[e]] * w[e];
+  }
   rhs[tgt[r]] += dt * acc;   // each target appears in one row only
 }
 
@@ -1102,7 +1176,7 @@ void mixed_destroy(Ctx& c) {
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
-  dfree(X->fr_row); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
+  dfree(X->fr_row); dfree(X->fr_len); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
   delete X;
   c.mixed = nullptr;
 }
@@ -1200,16 +1274,38 @@ void mixed_set_markers(Ctx& c) {
     for (auto& pr : kv.second) { fent.push_back(pr.first); fw.push_back(pr.second); }
     frow.push_back((int)fent.size());
   }
-  dfree(X.fr_row); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w);
-  X.fr_rows = (int)ftgt.size();
-  dalloc(&X.fr_row, frow.size()); dalloc(&X.fr_tgt, std::max<size_t>(1, ftgt.size()));
-  dalloc(&X.fr_ent, std::max<size_t>(1, fent.size())); dalloc(&X.fr_w, std::max<size_t>(1, fw.size()));
-  FP_HIP_CHECK(hipMemcpy(X.fr_row, frow.data(), sizeof(int) * frow.size(), hipMemcpyHostToDevice));
-  if (!ftgt.empty()) {
-    FP_HIP_CHECK(hipMemcpy(X.fr_tgt, ftgt.data(), sizeof(long) * ftgt.size(), hipMemcpyHostToDevice));
-    FP_HIP_CHECK(hipMemcpy(X.fr_ent, fent.data(), sizeof(int) * fent.size(), hipMemcpyHostToDevice));
-    FP_HIP_CHECK(hipMemcpy(X.fr_w, fw.data(), sizeof(float) * fw.size(), hipMemcpyHostToDevice));
+  // CSR -> sliced ELL (64-row slices, slot-major, zero-weight padding)
+  const int nrows = (int)ftgt.size();
+  const int nsl = (nrows + 63) / 64;
+  std::vector<int> soff(std::max(1, nsl)), slen(std::max(1, nsl));
+  long total = 0;
+  for (int sl = 0; sl < nsl; sl++) {
+    int mx = 0;
+    for (int r = sl * 64; r < std::min(nrows, sl * 64 + 64); r++) mx = std::max(mx, frow[r + 1] - frow[r]);
+    soff[sl] = (int)total;
+    slen[sl] = mx;
+    total += 64L * mx;
   }
+  std::vector<int> eent(std::max<long>(1, total), 0);
+  std::vector<float> ew(std::max<long>(1, total), 0.0f);
+  for (int r = 0; r < nrows; r++) {
+    const int sl = r >> 6, lane = r & 63;
+    for (int e = frow[r]; e < frow[r + 1]; e++) {
+      const long pos = soff[sl] + 64L * (e - frow[r]) + lane;
+      eent[pos] = fent[e];
+      ew[pos] = fw[e];
+    }
+  }
+  dfree(X.fr_row); dfree(X.fr_len); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w);
+  X.fr_rows = nrows;
+  dalloc(&X.fr_row, soff.size()); dalloc(&X.fr_len, slen.size()); dalloc(&X.fr_tgt, std::max<size_t>(1, ftgt.size()));
+  dalloc(&X.fr_ent, eent.size()); dalloc(&X.fr_w, ew.size());
+  FP_HIP_CHECK(hipMemcpy(X.fr_row, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice));
+  FP_HIP_CHECK(hipMemcpy(X.fr_len, slen.data(), sizeof(int) * slen.size(), hipMemcpyHostToDevice));
+  FP_HIP_CHECK(hipMemcpy(X.fr_ent, eent.data(), sizeof(int) * eent.size(), hipMemcpyHostToDevice));
+  FP_HIP_CHECK(hipMemcpy(X.fr_w, ew.data(), sizeof(float) * ew.size(), hipMemcpyHostToDevice));
+  if (!ftgt.empty())
+    FP_HIP_CHECK(hipMemcpy(X.fr_tgt, ftgt.data(), sizeof(long) * ftgt.size(), hipMemcpyHostToDevice));
 }
 
 const double* mixed_marker_forces(Ctx& c) {
@@ -1266,9 +1362,11 @@ void mixed_step(Ctx& c) {
       MTimer t(c, "fenep_select");
       hipLaunchKernelGGL(m_select<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list,
                          X.amb_cap);
-      if (g.walls)
-        hipLaunchKernelGGL(m_select<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb,
-                           X.amb_list, X.amb_cap);
+    }
+    if (g.walls) {
+      MTimer t(c, "fenep_select_walls");
+      hipLaunchKernelGGL(m_select<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb,
+                         X.amb_list, X.amb_cap);
     }
     {
       MTimer t(c, "fenep_select_fp64_fix");
@@ -1303,10 +1401,10 @@ void mixed_step(Ctx& c) {
   } else {
     {
       MTimer t(c, "ibm");
-      hipLaunchKernelGGL(m_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
+      hipLaunchKernelGGL(m_ibm_interp, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s, g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
       if (X.fr_rows)
         hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
-                           X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
+                           X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
     }
     MTimer t(c, "helmholtz");
     helmholtz32(c, X.R, X.Up);

@@ -159,6 +159,14 @@ int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
  * Reductions in FP64 on the device. */
 int fp_time_stats(void* ctx, double* out);
 
+/* Shell-summed kinetic-energy spectrum of the regular-grid velocity
+ * (fft_energy_dif, fenep_mpi.f90:2566-2665, no subtracted mean): forward FFT
+ * / (NX NY NZ), |u^|^2 of the three components summed into shells
+ * nint(|k|) with half weight on the kx = 0 and Nyquist planes.  E[0..nbins)
+ * receives shells 0..nbins-1 (the reference writes 0..NX-1); periodic grids
+ * only.  sum(E) over all shells equals K of fp_time_stats (Parseval). */
+int fp_energy_spectrum(void* ctx, double* E, int nbins);
+
 /* ---- Sensitivities (FP64 path, periodic grids, FFT Poisson, methods 1 and
  * 2).  The reference has no derivative code: these entry points serve the
  * discrete adjoint of the step it advances (fenep.f90:2477-2527) with respect

@@ -153,6 +153,29 @@ class Oracle:
         lib().orc_time_stats(self.h, ptr(out))
         return out
 
+    def energy_spectrum(self, nbins=None):
+        """Shell spectrum of the regular-grid velocity (fft_energy_dif,
+        fenep_mpi.f90:2566-2665, no subtracted mean), with numpy's FFT as the
+        transform: |u^/N|^2 summed into shells nint(|k|), half weight on the
+        kx = 0 and Nyquist planes of the half spectrum."""
+        st = self.get_state()
+        nz, ny, nx = st.U.shape[0] - 2, st.U.shape[1] - 2, st.U.shape[2] - 2
+        N = nx * ny * nz
+        I = (slice(1, -1), slice(1, -1), slice(1, -1))
+        ureg = (st.U[1:-1, 1:-1, 1:-1] + st.U[1:-1, 1:-1, :-2]) / 2.0
+        vreg = (st.V[1:-1, 1:-1, 1:-1] + st.V[1:-1, :-2, 1:-1]) / 2.0
+        wreg = (st.W[1:-1, 1:-1, 1:-1] + st.W[:-2, 1:-1, 1:-1]) / 2.0
+        e = sum(np.abs(np.fft.rfftn(a) / N) ** 2 for a in (ureg, vreg, wreg))
+        kx = np.arange(nx // 2 + 1, dtype=float)[None, None, :]
+        ky = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny).astype(float)[None, :, None]
+        kz = np.where(np.arange(nz) <= nz // 2, np.arange(nz), np.arange(nz) - nz).astype(float)[:, None, None]
+        idx = np.floor(np.sqrt(kx ** 2 + ky ** 2 + kz ** 2) + 0.5).astype(int)
+        w = np.ones(nx // 2 + 1)
+        w[0] = w[-1] = 0.5
+        nb = nx if nbins is None else int(nbins)
+        E = np.bincount(idx.ravel(), weights=(e * w[None, None, :]).ravel(), minlength=idx.max() + 1)
+        return E[:nb] if len(E) >= nb else np.pad(E, (0, nb - len(E)))
+
     def selection_counts(self):
         c = np.zeros(4)
         lib().orc_selection_counts(self.h, ptr(c))

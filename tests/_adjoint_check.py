@@ -47,8 +47,8 @@ def make_state(x, b0):
     return State(pad_periodic(x[0]), pad_periodic(x[1]), pad_periodic(x[2]), pad_periodic(x[3]), C, hist, 1)
 
 
-def small_case():
-    case = configs.adjoint_case(nx=32, ny=32, nz=16)
+def small_case(**kw):
+    case = configs.adjoint_case(nx=32, ny=32, nz=16, **kw)
     g = api.load_case(case)
     g.step(4)   # developed histories and flow
     st = g.get_state()
@@ -84,20 +84,36 @@ def check_vjp():
         out["fd_h%g" % h] = fd
         out["rel_h%g" % h] = abs(fd - adj) / abs(adj)
     out["adjoint"] = adj
-    # per-block attribution: only the velocity planes / only C / only histories
-    for name, sl in (("uvwp", slice(0, 4)), ("c", slice(4, 10)), ("hist", slice(10, 16))):
+    # per-block attribution: only the velocity planes / only C / only the
+    # histories (which enter linearly: a large step is exact)
+    for name, sl, h in (("uvwp", slice(0, 4), 1e-5), ("c", slice(4, 10), 1e-5), ("hist", slice(10, 16), 1e-2)):
         wm = np.zeros_like(w)
         wm[sl] = w[sl]
         a = float(np.dot(bi.cpu().numpy(), wm.reshape(-1)))
-        h = 1e-6
         fd = (J(x0 + h * wm) - J(x0 - h * wm)) / (2 * h)
         out["rel_" + name] = abs(fd - a) / max(abs(a), 1e-300)
     print(json.dumps(out), flush=True)
 
 
+def central(J, h):
+    return (J(h) - J(-h)) / (2 * h)
+
+
+def richardson(J, h):
+    """central difference with the h^2 term removed: (4 D(h/2) - D(h)) / 3"""
+    return (4 * central(J, h / 2) - central(J, h)) / 3
+
+
+def stretch(x0, trc):
+    """conformation stretched along x to tr C ~ trc (L^2 then matters)"""
+    x = x0.copy()
+    x[4] += trc - 3.0
+    return x
+
+
 def check_params():
-    case, g, st = small_case()
-    x0 = flat_state(st)
+    case, g, st = small_case(newton_eps=1e-12, newton_itrmax=50)
+    x0 = stretch(flat_state(st), 30.0)
     b0 = st.hist[6:9].copy()
     nsteps, every = 12, 5
     rng = np.random.default_rng(11)
@@ -122,23 +138,25 @@ def check_params():
     out = {"check": "params", "J": J0}
     names = ("wi", "beta", "L2")
     for q in range(3):
-        h = 1e-6 * base[q]
-        hi = list(base); lo = list(base)
-        hi[q] += h; lo[q] -= h
-        fd = (run(*hi, fx, True) - run(*lo, fx, True)) / (2 * h)
+        def Jq(t):
+            x = list(base)
+            x[q] += t
+            return run(*x, fx, True)
         out["d" + names[q]] = grads[q]
+        fd6 = central(Jq, 1e-6 * base[q])
+        out["rel_%s_h1e-06" % names[q]] = abs(fd6 - grads[q]) / abs(grads[q])
+        fd = richardson(Jq, 1e-4 * base[q])
         out["fd_" + names[q]] = fd
         out["rel_" + names[q]] = abs(fd - grads[q]) / abs(grads[q])
-    h = 1e-6
-    fd = (run(*base, fx + h * dF, True) - run(*base, fx - h * dF, True)) / (2 * h)
+    fd = richardson(lambda t: run(*base, fx + t * dF, True), 1e-4)
     out["dF_dir"] = dirF
     out["fd_F_dir"] = fd
     out["rel_F"] = abs(fd - dirF) / abs(dirF)
     print(json.dumps(out), flush=True)
 
 
-def check_cfg5():
-    case = configs.adjoint_case()          # 64x64x32, Lz = pi
+def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000):
+    case = configs.adjoint_case(newton_eps=newton_eps, newton_itrmax=itrmax)   # 64x64x32, Lz = pi
     nsteps, every, last = 400, 20, 200
     p = case.params
     fx = np.stack([np.asarray(a) for a in case.force])
@@ -154,38 +172,49 @@ def check_cfg5():
     J.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    # plain forward of the same steps (no recording) for the cost ratio
-    g = run.g
-    g.set_physics(p.wi, p.beta, p.lp)
-    g.set_state(case.state)
-    g.set_body_force([fx[0].copy(), fx[1].copy(), fx[2].copy()])
-    t3 = time.perf_counter()
-    g.step(nsteps)
-    t4 = time.perf_counter()
-    out = {"check": "cfg5", "J": float(J), "t_record_s": t1 - t0, "t_backward_s": t2 - t1,
-           "t_forward_s": t4 - t3, "ratio_backward_forward": (t2 - t1) / (t4 - t3),
-           "ratio_record_forward": (t1 - t0) / (t4 - t3)}
+    if timing:
+        # plain forward of the same steps (no recording) for the cost ratio
+        g = run.g
+        g.set_physics(p.wi, p.beta, p.lp)
+        g.set_state(case.state)
+        g.set_body_force([fx[0].copy(), fx[1].copy(), fx[2].copy()])
+        t3 = time.perf_counter()
+        g.step(nsteps)
+        t4 = time.perf_counter()
+        out.update({"J": float(J), "t_record_s": t1 - t0, "t_backward_s": t2 - t1, "t_forward_s": t4 - t3,
+                    "ratio_backward_forward": (t2 - t1) / (t4 - t3), "ratio_record_forward": (t1 - t0) / (t4 - t3)})
     run.replay = True
     rng = np.random.default_rng(5)
     dF = rng.standard_normal(fx.shape)
     base = [p.wi, p.beta, p.lp ** 2]
     grads = [float(wi.grad), float(beta.grad), float(L2.grad)]
     for q, name in enumerate(("wi", "beta", "L2")):
-        for hrel in (1e-6, 1e-4):
-            h = hrel * base[q]
-            hi = list(base); lo = list(base)
-            hi[q] += h; lo[q] -= h
-            fd = (run.forward(*hi, fx)[0] - run.forward(*lo, fx)[0]) / (2 * h)
-            out["fd_%s_h%g" % (name, hrel)] = fd
-            out["rel_%s_h%g" % (name, hrel)] = abs(fd - grads[q]) / abs(grads[q])
-        out["d" + name] = grads[q]
+        def Jq(t):
+            x = list(base)
+            x[q] += t
+            return run.forward(*x, fx)[0]
+        out["%s_d%s" % (tag, name)] = grads[q]
+        for hrel in hrels:
+            fd = central(Jq, hrel * base[q])
+            out["%s_fd_%s_h%g" % (tag, name, hrel)] = fd
+            out["%s_rel_%s_h%g" % (tag, name, hrel)] = abs(fd - grads[q]) / abs(grads[q])
+        fd = richardson(Jq, 1e-4 * base[q])
+        out["%s_fd_%s_rich" % (tag, name)] = fd
+        out["%s_rel_%s_rich" % (tag, name)] = abs(fd - grads[q]) / abs(grads[q])
     dirF = float(torch.sum(F.grad.cpu() * torch.from_numpy(dF)))
-    h = 1e-6
-    fd = (run.forward(*base, fx + h * dF)[0] - run.forward(*base, fx - h * dF)[0]) / (2 * h)
-    out["dF_dir"] = dirF
-    out["fd_F_dir"] = fd
-    out["rel_F"] = abs(fd - dirF) / abs(dirF)
+    fd = central(lambda t: run.forward(*base, fx + t * dF)[0], 1e-6)
+    out["%s_dF_dir" % tag] = dirF
+    out["%s_rel_F" % tag] = abs(fd - dirF) / abs(dirF)
     run.close()
+
+
+def check_cfg5():
+    out = {"check": "cfg5"}
+    # the reference's Newton tolerance (1e-6): the stopping rule makes the
+    # step piecewise smooth, so the parameter differences use h = 1e-4
+    cfg5_run(1e-6, (1e-4, 1e-6), out, "ref", True)
+    # Newton converged to round-off: differences at h = 1e-6
+    cfg5_run(1e-12, (1e-6,), out, "tight", False, itrmax=50)
     print(json.dumps(out), flush=True)
 
 

@@ -93,4 +93,28 @@ def test_driver_ibm_channel_runs_and_logs_drag(tmp_path):
     assert ts.shape == (4, 13)
     assert np.array_equal(ts[:, 0], [5, 10, 15, 20])
     assert np.allclose(ts[:, 1], ts[:, 0] * 0.002)
-    assert np.all(np.isfinite(ts)) and np.all(ts[:, 3] > 0.5)   # K_staggered of a mean flow U ~ 1
+    assert np.all(np.isfinite(ts))
+    # the regular-grid K of the record equals the log line's K_energy
+    logk = [float(l.split("K_energy:")[1].split("|")[0]) for l in r.stdout.splitlines() if "K_energy:" in l]
+    assert ts[1, 2] == pytest.approx(logk[0], rel=1e-3) and ts[3, 2] == pytest.approx(logk[1], rel=1e-3)
+
+
+@pytest.mark.gpu
+def test_driver_periodic_time_series_and_spectrum(tmp_path):
+    nml = ("NX=32, NY=32, NZ=32, dt=0.002, Re=100, beta=0.9, Wi=2, Lp=10, method=1, flow_type=4, "
+           "Nstep=20, Lstep=10, koide_step=0, Tstep=5, Gstep=10, dir='./tg/'")
+    r = run_driver(tmp_path, nml)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    out = os.path.join(str(tmp_path), "tg", "Log")
+    ts = np.loadtxt(os.path.join(out, "time_same.d"))
+    assert ts.shape == (4, 13)
+    for step in (10, 20):
+        e = np.loadtxt(os.path.join(out, "energy_same_%08d.d" % step))
+        assert e.shape == (32, 2) and np.array_equal(e[:, 0], np.arange(32))
+        # the spectrum averages the records of steps step-5 and step; its sum
+        # is the mean of their regular-grid kinetic energies (Parseval)
+        rows = ts[(ts[:, 0] > step - 10) & (ts[:, 0] <= step)]
+        assert e[:, 1].sum() == pytest.approx(rows[:, 2].mean(), rel=1e-10)
+        # the 2D Taylor-Green field (k = (1, 1, 0)) lives in the shell nint(sqrt 2) = 1
+        assert np.argmax(e[:, 1]) == 1 and e[1, 1] > 0.999 * e[:, 1].sum()

@@ -24,11 +24,14 @@ def run_check(which, timeout=600):
 
 def test_adjoint_one_step_vjp_matches_finite_differences():
     out = run_check("vjp")
-    assert out["rel_h1e-06"] < 1e-6
+    assert out["rel_h1e-05"] < 1e-8
     assert out["rel_uvwp"] < 1e-6 and out["rel_c"] < 1e-6 and out["rel_hist"] < 1e-6
 
 
 def test_adjoint_parameter_and_force_gradients_multi_segment():
+    # stretched conformation (tr C ~ 30 of L^2 = 100), Newton converged to
+    # round-off; reference derivative: Richardson-extrapolated central
+    # differences (h = 1e-4, 5e-5)
     out = run_check("params")
     for k in ("rel_wi", "rel_beta", "rel_L2", "rel_F"):
         assert out[k] < 1e-6, k
@@ -36,6 +39,12 @@ def test_adjoint_parameter_and_force_gradients_multi_segment():
 
 def test_adjoint_cfg5_drag_gradients_and_cost():
     out = run_check("cfg5", timeout=900)
-    for k in ("rel_wi_h1e-06", "rel_beta_h1e-06", "rel_L2_h1e-06", "rel_F"):
-        assert out[k] < 1e-6, k
+    # configuration 5 as specified (reference Newton tolerance) and with the
+    # Newton converged to round-off; reference derivative: Richardson-
+    # extrapolated central differences.  dJ/d(L^2) vanishes here (tr C ~ 3):
+    # it is checked on the stretched state of the multi-segment test.
+    for tag in ("ref", "tight"):
+        assert out[tag + "_rel_wi_rich"] < 1e-5, tag
+        assert out[tag + "_rel_beta_rich"] < 1e-6, tag
+        assert out[tag + "_rel_F"] < 1e-6, tag
     assert out["ratio_backward_forward"] < 3.0

@@ -92,3 +92,26 @@ def test_mixed_full_size_short_run_is_finite_and_close():
     assert e["u"] < TOL and e["C"] < TOL
     d = g.diagnostics()
     assert d["nan_flag"] == 0
+
+
+def test_mixed_throughput_1000_steps_against_fp64():
+    """The chosen precision form at 256x128x128 with 8 cylinders after 1000
+    steps against the FP64 GPU step (bit-identical to the oracle): relative
+    L2 of u, p, C and the drag and kinetic energy within 1e-4."""
+    import json
+    steps = 1000
+    print("mixed vs FP64, 256x128x128, %d steps" % steps, flush=True)
+    g = api.load_case(mixed(configs.throughput()))
+    g.step(steps)
+    h = api.load_case(configs.throughput())
+    h.step(steps)
+    e = errors(h, g)
+    dg, dh = g.diagnostics(), h.diagnostics()
+    e["drag"] = abs(dg["drag"] - dh["drag"]) / abs(dh["drag"])
+    e["ke"] = abs(dg["ke"] - dh["ke"]) / abs(dh["ke"])
+    e["div_max_mixed"] = dg["div_max"]
+    e["div_max_f64"] = dh["div_max"]
+    print(json.dumps({k: float(v) for k, v in e.items()}), flush=True)
+    assert e["u"] < TOL and e["C"] < TOL and e["P_dyn"] < TOL
+    assert e["drag"] < TOL and e["ke"] < TOL
+    assert dg["nan_flag"] == 0

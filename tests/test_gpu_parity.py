@@ -196,3 +196,15 @@ def test_time_stats_match_oracle():
         np.testing.assert_allclose(sg, so, rtol=1e-12, atol=1e-300)
         g.close()
         o.close()
+
+
+def test_energy_spectrum_matches_oracle():
+    case = configs.kolmogorov(n=32)
+    print("energy spectrum", case.name, flush=True)
+    o, g = po.load_case(case), api.load_case(case)
+    o.step(3)
+    g.step(3)
+    Eo, Eg = o.energy_spectrum(), g.energy_spectrum()
+    np.testing.assert_allclose(Eg, Eo, rtol=1e-11, atol=1e-14 * Eo.sum())
+    g.close()
+    o.close()

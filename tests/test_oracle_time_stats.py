@@ -69,3 +69,30 @@ def test_elastic_energy_and_trace_statistics():
     assert s[10] == pytest.approx(ep, rel=1e-13)
     assert s[11] == pytest.approx(3.6, rel=1e-14)
     assert s[12] == pytest.approx(0.1, rel=1e-12)
+
+
+def test_energy_spectrum_parseval_and_single_mode():
+    # Parseval: the shells add up to K of the time statistics
+    case = configs.kolmogorov(n=16)
+    o = po.load_case(case)
+    o.step(2)
+    E = o.energy_spectrum(nbins=40)
+    assert E.sum() == pytest.approx(o.time_stats()[2], rel=1e-13)
+    o.close()
+    # a single Fourier mode u = cos(3 x) on the U faces: regular-grid
+    # amplitude cos(3h/2), all energy cos^2(3h/2)/4 in shell 3
+    n = 16
+    h = 2 * math.pi / n
+    case = configs.taylor_green(n=n)
+    xf = (np.arange(1, n + 1)) * h
+    u = np.broadcast_to(np.cos(3 * xf)[None, None, :], (n, n, n)).copy()
+    st = case.state
+    st.U = pad_periodic(u)
+    st.V = pad_periodic(np.zeros_like(u))
+    st.W = pad_periodic(np.zeros_like(u))
+    o = po.Oracle(case.params)
+    o.set_state(st)
+    E = o.energy_spectrum()
+    o.close()
+    assert E[3] == pytest.approx(math.cos(1.5 * h) ** 2 / 4, rel=1e-13)
+    assert np.abs(np.delete(E, 3)).max() < 1e-28
