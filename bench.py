@@ -46,6 +46,8 @@ def mixed_step_bytes(n_dc):
 
 def run_gpu(case, steps, warmup):
     g = api.load_case(case)
+    # warm-up with the phase events on (fills the context's event pool)
+    g.enable_timing(True)
     g.step(warmup, sync=True)
     # timed region: K steps with the phase events on the context's stream
     g.enable_timing(True)

@@ -102,7 +102,7 @@ struct Timer {
   hipEvent_t a = nullptr, b = nullptr;
   Timer(Ctx& cc, const char* nm) : c(cc), name(nm) {
     if (!c.timing) return;
-    (void)hipEventCreate(&a); (void)hipEventCreate(&b);
+    a = ev_get(c); b = ev_get(c);
     (void)hipEventRecord(a, c.stream);
   }
   ~Timer() {
@@ -123,8 +123,8 @@ void flush_timers(Ctx& c) {
     bool found = false;
     for (auto& kv : c.ktime) if (kv.first == nm) { kv.second += ms; found = true; }
     if (!found) c.ktime.push_back({nm, (double)ms});
-    (void)hipEventDestroy(c.ev[e].second);
-    (void)hipEventDestroy(c.ev[e + 1].second);
+    c.evpool.push_back(c.ev[e].second);
+    c.evpool.push_back(c.ev[e + 1].second);
   }
   c.ev.clear();
 }
@@ -177,7 +177,7 @@ int fp_default_params(fp_params* p) {
   p->method = FP_METHOD_FFT; p->walls_y = 0; p->poisson_variant = FP_POISSON_FFT; p->precision = FP_PREC_F64;
   p->newton_itrmax = 10000; p->newton_eps = 1.0e-6; p->newton_dc = 1.0e-5;
   p->sor_itrmax = 10000; p->sor_eps = 1.0e-5; p->sor_omega = 0.0;
-  p->n_dc_max = 2; p->dc_tol = 1.0e-6;
+  p->n_dc_max = 2; p->dc_tol = 1.0e-5;
   return FP_OK;
 }
 
@@ -246,6 +246,7 @@ int fp_destroy(void* vc) {
   if (!c) return FP_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->ev) (void)hipEventDestroy(kv.second);
+  for (auto e : c->evpool) (void)hipEventDestroy(e);
   if (c->mixed) mixed_destroy(*c);
   adj_free(*c);
   dfree(c->U); dfree(c->V); dfree(c->W); dfree(c->P); dfree(c->Phi);

@@ -66,8 +66,10 @@ struct Ctx {
   int sp_rows[3] = {0, 0, 0};
   // SOR
   double sor_itr = -1.0;
-  // timing
+  // timing: recorded (name, event) pairs and a pool of created events, so
+  // that timed steps do not create or destroy events
   std::vector<std::pair<std::string, hipEvent_t>> ev;
+  std::vector<hipEvent_t> evpool;
   std::vector<std::pair<std::string, double>> ktime;
   // mixed precision state (allocated only for precision == 1)
   void* mixed = nullptr;
@@ -102,6 +104,18 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out);
 void adj_backward(Ctx& c, const double* weights, const double* bar_final, double* bar_initial, double* bar_params,
                   double* bar_force);
 void adj_free(Ctx& c);
+
+// an event from the context's pool (created on first use)
+inline hipEvent_t ev_get(Ctx& c) {
+  if (c.evpool.empty()) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  hipEvent_t e = c.evpool.back();
+  c.evpool.pop_back();
+  return e;
+}
 
 }  // namespace fp
 

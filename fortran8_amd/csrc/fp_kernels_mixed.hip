@@ -170,6 +170,16 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
   for (int o = 0; o < 2 * R + 1; o++)
 #pragma unroll
     for (int q = 0; q < 6; q++) v[o][q] = C[q * g.n + m + off[o]];
+  if (R == 1 && d == 1 && g.walls) {
+    // the one ghost row of a wall (C_kabe): linear extrapolation 2 C_0 - C_1,
+    // in place of the wrapped row loaded above
+    const bool lo = j == 0, hi = j == g.ny - 1;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      v[0][q] = lo ? 2.0f * v[1][q] - v[2][q] : v[0][q];
+      v[2][q] = hi ? 2.0f * v[1][q] - v[0][q] : v[2][q];
+    }
+  }
 }
 
 // Stencil kernels come in two instances.  GEN = false covers the cells whose
@@ -223,7 +233,7 @@ __global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict_
                                                 unsigned* __restrict__ amb, int* __restrict__ amb_list, long amb_cap) {
   long m;
   int i, j, k;
-  if (!row_cell<GEN, 1>(g, m, i, j, k)) return;
+  if (!row_cell<GEN, 0>(g, m, i, j, k)) return;   // the wall rows too (see load_line)
   uint8_t s = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -330,8 +340,184 @@ __device__ __forceinline__ Nbr nbr(const Grid& g, int i, int j, int k) {
   return o;
 }
 
-// Cstar + Lyapunov Newton (analytic Jacobian, FP32) + Peterlin factor of the
-// new C.
+// candidate face values in the arithmetic type T
+template <typename T>
+__device__ __forceinline__ T cand_pt(int l, T cm, T c, T cp) {
+  if (l == 0) return c * T(1.5) - cp * T(0.5);
+  if (l == 1) return T(0.5) * (cm + c);
+  if (l == 2) return (T(0.25) * cm + c) - T(0.25) * cp;
+  return c;
+}
+template <typename T>
+__device__ __forceinline__ T cand_nt(int l, T cm, T c, T cp) {
+  if (l == 0) return T(0.5) * (c + cp);
+  if (l == 1) return T(-0.5) * cm + T(1.5) * c;
+  if (l == 2) return (c - T(0.25) * cm) + T(0.25) * cp;
+  return c;
+}
+__device__ __forceinline__ float rcp_t(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double rcp_t(double v) { return 1.0 / v; }
+template <typename T>
+__device__ __forceinline__ T peterlin_t(T L2, T s) {
+  if (s > T(0.99) * L2) return (T)(((double)L2 - 3.0) / ((double)L2 - (double)s));
+  return (L2 - T(3)) / (L2 - s);
+}
+template <typename T>
+__device__ __forceinline__ T dpeterlin_t(T L2, T s) {
+  if (s > T(0.99) * L2) {
+    const double den = (double)L2 - (double)s;
+    return (T)(((double)L2 - 3.0) / (den * den));
+  }
+  const T den = L2 - s;
+  return (L2 - T(3)) / (den * den);
+}
+
+// Cstar + Lyapunov Newton (analytic Jacobian) + Peterlin factor of the new C
+// for one cell, in the arithmetic type T (storage stays FP32; the step uses
+// FP32: an FP64 evaluation of the strongly stretched cells was measured to
+// leave the error against the FP64 step unchanged).
+template <bool GEN, typename T>
+__device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __restrict__ C,
+                                                  const uint8_t* __restrict__ sel, const float* __restrict__ U,
+                                                  const float* __restrict__ V, const float* __restrict__ W,
+                                                  float* __restrict__ Cn, float* __restrict__ fC, int* __restrict__ nitr,
+                                                  T eps, int itrmax, long m, int i, int j, int k) {
+  const T dt = (T)g.dt;
+  const bool fast = !GEN || !g.walls || (j > 0 && j < g.ny - 1);
+  const Nbr o = nbr(g, i, j, k);
+  T cx[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) cx[q] = T(0);
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    const T h = (T)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
+    T v1, v2;
+    if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
+    else if (d == 1) { v1 = fast ? V[m + o.y[0]] : ldb(V, g, i, j - 1, k, BC_V); v2 = fast ? V[m] : ldb(V, g, i, j, k, BC_V); }
+    else { v1 = W[m + o.z[0]]; v2 = W[m]; }
+    const T r2h = dt / (T(2) * h);
+    const T s1 = (-v2 + fabs(v2)) * r2h;
+    const T s2 = T(1) / T(6) + (-v2 - fabs(v2)) * r2h;
+    const T s3 = T(1) / T(6) + (v1 - fabs(v1)) * r2h;
+    const T s4 = (v1 + fabs(v1)) * r2h;
+    const long om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
+    const long op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
+    const int lm = (sel[m + om] >> (2 * d)) & 3, l0 = (sel[m] >> (2 * d)) & 3, lp = (sel[m + op] >> (2 * d)) & 3;
+    float v[5][6];
+    load_line<2, GEN>(C, g, m, i, j, k, d, v);
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const T c0 = v[0][q], c1 = v[1][q], c2 = v[2][q], c3 = v[3][q], c4 = v[4][q];
+      const T pl = cand_pt<T>(l0, c1, c2, c3);   // Cp(a-1): own left face
+      const T nr = cand_nt<T>(l0, c1, c2, c3);   // Cn(a):   own right face
+      const T pr = cand_pt<T>(lp, c2, c3, c4);   // Cp(a):   left face of a+1
+      const T nl = cand_nt<T>(lm, c0, c1, c2);   // Cn(a-1): right face of a-1
+      cx[q] += s3 * pl + s1 * pr + s4 * nl + s2 * nr;
+    }
+  }
+  const T dtw = dt / (T)g.wi;
+  T b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
+  const T dx = (T)g.dx, dy = (T)g.dy, dz = (T)g.dz;
+  // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
+  T a11, a12, a13, a21, a22, a23, a31, a32, a33;
+  if (fast) {
+#define UF(a, bb, c) ((T)U[m + o.x[a] + o.y[bb] + o.z[c]])
+#define VF(a, bb, c) ((T)V[m + o.x[a] + o.y[bb] + o.z[c]])
+#define WF(a, bb, c) ((T)W[m + o.x[a] + o.y[bb] + o.z[c]])
+    a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) / dx;
+    a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) / (T(4) * dx);
+    a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) / (T(4) * dx);
+    a12 = -dt * ((UF(0, 2, 1) + UF(1, 2, 1)) - (UF(0, 0, 1) + UF(1, 0, 1))) / (T(4) * dy);
+    a22 = -dt * (VF(1, 1, 1) - VF(1, 0, 1)) / dy;
+    a32 = -dt * ((WF(1, 2, 0) - WF(1, 0, 0)) + (WF(1, 2, 1) - WF(1, 0, 1))) / (T(4) * dy);
+    a13 = -dt * ((UF(0, 1, 2) + UF(1, 1, 2)) - (UF(0, 1, 0) + UF(1, 1, 0))) / (T(4) * dz);
+    a23 = -dt * ((VF(1, 0, 2) + VF(1, 1, 2)) - (VF(1, 0, 0) + VF(1, 1, 0))) / (T(4) * dz);
+    a33 = -dt * (WF(1, 1, 1) - WF(1, 1, 0)) / dz;
+#undef UF
+#undef VF
+#undef WF
+  } else {
+#define UL(a, bb, c) ((T)ldb(U, g, a, bb, c, BC_U))
+#define VL(a, bb, c) ((T)ldb(V, g, a, bb, c, BC_V))
+#define WL(a, bb, c) ((T)ldb(W, g, a, bb, c, BC_W))
+    a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
+    a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (T(4) * dx);
+    a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (T(4) * dx);
+    a12 = -dt * ((UL(i - 1, j + 1, k) + UL(i, j + 1, k)) - (UL(i - 1, j - 1, k) + UL(i, j - 1, k))) / (T(4) * dy);
+    a22 = -dt * (VL(i, j, k) - VL(i, j - 1, k)) / dy;
+    a32 = -dt * ((WL(i, j + 1, k - 1) - WL(i, j - 1, k - 1)) + (WL(i, j + 1, k) - WL(i, j - 1, k))) / (T(4) * dy);
+    a13 = -dt * ((UL(i - 1, j, k + 1) + UL(i, j, k + 1)) - (UL(i - 1, j, k - 1) + UL(i, j, k - 1))) / (T(4) * dz);
+    a23 = -dt * ((VL(i, j - 1, k + 1) + VL(i, j, k + 1)) - (VL(i, j - 1, k - 1) + VL(i, j, k - 1))) / (T(4) * dz);
+    a33 = -dt * (WL(i, j, k) - WL(i, j, k - 1)) / dz;
+#undef UL
+#undef VL
+#undef WL
+  }
+  const T L2 = (T)(g.lp * g.lp);
+  T x[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
+  int it;
+  for (it = 1; it <= itrmax; it++) {
+    const T s = x[0] + x[3] + x[5];
+    const T f = peterlin_t<T>(L2, s);
+    const T dg = T(0.5) * (T(1) + f * dtw);
+    const T A11 = a11 + dg, A22 = a22 + dg, A33 = a33 + dg;
+    // residual l = M x - b (fenep.f90:598-603)
+    T l[6];
+    l[0] = T(2) * (A11 * x[0] + a12 * x[1] + a13 * x[2]) - b[0];
+    l[1] = a21 * x[0] + (A22 + A11) * x[1] + a23 * x[2] + a12 * x[3] + a13 * x[4] - b[1];
+    l[2] = a31 * x[0] + a32 * x[1] + (A33 + A11) * x[2] + a12 * x[4] + a13 * x[5] - b[2];
+    l[3] = T(2) * (a21 * x[1] + A22 * x[3] + a23 * x[4]) - b[3];
+    l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (A33 + A22) * x[4] + a23 * x[5] - b[4];
+    l[5] = T(2) * (a31 * x[2] + a32 * x[4] + A33 * x[5]) - b[5];
+    // Jacobian: linear operator + rank one Peterlin term
+    const T gp = dtw * dpeterlin_t<T>(L2, s);
+    T J[6][6] = {
+        {T(2) * A11, T(2) * a12, T(2) * a13, T(0), T(0), T(0)},
+        {a21, A22 + A11, a23, a12, a13, T(0)},
+        {a31, a32, A33 + A11, T(0), a12, a13},
+        {T(0), T(2) * a21, T(0), T(2) * A22, T(2) * a23, T(0)},
+        {T(0), a31, a21, a32, A33 + A22, a23},
+        {T(0), T(0), T(2) * a31, T(0), T(2) * a32, T(2) * A33}};
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      const T gx = gp * x[r];
+      J[r][0] += gx; J[r][3] += gx; J[r][5] += gx;
+    }
+    // Gaussian elimination (diagonally dominant: no pivoting needed)
+#pragma unroll
+    for (int kk = 0; kk < 6; kk++) {
+      const T inv = rcp_t(J[kk][kk]);
+#pragma unroll
+      for (int r = kk + 1; r < 6; r++) {
+        const T fct = J[r][kk] * inv;
+#pragma unroll
+        for (int cc = kk + 1; cc < 6; cc++) J[r][cc] -= fct * J[kk][cc];
+        l[r] -= fct * l[kk];
+      }
+      J[kk][kk] = inv;
+    }
+    T rr[6];
+#pragma unroll
+    for (int r = 5; r >= 0; r--) {
+      T acc = l[r];
+#pragma unroll
+      for (int cc = r + 1; cc < 6; cc++) acc -= J[r][cc] * rr[cc];
+      rr[r] = acc * J[r][r];
+    }
+    T sr = T(0), sx = T(0);
+#pragma unroll
+    for (int q = 0; q < 6; q++) { x[q] -= rr[q]; sr += rr[q] * rr[q]; sx += x[q] * x[q]; }
+    if (sr < eps * eps * sx) break;
+  }
+  if (it > itrmax) it = itrmax;
+#pragma unroll
+  for (int q = 0; q < 6; q++) Cn[q * g.n + m] = (float)x[q];
+  fC[m] = (float)peterlin_t<T>(L2, x[0] + x[3] + x[5]);
+  nitr[m] = it;
+}
+
 template <bool GEN>
 __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
                                                       const float* __restrict__ U, const float* __restrict__ V,
@@ -341,139 +527,7 @@ __global__ __launch_bounds__(128) void m_cstar_newton(Grid g, const float* __res
   long m;
   int i, j, k;
   if (!row_cell<GEN, 2>(g, m, i, j, k)) return;
-  const float dt = (float)g.dt;
-  const bool fast = !GEN || !g.walls || (j > 0 && j < g.ny - 1);
-  const Nbr o = nbr(g, i, j, k);
-  float cx[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) cx[q] = 0.0f;
-#pragma unroll 1
-  for (int d = 0; d < 3; d++) {
-    const float h = (float)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
-    float v1, v2;
-    if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
-    else if (d == 1) { v1 = fast ? V[m + o.y[0]] : ldb(V, g, i, j - 1, k, BC_V); v2 = fast ? V[m] : ldb(V, g, i, j, k, BC_V); }
-    else { v1 = W[m + o.z[0]]; v2 = W[m]; }
-    const float r2h = dt / (2.0f * h);
-    const float s1 = (-v2 + fabsf(v2)) * r2h;
-    const float s2 = (1.0f / 6.0f) + (-v2 - fabsf(v2)) * r2h;
-    const float s3 = (1.0f / 6.0f) + (v1 - fabsf(v1)) * r2h;
-    const float s4 = (v1 + fabsf(v1)) * r2h;
-    const long om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
-    const long op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
-    const int lm = (sel[m + om] >> (2 * d)) & 3, l0 = (sel[m] >> (2 * d)) & 3, lp = (sel[m + op] >> (2 * d)) & 3;
-    float v[5][6];
-    load_line<2, GEN>(C, g, m, i, j, k, d, v);
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-      const float pl = cand_pf(l0, v[1][q], v[2][q], v[3][q]);   // Cp(a-1): own left face
-      const float nr = cand_nf(l0, v[1][q], v[2][q], v[3][q]);   // Cn(a):   own right face
-      const float pr = cand_pf(lp, v[2][q], v[3][q], v[4][q]);   // Cp(a):   left face of a+1
-      const float nl = cand_nf(lm, v[0][q], v[1][q], v[2][q]);   // Cn(a-1): right face of a-1
-      cx[q] += s3 * pl + s1 * pr + s4 * nl + s2 * nr;
-    }
-  }
-  const float dtw = dt / (float)g.wi;
-  float b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
-  const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz;
-  // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
-  float a11, a12, a13, a21, a22, a23, a31, a32, a33;
-  if (fast) {
-#define UF(a, bb, c) U[m + o.x[a] + o.y[bb] + o.z[c]]
-#define VF(a, bb, c) V[m + o.x[a] + o.y[bb] + o.z[c]]
-#define WF(a, bb, c) W[m + o.x[a] + o.y[bb] + o.z[c]]
-    a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) / dx;
-    a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) / (4.0f * dx);
-    a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) / (4.0f * dx);
-    a12 = -dt * ((UF(0, 2, 1) + UF(1, 2, 1)) - (UF(0, 0, 1) + UF(1, 0, 1))) / (4.0f * dy);
-    a22 = -dt * (VF(1, 1, 1) - VF(1, 0, 1)) / dy;
-    a32 = -dt * ((WF(1, 2, 0) - WF(1, 0, 0)) + (WF(1, 2, 1) - WF(1, 0, 1))) / (4.0f * dy);
-    a13 = -dt * ((UF(0, 1, 2) + UF(1, 1, 2)) - (UF(0, 1, 0) + UF(1, 1, 0))) / (4.0f * dz);
-    a23 = -dt * ((VF(1, 0, 2) + VF(1, 1, 2)) - (VF(1, 0, 0) + VF(1, 1, 0))) / (4.0f * dz);
-    a33 = -dt * (WF(1, 1, 1) - WF(1, 1, 0)) / dz;
-#undef UF
-#undef VF
-#undef WF
-  } else {
-#define UL(a, bb, c) ldb(U, g, a, bb, c, BC_U)
-#define VL(a, bb, c) ldb(V, g, a, bb, c, BC_V)
-#define WL(a, bb, c) ldb(W, g, a, bb, c, BC_W)
-    a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
-    a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (4.0f * dx);
-    a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (4.0f * dx);
-    a12 = -dt * ((UL(i - 1, j + 1, k) + UL(i, j + 1, k)) - (UL(i - 1, j - 1, k) + UL(i, j - 1, k))) / (4.0f * dy);
-    a22 = -dt * (VL(i, j, k) - VL(i, j - 1, k)) / dy;
-    a32 = -dt * ((WL(i, j + 1, k - 1) - WL(i, j - 1, k - 1)) + (WL(i, j + 1, k) - WL(i, j - 1, k))) / (4.0f * dy);
-    a13 = -dt * ((UL(i - 1, j, k + 1) + UL(i, j, k + 1)) - (UL(i - 1, j, k - 1) + UL(i, j, k - 1))) / (4.0f * dz);
-    a23 = -dt * ((VL(i, j - 1, k + 1) + VL(i, j, k + 1)) - (VL(i, j - 1, k - 1) + VL(i, j, k - 1))) / (4.0f * dz);
-    a33 = -dt * (WL(i, j, k) - WL(i, j, k - 1)) / dz;
-#undef UL
-#undef VL
-#undef WL
-  }
-  const float L2 = (float)(g.lp * g.lp);
-  float x[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
-  int it;
-  for (it = 1; it <= itrmax; it++) {
-    const float s = x[0] + x[3] + x[5];
-    const float f = peterlin_f(L2, s);
-    const float dg = 0.5f * (1.0f + f * dtw);
-    const float A11 = a11 + dg, A22 = a22 + dg, A33 = a33 + dg;
-    // residual l = M x - b (fenep.f90:598-603)
-    float l[6];
-    l[0] = 2.0f * (A11 * x[0] + a12 * x[1] + a13 * x[2]) - b[0];
-    l[1] = a21 * x[0] + (A22 + A11) * x[1] + a23 * x[2] + a12 * x[3] + a13 * x[4] - b[1];
-    l[2] = a31 * x[0] + a32 * x[1] + (A33 + A11) * x[2] + a12 * x[4] + a13 * x[5] - b[2];
-    l[3] = 2.0f * (a21 * x[1] + A22 * x[3] + a23 * x[4]) - b[3];
-    l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (A33 + A22) * x[4] + a23 * x[5] - b[4];
-    l[5] = 2.0f * (a31 * x[2] + a32 * x[4] + A33 * x[5]) - b[5];
-    // Jacobian: linear operator + rank one Peterlin term
-    const float gp = dtw * dpeterlin_f(L2, s);
-    float J[6][6] = {
-        {2.0f * A11, 2.0f * a12, 2.0f * a13, 0.0f, 0.0f, 0.0f},
-        {a21, A22 + A11, a23, a12, a13, 0.0f},
-        {a31, a32, A33 + A11, 0.0f, a12, a13},
-        {0.0f, 2.0f * a21, 0.0f, 2.0f * A22, 2.0f * a23, 0.0f},
-        {0.0f, a31, a21, a32, A33 + A22, a23},
-        {0.0f, 0.0f, 2.0f * a31, 0.0f, 2.0f * a32, 2.0f * A33}};
-#pragma unroll
-    for (int r = 0; r < 6; r++) {
-      const float gx = gp * x[r];
-      J[r][0] += gx; J[r][3] += gx; J[r][5] += gx;
-    }
-    // Gaussian elimination (diagonally dominant: no pivoting needed)
-#pragma unroll
-    for (int kk = 0; kk < 6; kk++) {
-      const float inv = __builtin_amdgcn_rcpf(J[kk][kk]);
-#pragma unroll
-      for (int r = kk + 1; r < 6; r++) {
-        const float fct = J[r][kk] * inv;
-#pragma unroll
-        for (int cc = kk + 1; cc < 6; cc++) J[r][cc] -= fct * J[kk][cc];
-        l[r] -= fct * l[kk];
-      }
-      J[kk][kk] = inv;
-    }
-    float rr[6];
-#pragma unroll
-    for (int r = 5; r >= 0; r--) {
-      float acc = l[r];
-#pragma unroll
-      for (int cc = r + 1; cc < 6; cc++) acc -= J[r][cc] * rr[cc];
-      rr[r] = acc * J[r][r];
-    }
-    float sr = 0.0f, sx = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 6; q++) { x[q] -= rr[q]; sr += rr[q] * rr[q]; sx += x[q] * x[q]; }
-    if (sr < eps * eps * sx) break;
-  }
-  if (it > itrmax) it = itrmax;
-#pragma unroll
-  for (int q = 0; q < 6; q++) Cn[q * g.n + m] = x[q];
-  fC[m] = peterlin_f(L2, x[0] + x[3] + x[5]);
-  nitr[m] = it;
+  cstar_newton_cell<GEN, float>(g, C, sel, U, V, W, Cn, fC, nitr, eps, itrmax, m, i, j, k);
 }
 
 // ============================================================= predictor ===
@@ -1315,7 +1369,15 @@ const double* mixed_marker_forces(Ctx& c) {
   return X.mf64;
 }
 
-double mixed_last_ndc(Ctx& c) { return (double)std::max(1, c.p.n_dc_max); }
+// defect-correction sweeps the last step ran: the later sweeps are skipped on
+// the device once the FP64 residual is within dc_tol
+double mixed_last_ndc(Ctx& c) {
+  const int ndc = std::max(1, c.p.n_dc_max);
+  if (ndc == 1) return 1.0;
+  int skip = 0;
+  FP_HIP_CHECK(hipMemcpy(&skip, MX(c).skip, sizeof(int), hipMemcpyDeviceToHost));
+  return skip ? 1.0 : (double)ndc;
+}
 
 // statistics of the last step: [0] |b|^2, [1] |r|^2 before the last
 // correction, [2] guarded selections re-decided in FP64
@@ -1336,7 +1398,7 @@ struct MTimer {
   hipEvent_t a = nullptr, b = nullptr;
   MTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
     if (!c.timing) return;
-    (void)hipEventCreate(&a); (void)hipEventCreate(&b);
+    a = ev_get(c); b = ev_get(c);
     (void)hipEventRecord(a, c.stream);
   }
   ~MTimer() {
@@ -1362,11 +1424,6 @@ void mixed_step(Ctx& c) {
       MTimer t(c, "fenep_select");
       hipLaunchKernelGGL(m_select<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list,
                          X.amb_cap);
-    }
-    if (g.walls) {
-      MTimer t(c, "fenep_select_walls");
-      hipLaunchKernelGGL(m_select<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb,
-                         X.amb_list, X.amb_cap);
     }
     {
       MTimer t(c, "fenep_select_fp64_fix");

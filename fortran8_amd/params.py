@@ -67,7 +67,7 @@ def default_params(**kw):
     p.newton_itrmax, p.newton_eps, p.newton_dc = 10000, 1.0e-6, 1.0e-5
     p.sor_itrmax, p.sor_eps, p.sor_omega = 10000, 1.0e-5, 0.0
     p.body_force[0] = p.body_force[1] = p.body_force[2] = 0.0
-    p.n_dc_max, p.dc_tol = 2, 1.0e-6
+    p.n_dc_max, p.dc_tol = 2, 1.0e-5
     for k, v in kw.items():
         if k == "body_force":
             for i in range(3):
