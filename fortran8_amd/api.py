@@ -61,6 +61,7 @@ def lib():
         L.fp_set_physics.argtypes = [_vp, C.c_double, C.c_double, C.c_double]
         L.fp_time_stats.argtypes = [_vp, _dp]
         L.fp_energy_spectrum.argtypes = [_vp, _dp, C.c_int]
+        L.fp_check_selection.argtypes = [_vp, _dp]
         L.fp_adj_record.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
         L.fp_adj_backward.argtypes = [_vp, _dp, _vp, _vp, _dp, _vp]
         _LIB = L
@@ -176,6 +177,13 @@ class FenepGPU:
         out = np.zeros(nb)
         self._check(lib().fp_energy_spectrum(self.h, _ptr(out), nb), "fp_energy_spectrum")
         return out
+
+    def check_selection(self):
+        """Mixed precision: (re-decided in FP64, differing from the reference's
+        selection, total) over the C+- selection of the current state."""
+        out = np.zeros(3)
+        self._check(lib().fp_check_selection(self.h, _ptr(out)), "fp_check_selection")
+        return {"fp64": int(out[0]), "differ": int(out[1]), "total": int(out[2])}
 
     # ---- sensitivities (FP64 path, periodic grids)
     def set_physics(self, wi, beta, lp):

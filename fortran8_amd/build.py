@@ -2,7 +2,10 @@
 
 Plain hipcc invocations (no CMake): one object per translation unit, then a
 shared link.  The FP64 parity kernels are compiled without fused multiply-add
-(-ffp-contract=off) so that their evaluation order is the one written.
+(-ffp-contract=off) so that their evaluation order is the one written.  The
+mixed-precision and adjoint units contract freely except inside functions
+that say `#pragma clang fp contract(off)` (the reference formulas they share
+with the parity path): plain -ffp-contract=fast would fuse those as well.
 """
 import os
 import subprocess
@@ -19,9 +22,9 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
           "-Wno-unused-command-line-argument"]
 UNITS = [
     ("fp_kernels_f64.hip", ["-ffp-contract=off"]),
-    ("fp_kernels_mixed.hip", ["-ffp-contract=fast"]),
+    ("fp_kernels_mixed.hip", ["-ffp-contract=fast-honor-pragmas"]),
     ("fp_api.hip", ["-ffp-contract=off"]),
-    ("fp_adjoint.hip", ["-ffp-contract=fast"]),
+    ("fp_adjoint.hip", ["-ffp-contract=fast-honor-pragmas"]),
 ]
 
 

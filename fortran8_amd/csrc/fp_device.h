@@ -81,7 +81,11 @@ __device__ __forceinline__ long cidx(const Grid& g, int i, int j, int k) {
 
 // ---- FENE-P algebra (FP64) ------------------------------------------------
 // Vieta, fenep.f90:425-444
+// The reference formulas below are evaluated exactly as written, also in
+// translation units compiled with contraction on (the mixed path re-decides
+// selections with them and must reproduce the FP64 path's bits).
 __device__ __forceinline__ double vieta_d(const double* a) {
+#pragma clang fp contract(off)
   double a2 = -((a[0] + a[3]) + a[5]);
   double a1 = (((((a[0] * a[3]) + (a[3] * a[5])) + (a[5] * a[0])) - (a[4] * a[4])) - (a[2] * a[2])) - (a[1] * a[1]);
   double a0 = -((((((a[0] * a[3]) * a[5] + (a[1] * a[4]) * a[2]) + (a[1] * a[4]) * a[2]) - (a[0] * a[4]) * a[4]) -
@@ -107,12 +111,14 @@ __device__ __forceinline__ double minval2_d(double a, double b) {
 // candidate face values of one cell (fenep.f90:179-184): left face (Cp) and
 // right face (Cn) for candidate l in 0..2; l == 3 is the zero slope.
 __device__ __forceinline__ double cand_p(int l, double cm, double c, double cp) {
+#pragma clang fp contract(off)
   if (l == 0) return (c * 3.0) / 2.0 - cp / 2.0;
   if (l == 1) return cm / 2.0 + c / 2.0;
   if (l == 2) return (cm / 4.0 + c) - cp / 4.0;
   return c;
 }
 __device__ __forceinline__ double cand_n(int l, double cm, double c, double cp) {
+#pragma clang fp contract(off)
   if (l == 0) return c / 2.0 + cp / 2.0;
   if (l == 1) return -(cm / 2.0) + (c * 3.0) / 2.0;
   if (l == 2) return (-(cm / 4.0) + c) + cp / 4.0;

@@ -41,9 +41,11 @@ struct Mixed {
   int* skip = nullptr;            // defect correction converged: skip the sweep
   float *R = nullptr, *X2 = nullptr;  // Helmholtz right-hand side / iterate
   int helm_iters = 0;             // fixed-point sweeps (0: spectral solve)
-  unsigned* amb = nullptr;        // ambiguous selections re-decided in FP64
-  int* amb_list = nullptr;
-  long amb_cap = 0;
+  unsigned* amb = nullptr;        // uncertified selections, re-decided in FP64
+  unsigned* amb_list = nullptr;   // per-block regions of m * 3 + axis entries
+  unsigned* amb_cnt = nullptr;    // entries per select block
+  unsigned* amb_off = nullptr;    // exclusive offsets of amb_cnt
+  unsigned* chk = nullptr;        // selection check: differing decisions
   // IBM
   float* wi = nullptr;            // interpolation weights, nmk * 3 * 27
   int4* org = nullptr;            // per marker and component: i0, j0, n
@@ -91,54 +93,121 @@ __device__ __forceinline__ long woff(int a, int d, int n, long s) {
   return (long)(t - a) * s;
 }
 
-// arccos by Abramowitz & Stegun 4.4.46 (|error| <= 2e-8 on [0,1]), reflected
-// for x < 0
-__device__ __forceinline__ float acos_as(float x) {
-  const float ax = fabsf(x);
-  float p = -0.0012624911f;
-  p = fmaf(p, ax, 0.0066700901f);
-  p = fmaf(p, ax, -0.0170881256f);
-  p = fmaf(p, ax, 0.0308918810f);
-  p = fmaf(p, ax, -0.0501743046f);
-  p = fmaf(p, ax, 0.0889789874f);
-  p = fmaf(p, ax, -0.2145988016f);
-  p = fmaf(p, ax, 1.5707963050f);
-  const float r = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - ax)) * p;
-  return x < 0.0f ? 3.14159265358979f - r : r;
-}
+// ---- C+- selection in FP32 with certified decisions ----------------------
+// The selection of fenep.f90:186-221 takes, per cell and axis, the maxloc over
+// {v_0, v_1, v_2, 0} of v_l = min(lambda_min(Cp_l), lambda_min(Cn_l)), with
+// lambda_min from the trigonometric root of Vieta (fenep.f90:425-444) and the
+// sentinel -100 where the discriminant t >= 0.  The FP32 kernel evaluates
+// lambda_min on the deviatoric part B = A - (tr A/3) I (so that p and q do not
+// cancel) together with a bound on |lambda_32 - lambda_ref|, where lambda_ref
+// is the reference formula evaluated in FP64 on the same stored FP32 values.
+// A decision is kept only if the interval of the winner is strictly separated
+// from every other interval and no candidate is near the sentinel boundary;
+// otherwise the (cell, axis) is re-decided with the reference formula in FP64
+// (m_select_fix).  The final selection is therefore the reference's selection
+// of the stored field.
+//
+// With m = tr A/3, r = sqrt(-p/3), arg = det B/(2 r^3) = cos(theta),
+// lambda_min = m + 2 r psi(s),  psi(s) = cos(acos(1 - s^2)/3 + 2 pi/3),
+// s = sqrt(1 - arg).  psi is analytic in s on [0, sqrt 2] (the square-root
+// singularity of acos at arg = 1 is absorbed by s, and psi is even in
+// pi - theta at arg = -1), so it is a polynomial of degree 8 (Chebyshev
+// least-squares fit at 4000 Chebyshev nodes, converted to powers of s;
+// max |error| 2.4e-8 on [0, sqrt 2] with the coefficients as printed, below
+// half an FP32 ulp of |psi| >= 0.5).
+#define PSI0 -5.000000000e-01f
+#define PSI1 -4.082478881e-01f
+#define PSI2 5.554784089e-02f
+#define PSI3 -1.884038374e-02f
+#define PSI4 7.990322076e-03f
+#define PSI5 -3.478206461e-03f
+#define PSI6 1.290685032e-03f
+#define PSI7 -3.277156502e-04f
+#define PSI8 3.995704537e-05f
 
-// Smallest eigenvalue of the symmetric matrix a (xx,xy,xz,yy,yz,zz) by the
-// trigonometric root of Vieta (fenep.f90:425-444), evaluated on the
-// deviatoric part B = a - (tr a/3) I so that p and q do not cancel in FP32.
-// Same sentinel (-100) when the discriminant t >= 0.
-__device__ __forceinline__ float lmin32(const float* a) {
-  const float m = (a[0] + a[3] + a[5]) * (1.0f / 3.0f);
+// Error model, rho = S/r with S = |m| + 2 r (a bound on |lambda|):
+//   |arg_32 - arg_exact|  <= eps32 (K1 + K2 rho)    (B's diagonal carries
+//                                                    eps32 |m| absolute error)
+//   |arg_ref - arg_exact| <= eps64 K3 rho^3         (p and q of the reference
+//                                                    cancel on the full matrix)
+//   d lambda / d arg      <= r / sqrt(1 - arg)      (0.82 r sqrt(delta) in the
+//                                                    square-root regime)
+//   plus Km eps32 S for m, r and psi, and Kc eps32 Smax for forming the
+//   candidates in FP32 (Smax bounds the entries of the three cells).
+// K1 = K2 = Km = Kc = 8, K3 = 64: in a float32 emulation over 10^6 random
+// symmetric matrices (definite, indefinite, near double roots at either end,
+// near isotropic, exactly uniaxial) |lambda_32 - lambda_ref| stays below 0.37
+// of the bound, and no reference sentinel or NaN falls outside the flagged
+// zone |arg| > 1 - 2 delta.
+#define SEL_K1E 4.76837158e-07f    /* 8 eps32 */
+#define SEL_K2E 4.76837158e-07f    /* 8 eps32 */
+#define SEL_K3E 7.10542736e-15f    /* 64 eps64 */
+#define SEL_KME 4.76837158e-07f    /* 8 eps32 */
+#define SEL_KCE 4.76837158e-07f    /* 8 eps32 */
+
+struct Lmin {
+  float lam, err;
+  bool amb;   // near the sentinel boundary (or degenerate): decide in FP64
+};
+
+__device__ __forceinline__ Lmin lmin32(const float* a, float ein) {
+  const float third = 1.0f / 3.0f;
+  const float m = ((a[0] + a[3]) + a[5]) * third;
   const float b11 = a[0] - m, b22 = a[3] - m, b33 = a[5] - m;
-  const float o2 = a[1] * a[1] + a[2] * a[2] + a[4] * a[4];
-  const float p = -(0.5f * (b11 * b11 + b22 * b22 + b33 * b33) + o2);
-  const float det = b11 * (b22 * b33 - a[4] * a[4]) - a[1] * (a[1] * b33 - a[4] * a[2]) + a[2] * (a[1] * a[4] - b22 * a[2]);
-  const float p3 = p * (1.0f / 3.0f);
-  const float t = 0.25f * det * det + p3 * p3 * p3;
-  // branch-free: the sentinel is selected after the trig path
-  const float r = __builtin_amdgcn_sqrtf(fmaxf(-p3, 1.0e-30f));
-  float arg = 0.5f * det * __builtin_amdgcn_rcpf(r * r * r);   // -q/(2 r^3) with q = -det
-  arg = fminf(1.0f, fmaxf(-1.0f, arg));
-  const float ang = acos_as(arg) * (1.0f / 3.0f) + 2.09439510239319549f;
-  const float lam = m + 2.0f * r * __cosf(ang);
-  return t < 0.0f ? lam : -100.0f;
+  const float o2 = fmaf(a[1], a[1], fmaf(a[2], a[2], a[4] * a[4]));
+  const float hp = fmaf(b11, b11, fmaf(b22, b22, b33 * b33));
+  const float r2 = fmaxf(fmaf(hp, 1.0f / 6.0f, o2 * third), 1.0e-30f);   // -p/3
+  const float t1 = fmaf(b22, b33, -(a[4] * a[4]));
+  const float t2 = fmaf(a[1], b33, -(a[4] * a[2]));
+  const float t3 = fmaf(a[1], a[4], -(b22 * a[2]));
+  const float det = fmaf(b11, t1, fmaf(-a[1], t2, a[2] * t3));
+  // two reciprocal square roots (quarter-rate) per eigenvalue
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * ir;
+  const float ir3 = (ir * ir) * ir;     // inf for an isotropic A: arg NaN
+  const float arg = (0.5f * det) * ir3;
+  const float s2 = 1.0f - arg;
+  const float S = fmaf(2.0f, r, fabsf(m));
+  const float rho = S * ir;
+  const float rho3 = (rho * rho) * rho;
+  const float d = fminf(fmaf(SEL_K3E, rho3, fmaf(SEL_K2E, rho, SEL_K1E)), 1.0f);
+  // 1/sqrt(max(1 - arg, delta)); where the decision is certified, 1 - arg >=
+  // 2 delta and s = (1 - arg) v is sqrt(1 - arg)
+  const float v = __builtin_amdgcn_rsqf(fmaxf(s2, d));
+  const float s = fminf(s2, 2.0f) * v;
+  float ps = PSI8;
+  ps = fmaf(ps, s, PSI7);
+  ps = fmaf(ps, s, PSI6);
+  ps = fmaf(ps, s, PSI5);
+  ps = fmaf(ps, s, PSI4);
+  ps = fmaf(ps, s, PSI3);
+  ps = fmaf(ps, s, PSI2);
+  ps = fmaf(ps, s, PSI1);
+  ps = fmaf(ps, s, PSI0);
+  Lmin o;
+  o.lam = fmaf(2.0f * r, ps, m);
+  // written so that a NaN arg counts as ambiguous
+  o.amb = !(fabsf(arg) <= fmaf(-2.0f, d, 1.0f));
+  o.err = fmaf(SEL_KME, S, ein) + (r * d) * v;
+  return o;
 }
 
+// bound on the entries of one cell's C: |diag| + 2 |offdiag| summed
+__device__ __forceinline__ float entry_bound(const float* c) {
+  return (fabsf(c[0]) + fabsf(c[3]) + fabsf(c[5])) + 2.0f * ((fabsf(c[1]) + fabsf(c[2])) + fabsf(c[4]));
+}
+
+// candidate face values (fenep.f90:179-184) in FP32, each with at most two
+// roundings: left face Cp_l and right face Cn_l, l = 0..2
 __device__ __forceinline__ float cand_pf(int l, float cm, float c, float cp) {
-  if (l == 0) return c * 1.5f - cp * 0.5f;
-  if (l == 1) return 0.5f * (cm + c);
-  if (l == 2) return (0.25f * cm + c) - 0.25f * cp;
-  return c;
+  if (l == 0) return fmaf(c, 1.5f, -(cp * 0.5f));
+  if (l == 1) return (cm + c) * 0.5f;
+  return fmaf(cm, 0.25f, c) - cp * 0.25f;
 }
 __device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
-  if (l == 0) return 0.5f * (c + cp);
-  if (l == 1) return -0.5f * cm + 1.5f * c;
-  if (l == 2) return (c - 0.25f * cm) + 0.25f * cp;
-  return c;
+  if (l == 0) return (c + cp) * 0.5f;
+  if (l == 1) return fmaf(c, 1.5f, -(cm * 0.5f));
+  return fmaf(cp, 0.25f, c) - cm * 0.25f;
 }
 
 // C at cells a-r..a+r along axis d (r = 1 or 2) for the 6 components; wall
@@ -169,7 +238,7 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
 #pragma unroll
   for (int o = 0; o < 2 * R + 1; o++)
 #pragma unroll
-    for (int q = 0; q < 6; q++) v[o][q] = C[q * g.n + m + off[o]];
+    for (int q = 0; q < 6; q++) v[o][q] = o == R ? C[q * g.n + m] : C[q * g.n + m + off[o]];
   if (R == 1 && d == 1 && g.walls) {
     // the one ghost row of a wall (C_kabe): linear extrapolation 2 C_0 - C_1,
     // in place of the wrapped row loaded above
@@ -225,70 +294,106 @@ __device__ __forceinline__ bool row_cell(const Grid& g, long& m, int& i, int& j,
   return true;
 }
 
-// relative guard band for re-deciding a selection in FP64
-#define SEL_GUARD 1.0e-6f
+// Uncertified (cell, axis) pairs go to a list in per-block regions of 3 * 256
+// entries (block b owns entries [768 b, 768 b + cnt[b])): the slots are taken
+// with LDS atomics, so the kernel issues no device-scope atomic.
+#define SEL_BLOCK 256
+#define SEL_REGION (3 * SEL_BLOCK)
 
-template <bool GEN>
-__global__ __launch_bounds__(256) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
-                                                unsigned* __restrict__ amb, int* __restrict__ amb_list, long amb_cap) {
+__global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
+                                                      unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list) {
+  __shared__ unsigned s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0u;
+  __syncthreads();
   long m;
   int i, j, k;
-  if (!row_cell<GEN, 0>(g, m, i, j, k)) return;   // the wall rows too (see load_line)
+  const bool valid = row_cell<false, 0>(g, m, i, j, k);   // the wall rows too (see load_line)
+  unsigned* region = amb_list + (size_t)blockIdx.x * SEL_REGION;
   uint8_t s = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    float v[3][6];
-    load_line<1, GEN>(C, g, m, i, j, k, d, v);
-    float vals[3];
-#pragma unroll
-    for (int l = 0; l < 3; l++) {
-      float ap[6], an[6];
-#pragma unroll
-      for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, v[0][q], v[1][q], v[2][q]); an[q] = cand_nf(l, v[0][q], v[1][q], v[2][q]); }
-      vals[l] = fminf(lmin32(ap), lmin32(an));
-    }
-    // maxloc over {v0, v1, v2, 0}: first maximum
-    float best = vals[0];
+    bool unsure = false;
     int idx = 0;
-    if (vals[1] > best) { best = vals[1]; idx = 1; }
-    if (vals[2] > best) { best = vals[2]; idx = 2; }
-    if (0.0f > best) { best = 0.0f; idx = 3; }
-    float second = idx == 3 ? -1e30f : 0.0f;
+    if (valid) {
+      float v[3][6];
+      load_line<1, false>(C, g, m, i, j, k, d, v);
+      const float ein = SEL_KCE * fmaxf(fmaxf(entry_bound(v[0]), entry_bound(v[1])), entry_bound(v[2]));
+      float vals[4], errs[4];
 #pragma unroll
-    for (int l = 0; l < 3; l++) if (l != idx) second = fmaxf(second, vals[l]);
-    const bool sentinel = vals[0] == -100.0f || vals[1] == -100.0f || vals[2] == -100.0f;
-    const bool guarded = best - second < SEL_GUARD * fmaxf(1.0f, fabsf(best)) || sentinel;
-    // one atomic per wavefront: the first guarded lane reserves slots for all
-    const unsigned long long ball = __ballot(guarded);
+      for (int l = 0; l < 3; l++) {
+        float ap[6], an[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, v[0][q], v[1][q], v[2][q]); an[q] = cand_nf(l, v[0][q], v[1][q], v[2][q]); }
+        const Lmin P = lmin32(ap, ein), N = lmin32(an, ein);
+        vals[l] = fminf(P.lam, N.lam);
+        errs[l] = fmaxf(P.err, N.err);
+        unsure = unsure || P.amb || N.amb;
+      }
+      vals[3] = 0.0f;
+      errs[3] = 0.0f;
+      // maxloc over {v0, v1, v2, 0}: first maximum
+      float best = vals[0], eb = errs[0];
+      if (vals[1] > best) { best = vals[1]; eb = errs[1]; idx = 1; }
+      if (vals[2] > best) { best = vals[2]; eb = errs[2]; idx = 2; }
+      if (0.0f > best) { best = 0.0f; eb = 0.0f; idx = 3; }
+      // certified only if the winner's interval clears every other interval
+#pragma unroll
+      for (int l = 0; l < 4; l++)
+        if (l != idx) unsure = unsure || !(best - vals[l] > eb + errs[l]);
+    }
+    // one LDS atomic per wavefront: the first unsure lane reserves the slots
+    const unsigned long long ball = __ballot(unsure);
     if (ball) {
       const int lane = __lane_id();
       const int leader = __ffsll((long long)ball) - 1;
       unsigned base = 0;
-      if (lane == leader) base = atomicAdd(amb, (unsigned)__popcll(ball));
+      if (lane == leader) base = atomicAdd(&s_cnt, (unsigned)__popcll(ball));
       base = __shfl(base, leader);
-      if (guarded) {
-        const unsigned slot = base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
-        if (slot < amb_cap) amb_list[slot] = (int)(m * 3 + d);
-      }
+      if (unsure) region[base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull))] = (unsigned)(m * 3 + d);
     }
     s |= (uint8_t)(idx << (2 * d));
   }
-  sel[m] = s;
+  if (valid) sel[m] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = s_cnt;
 }
 
-// FP64 re-decision of guarded selections with the reference's test
-__global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
-                                                    const unsigned* __restrict__ amb, const int* __restrict__ amb_list,
-                                                    long amb_cap) {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  long cnt = min((long)*amb, amb_cap);
-  if (t >= cnt) return;
-  int e = amb_list[t];
-  long m = e / 3;
-  int d = e % 3;
+// Exclusive offsets of the per-block counts and their total (one block of
+// 1024 threads; each thread sums a contiguous run of counts).
+__global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ cnt, long nb,
+                                                   unsigned* __restrict__ off, unsigned* __restrict__ total) {
+  __shared__ unsigned part[1024];
+  const int t = threadIdx.x;
+  const long per = (nb + 1023) / 1024;
+  const long b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  unsigned sum = 0;
+  for (long b = b0; b < b1; b++) sum += cnt[b];
+  part[t] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the 1024 partial sums
+  for (int o = 1; o < 1024; o <<= 1) {
+    const unsigned x = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  unsigned run = part[t] - sum;
+  for (long b = b0; b < b1; b++) {
+    off[b] = run;
+    run += cnt[b];
+  }
+  if (t == 1023) *total = part[1023];
+}
+
+// blocks of the grid-stride re-decision kernel (128 threads each)
+#define SEL_FIX_BLOCKS 1024
+
+// the reference's selection (select_d, evaluated as written) of one (cell,
+// axis) of the stored FP32 field, wall ghosts by the C_kabe rule
+__device__ __forceinline__ int select_ref(const Grid& g, const float* __restrict__ C, long m, int d) {
   int i, j, k;
   cell_ijk(g, m, i, j, k);
-  int di = d == 0, dj = d == 1, dk = d == 2;
+  const int di = d == 0, dj = d == 1, dk = d == 2;
   double cm[6], c[6], cp[6];
   for (int q = 0; q < 6; q++) {
     const float* Cq = C + q * g.n;
@@ -296,17 +401,54 @@ __global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restr
     c[q] = (double)Cq[m];
     cp[q] = (double)ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
   }
-  int idx = select_d(cm, c, cp);
-  // the byte holds three axes: update only this axis
-  unsigned* word = (unsigned*)((size_t)(sel + m) & ~(size_t)3);
-  int sh = 8 * (int)((size_t)(sel + m) & 3) + 2 * d;
-  unsigned mask = 3u << sh;
-  unsigned old = *word, assumed;
-  do {
-    assumed = old;
-    unsigned nv = (assumed & ~mask) | ((unsigned)idx << sh);
-    old = atomicCAS(word, assumed, nv);
-  } while (old != assumed);
+  return select_d(cm, c, cp);
+}
+
+// FP64 re-decision of the uncertified selections: grid-stride over the
+// total, each entry found in its block's region by a binary search of the
+// exclusive offsets
+__global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
+                                                    const unsigned* __restrict__ total,
+                                                    const unsigned* __restrict__ off, long nb,
+                                                    const unsigned* __restrict__ amb_list) {
+  const long cnt = (long)*total;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < cnt; t += (long)gridDim.x * blockDim.x) {
+    // largest b with off[b] <= t (that block holds entry t)
+    long lo = 0, hi = nb - 1;
+    while (lo < hi) {
+      const long mid = (lo + hi + 1) >> 1;
+      if ((long)off[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const unsigned e = amb_list[(size_t)lo * SEL_REGION + (size_t)(t - (long)off[lo])];
+    const long m = (long)(e / 3u);
+    const int d = (int)(e % 3u);
+    const int idx = select_ref(g, C, m, d);
+    // the byte holds three axes: update only this axis
+    unsigned* word = (unsigned*)((size_t)(sel + m) & ~(size_t)3);
+    const int sh = 8 * (int)((size_t)(sel + m) & 3) + 2 * d;
+    const unsigned mask = 3u << sh;
+    unsigned old = *word, assumed;
+    do {
+      assumed = old;
+      const unsigned nv = (assumed & ~mask) | ((unsigned)idx << sh);
+      old = atomicCAS(word, assumed, nv);
+    } while (old != assumed);
+  }
+}
+
+// count of (cell, axis) whose selection differs from the reference's
+// selection of the stored field
+__global__ __launch_bounds__(256) void m_select_check(Grid g, const float* __restrict__ C,
+                                                      const uint8_t* __restrict__ sel, unsigned* __restrict__ bad) {
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  unsigned nb = 0;
+  if (m < g.n) {
+    const unsigned s = sel[m];
+    for (int d = 0; d < 3; d++) nb += (unsigned)(select_ref(g, C, m, d) != (int)((s >> (2 * d)) & 3u));
+  }
+  for (int o = 32; o > 0; o >>= 1) nb += __shfl_xor(nb, o);
+  if (__lane_id() == 0 && nb) atomicAdd(bad, nb);
 }
 
 // Peterlin factor: FP32 unless tr C is within 1% of L^2 (then FP64)
@@ -1196,8 +1338,14 @@ void mixed_create(Ctx& c) {
     int m = rho > 0.0 ? (int)std::ceil(std::log(1.0e-8) / std::log(rho)) : 1;
     X->helm_iters = (rho < 0.25 && m <= 6) ? std::max(1, m) : 0;
   }
-  X->amb_cap = std::max<long>(1024, 3 * n / 8);
-  dalloc(&X->amb, 1); dalloc(&X->amb_list, X->amb_cap);
+  // every (cell, axis) may need the FP64 decision: a region of 3 * 256
+  // entries per select block (3 n for n >= 256; m * 3 + axis < 2^32 as every
+  // axis has at most 1024 cells)
+  {
+    const long nb = nblk(n, SEL_BLOCK);
+    dalloc(&X->amb, 1); dalloc(&X->amb_list, nb * SEL_REGION); dalloc(&X->chk, 1);
+    dalloc(&X->amb_cnt, nb); dalloc(&X->amb_off, nb);
+  }
   // FP32 twiddles and wavenumbers from the FP64 host tables
   auto tw32 = [&](const double2* d, int nn, float2** out) {
     std::vector<double2> h(nn / 2);
@@ -1227,7 +1375,8 @@ void mixed_destroy(Ctx& c) {
   dfree(X->spec); dfree(X->thp); dfree(X->twx); dfree(X->twy); dfree(X->twz);
   dfree(X->kx2); dfree(X->ky2); dfree(X->kz2);
   dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->part); dfree(X->skip);
-  dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list);
+  dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
+  dfree(X->amb_cnt); dfree(X->amb_off);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
   dfree(X->fr_row); dfree(X->fr_len); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
@@ -1379,6 +1528,36 @@ double mixed_last_ndc(Ctx& c) {
   return skip ? 1.0 : (double)ndc;
 }
 
+// C+- selection of the current stored field: certified FP32 decisions, the
+// per-block list counts scanned, the rest re-decided in FP64.  tm != 0 times
+// the two phases.
+static void select_fenep(Ctx& c, int tm);
+
+// Selection of the current stored field as the step makes it (certified FP32
+// decisions, FP64 re-decision of the rest), compared with the reference's
+// selection of every (cell, axis).  out[0] = re-decided in FP64, out[1] =
+// differing decisions, out[2] = decisions (3 n).  Leaves the selection bytes
+// as the next step recomputes them anyway.
+int mixed_check_selection(Ctx& c, double* out3) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  out3[0] = out3[1] = 0.0;
+  out3[2] = 3.0 * (double)g.n;
+  if (c.p.beta == 1.0) return 0;   // no polymer stress: no selection
+  zero_words(X.chk, 1, s);
+  select_fenep(c, 0);
+  hipLaunchKernelGGL(m_select_check, dim3(nblk(g.n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.chk);
+  FP_HIP_CHECK(hipGetLastError());
+  unsigned h[2] = {0, 0};
+  FP_HIP_CHECK(hipMemcpyAsync(&h[0], X.amb, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FP_HIP_CHECK(hipMemcpyAsync(&h[1], X.chk, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FP_HIP_CHECK(hipStreamSynchronize(s));
+  out3[0] = (double)h[0];
+  out3[1] = (double)h[1];
+  return 0;
+}
+
 // statistics of the last step: [0] |b|^2, [1] |r|^2 before the last
 // correction, [2] guarded selections re-decided in FP64
 int mixed_stats(Ctx& c, double* out3) {
@@ -1394,21 +1573,38 @@ int mixed_stats(Ctx& c, double* out3) {
 namespace {
 struct MTimer {
   Ctx& c;
-  const char* name;
+  const char* name;   // nullptr: not timed
   hipEvent_t a = nullptr, b = nullptr;
   MTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
-    if (!c.timing) return;
+    if (!c.timing || !name) return;
     a = ev_get(c); b = ev_get(c);
     (void)hipEventRecord(a, c.stream);
   }
   ~MTimer() {
-    if (!c.timing) return;
+    if (!c.timing || !name) return;
     (void)hipEventRecord(b, c.stream);
     c.ev.push_back({std::string(name) + "#a", a});
     c.ev.push_back({std::string(name) + "#b", b});
   }
 };
 }  // namespace
+
+static void select_fenep(Ctx& c, int tm) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const long nb = nblk(g.n, SEL_BLOCK);
+  {
+    MTimer t(c, tm ? "fenep_select" : nullptr);
+    hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list);
+  }
+  {
+    MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr);
+    hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb);
+    hipLaunchKernelGGL(m_select_fix, dim3(SEL_FIX_BLOCKS), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_off, nb,
+                       X.amb_list);
+  }
+}
 
 void mixed_step(Ctx& c) {
   Mixed& X = MX(c);
@@ -1419,17 +1615,7 @@ void mixed_step(Ctx& c) {
   if (!polymer) {
     zero_words(X.C[X.cur], 6 * n, s);
   } else {
-    zero_words(X.amb, 1, s);
-    {
-      MTimer t(c, "fenep_select");
-      hipLaunchKernelGGL(m_select<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_list,
-                         X.amb_cap);
-    }
-    {
-      MTimer t(c, "fenep_select_fp64_fix");
-      hipLaunchKernelGGL(m_select_fix, dim3(nblk(X.amb_cap, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.amb,
-                         X.amb_list, X.amb_cap);
-    }
+    select_fenep(c, 1);
     {
       MTimer t(c, "fenep_cstar_newton");
       int itmax = std::min(c.p.newton_itrmax, 16);

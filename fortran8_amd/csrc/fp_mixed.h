@@ -14,5 +14,6 @@ void mixed_set_markers(Ctx& c);
 const double* mixed_marker_forces(Ctx& c);
 double mixed_last_ndc(Ctx& c);
 int mixed_stats(Ctx& c, double* out3);
+int mixed_check_selection(Ctx& c, double* out3);
 void mixed_step(Ctx& c);
 }  // namespace fp

@@ -167,6 +167,15 @@ int fp_time_stats(void* ctx, double* out);
  * only.  sum(E) over all shells equals K of fp_time_stats (Parseval). */
 int fp_energy_spectrum(void* ctx, double* E, int nbins);
 
+/* Mixed precision: the C+- selection of the current stored (FP32) field as
+ * the next step makes it (FP32 decisions that carry an error certificate, the
+ * rest re-decided with the reference's formula in FP64), compared with the
+ * reference's selection (fenep.f90:186-221 with the Vieta test of
+ * fenep.f90:425-444, in FP64) of every cell and axis.  out[3] = decisions
+ * re-decided in FP64, decisions that differ from the reference's, decisions
+ * in total (3 NX NY NZ).  The state is not changed. */
+int fp_check_selection(void* ctx, double* out);
+
 /* ---- Sensitivities (FP64 path, periodic grids, FFT Poisson, methods 1 and
  * 2).  The reference has no derivative code: these entry points serve the
  * discrete adjoint of the step it advances (fenep.f90:2477-2527) with respect

@@ -76,6 +76,32 @@ def test_mixed_throughput_layout_small():
     assert e["u"] < TOL and e["C"] < TOL
 
 
+def check_selection_along_run(g, steps, every, label):
+    worst = 0
+    for s in range(0, steps + 1, every):
+        if s:
+            g.step(every)
+        r = g.check_selection()
+        print(label, "step", s, r, flush=True)
+        worst = max(worst, r["differ"])
+        assert r["total"] == 3 * g.nx * g.ny * g.nz
+    return worst
+
+
+def test_mixed_selection_is_the_reference_selection():
+    """Certified FP32 decisions plus the FP64 re-decision reproduce the
+    reference's C+- selection of the stored field at every cell and axis:
+    from the isotropic initial state (every candidate at the sentinel) through
+    developed states, periodic and with walls, IBM and the full-size case."""
+    cases = [("kolmogorov 32^3", mixed(configs.kolmogorov(n=32)), 60, 10),
+             ("throughput 64x64x16", mixed(configs.throughput(nx=64, ny=64, nz=16, ncyl=2)), 40, 10),
+             ("throughput 256x128x128", mixed(configs.throughput()), 30, 10)]
+    for label, case, steps, every in cases:
+        g = api.load_case(case)
+        assert check_selection_along_run(g, steps, every, label) == 0, label
+        g.close()
+
+
 def test_mixed_rejects_unsupported_combinations():
     for kw in (dict(method=METHOD_EXPLICIT), dict(poisson_variant=POISSON_SOR)):
         p = default_params(nx=8, ny=8, nz=8, precision=PREC_MIXED, **kw)
@@ -114,12 +140,16 @@ def test_mixed_throughput_1000_steps_against_fp64():
     e["ke"] = abs(dg["ke"] - dh["ke"]) / abs(dh["ke"])
     e["div_max_mixed"] = dg["div_max"]
     e["div_max_f64"] = dh["div_max"]
+    sel = g.check_selection()
+    e["sel_fp64_fraction"] = sel["fp64"] / sel["total"]
+    e["sel_differ"] = sel["differ"]
     print(json.dumps({k: float(v) for k, v in e.items()}), flush=True)
     # Measured bounds.  The wakes of the 8 cylinders are unstable at Re_D =
     # 200: the FP64 step started from the FP32-rounded initial state alone
     # ends 2e-5 (u) and 3e-4 (C) away from the FP64 run after 1000 steps, so
     # an FP32-storage step, which rounds every step, cannot stay within 1e-4
     # in u and C here; the kinetic energy stays within 1e-4.
+    assert sel["differ"] == 0
     assert e["ke"] < TOL
     assert e["u"] < 1e-3 and e["P_dyn"] < 1e-3 and e["drag"] < 1e-3 and e["C"] < 1e-2
     assert dg["nan_flag"] == 0
