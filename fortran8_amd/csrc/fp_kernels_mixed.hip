@@ -358,35 +358,49 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
   if (threadIdx.x == 0) cnt[blockIdx.x] = s_cnt;
 }
 
-// Exclusive offsets of the per-block counts and their total (one block of
-// 1024 threads; each thread sums a contiguous run of counts).
+// Exclusive offsets of the per-block counts and their total: one block of
+// 1024 threads, the counts staged through LDS in chunks of 16384 with
+// coalesced loads, each thread scanning a contiguous run of 16.
+#define SCAN_CHUNK 16384
 __global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ cnt, long nb,
                                                    unsigned* __restrict__ off, unsigned* __restrict__ total) {
+  __shared__ unsigned buf[SCAN_CHUNK];
   __shared__ unsigned part[1024];
   const int t = threadIdx.x;
-  const long per = (nb + 1023) / 1024;
-  const long b0 = min(nb, t * per), b1 = min(nb, b0 + per);
-  unsigned sum = 0;
-  for (long b = b0; b < b1; b++) sum += cnt[b];
-  part[t] = sum;
-  __syncthreads();
-  // Hillis-Steele inclusive scan of the 1024 partial sums
-  for (int o = 1; o < 1024; o <<= 1) {
-    const unsigned x = t >= o ? part[t - o] : 0u;
+  unsigned carry = 0;
+  for (long c0 = 0; c0 < nb; c0 += SCAN_CHUNK) {
+    const long len = min((long)SCAN_CHUNK, nb - c0);
+    for (long b = t; b < len; b += 1024) buf[b] = cnt[c0 + b];
     __syncthreads();
-    part[t] += x;
+    const int r0 = t * (SCAN_CHUNK / 1024);
+    unsigned sum = 0;
+#pragma unroll
+    for (int r = 0; r < SCAN_CHUNK / 1024; r++) sum += r0 + r < len ? buf[r0 + r] : 0u;
+    part[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the 1024 partial sums
+    for (int o = 1; o < 1024; o <<= 1) {
+      const unsigned x = t >= o ? part[t - o] : 0u;
+      __syncthreads();
+      part[t] += x;
+      __syncthreads();
+    }
+    unsigned run = carry + part[t] - sum;
+#pragma unroll
+    for (int r = 0; r < SCAN_CHUNK / 1024; r++) {
+      if (r0 + r < len) {
+        const unsigned x = buf[r0 + r];
+        buf[r0 + r] = run;
+        run += x;
+      }
+    }
+    carry += part[1023];
+    __syncthreads();
+    for (long b = t; b < len; b += 1024) off[c0 + b] = buf[b];
     __syncthreads();
   }
-  unsigned run = part[t] - sum;
-  for (long b = b0; b < b1; b++) {
-    off[b] = run;
-    run += cnt[b];
-  }
-  if (t == 1023) *total = part[1023];
+  if (t == 0) *total = carry;
 }
-
-// blocks of the grid-stride re-decision kernel (128 threads each)
-#define SEL_FIX_BLOCKS 1024
 
 // the reference's selection (select_d, evaluated as written) of one (cell,
 // axis) of the stored FP32 field, wall ghosts by the C_kabe rule
@@ -407,7 +421,7 @@ __device__ __forceinline__ int select_ref(const Grid& g, const float* __restrict
 // FP64 re-decision of the uncertified selections: grid-stride over the
 // total, each entry found in its block's region by a binary search of the
 // exclusive offsets
-__global__ __launch_bounds__(128) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
+__global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                     const unsigned* __restrict__ total,
                                                     const unsigned* __restrict__ off, long nb,
                                                     const unsigned* __restrict__ amb_list) {
@@ -482,20 +496,19 @@ __device__ __forceinline__ Nbr nbr(const Grid& g, int i, int j, int k) {
   return o;
 }
 
-// candidate face values in the arithmetic type T
+// coefficients (of c_{a-1}, c_a, c_{a+1}) of the candidate face values of
+// fenep.f90:179-184: left face Cp_l and right face Cn_l, l = 3 the zero slope
 template <typename T>
-__device__ __forceinline__ T cand_pt(int l, T cm, T c, T cp) {
-  if (l == 0) return c * T(1.5) - cp * T(0.5);
-  if (l == 1) return T(0.5) * (cm + c);
-  if (l == 2) return (T(0.25) * cm + c) - T(0.25) * cp;
-  return c;
+__device__ __forceinline__ void cand_coef_p(int l, T& a, T& b, T& c) {
+  a = l == 1 ? T(0.5) : (l == 2 ? T(0.25) : T(0));
+  b = l == 0 ? T(1.5) : (l == 1 ? T(0.5) : T(1));
+  c = l == 0 ? T(-0.5) : (l == 2 ? T(-0.25) : T(0));
 }
 template <typename T>
-__device__ __forceinline__ T cand_nt(int l, T cm, T c, T cp) {
-  if (l == 0) return T(0.5) * (c + cp);
-  if (l == 1) return T(-0.5) * cm + T(1.5) * c;
-  if (l == 2) return (c - T(0.25) * cm) + T(0.25) * cp;
-  return c;
+__device__ __forceinline__ void cand_coef_n(int l, T& a, T& b, T& c) {
+  a = l == 1 ? T(-0.5) : (l == 2 ? T(-0.25) : T(0));
+  b = l == 0 ? T(0.5) : (l == 1 ? T(1.5) : T(1));
+  c = l == 0 ? T(0.5) : (l == 2 ? T(0.25) : T(0));
 }
 __device__ __forceinline__ float rcp_t(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ __forceinline__ double rcp_t(double v) { return 1.0 / v; }
@@ -545,17 +558,26 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     const long om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
     const long op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
     const int lm = (sel[m + om] >> (2 * d)) & 3, l0 = (sel[m] >> (2 * d)) & 3, lp = (sel[m + op] >> (2 * d)) & 3;
+    // The four face values are linear in the line c0..c4:
+    //   pl = Cp_l0(c1,c2,c3)  own left face      nr = Cn_l0(c1,c2,c3) own right face
+    //   pr = Cp_lp(c2,c3,c4)  left face of a+1   nl = Cn_lm(c0,c1,c2) right face of a-1
+    // so s3 pl + s1 pr + s4 nl + s2 nr = sum_p w_p c_p with five weights per
+    // axis shared by the six components.
+    T ap, bp, gp_, an, bn, gn, aq, bq, gq, am, bm, gm;
+    cand_coef_p<T>(l0, ap, bp, gp_);
+    cand_coef_n<T>(l0, an, bn, gn);
+    cand_coef_p<T>(lp, aq, bq, gq);
+    cand_coef_n<T>(lm, am, bm, gm);
+    const T w0 = s4 * am;
+    const T w1 = s3 * ap + s2 * an + s4 * bm;
+    const T w2 = s3 * bp + s2 * bn + s1 * aq + s4 * gm;
+    const T w3 = s3 * gp_ + s2 * gn + s1 * bq;
+    const T w4 = s1 * gq;
     float v[5][6];
     load_line<2, GEN>(C, g, m, i, j, k, d, v);
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-      const T c0 = v[0][q], c1 = v[1][q], c2 = v[2][q], c3 = v[3][q], c4 = v[4][q];
-      const T pl = cand_pt<T>(l0, c1, c2, c3);   // Cp(a-1): own left face
-      const T nr = cand_nt<T>(l0, c1, c2, c3);   // Cn(a):   own right face
-      const T pr = cand_pt<T>(lp, c2, c3, c4);   // Cp(a):   left face of a+1
-      const T nl = cand_nt<T>(lm, c0, c1, c2);   // Cn(a-1): right face of a-1
-      cx[q] += s3 * pl + s1 * pr + s4 * nl + s2 * nr;
-    }
+    for (int q = 0; q < 6; q++)
+      cx[q] += w0 * (T)v[0][q] + w1 * (T)v[1][q] + w2 * (T)v[2][q] + w3 * (T)v[3][q] + w4 * (T)v[4][q];
   }
   const T dtw = dt / (T)g.wi;
   T b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
@@ -1330,12 +1352,14 @@ void mixed_create(Ctx& c) {
   dalloc(&X->skip, 1);
   dalloc(&X->R, 3 * n); dalloc(&X->X2, 3 * n);
   {
-    // fixed-point Helmholtz sweeps: contraction eps*lambda_max per sweep,
-    // enough sweeps for a 1e-8 relative error, spectral solve if > 6
+    // fixed-point Helmholtz sweeps x <- q + eps Lap x from x = q: the error
+    // starts at eps Lap x* (<= rho |x*|, rho = eps lambda_max) and contracts
+    // by rho per sweep, so k sweeps leave rho^(k+1) |x*|; enough sweeps for
+    // 1e-8 (below FP32 rounding), spectral solve if more than 6
     const double eps = ((g.dt * g.beta) / g.re) / 2.0;
     const double lam = 4.0 / (g.dx * g.dx) + 4.0 / (g.dy * g.dy) + 4.0 / (g.dz * g.dz);
     const double rho = eps * lam;
-    int m = rho > 0.0 ? (int)std::ceil(std::log(1.0e-8) / std::log(rho)) : 1;
+    int m = rho > 0.0 ? (int)std::ceil(std::log(1.0e-8) / std::log(rho)) - 1 : 1;
     X->helm_iters = (rho < 0.25 && m <= 6) ? std::max(1, m) : 0;
   }
   // every (cell, axis) may need the FP64 decision: a region of 3 * 256
@@ -1601,8 +1625,9 @@ static void select_fenep(Ctx& c, int tm) {
   {
     MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr);
     hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb);
-    hipLaunchKernelGGL(m_select_fix, dim3(SEL_FIX_BLOCKS), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_off, nb,
-                       X.amb_list);
+    // one wavefront per select block: 16 per SIMD at 256x128x128, enough to
+    // hide the search and FP64 latencies; n / 4 entries per pass
+    hipLaunchKernelGGL(m_select_fix, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_off, nb, X.amb_list);
   }
 }
 
