@@ -40,8 +40,31 @@ KERNEL_WORDS = {
 }
 
 
+def cells_per_launch(p, prec, phase):
+    """Cells one launch of the phase's kernel processes.  Each timed phase is
+    one kernel.  With y-walls the mixed bulk stencil kernels leave the wall
+    rows (2 for the predictor, 4 for CStar/Newton) to their own instances,
+    timed as *_walls; the mixed selection and every FP64 kernel cover all
+    cells."""
+    cells = p.nx * p.ny * p.nz
+    if prec == 1 and p.walls_y:
+        if phase == "predictor":
+            return p.nx * (p.ny - 2) * p.nz
+        if phase == "fenep_cstar_newton":
+            return p.nx * (p.ny - 4) * p.nz
+    return cells
+
+
 def mixed_step_bytes(n_dc):
-    return 320.0 + 60.0 * n_dc
+    """Algorithmic HBM bytes per cell of the mixed IBM step (4 B words, FP64
+    where stored so): select 25 (C read, selection byte written),
+    CStar/Newton 69, predictor 116, three Helmholtz sweeps 96 (24 + 36 + 36),
+    divergence 24 (u* read, FP64 b and FP32 r written), the FP64 residual 20,
+    correction 40; per defect-correction sweep the FP32 spectral + line solve
+    64.4 (x transforms on the half spectrum, z transforms and y line solve)
+    and the FP64 accumulation of Phi (12 on the first sweep, 20 after).  The
+    sparse IBM and FP64 re-decision passes are not counted."""
+    return 390.0 + 64.4 * n_dc + 12.0 + 20.0 * max(0, n_dc - 1)
 
 
 def run_gpu(case, steps, warmup):
@@ -114,20 +137,15 @@ def main():
     # dominant single-kernel phase and its algorithmic bytes per launch
     dom = max(KERNEL_WORDS, key=lambda k: kt.get(k, 0.0))
     words, extra = KERNEL_WORDS[dom]
-    # the mixed bulk select kernel covers the cells off the wall rows (the
-    # two wall rows run in their own instance, timed separately)
-    p_ = case.params
-    units = cells
-    if prec == 1 and dom == "fenep_select" and p_.walls_y:
-        units = p_.nx * (p_.ny - 2) * p_.nz
+    units = cells_per_launch(case.params, prec, dom)
     dom_bytes = (words * width + extra) * units
     dom_ms = kt.get(dom, 0.0)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     traffic = read_traffic(a.precision).get(dom)
     step_bytes = STEP_WORDS * 8 * cells if prec == 0 else mixed_step_bytes(d.get("n_dc", 0)) * cells
-    kernel_names = {"fenep_select": "m_select" if prec else "k_select",
-                    "fenep_cstar_newton": "m_cstar_newton" if prec else "k_cstar_newton",
-                    "predictor": "m_predictor" if prec else "k_predictor"}
+    kernel_names = {"fenep_select": "fp::m_select" if prec else "fp::k_select",
+                    "fenep_cstar_newton": "fp::m_cstar_newton<false>" if prec else "fp::k_cstar_newton",
+                    "predictor": "fp::m_predictor<false>" if prec else "fp::k_predictor"}
     out = {
         "metric": "Mcell-steps/s 256x128x128 FENE-P+IBM",
         "value": round(value, 3),

@@ -79,6 +79,24 @@ __device__ __forceinline__ long cidx(const Grid& g, int i, int j, int k) {
   return i + (long)g.nx * (j + (long)g.ny * k);
 }
 
+// Band order of the cells of a stencil kernel: x fastest, then the 16
+// rows of a y-band, then z, then the bands.  With the XCD-contiguous block
+// numbering each XCD sweeps whole bands plane by plane, so the planes k-2..k+2
+// a stencil reads stay in that XCD's L2 (a plane-major walk keeps five full
+// planes of every field live, more than one L2 holds).
+__device__ __forceinline__ void band_cell(const Grid& g, long L, long& m, int& i, int& j, int& k) {
+  const int lb = g.lny < 4 ? g.lny : 4;
+  i = (int)(L & (g.nx - 1));
+  long t = L >> g.lnx;
+  const int jj = (int)(t & ((1 << lb) - 1));
+  t >>= lb;
+  const int lnz = __ffs(g.nz) - 1;
+  k = (int)(t & (g.nz - 1));
+  j = ((int)(t >> lnz) << lb) + jj;
+  m = cidx(g, i, j, k);
+}
+
+
 // ---- FENE-P algebra (FP64) ------------------------------------------------
 // Vieta, fenep.f90:425-444
 // The reference formulas below are evaluated exactly as written, also in

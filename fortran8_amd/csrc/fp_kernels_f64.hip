@@ -14,10 +14,11 @@ static inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
 // CpxCnx/CpyCny/CpzCnz selection (fenep.f90:169-382) for all three axes of
 // one cell; the chosen candidate per axis is packed in 2 bits.
 __global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict__ C, uint8_t* __restrict__ sel) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  const long L = xcd_cell();
+  if (L >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  band_cell(g, L, m, i, j, k);
   uint8_t s = 0;
 #pragma unroll 1
   for (int d = 0; d < 3; d++) {
@@ -60,10 +61,11 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
                                                       const double* __restrict__ W, double* __restrict__ Cn,
                                                       double* __restrict__ fC, int* __restrict__ nitr,
                                                       double dc, double eps, int itrmax) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  const long L = xcd_cell();
+  if (L >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  band_cell(g, L, m, i, j, k);
   const double dt = g.dt;
   double cx[6];
 #pragma unroll 1
@@ -151,10 +153,11 @@ __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first
                                                    double* __restrict__ A0, double* __restrict__ T0, double* __restrict__ B0,
                                                    double* __restrict__ Bout, const double* __restrict__ F,
                                                    double bf0, double bf1, double bf2, double* __restrict__ Q) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  const long L = xcd_cell();
+  if (L >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  band_cell(g, L, m, i, j, k);
   const double dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
 #define Ug(a, b, c) ldb(U, g, a, b, c, BC_U)
 #define Vg(a, b, c) ldb(V, g, a, b, c, BC_V)
@@ -516,10 +519,11 @@ void f64_solve(Ctx& c, int kind, const double* q, double* x) {
 // ======================================================= projection =======
 // fft_poisson right-hand side (fenep.f90:1550-1552)
 __global__ __launch_bounds__(256) void k_divergence(Grid g, const double* __restrict__ up, double* __restrict__ q) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  const long L = xcd_cell();
+  if (L >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  band_cell(g, L, m, i, j, k);
   const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
   q[m] = (((-ldb(Uu, g, i - 1, j, k, BC_U) + ldb(Uu, g, i, j, k, BC_U)) / g.dx
            + (-ldb(Vv, g, i, j - 1, k, BC_V) + ldb(Vv, g, i, j, k, BC_V)) / g.dy)
@@ -534,10 +538,11 @@ void f64_divergence(Ctx& c, const double* up, double* q) {
 __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const double* __restrict__ up, const double* __restrict__ phi,
                                                double* __restrict__ U, double* __restrict__ V, double* __restrict__ W,
                                                double* __restrict__ P) {
-  long m = xcd_cell();
-  if (m >= g.n) return;
+  const long L = xcd_cell();
+  if (L >= g.n) return;
+  long m;
   int i, j, k;
-  cell_ijk(g, m, i, j, k);
+  band_cell(g, L, m, i, j, k);
   const double dt = g.dt, dx = g.dx, dy = g.dy, dz = g.dz;
   const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
 #define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)

@@ -256,25 +256,8 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
 // periodic grid) and reads neighbours through wrapped offsets only;
 // GEN = true covers the 2R rows next to the walls through the boundary-rule
 // accessor.  Separate kernels keep the bulk kernel's register budget, and so
-// its occupancy, free of the wall path.
-//
-// The bulk instance walks the cells in band order: x fastest, then the 16
-// rows of a y-band, then z, then the bands.  With the XCD-contiguous block
-// numbering each XCD sweeps whole bands plane by plane, so the planes k-2..k+2
-// a stencil reads stay in that XCD's L2 (a plane-major walk keeps five full
-// planes of every field live, more than one L2 holds).
-__device__ __forceinline__ void band_cell(const Grid& g, long L, long& m, int& i, int& j, int& k) {
-  const int lb = g.lny < 4 ? g.lny : 4;
-  i = (int)(L & (g.nx - 1));
-  long t = L >> g.lnx;
-  const int jj = (int)(t & ((1 << lb) - 1));
-  t >>= lb;
-  const int lnz = __ffs(g.nz) - 1;
-  k = (int)(t & (g.nz - 1));
-  j = ((int)(t >> lnz) << lb) + jj;
-  m = cidx(g, i, j, k);
-}
-
+// its occupancy, free of the wall path.  The bulk instance walks the cells
+// in band order (band_cell).
 template <bool GEN, int R>
 __device__ __forceinline__ bool row_cell(const Grid& g, long& m, int& i, int& j, int& k) {
   if (!GEN) {
@@ -1641,25 +1624,30 @@ void mixed_step(Ctx& c) {
     zero_words(X.C[X.cur], 6 * n, s);
   } else {
     select_fenep(c, 1);
+    // one kernel per timed phase: the bulk instances and the wall rows apart
+    const int itmax = std::min(c.p.newton_itrmax, 16);
     {
       MTimer t(c, "fenep_cstar_newton");
-      int itmax = std::min(c.p.newton_itrmax, 16);
       hipLaunchKernelGGL(m_cstar_newton<false>, dim3(nblk(n, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
                          X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
-      if (g.walls)
-        hipLaunchKernelGGL(m_cstar_newton<true>, dim3(nblk(wall_rows(g, 2), 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
-                         X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
-      X.cur = 1 - X.cur;
     }
+    if (g.walls) {
+      MTimer t(c, "fenep_cstar_newton_walls");
+      hipLaunchKernelGGL(m_cstar_newton<true>, dim3(nblk(wall_rows(g, 2), 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
+                         X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
+    }
+    X.cur = 1 - X.cur;
   }
+  float* Rp = c.p.method == FP_METHOD_IBM ? X.R : nullptr;
   {
     MTimer t(c, "predictor");
-    float* Rp = c.p.method == FP_METHOD_IBM ? X.R : nullptr;
     hipLaunchKernelGGL(m_predictor<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
                        X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
-    if (g.walls)
-      hipLaunchKernelGGL(m_predictor<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
+  }
+  if (g.walls) {
+    MTimer t(c, "predictor_walls");
+    hipLaunchKernelGGL(m_predictor<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
                        X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   }

@@ -126,7 +126,8 @@ def test_mixed_full_size_short_run_is_finite_and_close():
 def test_mixed_throughput_1000_steps_against_fp64():
     """The chosen precision form at 256x128x128 with 8 cylinders after 1000
     steps against the FP64 GPU step (bit-identical to the oracle): relative
-    L2 of u, p, C and the drag and kinetic energy within 1e-4."""
+    L2 of u, the dynamic pressure, the drag and the kinetic energy within
+    1e-4; C within its FP32-storage bound."""
     import json
     steps = 1000
     print("mixed vs FP64, 256x128x128, %d steps" % steps, flush=True)
@@ -144,12 +145,14 @@ def test_mixed_throughput_1000_steps_against_fp64():
     e["sel_fp64_fraction"] = sel["fp64"] / sel["total"]
     e["sel_differ"] = sel["differ"]
     print(json.dumps({k: float(v) for k, v in e.items()}), flush=True)
-    # Measured bounds.  The wakes of the 8 cylinders are unstable at Re_D =
-    # 200: the FP64 step started from the FP32-rounded initial state alone
-    # ends 2e-5 (u) and 3e-4 (C) away from the FP64 run after 1000 steps, so
-    # an FP32-storage step, which rounds every step, cannot stay within 1e-4
-    # in u and C here; the kinetic energy stays within 1e-4.
+    # The C+- selection is the reference's selection of the stored field at
+    # every cell (certified FP32 decisions, FP64 re-decision of the rest), so
+    # u, p, the drag and the kinetic energy stay within 1e-4.  C does not:
+    # the wakes of the 8 cylinders are unstable at Re_D = 200, and the FP64
+    # step started from the FP32-rounded initial state alone ends 3.3e-4 away
+    # from the FP64 run in C after 1000 steps; an FP32-storage step rounds
+    # every step (measured 1.2e-3).
     assert sel["differ"] == 0
-    assert e["ke"] < TOL
-    assert e["u"] < 1e-3 and e["P_dyn"] < 1e-3 and e["drag"] < 1e-3 and e["C"] < 1e-2
+    assert e["ke"] < TOL and e["u"] < TOL and e["P_dyn"] < TOL and e["drag"] < TOL
+    assert e["C"] < 3e-3
     assert dg["nan_flag"] == 0
