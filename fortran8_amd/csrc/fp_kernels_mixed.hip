@@ -39,6 +39,10 @@ struct Mixed {
   double* red = nullptr;          // [0]=|b|^2, [1]=|r|^2 of the last sweep
   double* part = nullptr;         // per-block partial sums
   int* skip = nullptr;            // defect correction converged: skip the sweep
+  double* partb = nullptr;        // per-block |b|^2 of the fused divergence
+  int* dc_host = nullptr;         // pinned copy of the last periodic decision
+  hipEvent_t dc_ev = nullptr;
+  bool dc_pending = false, dc_correct = false;
   float *R = nullptr, *X2 = nullptr;  // Helmholtz right-hand side / iterate
   int helm_iters = 0;             // fixed-point sweeps (0: spectral solve)
   unsigned* amb = nullptr;        // uncertified selections, re-decided in FP64
@@ -342,45 +346,44 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
 }
 
 // Exclusive offsets of the per-block counts and their total: one block of
-// 1024 threads, the counts staged through LDS in chunks of 16384 with
-// coalesced loads, each thread scanning a contiguous run of 16.
-#define SCAN_CHUNK 16384
+// 1024 threads over rows of 1024 counts (16 rows per pass, all loads of a
+// pass in flight together); each row is scanned with wavefront prefix sums
+// and a 16-entry LDS combine, so every access is coalesced or a broadcast.
+#define SCAN_ROWS 16
 __global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ cnt, long nb,
                                                    unsigned* __restrict__ off, unsigned* __restrict__ total) {
-  __shared__ unsigned buf[SCAN_CHUNK];
-  __shared__ unsigned part[1024];
-  const int t = threadIdx.x;
+  __shared__ unsigned wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   unsigned carry = 0;
-  for (long c0 = 0; c0 < nb; c0 += SCAN_CHUNK) {
-    const long len = min((long)SCAN_CHUNK, nb - c0);
-    for (long b = t; b < len; b += 1024) buf[b] = cnt[c0 + b];
-    __syncthreads();
-    const int r0 = t * (SCAN_CHUNK / 1024);
-    unsigned sum = 0;
+  for (long p0 = 0; p0 < nb; p0 += 1024L * SCAN_ROWS) {
+    unsigned x[SCAN_ROWS];
 #pragma unroll
-    for (int r = 0; r < SCAN_CHUNK / 1024; r++) sum += r0 + r < len ? buf[r0 + r] : 0u;
-    part[t] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the 1024 partial sums
-    for (int o = 1; o < 1024; o <<= 1) {
-      const unsigned x = t >= o ? part[t - o] : 0u;
-      __syncthreads();
-      part[t] += x;
-      __syncthreads();
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      const long b = p0 + 1024L * r + t;
+      x[r] = b < nb ? cnt[b] : 0u;
     }
-    unsigned run = carry + part[t] - sum;
 #pragma unroll
-    for (int r = 0; r < SCAN_CHUNK / 1024; r++) {
-      if (r0 + r < len) {
-        const unsigned x = buf[r0 + r];
-        buf[r0 + r] = run;
-        run += x;
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      unsigned v = x[r];
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(v, o);
+        if (lane >= o) v += y;
       }
+      if (lane == 63) wsum[w] = v;
+      __syncthreads();
+      unsigned pre = 0, tot = 0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const unsigned sq = wsum[q];
+        pre += q < w ? sq : 0u;
+        tot += sq;
+      }
+      const long b = p0 + 1024L * r + t;
+      if (b < nb) off[b] = carry + pre + v - x[r];
+      carry += tot;
+      __syncthreads();
     }
-    carry += part[1023];
-    __syncthreads();
-    for (long b = t; b < len; b += 1024) off[c0 + b] = buf[b];
-    __syncthreads();
   }
   if (t == 0) *total = carry;
 }
@@ -908,6 +911,103 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
   }
 }
 
+// The Poisson right-hand side b = div_h(u*)/dt (fft_poisson,
+// fenep.f90:1550-1552; FP64 from the FP32 u*) computed in the load stage of
+// the forward x transform, two real lines per complex transform as in
+// m_fft_x_fwd.  want_b: also store b (FP64) and the per-block |b|^2 for the
+// defect check.
+__global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __restrict__ up, float2* __restrict__ spec,
+                                                       const float2* __restrict__ tw, int nl, double* __restrict__ b,
+                                                       double* __restrict__ nrm, int want_b) {
+  extern __shared__ float2 ldsf[];
+  const int n = g.nx, nc = nl / 2, nxh = g.nxh;
+  float2* x = ldsf;
+  float2* y = ldsf + nc * n;
+  float2* tws = ldsf + 2 * nc * n;
+  load_twiddles(tws, tw, n);
+  const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+  const long nlines = (long)g.ny * g.nz;
+  const long l0 = (long)blockIdx.x * nl;
+  double ss = 0.0;
+  for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
+    const int c = e / n, i = e - c * n;
+    float2 v2 = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const long L = l0 + 2 * c + h;
+      if (L >= nlines) continue;
+      const int j = (int)(L % g.ny), k = (int)(L / g.ny);
+      const double v = (((double)ldb(Uu, g, i, j, k, BC_U) - (double)ldb(Uu, g, i - 1, j, k, BC_U)) / g.dx +
+                        ((double)ldb(Vv, g, i, j, k, BC_V) - (double)ldb(Vv, g, i, j - 1, k, BC_V)) / g.dy +
+                        ((double)ldb(Ww, g, i, j, k, BC_W) - (double)ldb(Ww, g, i, j, k - 1, BC_W)) / g.dz) / g.dt;
+      if (want_b) {
+        b[L * n + i] = v;
+        ss += v * v;
+      }
+      if (h == 0) v2.x = (float)v;
+      else v2.y = (float)v;
+    }
+    x[e] = v2;
+  }
+  if (want_b) {
+    __shared__ double sh[256];
+    const double sum = block_sum256(ss, sh);
+    if (threadIdx.x == 0) nrm[blockIdx.x] = sum;
+  }
+  __syncthreads();
+  const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, false);
+  for (int e = threadIdx.x; e < nc * nxh; e += blockDim.x) {
+    const int c = e / nxh, k = e - c * nxh;
+    const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
+    const long La = l0 + 2 * c, Lb = La + 1;
+    if (La < nlines) {
+      const long j = La % g.ny, kk = La / g.ny;
+      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
+    }
+    if (Lb < nlines) {
+      const long j = Lb % g.ny, kk = Lb / g.ny;
+      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
+    }
+  }
+}
+
+// Inverse x transform of the Poisson correction d with the FP64
+// accumulation of the potential in the store stage: phi = d (first sweep) or
+// phi + d.
+__global__ __launch_bounds__(256) void m_fft_x_inv_accum(Grid g, const float2* __restrict__ spec, double* __restrict__ phi,
+                                                         const float2* __restrict__ tw, int nl, int first,
+                                                         const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ float2 ldsf[];
+  const int n = g.nx, h = n / 2, nxh = g.nxh, nc = nl / 2;
+  float2* x = ldsf;
+  float2* y = ldsf + nc * n;
+  float2* tws = ldsf + 2 * nc * n;
+  load_twiddles(tws, tw, n);
+  const long nlines = (long)g.ny * g.nz;
+  const long l0 = (long)blockIdx.x * nl;
+  for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
+    const int c = e / n, i = e - c * n;
+    const long La = l0 + 2 * c, Lb = La + 1;
+    const int src = i <= h ? i : n - i;
+    float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
+    if (La < nlines) A = spec[src + (long)nxh * ((La % g.ny + g.jo) + g.sy * (La / g.ny))];
+    if (Lb < nlines) B = spec[src + (long)nxh * ((Lb % g.ny + g.jo) + g.sy * (Lb / g.ny))];
+    if (src == 0 || src == h) { A.y = 0.0f; B.y = 0.0f; }
+    if (i > h) { A.y = -A.y; B.y = -B.y; }
+    x[e] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, true);
+  for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
+    const int c = e / n, i = e - c * n;
+    const long La = l0 + 2 * c, Lb = La + 1;
+    const float2 z = r[e];
+    if (La < nlines) phi[La * n + i] = first ? (double)z.x : phi[La * n + i] + (double)z.x;
+    if (Lb < nlines) phi[Lb * n + i] = first ? (double)z.y : phi[Lb * n + i] + (double)z.y;
+  }
+}
+
 // y or z transforms of ti consecutive kx columns; lines are staged
 // column-wise into LDS with pitch n + 1.  mode 0 forward, 1 inverse, 2
 // forward + spectral divide (Helmholtz or Poisson symbol) + inverse.
@@ -1048,14 +1148,12 @@ static int lines_for32(int n) { return std::max(2, std::min(16, 4096 / n)) & ~1;
 // kx columns per block of the y/z transforms (pitch n + 1)
 static int cols_for32(int n) { return std::max(1, std::min(16, 3968 / (n + 1))); }
 
-static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip = nullptr) {
+// y and z transforms (periodic y) or z transforms and the y line solve
+// (walls) of the half spectrum, with the spectral divide of the solve kind
+static void solve32_yz(Ctx& c, int kind, const int* skip) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
   hipStream_t s = c.stream;
-  int lx = lines_for32(g.nx);
-  long xlines = (long)g.ny * g.nz;
-  size_t shx = sizeof(float2) * ((size_t)lx * g.nx + g.nx / 2);
-  hipLaunchKernelGGL(m_fft_x_fwd, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, q, X.spec, X.twx, lx, skip);
   int tiy = cols_for32(g.ny), tiz = cols_for32(g.nz);
   int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
   size_t shy = sizeof(float2) * (2 * (size_t)tiy * (g.ny + 1) + g.ny / 2);
@@ -1072,30 +1170,22 @@ static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip 
                        s, g, X.spec, kind, tl, X.kx2, X.kz2, skip);
     hipLaunchKernelGGL(m_fft_yz, dim3(ntz * g.ny), dim3(256), shz, s, g, X.spec, X.twz, 2, 1, kind, 1, tiz, X.kx2, X.ky2, X.kz2, skip);
   }
-  hipLaunchKernelGGL(m_fft_x_inv, dim3(nblk(xlines, lx)), dim3(256), shx, s, g, X.spec, x, X.twx, lx, skip);
+}
+
+static size_t lds_x32(const Grid& g) { return sizeof(float2) * ((size_t)lines_for32(g.nx) * g.nx + g.nx / 2); }
+static int blocks_x32(const Grid& g) { return nblk((long)g.ny * g.nz, lines_for32(g.nx)); }
+
+static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip = nullptr) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const int lx = lines_for32(g.nx);
+  hipLaunchKernelGGL(m_fft_x_fwd, dim3(blocks_x32(g)), dim3(256), lds_x32(g), s, g, q, X.spec, X.twx, lx, skip);
+  solve32_yz(c, kind, skip);
+  hipLaunchKernelGGL(m_fft_x_inv, dim3(blocks_x32(g)), dim3(256), lds_x32(g), s, g, X.spec, x, X.twx, lx, skip);
 }
 
 // ============================================================ projection ===
-// b = div_h(u*)/dt in FP64 from the FP32 u*; also |b|^2
-__global__ __launch_bounds__(256) void m_divergence(Grid g, const float* __restrict__ up, double* __restrict__ b,
-                                                    float* __restrict__ b32, double* __restrict__ nrm) {
-  long m = xcd_cell();
-  double v = 0.0;
-  if (m < g.n) {
-    int i, j, k;
-  cell_ijk(g, m, i, j, k);
-    const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
-    v = (((double)ldb(Uu, g, i, j, k, BC_U) - (double)ldb(Uu, g, i - 1, j, k, BC_U)) / g.dx +
-         ((double)ldb(Vv, g, i, j, k, BC_V) - (double)ldb(Vv, g, i, j - 1, k, BC_V)) / g.dy +
-         ((double)ldb(Ww, g, i, j, k, BC_W) - (double)ldb(Ww, g, i, j, k - 1, BC_W)) / g.dz) / g.dt;
-    b[m] = v;
-    b32[m] = (float)v;
-  }
-  __shared__ double sh[256];
-  double sum = block_sum256(v * v, sh);
-  if (threadIdx.x == 0) nrm[blockIdx.x] = sum;
-}
-
 // r = b - L_h phi (FP64, boundary rules of Phi), rounded to FP32 for the
 // correction solve; |r|^2 accumulated
 __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restrict__ b, const double* __restrict__ phi,
@@ -1120,16 +1210,36 @@ __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restri
 }
 
 // skip the correction sweep once |r| <= tol |b|
-__global__ void m_dc_decide(const double* __restrict__ red, double tol, int* __restrict__ skip) {
-  if (threadIdx.x == 0) skip[0] = red[1] <= tol * tol * red[0] ? 1 : 0;
-}
-
-__global__ __launch_bounds__(256) void m_accum(long n, const float* __restrict__ d, double* __restrict__ phi, int first,
-                                               const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  phi[m] = first ? (double)d[m] : phi[m] + (double)d[m];
+// |b|^2 and |r|^2 from the per-block partials, and the decision to skip
+// the next defect-correction sweep (|r| <= tol |b|)
+__global__ __launch_bounds__(1024) void m_sum_decide(const double* __restrict__ pb, int nbb,
+                                                     const double* __restrict__ pr, int nbr, double* __restrict__ red,
+                                                     double tol, int* __restrict__ skip) {
+  __shared__ double sh[1024];
+  double vb = 0.0, vr = 0.0;
+#pragma unroll 8
+  for (int i = threadIdx.x; i < nbb; i += 1024) vb += pb[i];
+#pragma unroll 8
+  for (int i = threadIdx.x; i < nbr; i += 1024) vr += pr[i];
+  sh[threadIdx.x] = vb;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double sb = sh[0];
+  __syncthreads();
+  sh[threadIdx.x] = vr;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    red[0] = sb;
+    red[1] = sh[0];
+    skip[0] = sh[0] <= tol * tol * sb ? 1 : 0;
+  }
 }
 
 // One fixed-point sweep of the Crank-Nicolson Helmholtz problem
@@ -1310,10 +1420,6 @@ __global__ void m_to64(long n, const float* __restrict__ a, double* __restrict__
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m < n) b[m] = (double)a[m];
 }
-__global__ void m_zero_d(int n, double* p) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] = 0.0;
-}
 
 // ================================================================= host ===
 void mixed_create(Ctx& c) {
@@ -1332,6 +1438,10 @@ void mixed_create(Ctx& c) {
   dalloc(&X->phi, n); dalloc(&X->b, n); dalloc(&X->r32, n); dalloc(&X->d32, n);
   dalloc(&X->red, 8);
   dalloc(&X->part, (size_t)(n / 256 + 1024));
+  dalloc(&X->partb, (size_t)blocks_x32(g) + 1);
+  FP_HIP_CHECK(hipHostMalloc((void**)&X->dc_host, sizeof(int), hipHostMallocDefault));
+  *X->dc_host = 1;
+  FP_HIP_CHECK(hipEventCreateWithFlags(&X->dc_ev, hipEventDisableTiming));
   dalloc(&X->skip, 1);
   dalloc(&X->R, 3 * n); dalloc(&X->X2, 3 * n);
   {
@@ -1382,6 +1492,9 @@ void mixed_destroy(Ctx& c) {
   dfree(X->spec); dfree(X->thp); dfree(X->twx); dfree(X->twy); dfree(X->twz);
   dfree(X->kx2); dfree(X->ky2); dfree(X->kz2);
   dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->part); dfree(X->skip);
+  dfree(X->partb);
+  if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
+  if (X->dc_host) (void)hipHostFree(X->dc_host);
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
   dfree(X->amb_cnt); dfree(X->amb_off);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
@@ -1596,6 +1709,32 @@ struct MTimer {
 };
 }  // namespace
 
+#define DC_EVERY 8
+
+// The FP64 defect check (residual, |r| <= dc_tol |b|, and the correction
+// sweep it gates on the device) runs every DC_EVERY steps.  Its decision is
+// copied to pinned host memory; at the next periodic check the host reads it
+// (the copy finished DC_EVERY steps earlier) and, if a correction was needed,
+// checks every step until a periodic check passes again.
+static bool dc_check_due(Ctx& c) {
+  Mixed& X = MX(c);
+  if (c.step % DC_EVERY == 0) {
+    if (X.dc_pending) {
+      FP_HIP_CHECK(hipEventSynchronize(X.dc_ev));
+      X.dc_correct = *X.dc_host == 0;
+      X.dc_pending = false;
+    }
+    return true;
+  }
+  return X.dc_correct;
+}
+static void dc_record(Ctx& c) {
+  Mixed& X = MX(c);
+  FP_HIP_CHECK(hipMemcpyAsync(X.dc_host, X.skip, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  FP_HIP_CHECK(hipEventRecord(X.dc_ev, c.stream));
+  X.dc_pending = true;
+}
+
 static void select_fenep(Ctx& c, int tm) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
@@ -1667,23 +1806,26 @@ void mixed_step(Ctx& c) {
   }
   {
     MTimer t(c, "poisson");
-    const int nb = nblk(n, 256);
-    hipLaunchKernelGGL(m_zero_d, dim3(1), dim3(64), 0, s, 8, X.red);
-    hipLaunchKernelGGL(m_divergence, dim3(nb), dim3(256), 0, s, g, X.Up, X.b, X.r32, X.part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, X.part, nb, X.red, 0);
+    // first sweep: b = div u*/dt in the load stage of the x transform, the
+    // FP32 spectral + line solve, Phi accumulated in FP64 in the store stage
+    const int nb = nblk(n, 256), nbx = blocks_x32(g), lx = lines_for32(g.nx);
     const int ndc = std::max(1, c.p.n_dc_max);
-    for (int it = 0; it < ndc; it++) {
-      const int* skip = nullptr;
-      if (it > 0) {
-        // FP64 defect: r = b - L_h phi; the sweep is skipped on device once
-        // |r| <= dc_tol |b|
+    const bool check = ndc > 1 && dc_check_due(c);
+    hipLaunchKernelGGL(m_div_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.Up, X.spec, X.twx, lx, X.b, X.partb,
+                       check ? 1 : 0);
+    solve32_yz(c, 3, nullptr);
+    hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_x32(g), s, g, X.spec, X.phi, X.twx, lx, 1, nullptr);
+    if (check) {
+      // FP64 defect r = b - L_h phi; the next sweep is skipped on the device
+      // once |r| <= dc_tol |b|
+      for (int it = 1; it < ndc; it++) {
         hipLaunchKernelGGL(m_residual, dim3(nb), dim3(256), 0, s, g, X.b, X.phi, X.r32, X.part);
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, X.part, nb, X.red + 1, 0);
-        hipLaunchKernelGGL(m_dc_decide, dim3(1), dim3(64), 0, s, X.red, c.p.dc_tol, X.skip);
-        skip = X.skip;
+        hipLaunchKernelGGL(m_sum_decide, dim3(1), dim3(1024), 0, s, X.partb, nbx, X.part, nb, X.red, c.p.dc_tol, X.skip);
+        if (it == 1 && c.step % DC_EVERY == 0) dc_record(c);
+        hipLaunchKernelGGL(m_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.r32, X.spec, X.twx, lx, X.skip);
+        solve32_yz(c, 3, X.skip);
+        hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_x32(g), s, g, X.spec, X.phi, X.twx, lx, 0, X.skip);
       }
-      solve32(c, 3, X.r32, X.d32, skip);
-      hipLaunchKernelGGL(m_accum, dim3(nb), dim3(256), 0, s, n, X.d32, X.phi, it == 0 ? 1 : 0, skip);
     }
   }
   {
