@@ -40,6 +40,8 @@ struct Mixed {
   double* part = nullptr;         // per-block partial sums
   int* skip = nullptr;            // defect correction converged: skip the sweep
   double* partb = nullptr;        // per-block |b|^2 of the fused divergence
+  int slab = 0;                   // walls: Poisson through m_poisson_slab
+  float* pbuf = nullptr;          // its line-solve coefficients
   int* dc_host = nullptr;         // pinned copy of the last periodic decision
   hipEvent_t dc_ev = nullptr;
   bool dc_pending = false, dc_correct = false;
@@ -840,14 +842,43 @@ __device__ __forceinline__ void load_twiddles(float2* tws, const float2* __restr
   for (int e = threadIdx.x; e < n / 2; e += blockDim.x) tws[e] = tw[e];
 }
 
+// Half-spectrum element (kx, row, kz) in the x-major layout (kx fastest) or
+// the transposed layout of the slab solve (row fastest, then kz, then kx: one
+// kx slab is contiguous).
+__device__ __forceinline__ long hs_index(const Grid& g, int tr, long k, long row, long kk) {
+  return tr ? row + (long)g.sy * (kk + (long)g.nz * k) : k + (long)g.nxh * (row + (long)g.sy * kk);
+}
+
+// Store the half spectra of nl real x-lines (two per complex transform r:
+// A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i).  In the
+// transposed layout consecutive threads take consecutive lines of one kx, so
+// each kx receives a contiguous run of rows.
+__device__ __forceinline__ void store_half_spectra(const Grid& g, const float2* r, float2* __restrict__ spec, int nl,
+                                                   long l0, int tr) {
+  const int n = g.nx, nc = nl / 2, nxh = g.nxh;
+  const long nlines = (long)g.ny * g.nz;
+  for (int e = threadIdx.x; e < nc * nxh; e += blockDim.x) {
+    int c, k;
+    if (tr) { k = e / nc; c = e - k * nc; }
+    else { c = e / nxh; k = e - c * nxh; }
+    const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
+    const long La = l0 + 2 * c, Lb = La + 1;
+    if (La < nlines)
+      spec[hs_index(g, tr, k, La % g.ny + g.jo, La / g.ny)] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
+    if (Lb < nlines)
+      spec[hs_index(g, tr, k, Lb % g.ny + g.jo, Lb / g.ny)] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
+  }
+}
+
 // Real x-lines -> half-spectrum rows, two real lines per complex transform:
 // z = a + i b, A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i.
 // nl (even) real lines per block.
 __global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restrict__ q, float2* __restrict__ spec,
-                                                   const float2* __restrict__ tw, int nl, const int* __restrict__ skip) {
+                                                   const float2* __restrict__ tw, int nl, const int* __restrict__ skip,
+                                                   int tr) {
   if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
-  const int n = g.nx, nc = nl / 2, nxh = g.nxh;
+  const int n = g.nx, nc = nl / 2;
   float2* x = ldsf;
   float2* y = ldsf + nc * n;
   float2* tws = ldsf + 2 * nc * n;
@@ -861,19 +892,7 @@ __global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restri
   }
   __syncthreads();
   const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, false);
-  for (int e = threadIdx.x; e < nc * nxh; e += blockDim.x) {
-    const int c = e / nxh, k = e - c * nxh;
-    const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
-    const long La = l0 + 2 * c, Lb = La + 1;
-    if (La < nlines) {
-      const long j = La % g.ny, kk = La / g.ny;
-      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
-    }
-    if (Lb < nlines) {
-      const long j = Lb % g.ny, kk = Lb / g.ny;
-      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
-    }
-  }
+  store_half_spectra(g, r, spec, nl, l0, tr);
 }
 
 // Half-spectrum rows -> real x-lines (unnormalised), two lines per complex
@@ -918,9 +937,9 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
 // defect check.
 __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __restrict__ up, float2* __restrict__ spec,
                                                        const float2* __restrict__ tw, int nl, double* __restrict__ b,
-                                                       double* __restrict__ nrm, int want_b) {
+                                                       double* __restrict__ nrm, int want_b, int tr) {
   extern __shared__ float2 ldsf[];
-  const int n = g.nx, nc = nl / 2, nxh = g.nxh;
+  const int n = g.nx, nc = nl / 2;
   float2* x = ldsf;
   float2* y = ldsf + nc * n;
   float2* tws = ldsf + 2 * nc * n;
@@ -929,6 +948,7 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
   double ss = 0.0;
+#pragma unroll 2
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
     const int c = e / n, i = e - c * n;
     float2 v2 = make_float2(0.0f, 0.0f);
@@ -956,19 +976,7 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
   }
   __syncthreads();
   const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, false);
-  for (int e = threadIdx.x; e < nc * nxh; e += blockDim.x) {
-    const int c = e / nxh, k = e - c * nxh;
-    const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
-    const long La = l0 + 2 * c, Lb = La + 1;
-    if (La < nlines) {
-      const long j = La % g.ny, kk = La / g.ny;
-      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
-    }
-    if (Lb < nlines) {
-      const long j = Lb % g.ny, kk = Lb / g.ny;
-      spec[k + (long)nxh * ((j + g.jo) + g.sy * kk)] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
-    }
-  }
+  store_half_spectra(g, r, spec, nl, l0, tr);
 }
 
 // Inverse x transform of the Poisson correction d with the FP64
@@ -976,23 +984,40 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
 // phi + d.
 __global__ __launch_bounds__(256) void m_fft_x_inv_accum(Grid g, const float2* __restrict__ spec, double* __restrict__ phi,
                                                          const float2* __restrict__ tw, int nl, int first,
-                                                         const int* __restrict__ skip) {
+                                                         const int* __restrict__ skip, int tr) {
   if (skip && *skip) return;
   extern __shared__ float2 ldsf[];
   const int n = g.nx, h = n / 2, nxh = g.nxh, nc = nl / 2;
   float2* x = ldsf;
   float2* y = ldsf + nc * n;
   float2* tws = ldsf + 2 * nc * n;
+  float2* hs = tws + n / 2;        // transposed layout: [line][kx] staging
   load_twiddles(tws, tw, n);
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
+  if (tr) {
+    // consecutive threads read consecutive rows of one kx (coalesced); the
+    // unrolled loop keeps several loads of a thread in flight
+#pragma unroll 4
+    for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
+      const int k = e / nl, jl = e - k * nl;
+      const long L = l0 + jl;
+      hs[jl * nxh + k] = L < nlines ? spec[hs_index(g, 1, k, L % g.ny + g.jo, L / g.ny)] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+  }
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
     const int c = e / n, i = e - c * n;
     const long La = l0 + 2 * c, Lb = La + 1;
     const int src = i <= h ? i : n - i;
     float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
-    if (La < nlines) A = spec[src + (long)nxh * ((La % g.ny + g.jo) + g.sy * (La / g.ny))];
-    if (Lb < nlines) B = spec[src + (long)nxh * ((Lb % g.ny + g.jo) + g.sy * (Lb / g.ny))];
+    if (tr) {
+      A = hs[(2 * c) * nxh + src];
+      B = hs[(2 * c + 1) * nxh + src];
+    } else {
+      if (La < nlines) A = spec[src + (long)nxh * ((La % g.ny + g.jo) + g.sy * (La / g.ny))];
+      if (Lb < nlines) B = spec[src + (long)nxh * ((Lb % g.ny + g.jo) + g.sy * (Lb / g.ny))];
+    }
     if (src == 0 || src == h) { A.y = 0.0f; B.y = 0.0f; }
     if (i > h) { A.y = -A.y; B.y = -B.y; }
     x[e] = make_float2(A.x - B.y, A.y + B.x);
@@ -1059,6 +1084,168 @@ __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spe
     const int l = e % ti, r = e / ti;
     const int i = i0 + l;
     if (i < nxh) base[i + stride * r] = res[l * pitch + r];
+  }
+}
+
+// In-place radix-2 transforms (decimation in time: bit-reversed input,
+// natural-order output) of R rows of n points with row pitch P, all rows in
+// one pass per stage; tws[t] = exp(-2 pi i t / n), t < n/2; inv: conjugate
+// twiddles (unnormalised).  Called by every thread of the block.
+__device__ void fft_rows_inplace(float2* S, int R, int P, int n, int logn, const float2* tws, bool inv) {
+  for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
+    const int row = e >> logn, i = e & (n - 1);
+    const int j = (int)(__brev((unsigned)i) >> (32 - logn));
+    if (i < j) {
+      float2* x = S + (size_t)row * P;
+      const float2 t = x[i];
+      x[i] = x[j];
+      x[j] = t;
+    }
+  }
+  __syncthreads();
+  for (int st = 1; st <= logn; st++) {
+    const int half = 1 << (st - 1), str = n >> st;
+    for (int b = threadIdx.x; b < R * (n >> 1); b += blockDim.x) {
+      const int row = b >> (logn - 1), t = b & ((n >> 1) - 1);
+      const int k = t & (half - 1);
+      const int i = ((t >> (st - 1)) << st) + k, j = i + half;
+      float2* x = S + (size_t)row * P;
+      const float2 w = tws[k * str];
+      const float wr = w.x, wi = inv ? -w.y : w.y;
+      const float2 a = x[i], bb = x[j];
+      const float tr = bb.x * wr - bb.y * wi, ti = bb.x * wi + bb.y * wr;
+      x[i] = make_float2(a.x + tr, a.y + ti);
+      x[j] = make_float2(a.x - tr, a.y - ti);
+    }
+    __syncthreads();
+  }
+}
+
+// Pressure Poisson solve of one kx slab with y-walls (fft_poisson_kabe and
+// tdma, fenep_2d.f90:1145-1177, 1179-1342): forward z transforms of the
+// interior rows, the line solve along y of every kz (boundary rows x_0 = x_1,
+// x_{NY+1} = x_NY), inverse z transforms.  The slab, held [row][kz] with pitch
+// nz + 1 in LDS, is read and written once from the transposed half spectrum
+// (row fastest) and transformed in place; the line solve keeps its P
+// coefficients in pbuf (L2).
+static size_t lds_slab(const Grid& g) { return sizeof(float2) * ((size_t)g.sy * (g.nz + 1) + g.nz / 2); }
+__global__ __launch_bounds__(1024) void m_poisson_slab(Grid g, float2* __restrict__ spec, const float2* __restrict__ tw,
+                                                      const float* __restrict__ kx2, const float* __restrict__ kz2,
+                                                      float* __restrict__ pbuf, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ float2 lds[];
+  const int nz = g.nz, sy = g.sy, ny = g.ny, pitch = nz + 1;
+  const long kx = blockIdx.x;
+  float2* S = lds;                              // [sy][pitch]
+  float2* tws = lds + (size_t)sy * pitch;
+  load_twiddles(tws, tw, nz);
+  float2* slab = spec + (size_t)sy * nz * kx;   // [kz][row]
+  // one workgroup per CU: issue a thread's loads together, then store them
+  constexpr int LU = 16;
+  for (int e0 = 0; e0 < ny * nz; e0 += 1024 * LU) {
+    float2 v[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int e = e0 + 1024 * u + (int)threadIdx.x;
+      if (e < ny * nz) {
+        const int kk = e / ny, j = e - kk * ny;
+        v[u] = slab[(size_t)sy * kk + j + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int e = e0 + 1024 * u + (int)threadIdx.x;
+      if (e < ny * nz) {
+        const int kk = e / ny, j = e - kk * ny;
+        S[(j + 1) * pitch + kk] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int logn = ilog2f(nz);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      // line solve along y, one thread per kz; rows go in batches of TB with
+      // the next batch's operands loaded while the current one is computed
+      // (the back substitution reads P from L2)
+      if ((int)threadIdx.x < nz) {
+        constexpr int TB = 8;
+        const int kk = threadIdx.x;
+        float* P = pbuf + (size_t)sy * nz * kx;   // [row][kz]
+        const float inxz = (float)(1.0 / ((double)g.nx * g.nz));
+        const float idy2 = (float)(1.0 / (g.dy * g.dy));
+        const float kq = kx2[kx] + kz2[kk];
+        const float ai = -(kq + 2.0f * idy2);
+        float Pp = 1.0f, Qr = 0.0f, Qi = 0.0f;   // row 0: a = 1, b = -1
+        P[kk] = Pp;
+        S[kk] = make_float2(0.0f, 0.0f);
+        float2 cur[TB];
+#pragma unroll
+        for (int t = 0; t < TB; t++) cur[t] = 1 + t < sy ? S[(1 + t) * pitch + kk] : make_float2(0.0f, 0.0f);
+        for (int r0 = 1; r0 < sy; r0 += TB) {
+          float2 nxt[TB];
+#pragma unroll
+          for (int t = 0; t < TB; t++) nxt[t] = r0 + TB + t < sy ? S[(r0 + TB + t) * pitch + kk] : make_float2(0.0f, 0.0f);
+#pragma unroll
+          for (int t = 0; t < TB; t++) {
+            const int r = r0 + t;
+            if (r < sy) {
+              const bool last = r == sy - 1;
+              const float a = last ? 1.0f : ai;
+              const float bb = last ? 0.0f : idy2;
+              const float cc = last ? -1.0f : idy2;
+              const float dr = last ? 0.0f : cur[t].x * inxz, di = last ? 0.0f : cur[t].y * inxz;
+              const float den = a + cc * Pp;
+              if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
+              else { const float id = __builtin_amdgcn_rcpf(den); Pp = -bb * id; Qr = (dr - cc * Qr) * id; Qi = (di - cc * Qi) * id; }
+              P[r * nz + kk] = Pp;
+              S[r * pitch + kk] = make_float2(Qr, Qi);
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < TB; t++) cur[t] = nxt[t];
+        }
+        // back substitution x_r = P_r x_{r+1} + Q_r, r = sy-2 .. 0
+        float xr = Qr, xi = Qi;
+        float pc[TB];
+        float2 qc[TB];
+#pragma unroll
+        for (int t = 0; t < TB; t++) {
+          const int r = sy - 2 - t;
+          pc[t] = r >= 0 ? P[r * nz + kk] : 0.0f;
+          qc[t] = r >= 0 ? S[r * pitch + kk] : make_float2(0.0f, 0.0f);
+        }
+        for (int r0 = sy - 2; r0 >= 0; r0 -= TB) {
+          float pn[TB];
+          float2 qn[TB];
+#pragma unroll
+          for (int t = 0; t < TB; t++) {
+            const int r = r0 - TB - t;
+            pn[t] = r >= 0 ? P[r * nz + kk] : 0.0f;
+            qn[t] = r >= 0 ? S[r * pitch + kk] : make_float2(0.0f, 0.0f);
+          }
+#pragma unroll
+          for (int t = 0; t < TB; t++) {
+            const int r = r0 - t;
+            if (r >= 0) {
+              xr = pc[t] * xr + qc[t].x;
+              xi = pc[t] * xi + qc[t].y;
+              S[r * pitch + kk] = make_float2(xr, xi);
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < TB; t++) { pc[t] = pn[t]; qc[t] = qn[t]; }
+        }
+      }
+      __syncthreads();
+    }
+    // z transforms of the interior rows
+    fft_rows_inplace(S + pitch, ny, pitch, nz, logn, tws, pass == 1);
+    if (pass == 1) break;
+  }
+  for (int e = threadIdx.x; e < ny * nz; e += blockDim.x) {
+    const int kk = e / ny, j = e - kk * ny;
+    slab[(size_t)sy * kk + j + 1] = S[(j + 1) * pitch + kk];
   }
 }
 
@@ -1150,10 +1337,16 @@ static int cols_for32(int n) { return std::max(1, std::min(16, 3968 / (n + 1)));
 
 // y and z transforms (periodic y) or z transforms and the y line solve
 // (walls) of the half spectrum, with the spectral divide of the solve kind
-static void solve32_yz(Ctx& c, int kind, const int* skip) {
+static void solve32_yz(Ctx& c, int kind, const int* skip, int tr = 0) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
   hipStream_t s = c.stream;
+  if (tr) {
+    // transposed half spectrum, walls, pressure Poisson
+    hipLaunchKernelGGL(m_poisson_slab, dim3(g.nxh), dim3(1024), lds_slab(g), s, g, X.spec, X.twz, X.kx2, X.kz2, X.pbuf,
+                       skip);
+    return;
+  }
   int tiy = cols_for32(g.ny), tiz = cols_for32(g.nz);
   int nty = (g.nxh + tiy - 1) / tiy, ntz = (g.nxh + tiz - 1) / tiz;
   size_t shy = sizeof(float2) * (2 * (size_t)tiy * (g.ny + 1) + g.ny / 2);
@@ -1174,13 +1367,16 @@ static void solve32_yz(Ctx& c, int kind, const int* skip) {
 
 static size_t lds_x32(const Grid& g) { return sizeof(float2) * ((size_t)lines_for32(g.nx) * g.nx + g.nx / 2); }
 static int blocks_x32(const Grid& g) { return nblk((long)g.ny * g.nz, lines_for32(g.nx)); }
+// the inverse x transform reading the transposed layout stages its lines' half
+// spectra in LDS as well
+static size_t lds_x32_tr(const Grid& g) { return lds_x32(g) + sizeof(float2) * (size_t)lines_for32(g.nx) * g.nxh; }
 
 static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip = nullptr) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
   hipStream_t s = c.stream;
   const int lx = lines_for32(g.nx);
-  hipLaunchKernelGGL(m_fft_x_fwd, dim3(blocks_x32(g)), dim3(256), lds_x32(g), s, g, q, X.spec, X.twx, lx, skip);
+  hipLaunchKernelGGL(m_fft_x_fwd, dim3(blocks_x32(g)), dim3(256), lds_x32(g), s, g, q, X.spec, X.twx, lx, skip, 0);
   solve32_yz(c, kind, skip);
   hipLaunchKernelGGL(m_fft_x_inv, dim3(blocks_x32(g)), dim3(256), lds_x32(g), s, g, X.spec, x, X.twx, lx, skip);
 }
@@ -1439,6 +1635,19 @@ void mixed_create(Ctx& c) {
   dalloc(&X->red, 8);
   dalloc(&X->part, (size_t)(n / 256 + 1024));
   dalloc(&X->partb, (size_t)blocks_x32(g) + 1);
+  if (g.walls) {
+    // the slab solve holds one kx slab in LDS: use it where the device's
+    // per-block LDS takes it
+    int lds_max = 0, dev = 0;
+    FP_HIP_CHECK(hipGetDevice(&dev));
+    FP_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (lds_slab(g) <= (size_t)lds_max && g.nz <= 1024) {
+      FP_HIP_CHECK(hipFuncSetAttribute((const void*)m_poisson_slab, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_slab(g)));
+      X->slab = 1;
+      dalloc(&X->pbuf, (size_t)g.nxh * g.sy * g.nz);
+    }
+  }
   FP_HIP_CHECK(hipHostMalloc((void**)&X->dc_host, sizeof(int), hipHostMallocDefault));
   *X->dc_host = 1;
   FP_HIP_CHECK(hipEventCreateWithFlags(&X->dc_ev, hipEventDisableTiming));
@@ -1447,12 +1656,13 @@ void mixed_create(Ctx& c) {
   {
     // fixed-point Helmholtz sweeps x <- q + eps Lap x from x = q: the error
     // starts at eps Lap x* (<= rho |x*|, rho = eps lambda_max) and contracts
-    // by rho per sweep, so k sweeps leave rho^(k+1) |x*|; enough sweeps for
-    // 1e-8 (below FP32 rounding), spectral solve if more than 6
+    // by rho per sweep, so k sweeps leave rho^(k+1) |x*|; enough sweeps to
+    // bring that below the FP32 unit roundoff 2^-24 of the stored result,
+    // spectral solve if more than 6
     const double eps = ((g.dt * g.beta) / g.re) / 2.0;
     const double lam = 4.0 / (g.dx * g.dx) + 4.0 / (g.dy * g.dy) + 4.0 / (g.dz * g.dz);
     const double rho = eps * lam;
-    int m = rho > 0.0 ? (int)std::ceil(std::log(1.0e-8) / std::log(rho)) - 1 : 1;
+    int m = rho > 0.0 ? (int)std::ceil(std::log(std::ldexp(1.0, -24)) / std::log(rho)) - 1 : 1;
     X->helm_iters = (rho < 0.25 && m <= 6) ? std::max(1, m) : 0;
   }
   // every (cell, axis) may need the FP64 decision: a region of 3 * 256
@@ -1493,6 +1703,7 @@ void mixed_destroy(Ctx& c) {
   dfree(X->kx2); dfree(X->ky2); dfree(X->kz2);
   dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->part); dfree(X->skip);
   dfree(X->partb);
+  dfree(X->pbuf);
   if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
   if (X->dc_host) (void)hipHostFree(X->dc_host);
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
@@ -1811,10 +2022,12 @@ void mixed_step(Ctx& c) {
     const int nb = nblk(n, 256), nbx = blocks_x32(g), lx = lines_for32(g.nx);
     const int ndc = std::max(1, c.p.n_dc_max);
     const bool check = ndc > 1 && dc_check_due(c);
+    const int tr = X.slab;
+    const size_t lds_inv = tr ? lds_x32_tr(g) : lds_x32(g);
     hipLaunchKernelGGL(m_div_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.Up, X.spec, X.twx, lx, X.b, X.partb,
-                       check ? 1 : 0);
-    solve32_yz(c, 3, nullptr);
-    hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_x32(g), s, g, X.spec, X.phi, X.twx, lx, 1, nullptr);
+                       check ? 1 : 0, tr);
+    solve32_yz(c, 3, nullptr, tr);
+    hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_inv, s, g, X.spec, X.phi, X.twx, lx, 1, nullptr, tr);
     if (check) {
       // FP64 defect r = b - L_h phi; the next sweep is skipped on the device
       // once |r| <= dc_tol |b|
@@ -1822,9 +2035,9 @@ void mixed_step(Ctx& c) {
         hipLaunchKernelGGL(m_residual, dim3(nb), dim3(256), 0, s, g, X.b, X.phi, X.r32, X.part);
         hipLaunchKernelGGL(m_sum_decide, dim3(1), dim3(1024), 0, s, X.partb, nbx, X.part, nb, X.red, c.p.dc_tol, X.skip);
         if (it == 1 && c.step % DC_EVERY == 0) dc_record(c);
-        hipLaunchKernelGGL(m_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.r32, X.spec, X.twx, lx, X.skip);
-        solve32_yz(c, 3, X.skip);
-        hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_x32(g), s, g, X.spec, X.phi, X.twx, lx, 0, X.skip);
+        hipLaunchKernelGGL(m_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.r32, X.spec, X.twx, lx, X.skip, tr);
+        solve32_yz(c, 3, X.skip, tr);
+        hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_inv, s, g, X.spec, X.phi, X.twx, lx, 0, X.skip, tr);
       }
     }
   }
