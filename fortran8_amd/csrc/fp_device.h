@@ -13,36 +13,40 @@ __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >
 // periodic copies (PBM, fenep.f90:147-155) in x and z; in y either periodic or
 // the wall rules U_kabe/V_kabe/P_kabe/C_kabe (fenep_2d.f90:1098-1143, NA=0,
 // NB=NY, Uwall=0).
-template <typename T>
-__device__ __forceinline__ T ldb(const T* __restrict__ A, const Grid& g, int i, int j, int k, int kind) {
+template <bool WALLS, typename T>
+__device__ __forceinline__ T ldbw(const T* __restrict__ A, const Grid& g, int i, int j, int k, int kind) {
   i = wrapi(i, g.nx);
   k = wrapi(k, g.nz);
   const long plane = (long)g.nx * g.ny;
-  if (!g.walls) {
+  if constexpr (!WALLS) {
     j = wrapi(j, g.ny);
     return A[i + (long)g.nx * j + plane * k];
-  }
-  const T* col = A + i + plane * k;
-  if (j >= 0 && j < g.ny) {
-    if (kind == BC_V && j == g.ny - 1) return T(0);
-    return col[(long)g.nx * j];
-  }
-  const int nx = g.nx;
-  if (j < 0) {
-    switch (kind) {
-      case BC_U: case BC_W: return -col[0];
-      case BC_V: return T(0);
-      case BC_P: return col[0];
-      default: return T(2) * col[0] - col[nx];
+  } else {
+    // y-walls (U_kabe .. C_kabe, fenep_2d.f90:1098-1143), without branches
+    // so that a caller's loads can be issued together: every rule is one
+    // load of the nearest interior row, negated or replaced by zero, or (C)
+    // the linear extrapolation from the two nearest rows
+    const T* col = A + i + plane * k;
+    const int nx = g.nx, ny = g.ny;
+    const bool lo = j < 0, hi = j >= ny, ghost = lo || hi;
+    const int jc = lo ? 0 : (hi ? ny - 1 : j);
+    if (kind == BC_U || kind == BC_V || kind == BC_W || kind == BC_P) {
+      const T v = col[(long)nx * jc];
+      // U, W ghosts: -U_1 below, 2*0 - U_NY above (the reference's forms,
+      // which differ for a zero in the sign of the result)
+      if (kind == BC_U || kind == BC_W) return lo ? -v : (hi ? T(2) * T(0) - v : v);
+      const bool zero = kind == BC_V && (ghost || j == ny - 1);
+      return zero ? T(0) : v;
     }
+    const int j2 = lo ? 1 : (hi ? ny - 2 : jc);
+    const T a = col[(long)nx * jc], b = col[(long)nx * j2];
+    return ghost ? T(2) * a - b : a;
   }
-  const long top = (long)nx * (g.ny - 1);
-  switch (kind) {
-    case BC_U: case BC_W: return T(2) * T(0) - col[top];
-    case BC_V: return T(0);
-    case BC_P: return col[top];
-    default: return T(2) * col[top] - col[top - nx];
-  }
+}
+// the same with the wall flag read at run time
+template <typename T>
+__device__ __forceinline__ T ldb(const T* __restrict__ A, const Grid& g, int i, int j, int k, int kind) {
+  return g.walls ? ldbw<true>(A, g, i, j, k, kind) : ldbw<false>(A, g, i, j, k, kind);
 }
 
 // Workgroups of a launch are handed to the 8 XCDs of the chip in turn, and

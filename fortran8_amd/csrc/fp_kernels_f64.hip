@@ -13,6 +13,7 @@ static inline int nblk(long n, int t) { return (int)((n + t - 1) / t); }
 // ============================================================ FENE-P ======
 // CpxCnx/CpyCny/CpzCnz selection (fenep.f90:169-382) for all three axes of
 // one cell; the chosen candidate per axis is packed in 2 bits.
+template <bool WL>
 __global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict__ C, uint8_t* __restrict__ sel) {
   const long L = xcd_cell();
   if (L >= g.n) return;
@@ -27,9 +28,9 @@ __global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict
 #pragma unroll
     for (int q = 0; q < 6; q++) {
       const double* Cq = C + q * g.n;
-      cm[q] = ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+      cm[q] = ldbw<WL>(Cq, g, i - di, j - dj, k - dk, BC_C);
       c[q] = Cq[m];
-      cp[q] = ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+      cp[q] = ldbw<WL>(Cq, g, i + di, j + dj, k + dk, BC_C);
     }
     s |= (uint8_t)(select_d(cm, c, cp) << (2 * d));
   }
@@ -38,6 +39,7 @@ __global__ __launch_bounds__(256) void k_select(Grid g, const double* __restrict
 
 // face value of cell a (wrapped) along axis d with its own selection:
 // left (Cp) when left=true, right (Cn) otherwise, component q.
+template <bool WL>
 __device__ __forceinline__ void face_vals(const Grid& g, const double* __restrict__ C, const uint8_t* __restrict__ sel,
                                           int d, int i, int j, int k, bool left, double* out) {
   i = wrapi(i, g.nx); j = wrapi(j, g.ny); k = wrapi(k, g.nz);
@@ -48,14 +50,15 @@ __device__ __forceinline__ void face_vals(const Grid& g, const double* __restric
   for (int q = 0; q < 6; q++) {
     const double* Cq = C + q * g.n;
     double c = Cq[m];
-    double cm = l == 3 ? 0.0 : ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
-    double cp = l == 3 ? 0.0 : ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+    double cm = l == 3 ? 0.0 : ldbw<WL>(Cq, g, i - di, j - dj, k - dk, BC_C);
+    double cp = l == 3 ? 0.0 : ldbw<WL>(Cq, g, i + di, j + dj, k + dk, BC_C);
     out[q] = left ? cand_p(l, cm, c, cp) : cand_n(l, cm, c, cp);
   }
 }
 
 // Cstar (fenep.f90:464-507) + Lyapunov Newton (fenep.f90:518-585) + the
 // Peterlin factor of the new C (f, fenep.f90:510-515, used by polymer_stress).
+template <bool WL>
 __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __restrict__ C, const uint8_t* __restrict__ sel,
                                                       const double* __restrict__ U, const double* __restrict__ V,
                                                       const double* __restrict__ W, double* __restrict__ Cn,
@@ -72,19 +75,19 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
   for (int d = 0; d < 3; d++) {
     double h = d == 0 ? g.dx : (d == 1 ? g.dy : g.dz);
     double v1, v2;
-    if (d == 0) { v1 = ldb(U, g, i - 1, j, k, BC_U); v2 = U[m]; }
-    else if (d == 1) { v1 = ldb(V, g, i, j - 1, k, BC_V); v2 = ldb(V, g, i, j, k, BC_V); }
-    else { v1 = ldb(W, g, i, j, k - 1, BC_W); v2 = W[m]; }
+    if (d == 0) { v1 = ldbw<WL>(U, g, i - 1, j, k, BC_U); v2 = U[m]; }
+    else if (d == 1) { v1 = ldbw<WL>(V, g, i, j - 1, k, BC_V); v2 = ldbw<WL>(V, g, i, j, k, BC_V); }
+    else { v1 = ldbw<WL>(W, g, i, j, k - 1, BC_W); v2 = W[m]; }
     double s1 = ((-v2 + fabs(v2)) * dt) / (2.0 * h);
     double s2 = 1.0 / 6.0 + ((-v2 - fabs(v2)) * dt) / (2.0 * h);
     double s3 = 1.0 / 6.0 + ((v1 - fabs(v1)) * dt) / (2.0 * h);
     double s4 = ((v1 + fabs(v1)) * dt) / (2.0 * h);
     int di = d == 0, dj = d == 1, dk = d == 2;
     double pl[6], pr[6], nl[6], nr[6];
-    face_vals(g, C, sel, d, i, j, k, true, pl);                    // Cp(a-1): own left face
-    face_vals(g, C, sel, d, i + di, j + dj, k + dk, true, pr);     // Cp(a): left face of a+1
-    face_vals(g, C, sel, d, i - di, j - dj, k - dk, false, nl);    // Cn(a-1): right face of a-1
-    face_vals(g, C, sel, d, i, j, k, false, nr);                   // Cn(a): own right face
+    face_vals<WL>(g, C, sel, d, i, j, k, true, pl);                    // Cp(a-1): own left face
+    face_vals<WL>(g, C, sel, d, i + di, j + dj, k + dk, true, pr);     // Cp(a): left face of a+1
+    face_vals<WL>(g, C, sel, d, i - di, j - dj, k - dk, false, nl);    // Cn(a-1): right face of a-1
+    face_vals<WL>(g, C, sel, d, i, j, k, false, nr);                   // Cn(a): own right face
 #pragma unroll
     for (int q = 0; q < 6; q++) {
       double acc = d == 0 ? s3 * pl[q] : cx[q] + s3 * pl[q];
@@ -102,9 +105,9 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
   b[3] = b[3] + dt / g.wi;
   b[5] = b[5] + dt / g.wi;
   const double dx = g.dx, dy = g.dy, dz = g.dz;
-#define UL(a, bb, c) ldb(U, g, a, bb, c, BC_U)
-#define VL(a, bb, c) ldb(V, g, a, bb, c, BC_V)
-#define WL(a, bb, c) ldb(W, g, a, bb, c, BC_W)
+#define UL(a, bb, c) ldbw<WL>(U, g, a, bb, c, BC_U)
+#define VL(a, bb, c) ldbw<WL>(V, g, a, bb, c, BC_V)
+#define WL(a, bb, c) ldbw<WL>(W, g, a, bb, c, BC_W)
   double a11 = -((dt * (-UL(i - 1, j, k) + UL(i, j, k))) / dx);
   double a12 = -((dt * (((-VL(i - 1, j - 1, k) + VL(i + 1, j - 1, k)) - VL(i - 1, j, k)) + VL(i + 1, j, k))) / (4.0 * dx));
   double a13 = -((dt * (((-WL(i - 1, j, k - 1) + WL(i + 1, j, k - 1)) - WL(i - 1, j, k)) + WL(i + 1, j, k))) / (4.0 * dx));
@@ -130,13 +133,14 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
 
 // ========================================================= predictor ======
 // C0 = f(C) C - E at (i,j,k) with ghosts (polymer_stress, fenep.f90:655-659)
+template <bool WL>
 __device__ __forceinline__ double c0_at(const Grid& g, const double* __restrict__ C, const double* __restrict__ fC,
                                         int q, int i, int j, int k) {
   const double E = (q == 0 || q == 3 || q == 5) ? 1.0 : 0.0;
   if (g.walls && (j < 0 || j >= g.ny)) {
-    double c0 = ldb(C, g, i, j, k, BC_C), c3 = ldb(C + 3 * g.n, g, i, j, k, BC_C), c5 = ldb(C + 5 * g.n, g, i, j, k, BC_C);
+    double c0 = ldbw<WL>(C, g, i, j, k, BC_C), c3 = ldbw<WL>(C + 3 * g.n, g, i, j, k, BC_C), c5 = ldbw<WL>(C + 5 * g.n, g, i, j, k, BC_C);
     double f = peterlin_d(g.lp, c0, c3, c5);
-    return f * ldb(C + q * g.n, g, i, j, k, BC_C) - E;
+    return f * ldbw<WL>(C + q * g.n, g, i, j, k, BC_C) - E;
   }
   long m = cidx(g, wrapi(i, g.nx), wrapi(j, g.ny), wrapi(k, g.nz));
   return fC[m] * C[q * g.n + m] - E;
@@ -146,6 +150,7 @@ __device__ __forceinline__ double c0_at(const Grid& g, const double* __restrict_
 // (645-691) and the predictor right-hand side of navier (803-826),
 // fft_navier (1466-1511) or ibm_preliminary (1994-2013), with the history
 // updates A0=A, T0=T, B0=B done in place.
+template <bool WL>
 __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first, int polymer,
                                                    const double* __restrict__ U, const double* __restrict__ V,
                                                    const double* __restrict__ W, const double* __restrict__ P,
@@ -159,9 +164,9 @@ __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first
   int i, j, k;
   band_cell(g, L, m, i, j, k);
   const double dx = g.dx, dy = g.dy, dz = g.dz, dt = g.dt;
-#define Ug(a, b, c) ldb(U, g, a, b, c, BC_U)
-#define Vg(a, b, c) ldb(V, g, a, b, c, BC_V)
-#define Wg(a, b, c) ldb(W, g, a, b, c, BC_W)
+#define Ug(a, b, c) ldbw<WL>(U, g, a, b, c, BC_U)
+#define Vg(a, b, c) ldbw<WL>(V, g, a, b, c, BC_V)
+#define Wg(a, b, c) ldbw<WL>(W, g, a, b, c, BC_W)
   double a[3], bv[3], t[3];
   {
     double h1 = (Ug(i - 1, j, k) + Ug(i, j, k)) / 2.0, h2 = (Ug(i, j, k) + Ug(i + 1, j, k)) / 2.0;
@@ -189,14 +194,14 @@ __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first
   for (int c = 0; c < 3; c++) {
     const double* X = c == 0 ? U : (c == 1 ? V : W);
     int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
-    double x0 = ldb(X, g, i, j, k, kind);
-    double bb = (((ldb(X, g, i - 1, j, k, kind) - 2.0 * x0) + ldb(X, g, i + 1, j, k, kind)) / (dx * dx)
-                 + ((ldb(X, g, i, j - 1, k, kind) - 2.0 * x0) + ldb(X, g, i, j + 1, k, kind)) / (dy * dy))
-                + ((ldb(X, g, i, j, k - 1, kind) - 2.0 * x0) + ldb(X, g, i, j, k + 1, kind)) / (dz * dz);
+    double x0 = ldbw<WL>(X, g, i, j, k, kind);
+    double bb = (((ldbw<WL>(X, g, i - 1, j, k, kind) - 2.0 * x0) + ldbw<WL>(X, g, i + 1, j, k, kind)) / (dx * dx)
+                 + ((ldbw<WL>(X, g, i, j - 1, k, kind) - 2.0 * x0) + ldbw<WL>(X, g, i, j + 1, k, kind)) / (dy * dy))
+                + ((ldbw<WL>(X, g, i, j, k - 1, kind) - 2.0 * x0) + ldbw<WL>(X, g, i, j, k + 1, kind)) / (dz * dz);
     bv[c] = cb * bb;
   }
   if (polymer) {
-#define C0g(q, aa, bb, cc) c0_at(g, C, fC, q, aa, bb, cc)
+#define C0g(q, aa, bb, cc) c0_at<WL>(g, C, fC, q, aa, bb, cc)
     double tx = ((-C0g(0, i, j, k) + C0g(0, i + 1, j, k)) / dx
                  + (((-C0g(1, i, j - 1, k) - C0g(1, i + 1, j - 1, k)) + C0g(1, i, j + 1, k)) + C0g(1, i + 1, j + 1, k)) / (4.0 * dy))
                 + (((-C0g(2, i, j, k - 1) - C0g(2, i + 1, j, k - 1)) + C0g(2, i, j, k + 1)) + C0g(2, i + 1, j, k + 1)) / (4.0 * dz);
@@ -221,8 +226,8 @@ __global__ __launch_bounds__(256) void k_predictor(Grid g, int method, int first
     double a0 = first ? a[c] : A0[o];
     double t0 = first ? t[c] : T0[o];
     double b0 = first ? bv[c] : B0[o];
-    double u = ldb(X, g, i, j, k, kind);
-    double p1 = c == 0 ? ldb(P, g, i + 1, j, k, BC_P) : (c == 1 ? ldb(P, g, i, j + 1, k, BC_P) : ldb(P, g, i, j, k + 1, BC_P));
+    double u = ldbw<WL>(X, g, i, j, k, kind);
+    double p1 = c == 0 ? ldbw<WL>(P, g, i + 1, j, k, BC_P) : (c == 1 ? ldbw<WL>(P, g, i, j + 1, k, BC_P) : ldbw<WL>(P, g, i, j, k + 1, BC_P));
     double h = c == 0 ? dx : (c == 1 ? dy : dz);
     double dp = (dt * (-P[m] + p1)) / h;
     double fv = (F ? F[o] : 0.0) + bfs[c];
@@ -518,6 +523,7 @@ void f64_solve(Ctx& c, int kind, const double* q, double* x) {
 
 // ======================================================= projection =======
 // fft_poisson right-hand side (fenep.f90:1550-1552)
+template <bool WL>
 __global__ __launch_bounds__(256) void k_divergence(Grid g, const double* __restrict__ up, double* __restrict__ q) {
   const long L = xcd_cell();
   if (L >= g.n) return;
@@ -525,16 +531,17 @@ __global__ __launch_bounds__(256) void k_divergence(Grid g, const double* __rest
   int i, j, k;
   band_cell(g, L, m, i, j, k);
   const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
-  q[m] = (((-ldb(Uu, g, i - 1, j, k, BC_U) + ldb(Uu, g, i, j, k, BC_U)) / g.dx
-           + (-ldb(Vv, g, i, j - 1, k, BC_V) + ldb(Vv, g, i, j, k, BC_V)) / g.dy)
-          + (-ldb(Ww, g, i, j, k - 1, BC_W) + ldb(Ww, g, i, j, k, BC_W)) / g.dz) / g.dt;
+  q[m] = (((-ldbw<WL>(Uu, g, i - 1, j, k, BC_U) + ldbw<WL>(Uu, g, i, j, k, BC_U)) / g.dx
+           + (-ldbw<WL>(Vv, g, i, j - 1, k, BC_V) + ldbw<WL>(Vv, g, i, j, k, BC_V)) / g.dy)
+          + (-ldbw<WL>(Ww, g, i, j, k - 1, BC_W) + ldbw<WL>(Ww, g, i, j, k, BC_W)) / g.dz) / g.dt;
 }
 
 void f64_divergence(Ctx& c, const double* up, double* q) {
-  hipLaunchKernelGGL(k_divergence, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g, up, q);
+  hipLaunchKernelGGL(c.g.walls ? k_divergence<true> : k_divergence<false>, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g, up, q);
 }
 
 // march / fft_march (fenep.f90:907-946, 1562-1604)
+template <bool WL>
 __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const double* __restrict__ up, const double* __restrict__ phi,
                                                double* __restrict__ U, double* __restrict__ V, double* __restrict__ W,
                                                double* __restrict__ P) {
@@ -545,11 +552,11 @@ __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const dou
   band_cell(g, L, m, i, j, k);
   const double dt = g.dt, dx = g.dx, dy = g.dy, dz = g.dz;
   const double *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
-#define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)
+#define PH(a, b, c) ldbw<WL>(phi, g, a, b, c, BC_P)
   double ph = phi[m];
-  U[m] = ldb(Uu, g, i, j, k, BC_U) - (dt * (-ph + PH(i + 1, j, k))) / dx;
-  V[m] = ldb(Vv, g, i, j, k, BC_V) - (dt * (-ph + PH(i, j + 1, k))) / dy;
-  W[m] = ldb(Ww, g, i, j, k, BC_W) - (dt * (-ph + PH(i, j, k + 1))) / dz;
+  U[m] = ldbw<WL>(Uu, g, i, j, k, BC_U) - (dt * (-ph + PH(i + 1, j, k))) / dx;
+  V[m] = ldbw<WL>(Vv, g, i, j, k, BC_V) - (dt * (-ph + PH(i, j + 1, k))) / dy;
+  W[m] = ldbw<WL>(Ww, g, i, j, k, BC_W) - (dt * (-ph + PH(i, j, k + 1))) / dz;
   if (explicit_p) {
     P[m] = P[m] + ph;
   } else {
@@ -816,11 +823,11 @@ int f64_sor(Ctx& c, const double* q, double* phi) {
 
 // ======================================================= per step =========
 void f64_select(Ctx& c) {
-  hipLaunchKernelGGL(k_select, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g, c.C[c.cur], c.sel);
+  hipLaunchKernelGGL(c.g.walls ? k_select<true> : k_select<false>, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g, c.C[c.cur], c.sel);
 }
 
 void f64_cstar_newton(Ctx& c) {
-  hipLaunchKernelGGL(k_cstar_newton, dim3(nblk(c.g.n, 128)), dim3(128), 0, c.stream, c.g, c.C[c.cur], c.sel, c.U, c.V, c.W,
+  hipLaunchKernelGGL(c.g.walls ? k_cstar_newton<true> : k_cstar_newton<false>, dim3(nblk(c.g.n, 128)), dim3(128), 0, c.stream, c.g, c.C[c.cur], c.sel, c.U, c.V, c.W,
                      c.C[1 - c.cur], c.fC, c.nitr, c.p.newton_dc, c.p.newton_eps, c.p.newton_itrmax);
   c.cur = 1 - c.cur;
 }
@@ -829,13 +836,13 @@ void f64_predictor(Ctx& c) {
   const Grid& g = c.g;
   int polymer = c.p.beta != 1.0;
   double* Bout = c.p.method == FP_METHOD_IBM ? c.B : nullptr;
-  hipLaunchKernelGGL(k_predictor, dim3(nblk(g.n, 256)), dim3(256), 0, c.stream, g, c.p.method, c.has_history ? 0 : 1,
+  hipLaunchKernelGGL(c.g.walls ? k_predictor<true> : k_predictor<false>, dim3(nblk(g.n, 256)), dim3(256), 0, c.stream, g, c.p.method, c.has_history ? 0 : 1,
                      polymer, c.U, c.V, c.W, c.P, c.C[c.cur], c.fC, c.A0, c.T0, c.B0, Bout, c.F,
                      c.p.body_force[0], c.p.body_force[1], c.p.body_force[2], c.Q);
 }
 
 void f64_march(Ctx& c) {
-  hipLaunchKernelGGL(k_march, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g,
+  hipLaunchKernelGGL(c.g.walls ? k_march<true> : k_march<false>, dim3(nblk(c.g.n, 256)), dim3(256), 0, c.stream, c.g,
                      c.p.method == FP_METHOD_EXPLICIT ? 1 : 0, c.Up, c.Phi, c.U, c.V, c.W, c.P);
 }
 

@@ -229,7 +229,7 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
 #pragma unroll
       for (int q = 0; q < 6; q++) {
         const float* Cq = C + q * g.n;
-        v[o + R][q] = (jj >= -1 && jj <= g.ny) ? ldb(Cq, g, i, jj, k, BC_C) : Cq[m];
+        v[o + R][q] = (jj >= -1 && jj <= g.ny) ? ldbw<true>(Cq, g, i, jj, k, BC_C) : Cq[m];
       }
     }
     return;
@@ -392,6 +392,7 @@ __global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ 
 
 // the reference's selection (select_d, evaluated as written) of one (cell,
 // axis) of the stored FP32 field, wall ghosts by the C_kabe rule
+template <bool WL>
 __device__ __forceinline__ int select_ref(const Grid& g, const float* __restrict__ C, long m, int d) {
   int i, j, k;
   cell_ijk(g, m, i, j, k);
@@ -399,9 +400,9 @@ __device__ __forceinline__ int select_ref(const Grid& g, const float* __restrict
   double cm[6], c[6], cp[6];
   for (int q = 0; q < 6; q++) {
     const float* Cq = C + q * g.n;
-    cm[q] = (double)ldb(Cq, g, i - di, j - dj, k - dk, BC_C);
+    cm[q] = (double)ldbw<WL>(Cq, g, i - di, j - dj, k - dk, BC_C);
     c[q] = (double)Cq[m];
-    cp[q] = (double)ldb(Cq, g, i + di, j + dj, k + dk, BC_C);
+    cp[q] = (double)ldbw<WL>(Cq, g, i + di, j + dj, k + dk, BC_C);
   }
   return select_d(cm, c, cp);
 }
@@ -409,6 +410,7 @@ __device__ __forceinline__ int select_ref(const Grid& g, const float* __restrict
 // FP64 re-decision of the uncertified selections: grid-stride over the
 // total, each entry found in its block's region by a binary search of the
 // exclusive offsets
+template <bool WL>
 __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                     const unsigned* __restrict__ total,
                                                     const unsigned* __restrict__ off, long nb,
@@ -425,7 +427,7 @@ __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restri
     const unsigned e = amb_list[(size_t)lo * SEL_REGION + (size_t)(t - (long)off[lo])];
     const long m = (long)(e / 3u);
     const int d = (int)(e % 3u);
-    const int idx = select_ref(g, C, m, d);
+    const int idx = select_ref<WL>(g, C, m, d);
     // the byte holds three axes: update only this axis
     unsigned* word = (unsigned*)((size_t)(sel + m) & ~(size_t)3);
     const int sh = 8 * (int)((size_t)(sel + m) & 3) + 2 * d;
@@ -447,7 +449,8 @@ __global__ __launch_bounds__(256) void m_select_check(Grid g, const float* __res
   unsigned nb = 0;
   if (m < g.n) {
     const unsigned s = sel[m];
-    for (int d = 0; d < 3; d++) nb += (unsigned)(select_ref(g, C, m, d) != (int)((s >> (2 * d)) & 3u));
+    for (int d = 0; d < 3; d++)
+      nb += (unsigned)((g.walls ? select_ref<true>(g, C, m, d) : select_ref<false>(g, C, m, d)) != (int)((s >> (2 * d)) & 3u));
   }
   for (int o = 32; o > 0; o >>= 1) nb += __shfl_xor(nb, o);
   if (__lane_id() == 0 && nb) atomicAdd(bad, nb);
@@ -536,7 +539,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     const T h = (T)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
     T v1, v2;
     if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
-    else if (d == 1) { v1 = fast ? V[m + o.y[0]] : ldb(V, g, i, j - 1, k, BC_V); v2 = fast ? V[m] : ldb(V, g, i, j, k, BC_V); }
+    else if (d == 1) { v1 = fast ? V[m + o.y[0]] : ldbw<true>(V, g, i, j - 1, k, BC_V); v2 = fast ? V[m] : ldbw<true>(V, g, i, j, k, BC_V); }
     else { v1 = W[m + o.z[0]]; v2 = W[m]; }
     const T r2h = dt / (T(2) * h);
     const T s1 = (-v2 + fabs(v2)) * r2h;
@@ -589,9 +592,9 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
 #undef VF
 #undef WF
   } else {
-#define UL(a, bb, c) ((T)ldb(U, g, a, bb, c, BC_U))
-#define VL(a, bb, c) ((T)ldb(V, g, a, bb, c, BC_V))
-#define WL(a, bb, c) ((T)ldb(W, g, a, bb, c, BC_W))
+#define UL(a, bb, c) ((T)ldbw<true>(U, g, a, bb, c, BC_U))
+#define VL(a, bb, c) ((T)ldbw<true>(V, g, a, bb, c, BC_V))
+#define WL(a, bb, c) ((T)ldbw<true>(W, g, a, bb, c, BC_W))
     a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
     a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (T(4) * dx);
     a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (T(4) * dx);
@@ -687,9 +690,9 @@ __device__ __forceinline__ float c0_atf(const Grid& g, const float* __restrict__
                                         int q, int i, int j, int k) {
   const float E = (q == 0 || q == 3 || q == 5) ? 1.0f : 0.0f;
   if (g.walls && (j < 0 || j >= g.ny)) {
-    float c0 = ldb(C, g, i, j, k, BC_C), c3 = ldb(C + 3 * g.n, g, i, j, k, BC_C), c5 = ldb(C + 5 * g.n, g, i, j, k, BC_C);
+    float c0 = ldbw<true>(C, g, i, j, k, BC_C), c3 = ldbw<true>(C + 3 * g.n, g, i, j, k, BC_C), c5 = ldbw<true>(C + 5 * g.n, g, i, j, k, BC_C);
     float f = peterlin_f((float)(g.lp * g.lp), c0 + c3 + c5);
-    return f * ldb(C + q * g.n, g, i, j, k, BC_C) - E;
+    return f * ldbw<true>(C + q * g.n, g, i, j, k, BC_C) - E;
   }
   long m = cidx(g, wrapi(i, g.nx), wrapi(j, g.ny), wrapi(k, g.nz));
   return fC[m] * C[q * g.n + m] - E;
@@ -710,7 +713,7 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
   Nbr o;
   if (FAST) o = nbr(g, i, j, k);
   // (a, b, c) in {-1, 0, 1}
-#define NB(A, a, b, c, kind) (FAST ? A[m + o.x[(a) + 1] + o.y[(b) + 1] + o.z[(c) + 1]] : ldb(A, g, i + (a), j + (b), k + (c), kind))
+#define NB(A, a, b, c, kind) (FAST ? A[m + o.x[(a) + 1] + o.y[(b) + 1] + o.z[(c) + 1]] : ldbw<true>(A, g, i + (a), j + (b), k + (c), kind))
 #define Ug(a, b, c) NB(U, a, b, c, BC_U)
 #define Vg(a, b, c) NB(V, a, b, c, BC_V)
 #define Wg(a, b, c) NB(W, a, b, c, BC_W)
@@ -791,8 +794,11 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
 #undef Pg
 }
 
+// At most four waves per SIMD: with the registers that frees, the compiler
+// issues the ~70 stencil loads of a cell in one batch instead of ~15 dependent
+// groups of two to four (the kernel waits on L2 latency, not on bandwidth).
 template <bool GEN>
-__global__ __launch_bounds__(256) void m_predictor(Grid g, int method, int first, int polymer,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void m_predictor(Grid g, int method, int first, int polymer,
                                                    const float* __restrict__ U, const float* __restrict__ V,
                                                    const float* __restrict__ W, const float* __restrict__ P,
                                                    const float* __restrict__ C, const float* __restrict__ fC,
@@ -935,6 +941,7 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
 // the forward x transform, two real lines per complex transform as in
 // m_fft_x_fwd.  want_b: also store b (FP64) and the per-block |b|^2 for the
 // defect check.
+template <bool WL>
 __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __restrict__ up, float2* __restrict__ spec,
                                                        const float2* __restrict__ tw, int nl, double* __restrict__ b,
                                                        double* __restrict__ nrm, int want_b, int tr) {
@@ -957,9 +964,9 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
       const long L = l0 + 2 * c + h;
       if (L >= nlines) continue;
       const int j = (int)(L % g.ny), k = (int)(L / g.ny);
-      const double v = (((double)ldb(Uu, g, i, j, k, BC_U) - (double)ldb(Uu, g, i - 1, j, k, BC_U)) / g.dx +
-                        ((double)ldb(Vv, g, i, j, k, BC_V) - (double)ldb(Vv, g, i, j - 1, k, BC_V)) / g.dy +
-                        ((double)ldb(Ww, g, i, j, k, BC_W) - (double)ldb(Ww, g, i, j, k - 1, BC_W)) / g.dz) / g.dt;
+      const double v = (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) / g.dx +
+                        ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) / g.dy +
+                        ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) / g.dz) / g.dt;
       if (want_b) {
         b[L * n + i] = v;
         ss += v * v;
@@ -1384,6 +1391,7 @@ static void solve32(Ctx& c, int kind, const float* q, float* x, const int* skip 
 // ============================================================ projection ===
 // r = b - L_h phi (FP64, boundary rules of Phi), rounded to FP32 for the
 // correction solve; |r|^2 accumulated
+template <bool WL>
 __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restrict__ b, const double* __restrict__ phi,
                                                   float* __restrict__ r32, double* __restrict__ nrm) {
   long m = xcd_cell();
@@ -1392,7 +1400,7 @@ __global__ __launch_bounds__(256) void m_residual(Grid g, const double* __restri
     int i, j, k;
   cell_ijk(g, m, i, j, k);
     double p = phi[m];
-#define PH(a, bb, c) ldb(phi, g, a, bb, c, BC_P)
+#define PH(a, bb, c) ldbw<WL>(phi, g, a, bb, c, BC_P)
     double lap = (PH(i - 1, j, k) - 2.0 * p + PH(i + 1, j, k)) / (g.dx * g.dx) +
                  (PH(i, j - 1, k) - 2.0 * p + PH(i, j + 1, k)) / (g.dy * g.dy) +
                  (PH(i, j, k - 1) - 2.0 * p + PH(i, j, k + 1)) / (g.dz * g.dz);
@@ -1442,6 +1450,7 @@ __global__ __launch_bounds__(1024) void m_sum_decide(const double* __restrict__ 
 // [1 - eps lap_h] x = q for the three velocity components at once:
 // x_out = q + eps lap_h x_in (x_in = q on the first sweep).  With
 // eps * |lap_h| << 1 the error contracts by eps*lambda_max per sweep.
+template <bool WL>
 __global__ __launch_bounds__(256) void m_helm_sweep(Grid g, float eps, const float* __restrict__ q,
                                                     const float* __restrict__ xin, float* __restrict__ xout) {
   long m = xcd_cell();
@@ -1453,12 +1462,12 @@ __global__ __launch_bounds__(256) void m_helm_sweep(Grid g, float eps, const flo
   for (int c = 0; c < 3; c++) {
     const float* X = xin + c * g.n;
     int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
-    float x0 = ldb(X, g, i, j, k, kind);
-    float lap = (ldb(X, g, i - 1, j, k, kind) - 2.0f * x0 + ldb(X, g, i + 1, j, k, kind)) * ix2 +
-                (ldb(X, g, i, j - 1, k, kind) - 2.0f * x0 + ldb(X, g, i, j + 1, k, kind)) * iy2 +
-                (ldb(X, g, i, j, k - 1, kind) - 2.0f * x0 + ldb(X, g, i, j, k + 1, kind)) * iz2;
+    float x0 = ldbw<WL>(X, g, i, j, k, kind);
+    float lap = (ldbw<WL>(X, g, i - 1, j, k, kind) - 2.0f * x0 + ldbw<WL>(X, g, i + 1, j, k, kind)) * ix2 +
+                (ldbw<WL>(X, g, i, j - 1, k, kind) - 2.0f * x0 + ldbw<WL>(X, g, i, j + 1, k, kind)) * iy2 +
+                (ldbw<WL>(X, g, i, j, k - 1, kind) - 2.0f * x0 + ldbw<WL>(X, g, i, j, k + 1, kind)) * iz2;
     float v = q[c * g.n + m] + eps * lap;
-    if (c == 1 && g.walls && j == g.ny - 1) v = 0.0f;   // wall face of V
+    if (c == 1 && WL && j == g.ny - 1) v = 0.0f;   // wall face of V
     xout[c * g.n + m] = v;
   }
 }
@@ -1479,12 +1488,14 @@ static void helmholtz32(Ctx& c, const float* q, float* x) {
   for (int it = 0; it < X.helm_iters; it++) {
     // the last sweep writes x; earlier ones alternate so that it does
     float* out = ((X.helm_iters - 1 - it) % 2 == 0) ? x : X.X2;
-    hipLaunchKernelGGL(m_helm_sweep, dim3(nblk(n, 256)), dim3(256), 0, c.stream, g, eps, q, in, out);
+    if (g.walls) hipLaunchKernelGGL(m_helm_sweep<true>, dim3(nblk(n, 256)), dim3(256), 0, c.stream, g, eps, q, in, out);
+    else hipLaunchKernelGGL(m_helm_sweep<false>, dim3(nblk(n, 256)), dim3(256), 0, c.stream, g, eps, q, in, out);
     in = out;
   }
 }
 
 // fft_march (fenep.f90:1562-1604) with FP64 potential, FP32 storage
+template <bool WL>
 __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__ up, const double* __restrict__ phi,
                                                float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
                                                float* __restrict__ P) {
@@ -1494,11 +1505,11 @@ __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__
   cell_ijk(g, m, i, j, k);
   const double dt = g.dt;
   const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
-#define PH(a, b, c) ldb(phi, g, a, b, c, BC_P)
+#define PH(a, b, c) ldbw<WL>(phi, g, a, b, c, BC_P)
   double ph = phi[m];
-  U[m] = (float)((double)ldb(Uu, g, i, j, k, BC_U) - dt * (PH(i + 1, j, k) - ph) / g.dx);
-  V[m] = (float)((double)ldb(Vv, g, i, j, k, BC_V) - dt * (PH(i, j + 1, k) - ph) / g.dy);
-  W[m] = (float)((double)ldb(Ww, g, i, j, k, BC_W) - dt * (PH(i, j, k + 1) - ph) / g.dz);
+  U[m] = (float)((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - dt * (PH(i + 1, j, k) - ph) / g.dx);
+  V[m] = (float)((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - dt * (PH(i, j + 1, k) - ph) / g.dy);
+  W[m] = (float)((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - dt * (PH(i, j, k + 1) - ph) / g.dz);
   const double coef = ((dt * g.beta) / g.re) / 2.0;
   double lap = (PH(i - 1, j, k) - 2.0 * ph + PH(i + 1, j, k)) / (g.dx * g.dx) +
                (PH(i, j - 1, k) - 2.0 * ph + PH(i, j + 1, k)) / (g.dy * g.dy) +
@@ -1561,6 +1572,7 @@ __global__ void m_ibm_setup_spread(Grid g, int comp, int rows, const int* __rest
 
 // interpolation of Ua at the markers and the direct forcing Fc = (Uc - Ub)/dt
 // (ibm_Helmholtz, fenep.f90:2076-2119), one thread per (component, marker)
+template <bool WL>
 __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const float* __restrict__ wi, const int4* __restrict__ org,
                                                     const float* __restrict__ uc, const float* __restrict__ ua,
                                                     float* __restrict__ mf) {
@@ -1581,7 +1593,7 @@ __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const floa
     for (int jj = 0; jj < 3; jj++)
 #pragma unroll
       for (int ii = 0; ii < 3; ii++, s++)
-        acc += w[(long)s * nmk] * ldb(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
+        acc += w[(long)s * nmk] * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
   mf[t] = (uc[t] - acc) * idt;
 }
 
@@ -1960,7 +1972,8 @@ static void select_fenep(Ctx& c, int tm) {
     hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb);
     // one wavefront per select block: 16 per SIMD at 256x128x128, enough to
     // hide the search and FP64 latencies; n / 4 entries per pass
-    hipLaunchKernelGGL(m_select_fix, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_off, nb, X.amb_list);
+    hipLaunchKernelGGL(g.walls ? m_select_fix<true> : m_select_fix<false>, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel,
+                       X.amb, X.amb_off, nb, X.amb_list);
   }
 }
 
@@ -2007,7 +2020,8 @@ void mixed_step(Ctx& c) {
   } else {
     {
       MTimer t(c, "ibm");
-      hipLaunchKernelGGL(m_ibm_interp, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s, g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
+      hipLaunchKernelGGL(g.walls ? m_ibm_interp<true> : m_ibm_interp<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s, g,
+                         c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
       if (X.fr_rows)
         hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
                            X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
@@ -2024,15 +2038,16 @@ void mixed_step(Ctx& c) {
     const bool check = ndc > 1 && dc_check_due(c);
     const int tr = X.slab;
     const size_t lds_inv = tr ? lds_x32_tr(g) : lds_x32(g);
-    hipLaunchKernelGGL(m_div_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.Up, X.spec, X.twx, lx, X.b, X.partb,
-                       check ? 1 : 0, tr);
+    hipLaunchKernelGGL(g.walls ? m_div_fft_x_fwd<true> : m_div_fft_x_fwd<false>, dim3(nbx), dim3(256), lds_x32(g), s, g,
+                       X.Up, X.spec, X.twx, lx, X.b, X.partb, check ? 1 : 0, tr);
     solve32_yz(c, 3, nullptr, tr);
     hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_inv, s, g, X.spec, X.phi, X.twx, lx, 1, nullptr, tr);
     if (check) {
       // FP64 defect r = b - L_h phi; the next sweep is skipped on the device
       // once |r| <= dc_tol |b|
       for (int it = 1; it < ndc; it++) {
-        hipLaunchKernelGGL(m_residual, dim3(nb), dim3(256), 0, s, g, X.b, X.phi, X.r32, X.part);
+        hipLaunchKernelGGL(g.walls ? m_residual<true> : m_residual<false>, dim3(nb), dim3(256), 0, s, g, X.b, X.phi, X.r32,
+                           X.part);
         hipLaunchKernelGGL(m_sum_decide, dim3(1), dim3(1024), 0, s, X.partb, nbx, X.part, nb, X.red, c.p.dc_tol, X.skip);
         if (it == 1 && c.step % DC_EVERY == 0) dc_record(c);
         hipLaunchKernelGGL(m_fft_x_fwd, dim3(nbx), dim3(256), lds_x32(g), s, g, X.r32, X.spec, X.twx, lx, X.skip, tr);
@@ -2043,7 +2058,8 @@ void mixed_step(Ctx& c) {
   }
   {
     MTimer t(c, "march");
-    hipLaunchKernelGGL(m_march, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.phi, X.U, X.V, X.W, X.P);
+    hipLaunchKernelGGL(g.walls ? m_march<true> : m_march<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.phi, X.U,
+                       X.V, X.W, X.P);
   }
   c.has_history = 1;
   c.step++;
