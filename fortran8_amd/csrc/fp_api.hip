@@ -430,6 +430,7 @@ int fp_step(void* vc, int nsteps, int sync) {
       if (c->mixed) mixed_step(*c);
       else step_f64(*c);
     }
+    if (c->mixed) mixed_join(*c);
     if (sync || c->timing) FP_HIP_CHECK(hipStreamSynchronize(c->stream));
     flush_timers(*c);
   } catch (const std::string& e) {

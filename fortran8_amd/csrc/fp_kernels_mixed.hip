@@ -24,6 +24,12 @@ namespace fp {
 namespace {
 
 struct Mixed {
+  // The C+- selection of the C a step produces is made on a second stream
+  // (default priority) beside the rest of that step; the next step's
+  // CStar/Newton waits for it (sel_done).
+  hipStream_t side = nullptr;
+  hipEvent_t ev_cstar = nullptr, sel_done = nullptr;
+  int sel_ready = 0;              // c.sel holds the selection of the current C
   float *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr;
   float* C[2] = {nullptr, nullptr};
   int cur = 0;
@@ -1637,6 +1643,9 @@ void mixed_create(Ctx& c) {
   c.mixed = X;
   const Grid& g = c.g;
   const long n = g.n;
+  FP_HIP_CHECK(hipStreamCreateWithFlags(&X->side, hipStreamNonBlocking));
+  FP_HIP_CHECK(hipEventCreateWithFlags(&X->ev_cstar, hipEventDisableTiming));
+  FP_HIP_CHECK(hipEventCreateWithFlags(&X->sel_done, hipEventDisableTiming));
   dalloc(&X->U, n); dalloc(&X->V, n); dalloc(&X->W, n); dalloc(&X->P, n);
   dalloc(&X->C[0], 6 * n); dalloc(&X->C[1], 6 * n); dalloc(&X->f, n);
   dalloc(&X->A0, 3 * n); dalloc(&X->T0, 3 * n); dalloc(&X->B, 3 * n);
@@ -1709,6 +1718,10 @@ void mixed_create(Ctx& c) {
 void mixed_destroy(Ctx& c) {
   Mixed* X = (Mixed*)c.mixed;
   if (!X) return;
+  if (X->side) (void)hipStreamSynchronize(X->side);
+  if (X->ev_cstar) (void)hipEventDestroy(X->ev_cstar);
+  if (X->sel_done) (void)hipEventDestroy(X->sel_done);
+  if (X->side) (void)hipStreamDestroy(X->side);
   dfree(X->U); dfree(X->V); dfree(X->W); dfree(X->P); dfree(X->C[0]); dfree(X->C[1]); dfree(X->f);
   dfree(X->A0); dfree(X->T0); dfree(X->B); dfree(X->Q); dfree(X->Up); dfree(X->F);
   dfree(X->spec); dfree(X->thp); dfree(X->twx); dfree(X->twy); dfree(X->twz);
@@ -1731,6 +1744,9 @@ void mixed_load_from_f64(Ctx& c) {
   Mixed& X = MX(c);
   const long n = c.g.n;
   hipStream_t s = c.stream;
+  // a selection made ahead still reads the old C; it no longer applies
+  if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(s, X.sel_done, 0));
+  X.sel_ready = 0;
   hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.U, X.U);
   hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.V, X.V);
   hipLaunchKernelGGL(m_to32, dim3(nblk(n, 256)), dim3(256), 0, s, n, c.W, X.W);
@@ -1874,7 +1890,7 @@ double mixed_last_ndc(Ctx& c) {
 // C+- selection of the current stored field: certified FP32 decisions, the
 // per-block list counts scanned, the rest re-decided in FP64.  tm != 0 times
 // the two phases.
-static void select_fenep(Ctx& c, int tm);
+static void select_fenep(Ctx& c, int tm, hipStream_t s);
 
 // Selection of the current stored field as the step makes it (certified FP32
 // decisions, FP64 re-decision of the rest), compared with the reference's
@@ -1888,8 +1904,10 @@ int mixed_check_selection(Ctx& c, double* out3) {
   out3[0] = out3[1] = 0.0;
   out3[2] = 3.0 * (double)g.n;
   if (c.p.beta == 1.0) return 0;   // no polymer stress: no selection
+  if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(s, X.sel_done, 0));
   zero_words(X.chk, 1, s);
-  select_fenep(c, 0);
+  select_fenep(c, 0, s);
+  X.sel_ready = 1;
   hipLaunchKernelGGL(m_select_check, dim3(nblk(g.n, 256)), dim3(256), 0, s, g, X.C[X.cur], c.sel, X.chk);
   FP_HIP_CHECK(hipGetLastError());
   unsigned h[2] = {0, 0};
@@ -1903,8 +1921,16 @@ int mixed_check_selection(Ctx& c, double* out3) {
 
 // statistics of the last step: [0] |b|^2, [1] |r|^2 before the last
 // correction, [2] guarded selections re-decided in FP64
+// The context's stream waits for work made ahead on the side stream, so that
+// synchronizing the context's stream covers it.
+void mixed_join(Ctx& c) {
+  Mixed& X = MX(c);
+  if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(c.stream, X.sel_done, 0));
+}
+
 int mixed_stats(Ctx& c, double* out3) {
   Mixed& X = MX(c);
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
   double h[8];
   FP_HIP_CHECK(hipMemcpy(h, X.red, sizeof(h), hipMemcpyDeviceToHost));
   unsigned amb = 0;
@@ -1917,15 +1943,16 @@ namespace {
 struct MTimer {
   Ctx& c;
   const char* name;   // nullptr: not timed
+  hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
-  MTimer(Ctx& cc, const char* nm) : c(cc), name(nm) {
+  MTimer(Ctx& cc, const char* nm, hipStream_t s = nullptr) : c(cc), name(nm), st(s ? s : cc.stream) {
     if (!c.timing || !name) return;
     a = ev_get(c); b = ev_get(c);
-    (void)hipEventRecord(a, c.stream);
+    (void)hipEventRecord(a, st);
   }
   ~MTimer() {
     if (!c.timing || !name) return;
-    (void)hipEventRecord(b, c.stream);
+    (void)hipEventRecord(b, st);
     c.ev.push_back({std::string(name) + "#a", a});
     c.ev.push_back({std::string(name) + "#b", b});
   }
@@ -1958,23 +1985,35 @@ static void dc_record(Ctx& c) {
   X.dc_pending = true;
 }
 
-static void select_fenep(Ctx& c, int tm) {
+static void select_fenep(Ctx& c, int tm, hipStream_t s) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
-  hipStream_t s = c.stream;
   const long nb = nblk(g.n, SEL_BLOCK);
   {
-    MTimer t(c, tm ? "fenep_select" : nullptr);
+    MTimer t(c, tm ? "fenep_select" : nullptr, s);
     hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list);
   }
   {
-    MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr);
+    MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr, s);
     hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb);
     // one wavefront per select block: 16 per SIMD at 256x128x128, enough to
     // hide the search and FP64 latencies; n / 4 entries per pass
     hipLaunchKernelGGL(g.walls ? m_select_fix<true> : m_select_fix<false>, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel,
                        X.amb, X.amb_off, nb, X.amb_list);
   }
+}
+
+// Start the selection of the C this step produced on the side stream: it
+// waits for everything enqueued on the context's stream so far and runs
+// beside what follows (its VALU work beside the memory-bound kernels and the
+// Poisson slab solve, which leaves half of the CUs free).
+static void select_ahead(Ctx& c) {
+  Mixed& X = MX(c);
+  FP_HIP_CHECK(hipEventRecord(X.ev_cstar, c.stream));
+  FP_HIP_CHECK(hipStreamWaitEvent(X.side, X.ev_cstar, 0));
+  select_fenep(c, 1, X.side);
+  FP_HIP_CHECK(hipEventRecord(X.sel_done, X.side));
+  X.sel_ready = 1;
 }
 
 void mixed_step(Ctx& c) {
@@ -1986,7 +2025,9 @@ void mixed_step(Ctx& c) {
   if (!polymer) {
     zero_words(X.C[X.cur], 6 * n, s);
   } else {
-    select_fenep(c, 1);
+    // the selection of this C: made ahead beside the previous step, or now
+    if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(s, X.sel_done, 0));
+    else select_fenep(c, 1, s);
     // one kernel per timed phase: the bulk instances and the wall rows apart
     const int itmax = std::min(c.p.newton_itrmax, 16);
     {
@@ -2030,6 +2071,9 @@ void mixed_step(Ctx& c) {
     helmholtz32(c, X.R, X.Up);
   }
   {
+    // the next step's selection beside the Poisson solve (measured best:
+    // the slab solve occupies one CU per kx)
+    if (polymer) select_ahead(c);
     MTimer t(c, "poisson");
     // first sweep: b = div u*/dt in the load stage of the x transform, the
     // FP32 spectral + line solve, Phi accumulated in FP64 in the store stage

@@ -16,4 +16,5 @@ double mixed_last_ndc(Ctx& c);
 int mixed_stats(Ctx& c, double* out3);
 int mixed_check_selection(Ctx& c, double* out3);
 void mixed_step(Ctx& c);
+void mixed_join(Ctx& c);
 }  // namespace fp
