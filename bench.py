@@ -7,9 +7,11 @@ the state resident in HBM.  Prints ONE JSON line.
   python bench.py [--steps K] [--warmup W] [--precision mixed|f64]
                   [--no-cpu] [--cpu-steps S] [--small]
 
-The per-kernel durations come from HIP events recorded on the context's own
-stream around each phase, inside the timed region.  HBM traffic per launch
-of the dominant kernel is the rocprofv3 FETCH_SIZE/WRITE_SIZE reading kept in
+The roofline kernel's duration comes from HIP events recorded around it on
+its stream inside the timed region (the only events there: every phase
+timed would add ~7 % to the step); the per-phase breakdown comes from a few
+steps run afterwards with every phase timed.  HBM traffic per launch of the
+roofline kernel is the rocprofv3 FETCH_SIZE/WRITE_SIZE reading kept in
 profiles/traffic.txt (already scaled by the copy-kernel calibration).
 """
 import argparse
@@ -70,21 +72,26 @@ def mixed_step_bytes(n_dc):
 
 
 def run_gpu(case, steps, warmup):
+    """Timed region: `steps` steps with HIP events only around the predictor
+    (the roofline kernel; two events per step).  Then, outside the timed
+    region, a few steps with every phase timed for the breakdown."""
     g = api.load_case(case)
-    # warm-up with the phase events on (fills the context's event pool)
-    g.enable_timing(True)
-    g.step(warmup, sync=True)
-    # timed region: K steps with the phase events on the context's stream
-    g.enable_timing(True)
+    g.enable_timing(2)
+    g.step(warmup, sync=True)           # warm-up (also fills the event pool)
+    g.enable_timing(2)
     t0 = time.perf_counter()
     g.step(steps, sync=True)
     t1 = time.perf_counter()
-    kt = {k: v / steps for k, v in g.kernel_times().items()}
-    g.enable_timing(False)
-    wall = (t1 - t0) / steps
+    kt_timed = {k: v / steps for k, v in g.kernel_times().items()}
     d = g.diagnostics()
+    nb = max(1, min(steps, 10))
+    g.enable_timing(1)
+    g.step(nb, sync=True)
+    kt = {k: v / nb for k, v in g.kernel_times().items()}
+    g.enable_timing(0)
     g.close()
-    return wall, kt, d
+    wall = (t1 - t0) / steps
+    return wall, kt_timed, kt, d
 
 
 def read_traffic(precision):
@@ -132,7 +139,7 @@ def main():
         case = configs.throughput(nx=64, ny=32, nz=32, precision=prec, ncyl=1)
     else:
         case = configs.throughput(precision=prec)
-    wall, kt, d = run_gpu(case, a.steps, a.warmup)
+    wall, kt_timed, kt, d = run_gpu(case, a.steps, a.warmup)
     cells = case.cells
     value = cells / wall / 1e6
     width = 8 if prec == 0 else 4
@@ -145,7 +152,7 @@ def main():
     words, extra = KERNEL_WORDS[dom]
     units = cells_per_launch(case.params, prec, dom)
     dom_bytes = (words * width + extra) * units
-    dom_ms = kt.get(dom, 0.0)
+    dom_ms = kt_timed.get(dom, 0.0)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     traffic = read_traffic(a.precision).get(dom)
     step_bytes = STEP_WORDS * 8 * cells if prec == 0 else mixed_step_bytes(d.get("n_dc", 0)) * cells
@@ -168,6 +175,8 @@ def main():
         "config": {"workload": case.name, "grid": [case.params.nx, case.params.ny, case.params.nz],
                    "cylinders": len(case.markers.centers), "precision": a.precision,
                    "poisson": "fft+tridiagonal", "parallelism": "single GPU"},
+        # phase breakdown from steps run after the timed region with every
+        # phase timed (the timed region times the predictor only)
         "phases_ms": {k: round(v, 4) for k, v in kt.items()},
         "roofline": {
             "bound": "hbm",

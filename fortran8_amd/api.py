@@ -207,7 +207,7 @@ class FenepGPU:
         return out
 
     def enable_timing(self, on=True):
-        lib().fp_enable_timing(self.h, 1 if on else 0)
+        lib().fp_enable_timing(self.h, int(on) if not isinstance(on, bool) else (1 if on else 0))
 
     def kernel_times(self):
         ms = np.zeros(64)

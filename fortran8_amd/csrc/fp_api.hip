@@ -101,12 +101,12 @@ struct Timer {
   const char* name;
   hipEvent_t a = nullptr, b = nullptr;
   Timer(Ctx& cc, const char* nm) : c(cc), name(nm) {
-    if (!c.timing) return;
+    if (!phase_timed(c, name)) return;
     a = ev_get(c); b = ev_get(c);
     (void)hipEventRecord(a, c.stream);
   }
   ~Timer() {
-    if (!c.timing) return;
+    if (!phase_timed(c, name)) return;
     (void)hipEventRecord(b, c.stream);
     c.ev.push_back({std::string(name) + "#a", a});
     c.ev.push_back({std::string(name) + "#b", b});
@@ -518,7 +518,8 @@ int fp_get_state_device(void* vc, double* U, double* V, double* W, double* P, do
 int fp_enable_timing(void* vc, int on) {
   Ctx* c = (Ctx*)vc;
   if (!c) return FP_E_ARG;
-  c->timing = on ? 1 : 0;
+  if (on < 0 || on > 2) return fail(c, FP_E_ARG, "timing level is 0, 1 or 2");
+  c->timing = on;
   c->ktime.clear();
   return FP_OK;
 }

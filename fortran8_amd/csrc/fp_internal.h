@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cstring>
 #include <string>
 #include <vector>
 #include "../../include/fenep_gpu.h"
@@ -106,6 +107,12 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
 void adj_free(Ctx& c);
 
 // an event from the context's pool (created on first use)
+// Phase timing: level 1 records every phase, level 2 only the predictor (the
+// kernel whose roofline the benchmark reports), so that a timed run carries
+// two events per step.
+inline bool phase_timed(const Ctx& c, const char* name) {
+  return name && (c.timing == 1 || (c.timing == 2 && std::strcmp(name, "predictor") == 0));
+}
 inline hipEvent_t ev_get(Ctx& c) {
   if (c.evpool.empty()) {
     hipEvent_t e = nullptr;

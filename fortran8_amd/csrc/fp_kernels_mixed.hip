@@ -1946,12 +1946,12 @@ struct MTimer {
   hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   MTimer(Ctx& cc, const char* nm, hipStream_t s = nullptr) : c(cc), name(nm), st(s ? s : cc.stream) {
-    if (!c.timing || !name) return;
+    if (!phase_timed(c, name)) return;
     a = ev_get(c); b = ev_get(c);
     (void)hipEventRecord(a, st);
   }
   ~MTimer() {
-    if (!c.timing || !name) return;
+    if (!phase_timed(c, name)) return;
     (void)hipEventRecord(b, st);
     c.ev.push_back({std::string(name) + "#a", a});
     c.ev.push_back({std::string(name) + "#b", b});

@@ -146,7 +146,9 @@ int fp_get_state_device(void* ctx, double* U, double* V, double* W, double* P,
 
 /* Per-kernel timing of the dominant kernels over the last fp_step call with
  * timing enabled (milliseconds, summed over steps). names is filled with a
- * comma-separated list; returns the number of entries written to ms. */
+ * comma-separated list; returns the number of entries written to ms.
+ * on: 0 off, 1 every phase, 2 only the predictor phase (two events per
+ * step, for timing runs). */
 int fp_enable_timing(void* ctx, int on);
 int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
                     int names_len);
