@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["mixed", "f64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--small", action="store_true", help="64x32x32 smoke size")
     a = ap.parse_args()
     if a.gpus != 1:

@@ -2055,7 +2055,12 @@ void mixed_step(Ctx& c) {
                        X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   }
+  // The next step's selection (it reads only the new C) forks onto the second
+  // stream here, beside the Helmholtz sweeps and the Poisson solve.  Forking
+  // before the IBM phase measured slower, forking after the sweeps leaves the
+  // selection chain longer than what overlaps it.
   if (c.p.method == FP_METHOD_FFT) {
+    if (polymer) select_ahead(c);
     MTimer t(c, "helmholtz");
     helmholtz32(c, X.Q, X.Up);
   } else {
@@ -2067,13 +2072,11 @@ void mixed_step(Ctx& c) {
         hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
                            X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
     }
+    if (polymer) select_ahead(c);
     MTimer t(c, "helmholtz");
     helmholtz32(c, X.R, X.Up);
   }
   {
-    // the next step's selection beside the Poisson solve (measured best:
-    // the slab solve occupies one CU per kx)
-    if (polymer) select_ahead(c);
     MTimer t(c, "poisson");
     // first sweep: b = div u*/dt in the load stage of the x transform, the
     // FP32 spectral + line solve, Phi accumulated in FP64 in the store stage
