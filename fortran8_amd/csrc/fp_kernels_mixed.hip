@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <unordered_map>
 #include <vector>
 
 namespace fp {
@@ -73,6 +74,19 @@ struct Mixed {
   int* fr_ent = nullptr;          // c*nmk + marker
   float* fr_w = nullptr;
   int fr_rows = 0;
+  // markers repeated along z (every layer is layer 0 moved up by whole
+  // planes, checked on the weights themselves): the interpolation weights of
+  // layer 0 and the spreading rows of plane 0 serve every plane
+  int zinv = 0;
+  long zP = 0;                    // markers per layer
+  float* zi_w = nullptr;          // interpolation weights of layer 0, 3 * 27 * zP
+  int2* zi_org = nullptr;         // i0, j0 per component and layer-0 marker
+  int* zs_off = nullptr;          // plane rows as sliced ELL (as fr_*)
+  int* zs_len = nullptr;
+  long* zs_tgt = nullptr;         // c*n + in-plane interior index
+  int* zs_ent = nullptr;          // 4 * (c*nmk + layer-0 marker) + layer offset + 1
+  float* zs_w = nullptr;
+  int zs_rows = 0, zs_rows64 = 0;
   double last_ndc = 0.0;
 };
 
@@ -1603,6 +1617,9 @@ __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const floa
   mf[t] = (uc[t] - acc) * idt;
 }
 
+// entries of a sliced-ELL slice come in groups of this many
+#define ELL_UNROLL 8
+
 // spreading gathered onto interior targets with the ghost images folded in
 // (ibm_PBM, fenep.f90:2195-2205), FP32 accumulation
 __global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* __restrict__ slice_off,
@@ -1611,18 +1628,91 @@ __global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* 
                                                            const float* __restrict__ mf, float dt,
                                                            float* __restrict__ rhs) {
   // sliced ELL: the rows of a 64-row slice store their entries slot-major
-  // (entry k of lane l at off + 64 k + l), padded with zero weights, so each
-  // wave reads its entries with contiguous loads
+  // (entry k of lane l at off + 64 k + l), padded with zero weights to a
+  // multiple of ELL_UNROLL entries, so each wave reads its entries with
+  // contiguous loads, ELL_UNROLL entries' loads in flight together
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   const int sl = r >> 6, lane = r & 63;
   const int off = slice_off[sl], len = slice_len[sl];
   float acc = 0.0f;
-  for (int k = 0; k < len; k++) {
-    const int e = off + 64 * k + lane;
-    acc += mf[ent[e]] * w[e];
+  for (int k = 0; k < len; k += ELL_UNROLL) {
+    int pk[ELL_UNROLL];
+    float wv[ELL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < ELL_UNROLL; u++) {
+      const int e = off + 64 * (k + u) + lane;
+      pk[u] = ent[e];
+      wv[u] = w[e];
+    }
+#pragma unroll
+    for (int u = 0; u < ELL_UNROLL; u++) acc += mf[pk[u]] * wv[u];
   }
   rhs[tgt[r]] += dt * acc;   // each target appears in one row only
+}
+
+// The same two passes for markers repeated along z.  Interpolation: the
+// weights and the in-plane origin of the layer-0 marker, the layer from the
+// marker index (markers are numbered layer by layer, zP to a layer).
+template <bool WL>
+__global__ __launch_bounds__(256) void m_ibm_interp_z(Grid g, long nmk, long zP, const float* __restrict__ wz,
+                                                      const int2* __restrict__ orgz, const float* __restrict__ uc,
+                                                      const float* __restrict__ ua, float* __restrict__ mf) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 3 * nmk) return;
+  const int c = (int)(t / nmk);
+  const long id = t - c * nmk;
+  const int L = (int)(id / zP);
+  const long p = id - (long)L * zP;
+  const float idt = (float)(1.0 / g.dt);
+  const int2 o = orgz[c * zP + p];
+  const float* X = ua + c * g.n;
+  const int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+  const float* w = wz + (long)c * 27 * zP + p;
+  float acc = 0.0f;
+  int s = 0;
+#pragma unroll
+  for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 3; ii++, s++)
+        acc += w[(long)s * zP] * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, L + kk - 1, kind);
+  mf[t] = (uc[t] - acc) * idt;
+}
+
+// Spreading: thread (plane k, row r of plane 0); an entry names the layer-0
+// marker and the layer offset d in {-1, 0, 1} (periodic in z), so the marker
+// it takes is that of layer k + d.  Rows are padded to whole slices per plane.
+__global__ __launch_bounds__(256) void m_ibm_spread_z(int rows, int rows64, int nz, long zP, long plane,
+                                                      const int* __restrict__ slice_off,
+                                                      const int* __restrict__ slice_len, const long* __restrict__ tgt,
+                                                      const int* __restrict__ ent, const float* __restrict__ w,
+                                                      const float* __restrict__ mf, float dt, float* __restrict__ rhs) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int k = (int)(t / rows64);
+  const int r = (int)(t - (long)k * rows64);
+  if (k >= nz || r >= rows) return;
+  const int sl = r >> 6, lane = r & 63;
+  const int off = slice_off[sl], len = slice_len[sl];
+  float acc = 0.0f;
+  for (int q = 0; q < len; q += ELL_UNROLL) {
+    int pk[ELL_UNROLL];
+    float wv[ELL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < ELL_UNROLL; u++) {
+      const int e = off + 64 * (q + u) + lane;
+      pk[u] = ent[e];
+      wv[u] = w[e];
+    }
+#pragma unroll
+    for (int u = 0; u < ELL_UNROLL; u++) {
+      int L = k + (pk[u] & 3) - 1;
+      L = L < 0 ? L + nz : (L >= nz ? L - nz : L);
+      acc += mf[(long)(pk[u] >> 2) + (long)L * zP] * wv[u];
+    }
+  }
+  rhs[tgt[r] + (long)k * plane] += dt * acc;   // each target appears in one row only
 }
 
 // ========================================================== conversions ===
@@ -1736,6 +1826,8 @@ void mixed_destroy(Ctx& c) {
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
   dfree(X->fr_row); dfree(X->fr_len); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
+  dfree(X->zi_w); dfree(X->zi_org);
+  dfree(X->zs_off); dfree(X->zs_len); dfree(X->zs_tgt); dfree(X->zs_ent); dfree(X->zs_w);
   delete X;
   c.mixed = nullptr;
 }
@@ -1828,6 +1920,145 @@ void mixed_set_markers(Ctx& c) {
       for (int e = rowp[r]; e < rowp[r + 1]; e++) dst.push_back({(int)(k * c.nmk + ent[e]), w[e]});
     }
   }
+  dfree(X.fr_row); dfree(X.fr_len); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w);
+  dfree(X.zi_w); dfree(X.zi_org);
+  dfree(X.zs_off); dfree(X.zs_len); dfree(X.zs_tgt); dfree(X.zs_ent); dfree(X.zs_w);
+  X.fr_rows = 0;
+  X.zinv = 0;
+  X.zs_rows = X.zs_rows64 = 0;
+  // CSR -> sliced ELL (64-row slices, slot-major, padded with zero weights and
+  // the entry `pad`), uploaded to the given buffers
+  auto upload_ell = [&](const std::vector<int>& frow, const std::vector<int>& fent, const std::vector<float>& fw,
+                        const std::vector<long>& ftgt, int pad, int*& d_off, int*& d_len, long*& d_tgt, int*& d_ent,
+                        float*& d_w) {
+    const int nrows = (int)ftgt.size();
+    const int nsl = (nrows + 63) / 64;
+    std::vector<int> soff(std::max(1, nsl)), slen(std::max(1, nsl));
+    long total = 0;
+    for (int sl = 0; sl < nsl; sl++) {
+      int mx = 0;
+      for (int r = sl * 64; r < std::min(nrows, sl * 64 + 64); r++) mx = std::max(mx, frow[r + 1] - frow[r]);
+      mx = (mx + ELL_UNROLL - 1) / ELL_UNROLL * ELL_UNROLL;
+      soff[sl] = (int)total;
+      slen[sl] = mx;
+      total += 64L * mx;
+    }
+    std::vector<int> eent(std::max<long>(1, total), pad);
+    std::vector<float> ew(std::max<long>(1, total), 0.0f);
+    for (int r = 0; r < nrows; r++) {
+      const int sl = r >> 6, lane = r & 63;
+      for (int e = frow[r]; e < frow[r + 1]; e++) {
+        const long pos = soff[sl] + 64L * (e - frow[r]) + lane;
+        eent[pos] = fent[e];
+        ew[pos] = fw[e];
+      }
+    }
+    dalloc(&d_off, soff.size()); dalloc(&d_len, slen.size()); dalloc(&d_tgt, std::max<size_t>(1, ftgt.size()));
+    dalloc(&d_ent, eent.size()); dalloc(&d_w, ew.size());
+    FP_HIP_CHECK(hipMemcpy(d_off, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice));
+    FP_HIP_CHECK(hipMemcpy(d_len, slen.data(), sizeof(int) * slen.size(), hipMemcpyHostToDevice));
+    FP_HIP_CHECK(hipMemcpy(d_ent, eent.data(), sizeof(int) * eent.size(), hipMemcpyHostToDevice));
+    FP_HIP_CHECK(hipMemcpy(d_w, ew.data(), sizeof(float) * ew.size(), hipMemcpyHostToDevice));
+    if (!ftgt.empty())
+      FP_HIP_CHECK(hipMemcpy(d_tgt, ftgt.data(), sizeof(long) * ftgt.size(), hipMemcpyHostToDevice));
+  };
+
+  // Markers repeated along z (ibm_init, fenep.f90:1826-1895, places every
+  // layer at the same x, y with Zc = (k - 0.25) dZ): checked on the tables
+  // themselves, every marker's interpolation origin and weights must be those
+  // of the layer-0 marker moved up by its layer, and every spreading row of
+  // plane k that of plane 0 with each marker moved up k layers (weights equal
+  // to within 2^-20, the rounding of the FP64 weights to FP32).
+  const long nmk = c.nmk, plane = (long)g.nx * g.ny;
+  const long P = nmk / g.nz;
+  bool zinv = g.nz >= 3 && P > 0 && P * g.nz == nmk;
+  auto close = [](float a, float b) { return a == b || fabsf(a - b) <= 9.5367431640625e-07f * fmaxf(fabsf(a), fabsf(b)); };
+  std::vector<float> wz;
+  std::vector<int2> oz;
+  if (zinv) {
+    std::vector<float> wih((size_t)nmk * 81);
+    std::vector<int4> orgh((size_t)nmk * 3);
+    FP_HIP_CHECK(hipMemcpy(wih.data(), X.wi, sizeof(float) * wih.size(), hipMemcpyDeviceToHost));
+    FP_HIP_CHECK(hipMemcpy(orgh.data(), X.org, sizeof(int4) * orgh.size(), hipMemcpyDeviceToHost));
+    for (int cc = 0; cc < 3 && zinv; cc++)
+      for (long id = 0; id < nmk && zinv; id++) {
+        const long L = id / P, pm = id - L * P;
+        const int4 a = orgh[cc * nmk + id], b = orgh[cc * nmk + pm];
+        if (a.x != b.x || a.y != b.y || (long)a.z != (long)b.z + L) zinv = false;
+        for (int s2 = 0; s2 < 27 && zinv; s2++)
+          if (!close(wih[(size_t)(cc * 27 + s2) * nmk + id], wih[(size_t)(cc * 27 + s2) * nmk + pm])) zinv = false;
+      }
+    if (zinv) {
+      wz.resize((size_t)81 * P);
+      oz.resize((size_t)3 * P);
+      for (int cs = 0; cs < 81; cs++)
+        for (long pm = 0; pm < P; pm++) wz[(size_t)cs * P + pm] = wih[(size_t)cs * nmk + pm];
+      for (int cc = 0; cc < 3; cc++)
+        for (long pm = 0; pm < P; pm++) oz[cc * P + pm] = make_int2(orgh[cc * nmk + pm].x, orgh[cc * nmk + pm].y);
+    }
+  }
+  // a row of plane k in layer-0 form: entries 4 (c nmk + p) + d + 1 for the
+  // marker p of layer k + d, d in {-1, 0, 1}, sorted
+  using Row = std::vector<std::pair<int, float>>;
+  auto plane_form = [&](long k, const Row& src, Row& out) -> bool {
+    out.clear();
+    for (auto& pr : src) {
+      const long cc = pr.first / nmk, id = pr.first - cc * nmk;
+      const long L = id / P, pm = id - L * P;
+      const long d = ((L - k) % g.nz + g.nz) % g.nz;
+      const int dl = d == 0 ? 0 : (d == 1 ? 1 : (d == g.nz - 1 ? -1 : 2));
+      if (dl == 2) return false;
+      out.push_back({(int)(4 * (cc * nmk + pm) + dl + 1), pr.second});
+    }
+    std::sort(out.begin(), out.end());
+    return true;
+  };
+  std::vector<Row> prow;
+  std::vector<long> ptgt;
+  if (zinv) {
+    std::unordered_map<long, int> key0;   // c * plane + in-plane index -> plane-0 row
+    Row tmp;
+    for (auto& kv : rows) {
+      const long cc = kv.first / g.n, rem = kv.first - cc * g.n;
+      if (rem / plane != 0) continue;
+      if (!plane_form(0, kv.second, tmp)) { zinv = false; break; }
+      key0[cc * plane + rem] = (int)prow.size();
+      prow.push_back(tmp);
+      ptgt.push_back(kv.first);
+    }
+    std::vector<long> count(g.nz, 0);
+    for (auto& kv : rows) {
+      if (!zinv) break;
+      const long cc = kv.first / g.n, rem = kv.first - cc * g.n, k = rem / plane;
+      const auto it = key0.find(cc * plane + (rem - k * plane));
+      if (it == key0.end() || !plane_form(k, kv.second, tmp)) { zinv = false; break; }
+      const Row& ref = prow[it->second];
+      if (ref.size() != tmp.size()) { zinv = false; break; }
+      for (size_t e = 0; e < tmp.size() && zinv; e++)
+        if (tmp[e].first != ref[e].first || !close(tmp[e].second, ref[e].second)) zinv = false;
+      count[k]++;
+    }
+    for (long k = 0; k < g.nz && zinv; k++)
+      if (count[k] != (long)prow.size()) zinv = false;
+  }
+  if (zinv) {
+    std::vector<int> frow(1, 0), fent;
+    std::vector<float> fw;
+    for (auto& r : prow) {
+      for (auto& pr : r) { fent.push_back(pr.first); fw.push_back(pr.second); }
+      frow.push_back((int)fent.size());
+    }
+    // padding: marker 0 of component 0 at offset 0, weight 0
+    upload_ell(frow, fent, fw, ptgt, 1, X.zs_off, X.zs_len, X.zs_tgt, X.zs_ent, X.zs_w);
+    dalloc(&X.zi_w, wz.size()); dalloc(&X.zi_org, oz.size());
+    FP_HIP_CHECK(hipMemcpy(X.zi_w, wz.data(), sizeof(float) * wz.size(), hipMemcpyHostToDevice));
+    FP_HIP_CHECK(hipMemcpy(X.zi_org, oz.data(), sizeof(int2) * oz.size(), hipMemcpyHostToDevice));
+    X.zinv = 1;
+    X.zP = P;
+    X.zs_rows = (int)prow.size();
+    X.zs_rows64 = (X.zs_rows + 63) / 64 * 64;
+    return;
+  }
   std::vector<int> frow(1, 0), fent;
   std::vector<long> ftgt;
   std::vector<float> fw;
@@ -1836,38 +2067,8 @@ void mixed_set_markers(Ctx& c) {
     for (auto& pr : kv.second) { fent.push_back(pr.first); fw.push_back(pr.second); }
     frow.push_back((int)fent.size());
   }
-  // CSR -> sliced ELL (64-row slices, slot-major, zero-weight padding)
-  const int nrows = (int)ftgt.size();
-  const int nsl = (nrows + 63) / 64;
-  std::vector<int> soff(std::max(1, nsl)), slen(std::max(1, nsl));
-  long total = 0;
-  for (int sl = 0; sl < nsl; sl++) {
-    int mx = 0;
-    for (int r = sl * 64; r < std::min(nrows, sl * 64 + 64); r++) mx = std::max(mx, frow[r + 1] - frow[r]);
-    soff[sl] = (int)total;
-    slen[sl] = mx;
-    total += 64L * mx;
-  }
-  std::vector<int> eent(std::max<long>(1, total), 0);
-  std::vector<float> ew(std::max<long>(1, total), 0.0f);
-  for (int r = 0; r < nrows; r++) {
-    const int sl = r >> 6, lane = r & 63;
-    for (int e = frow[r]; e < frow[r + 1]; e++) {
-      const long pos = soff[sl] + 64L * (e - frow[r]) + lane;
-      eent[pos] = fent[e];
-      ew[pos] = fw[e];
-    }
-  }
-  dfree(X.fr_row); dfree(X.fr_len); dfree(X.fr_tgt); dfree(X.fr_ent); dfree(X.fr_w);
-  X.fr_rows = nrows;
-  dalloc(&X.fr_row, soff.size()); dalloc(&X.fr_len, slen.size()); dalloc(&X.fr_tgt, std::max<size_t>(1, ftgt.size()));
-  dalloc(&X.fr_ent, eent.size()); dalloc(&X.fr_w, ew.size());
-  FP_HIP_CHECK(hipMemcpy(X.fr_row, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice));
-  FP_HIP_CHECK(hipMemcpy(X.fr_len, slen.data(), sizeof(int) * slen.size(), hipMemcpyHostToDevice));
-  FP_HIP_CHECK(hipMemcpy(X.fr_ent, eent.data(), sizeof(int) * eent.size(), hipMemcpyHostToDevice));
-  FP_HIP_CHECK(hipMemcpy(X.fr_w, ew.data(), sizeof(float) * ew.size(), hipMemcpyHostToDevice));
-  if (!ftgt.empty())
-    FP_HIP_CHECK(hipMemcpy(X.fr_tgt, ftgt.data(), sizeof(long) * ftgt.size(), hipMemcpyHostToDevice));
+  upload_ell(frow, fent, fw, ftgt, 0, X.fr_row, X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w);
+  X.fr_rows = (int)ftgt.size();
 }
 
 const double* mixed_marker_forces(Ctx& c) {
@@ -2066,11 +2267,20 @@ void mixed_step(Ctx& c) {
   } else {
     {
       MTimer t(c, "ibm");
-      hipLaunchKernelGGL(g.walls ? m_ibm_interp<true> : m_ibm_interp<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s, g,
-                         c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
-      if (X.fr_rows)
-        hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
-                           X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
+      if (X.zinv) {
+        hipLaunchKernelGGL(g.walls ? m_ibm_interp_z<true> : m_ibm_interp_z<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0,
+                           s, g, c.nmk, X.zP, X.zi_w, X.zi_org, X.uc, X.Q, X.mf);
+        if (X.zs_rows)
+          hipLaunchKernelGGL(m_ibm_spread_z, dim3(nblk((long)g.nz * X.zs_rows64, 256)), dim3(256), 0, s, X.zs_rows,
+                             X.zs_rows64, g.nz, X.zP, (long)g.nx * g.ny, X.zs_off, X.zs_len, X.zs_tgt, X.zs_ent, X.zs_w,
+                             X.mf, (float)g.dt, X.R);
+      } else {
+        hipLaunchKernelGGL(g.walls ? m_ibm_interp<true> : m_ibm_interp<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s,
+                           g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
+        if (X.fr_rows)
+          hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
+                             X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
+      }
     }
     if (polymer) select_ahead(c);
     MTimer t(c, "helmholtz");

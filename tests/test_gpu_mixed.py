@@ -66,6 +66,24 @@ def test_mixed_channel_cylinder_within_tolerance_and_drag():
     assert dg["div_max"] < 1e-3
 
 
+def test_mixed_channel_cylinder_markers_not_repeated_along_z():
+    """One marker layer moved by a tenth of a cell: the layers no longer
+    repeat along z, so the mixed step interpolates and spreads through the
+    general per-marker tables instead of the one-plane tables the other IBM
+    cases use."""
+    case = mixed(configs.channel_cylinder(nx=64, ny=32, nz=16))
+    dx = case.params.lx / case.params.nx
+    case.markers.xc[5] += 0.1 * dx
+    print("mixed channel 64x32x16, layer 5 of the markers moved, 40 steps", flush=True)
+    o, g = po.load_case(case), api.load_case(case)
+    o.step(40)
+    g.step(40)
+    e = errors(o, g)
+    print(e, flush=True)
+    assert e["u"] < TOL and e["C"] < TOL and e["P"] < TOL
+    assert g.diagnostics()["drag"] == pytest.approx(o.diagnostics()["drag"], rel=TOL)
+
+
 def test_mixed_throughput_layout_small():
     case = mixed(configs.throughput(nx=64, ny=64, nz=16, ncyl=2))
     o, g = po.load_case(case), api.load_case(case)
