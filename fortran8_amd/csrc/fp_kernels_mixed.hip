@@ -58,6 +58,7 @@ struct Mixed {
   unsigned* amb_list = nullptr;   // per-block regions of m * 3 + axis entries
   unsigned* amb_cnt = nullptr;    // entries per select block
   unsigned* amb_off = nullptr;    // exclusive offsets of amb_cnt
+  float* amb_data = nullptr;      // staged cell values, 18 * SEL_STAGE per select block
   unsigned* chk = nullptr;        // selection check: differing decisions
   // IBM
   float* wi = nullptr;            // interpolation weights, nmk * 3 * 27
@@ -305,12 +306,18 @@ __device__ __forceinline__ bool row_cell(const Grid& g, long& m, int& i, int& j,
 
 // Uncertified (cell, axis) pairs go to a list in per-block regions of 3 * 256
 // entries (block b owns entries [768 b, 768 b + cnt[b])): the slots are taken
-// with LDS atomics, so the kernel issues no device-scope atomic.
+// with LDS atomics, so the kernel issues no device-scope atomic.  The first
+// SEL_STAGE entries of a block also keep the 18 values the selection read
+// (the three cells' six components, plane-major in the block's staging area),
+// so that their re-decision reads them contiguously instead of gathering
+// them again; later entries gather.
 #define SEL_BLOCK 256
 #define SEL_REGION (3 * SEL_BLOCK)
+#define SEL_STAGE 64
 
 __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
-                                                      unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list) {
+                                                      unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list,
+                                                      float* __restrict__ amb_data) {
   __shared__ unsigned s_cnt;
   if (threadIdx.x == 0) s_cnt = 0u;
   __syncthreads();
@@ -318,13 +325,14 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
   int i, j, k;
   const bool valid = row_cell<false, 0>(g, m, i, j, k);   // the wall rows too (see load_line)
   unsigned* region = amb_list + (size_t)blockIdx.x * SEL_REGION;
+  float* stage = amb_data + (size_t)blockIdx.x * 18 * SEL_STAGE;
   uint8_t s = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     bool unsure = false;
     int idx = 0;
+    float v[3][6];
     if (valid) {
-      float v[3][6];
       load_line<1, false>(C, g, m, i, j, k, d, v);
       const float ein = SEL_KCE * fmaxf(fmaxf(entry_bound(v[0]), entry_bound(v[1])), entry_bound(v[2]));
       float vals[4], errs[4];
@@ -358,7 +366,16 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
       unsigned base = 0;
       if (lane == leader) base = atomicAdd(&s_cnt, (unsigned)__popcll(ball));
       base = __shfl(base, leader);
-      if (unsure) region[base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull))] = (unsigned)(m * 3 + d);
+      if (unsure) {
+        const unsigned slot = base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
+        region[slot] = (unsigned)(m * 3 + d);
+        if (slot < SEL_STAGE) {
+#pragma unroll
+          for (int o = 0; o < 3; o++)
+#pragma unroll
+            for (int q = 0; q < 6; q++) stage[(o * 6 + q) * SEL_STAGE + slot] = v[o][q];
+        }
+      }
     }
     s |= (uint8_t)(idx << (2 * d));
   }
@@ -427,6 +444,112 @@ __device__ __forceinline__ int select_ref(const Grid& g, const float* __restrict
   return select_d(cm, c, cp);
 }
 
+// The reference's selection of one (cell, axis), evaluating only the
+// candidates that can decide it.  The FP32 intervals of the six candidates
+// are recomputed as m_select makes them.  A candidate whose interval lies
+// above its partner's (the other face of its level) is not the minimum of its
+// level, and a level whose interval lies below another option's (another
+// level, or the zero slope) is not the maximum; the reference values keep
+// these strict orders, so the first maximum over the evaluated options is
+// select_d's answer.  A candidate near the sentinel boundary bounds nothing:
+// its level is evaluated in full.  Each lane walks its own needed candidates
+// (2.4 of 6 on average in the throughput case), one reference Vieta per trip.
+// The three cells of (cell m, axis d), wall ghosts by the C_kabe rule (the
+// values m_select reads through load_line).
+template <bool WL>
+__device__ __forceinline__ void sel_cells(const Grid& g, const float* __restrict__ C, long m, int d, float (&fm)[6],
+                                          float (&fc)[6], float (&fp)[6]) {
+  int i, j, k;
+  cell_ijk(g, m, i, j, k);
+  const int di = d == 0, dj = d == 1, dk = d == 2;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const float* Cq = C + q * g.n;
+    fm[q] = ldbw<WL>(Cq, g, i - di, j - dj, k - dk, BC_C);
+    fc[q] = Cq[m];
+    fp[q] = ldbw<WL>(Cq, g, i + di, j + dj, k + dk, BC_C);
+  }
+}
+
+__device__ __forceinline__ int select_pruned(const float (&fm)[6], const float (&fc)[6], const float (&fp)[6]) {
+  const float ein = SEL_KCE * fmaxf(fmaxf(entry_bound(fm), entry_bound(fc)), entry_bound(fp));
+  // one level per trip (a rolled loop keeps the register count, and so the
+  // occupancy that hides the FP64 latencies, down)
+  float lam[6], err[6];
+  bool amb[3];
+#pragma unroll 1
+  for (int l = 0; l < 3; l++) {
+    float ap[6], an[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, fm[q], fc[q], fp[q]); an[q] = cand_nf(l, fm[q], fc[q], fp[q]); }
+    const Lmin P = lmin32(ap, ein), N = lmin32(an, ein);
+#pragma unroll
+    for (int u = 0; u < 3; u++)
+      if (u == l) {
+        lam[2 * u] = P.lam; err[2 * u] = P.err;
+        lam[2 * u + 1] = N.lam; err[2 * u + 1] = N.err;
+        amb[u] = P.amb || N.amb;
+      }
+  }
+  float lo[4], hi[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    const float v = fminf(lam[2 * l], lam[2 * l + 1]), e = fmaxf(err[2 * l], err[2 * l + 1]);
+    lo[l] = amb[l] ? -INFINITY : v - e;
+    hi[l] = amb[l] ? INFINITY : v + e;
+  }
+  lo[3] = 0.0f;
+  unsigned need = 0u;
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    float other = -INFINITY;
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+      if (o != l) other = fmaxf(other, lo[o]);
+    if (!(other > hi[l])) {
+      const bool pP = amb[l] || !(lam[2 * l] - err[2 * l] > lam[2 * l + 1] + err[2 * l + 1]);
+      const bool pN = amb[l] || !(lam[2 * l + 1] - err[2 * l + 1] > lam[2 * l] + err[2 * l]);
+      need |= (pP ? 1u : 0u) << (2 * l);
+      need |= (pN ? 1u : 0u) << (2 * l + 1);
+    }
+  }
+  // Candidate t = 2 l + (right face) as (wa ua) sa + z + (wc uc) sc, the
+  // terms of cand_p / cand_n in their order ((c 3)/2 as (3 c) 0.5: the same
+  // rounding).  Levels are finished in increasing order, at their last
+  // needed candidate, which keeps select_d's first maximum.
+  int idx = -1;
+  double best = 0.0, vleft = 0.0;
+  unsigned rest = need;
+  while (rest) {
+    const int t = __builtin_ctz(rest);
+    rest &= rest - 1u;
+    const bool a_c = t < 2, c_c = t == 2 || t == 3, three = t >= 4;
+    const double wa = t == 0 ? 3.0 : (t == 3 || t == 5 ? -1.0 : 1.0);
+    const double wc = t == 3 ? 3.0 : (t == 0 || t == 4 ? -1.0 : 1.0);
+    const double sh = t >= 4 ? 0.25 : 0.5;
+    double a[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+#pragma clang fp contract(off)
+      const double cq = (double)fc[q];
+      const double A = (wa * (a_c ? cq : (double)fm[q])) * sh;
+      const double Cc = (wc * (c_c ? cq : (double)fp[q])) * sh;
+      a[q] = (A + (three ? cq : -0.0)) + Cc;
+    }
+    const double v = vieta_d(a);
+    const bool right = t & 1;
+    const bool pair = (need >> (t ^ 1)) & 1u;   // the other face of the level is needed too
+    if (!right && pair) {
+      vleft = v;
+      continue;
+    }
+    const double lv = right && pair ? minval2_d(vleft, v) : v;
+    if (!isnan(lv) && (idx < 0 || lv > best)) { idx = t >> 1; best = lv; }
+  }
+  if (idx < 0 || 0.0 > best) idx = 3;
+  return idx;
+}
+
 // FP64 re-decision of the uncertified selections: grid-stride over the
 // total, each entry found in its block's region by a binary search of the
 // exclusive offsets
@@ -434,7 +557,8 @@ template <bool WL>
 __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                     const unsigned* __restrict__ total,
                                                     const unsigned* __restrict__ off, long nb,
-                                                    const unsigned* __restrict__ amb_list) {
+                                                    const unsigned* __restrict__ amb_list,
+                                                    const float* __restrict__ amb_data) {
   const long cnt = (long)*total;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < cnt; t += (long)gridDim.x * blockDim.x) {
     // largest b with off[b] <= t (that block holds entry t)
@@ -444,10 +568,23 @@ __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restri
       if ((long)off[mid] <= t) lo = mid;
       else hi = mid - 1;
     }
-    const unsigned e = amb_list[(size_t)lo * SEL_REGION + (size_t)(t - (long)off[lo])];
+    const unsigned slot = (unsigned)(t - (long)off[lo]);
+    const unsigned e = amb_list[(size_t)lo * SEL_REGION + slot];
     const long m = (long)(e / 3u);
     const int d = (int)(e % 3u);
-    const int idx = select_ref<WL>(g, C, m, d);
+    float fm[6], fc[6], fp[6];
+    if (slot < SEL_STAGE) {
+      const float* src = amb_data + (size_t)lo * 18 * SEL_STAGE + slot;
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        fm[q] = src[q * SEL_STAGE];
+        fc[q] = src[(6 + q) * SEL_STAGE];
+        fp[q] = src[(12 + q) * SEL_STAGE];
+      }
+    } else {
+      sel_cells<WL>(g, C, m, d, fm, fc, fp);
+    }
+    const int idx = select_pruned(fm, fc, fp);
     // the byte holds three axes: update only this axis
     unsigned* word = (unsigned*)((size_t)(sel + m) & ~(size_t)3);
     const int sh = 8 * (int)((size_t)(sel + m) & 3) + 2 * d;
@@ -1783,6 +1920,7 @@ void mixed_create(Ctx& c) {
     const long nb = nblk(n, SEL_BLOCK);
     dalloc(&X->amb, 1); dalloc(&X->amb_list, nb * SEL_REGION); dalloc(&X->chk, 1);
     dalloc(&X->amb_cnt, nb); dalloc(&X->amb_off, nb);
+    dalloc(&X->amb_data, nb * 18 * SEL_STAGE);
   }
   // FP32 twiddles and wavenumbers from the FP64 host tables
   auto tw32 = [&](const double2* d, int nn, float2** out) {
@@ -1822,7 +1960,7 @@ void mixed_destroy(Ctx& c) {
   if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
   if (X->dc_host) (void)hipHostFree(X->dc_host);
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
-  dfree(X->amb_cnt); dfree(X->amb_off);
+  dfree(X->amb_cnt); dfree(X->amb_off); dfree(X->amb_data);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
   dfree(X->fr_row); dfree(X->fr_len); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
@@ -2192,7 +2330,8 @@ static void select_fenep(Ctx& c, int tm, hipStream_t s) {
   const long nb = nblk(g.n, SEL_BLOCK);
   {
     MTimer t(c, tm ? "fenep_select" : nullptr, s);
-    hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list);
+    hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list,
+                       X.amb_data);
   }
   {
     MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr, s);
@@ -2200,7 +2339,7 @@ static void select_fenep(Ctx& c, int tm, hipStream_t s) {
     // one wavefront per select block: 16 per SIMD at 256x128x128, enough to
     // hide the search and FP64 latencies; n / 4 entries per pass
     hipLaunchKernelGGL(g.walls ? m_select_fix<true> : m_select_fix<false>, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel,
-                       X.amb, X.amb_off, nb, X.amb_list);
+                       X.amb, X.amb_off, nb, X.amb_list, X.amb_data);
   }
 }
 
