@@ -42,15 +42,17 @@ KERNEL_WORDS = {
 }
 
 
-def cells_per_launch(p, prec, phase):
+def cells_per_launch(p, prec, phase, phases):
     """Cells one launch of the phase's kernel processes.  Each timed phase is
-    one kernel.  With y-walls the mixed bulk stencil kernels leave the wall
-    rows (2 for the predictor, 4 for CStar/Newton) to their own instances,
-    timed as *_walls; the mixed selection and every FP64 kernel cover all
+    one kernel.  With y-walls the mixed bulk CStar/Newton kernel leaves the
+    4 wall rows to its own instance (timed as fenep_cstar_newton_walls); so
+    does the per-cell mixed predictor (2 rows, predictor_walls), which runs
+    where the z-streamed predictor's tiling does not fit the grid.  The mixed
+    selection, the z-streamed predictor and every FP64 kernel cover all
     cells."""
     cells = p.nx * p.ny * p.nz
     if prec == 1 and p.walls_y:
-        if phase == "predictor":
+        if phase == "predictor" and "predictor_walls" in phases:
             return p.nx * (p.ny - 2) * p.nz
         if phase == "fenep_cstar_newton":
             return p.nx * (p.ny - 4) * p.nz
@@ -150,15 +152,17 @@ def main():
     # stream, so its duration there includes the sharing).
     dom = "predictor"
     words, extra = KERNEL_WORDS[dom]
-    units = cells_per_launch(case.params, prec, dom)
+    units = cells_per_launch(case.params, prec, dom, kt)
     dom_bytes = (words * width + extra) * units
     dom_ms = kt_timed.get(dom, 0.0)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     traffic = read_traffic(a.precision).get(dom)
     step_bytes = STEP_WORDS * 8 * cells if prec == 0 else mixed_step_bytes(d.get("n_dc", 0)) * cells
+    walls = "true" if case.params.walls_y else "false"
+    mixed_pred = "fp::m_predictor<false>" if "predictor_walls" in kt else "fp::m_predictor_z<%s>" % walls
     kernel_names = {"fenep_select": "fp::m_select" if prec else "fp::k_select",
-                    "fenep_cstar_newton": "fp::m_cstar_newton<false>" if prec else "fp::k_cstar_newton",
-                    "predictor": "fp::m_predictor<false>" if prec else "fp::k_predictor"}
+                    "fenep_cstar_newton": "fp::m_cstar_newton<false>" if prec else "fp::k_cstar_newton<%s>" % walls,
+                    "predictor": mixed_pred if prec else "fp::k_predictor<%s>" % walls}
     out = {
         "metric": "Mcell-steps/s 256x128x128 FENE-P+IBM",
         "value": round(value, 3),

@@ -855,26 +855,69 @@ __device__ __forceinline__ float c0_atf(const Grid& g, const float* __restrict__
   return fC[m] * C[q * g.n + m] - E;
 }
 
-// predictor body; FAST reads neighbours through wrapped offsets (no wall
-// row involved), otherwise through the boundary-rule accessor
-template <bool FAST>
-__device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int j, int k, int method, int first, int polymer,
-                                               const float* __restrict__ U, const float* __restrict__ V,
-                                               const float* __restrict__ W, const float* __restrict__ P,
-                                               const float* __restrict__ C, const float* __restrict__ fC,
-                                               float* __restrict__ A0, float* __restrict__ T0, float* __restrict__ Rp,
-                                               const float* __restrict__ F, float bf0, float bf1, float bf2,
+// Stencil access of the predictor, neighbour (a, b, c) in {-1, 0, 1}^3 of the
+// cell: u, v, w, p and c0(q) = f C_q - delta_q (the polymer stress deviation
+// of component q).  PredFast reads through wrapped offsets (no wall row in
+// the stencil), PredWalls through the boundary-rule accessor.
+__device__ __forceinline__ float kron6(int q) { return (q == 0 || q == 3 || q == 5) ? 1.0f : 0.0f; }
+
+struct PredFast {
+  const Grid& g;
+  long m;
+  Nbr o;
+  const float *U, *V, *W, *P, *C, *fC;
+  __device__ __forceinline__ long at(int a, int b, int c) const { return m + o.x[a + 1] + o.y[b + 1] + o.z[c + 1]; }
+  __device__ __forceinline__ float u(int a, int b, int c) const { return U[at(a, b, c)]; }
+  __device__ __forceinline__ float v(int a, int b, int c) const { return V[at(a, b, c)]; }
+  __device__ __forceinline__ float w(int a, int b, int c) const { return W[at(a, b, c)]; }
+  __device__ __forceinline__ float p(int a, int b, int c) const { return P[at(a, b, c)]; }
+  __device__ __forceinline__ float c0(int q, int a, int b, int c) const {
+    const long mm = at(a, b, c);
+    return fC[mm] * C[q * g.n + mm] - kron6(q);
+  }
+};
+
+struct PredWalls {
+  const Grid& g;
+  int i, j, k;
+  const float *U, *V, *W, *P, *C, *fC;
+  __device__ __forceinline__ float u(int a, int b, int c) const { return ldbw<true>(U, g, i + a, j + b, k + c, BC_U); }
+  __device__ __forceinline__ float v(int a, int b, int c) const { return ldbw<true>(V, g, i + a, j + b, k + c, BC_V); }
+  __device__ __forceinline__ float w(int a, int b, int c) const { return ldbw<true>(W, g, i + a, j + b, k + c, BC_W); }
+  __device__ __forceinline__ float p(int a, int b, int c) const { return ldbw<true>(P, g, i + a, j + b, k + c, BC_P); }
+  __device__ __forceinline__ float c0(int q, int a, int b, int c) const { return c0_atf(g, C, fC, q, i + a, j + b, k + c); }
+};
+
+// the cell's own inputs besides the stencil: the AB2/CN histories and the
+// forcing (body force added)
+struct PredCell {
+  float a0[3], t0[3], fv[3];
+};
+__device__ __forceinline__ void pred_cell_load(const Grid& g, long m, int first, const float* __restrict__ A0,
+                                               const float* __restrict__ T0, const float* __restrict__ F, float bf0,
+                                               float bf1, float bf2, PredCell& in) {
+  const float bfs[3] = {bf0, bf1, bf2};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long oo = c * g.n + m;
+    in.a0[c] = first ? 0.0f : A0[oo];
+    in.t0[c] = first ? 0.0f : T0[oo];
+    in.fv[c] = (F ? F[oo] : 0.0f) + bfs[c];
+  }
+}
+
+// predictor body over a stencil accessor S and the cell's own inputs
+template <class S_t>
+__device__ __forceinline__ void predictor_body(const Grid& g, const S_t& S, const PredCell& in, long m, int j,
+                                               int method, int first, int polymer, float* __restrict__ A0,
+                                               float* __restrict__ T0, float* __restrict__ Rp,
                                                float* __restrict__ Q) {
   const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz, dt = (float)g.dt;
   const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
-  Nbr o;
-  if (FAST) o = nbr(g, i, j, k);
-  // (a, b, c) in {-1, 0, 1}
-#define NB(A, a, b, c, kind) (FAST ? A[m + o.x[(a) + 1] + o.y[(b) + 1] + o.z[(c) + 1]] : ldbw<true>(A, g, i + (a), j + (b), k + (c), kind))
-#define Ug(a, b, c) NB(U, a, b, c, BC_U)
-#define Vg(a, b, c) NB(V, a, b, c, BC_V)
-#define Wg(a, b, c) NB(W, a, b, c, BC_W)
-#define Pg(a, b, c) NB(P, a, b, c, BC_P)
+#define Ug(a, b, c) S.u(a, b, c)
+#define Vg(a, b, c) S.v(a, b, c)
+#define Wg(a, b, c) S.w(a, b, c)
+#define Pg(a, b, c) S.p(a, b, c)
   float a[3], bv[3], t[3];
   const float uc = Ug(0, 0, 0), vc = Vg(0, 0, 0), wc = Wg(0, 0, 0);
   {
@@ -903,7 +946,7 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
                   (Wg(0, 0, -1) - 2.0f * wc + Wg(0, 0, 1)) * iz2);
   }
   if (polymer) {
-#define C0g(q, aa, bb, cc) (FAST ? (fC[m + o.x[(aa) + 1] + o.y[(bb) + 1] + o.z[(cc) + 1]] * C[(q) * g.n + m + o.x[(aa) + 1] + o.y[(bb) + 1] + o.z[(cc) + 1]] - (((q) == 0 || (q) == 3 || (q) == 5) ? 1.0f : 0.0f)) : c0_atf(g, C, fC, q, i + (aa), j + (bb), k + (cc)))
+#define C0g(q, aa, bb, cc) S.c0(q, aa, bb, cc)
     const float tx = (C0g(0, 1, 0, 0) - C0g(0, 0, 0, 0)) * idx_
                      + 0.25f * ((C0g(1, 0, 1, 0) + C0g(1, 1, 1, 0)) - (C0g(1, 0, -1, 0) + C0g(1, 1, -1, 0))) * idy
                      + 0.25f * ((C0g(2, 0, 0, 1) + C0g(2, 1, 0, 1)) - (C0g(2, 0, 0, -1) + C0g(2, 1, 0, -1))) * idz;
@@ -919,8 +962,7 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
   } else {
     t[0] = t[1] = t[2] = 0.0f;
   }
-  const float bfs[3] = {bf0, bf1, bf2};
-  const float p0 = P[m];
+  const float p0 = Pg(0, 0, 0);
   const float pn[3] = {Pg(1, 0, 0), Pg(0, 1, 0), Pg(0, 0, 1)};
   const float uu[3] = {uc, vc, wc};
   const float ih[3] = {idx_, idy, idz};
@@ -928,11 +970,10 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     long oo = c * g.n + m;
-    float a0 = first ? a[c] : A0[oo];
-    float t0 = first ? t[c] : T0[oo];
-    float fv = (F ? F[oo] : 0.0f) + bfs[c];
+    float a0 = first ? a[c] : in.a0[c];
+    float t0 = first ? t[c] : in.t0[c];
     const float qv = uu[c] - dt * (pn[c] - p0) * ih[c] + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] +
-                     0.5f * dt * (t[c] + t0) + dt * fv;
+                     0.5f * dt * (t[c] + t0) + dt * in.fv[c];
     Q[oo] = qv;
     A0[oo] = a[c];
     T0[oo] = t[c];
@@ -940,11 +981,10 @@ __device__ __forceinline__ void predictor_body(const Grid& g, long m, int i, int
       // ibm_predict right-hand side without the forcing (fenep.f90:2229):
       // Ua - dt B/2, with the wall face of V read as 0 (V_kabe); the spread
       // adds dt f at the cells the markers reach
-      const float ua = (!FAST && c == 1 && g.walls && j == g.ny - 1) ? 0.0f : qv;
+      const float ua = (c == 1 && g.walls && j == g.ny - 1) ? 0.0f : qv;
       Rp[oo] = ua - 0.5f * dt * bv[c];
     }
   }
-#undef NB
 #undef Ug
 #undef Vg
 #undef Wg
@@ -965,7 +1005,175 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
   long m;
   int i, j, k;
   if (!row_cell<GEN, 1>(g, m, i, j, k)) return;
-  predictor_body<!GEN>(g, m, i, j, k, method, first, polymer, U, V, W, P, C, fC, A0, T0, Rp, F, bf0, bf1, bf2, Q);
+  PredCell in;
+  pred_cell_load(g, m, first, A0, T0, F, bf0, bf1, bf2, in);
+  if (GEN) {
+    const PredWalls S{g, i, j, k, U, V, W, P, C, fC};
+    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Rp, Q);
+  } else {
+    const PredFast S{g, m, nbr(g, i, j, k), U, V, W, P, C, fC};
+    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Rp, Q);
+  }
+}
+
+// ---- predictor streamed along z through LDS -----------------------------
+// A block owns a PZ_TX x PZ_TY tile of cells in (x, y) and walks its column
+// of planes k0 .. k0 + kc - 1.  The LDS holds three planes (k-1, k, k+1) of
+// the tile with a one-cell halo for the ten stencil fields (u, v, w, p and
+// c0 of the six components); every neighbour the predictor reads lies there.
+// Plane k+2 is loaded into registers while plane k is computed, then takes
+// the place of plane k-1.  Each value is fetched from L2 about 1.7 times at
+// 64 x 4 tiles and 16 planes (halo rows and chunk ends), in whole rows,
+// where the per-cell stencil of m_predictor fetches it about 6 times.  The
+// values in LDS are those of the boundary-rule accessor (wall ghosts and the
+// wall face of V included), so one kernel covers the wall rows too.
+#define PZ_TX 64
+#define PZ_TY 4
+#define PZ_HX (PZ_TX + 2)
+#define PZ_PTS (PZ_HX * (PZ_TY + 2))
+#define PZ_NF 10
+#define PZ_THREADS (PZ_TX * PZ_TY)
+#define PZ_PER ((PZ_PTS + PZ_THREADS - 1) / PZ_THREADS)
+#define PZ_KC 32                  // planes per block (16 and 64 measured slower)
+
+struct PredLds {
+  const float *pm, *p0, *pp;   // planes k-1, k, k+1: PZ_NF fields of PZ_PTS
+  int c;                       // the cell's halo-tile index
+  __device__ __forceinline__ float at(int f, int a, int b, int cz) const {
+    const float* pl = cz < 0 ? pm : (cz > 0 ? pp : p0);
+    return pl[f * PZ_PTS + c + b * PZ_HX + a];
+  }
+  __device__ __forceinline__ float u(int a, int b, int c_) const { return at(0, a, b, c_); }
+  __device__ __forceinline__ float v(int a, int b, int c_) const { return at(1, a, b, c_); }
+  __device__ __forceinline__ float w(int a, int b, int c_) const { return at(2, a, b, c_); }
+  __device__ __forceinline__ float p(int a, int b, int c_) const { return at(3, a, b, c_); }
+  __device__ __forceinline__ float c0(int q, int a, int b, int c_) const { return at(4 + q, a, b, c_); }
+};
+
+// raw loads of the halo-tile points this thread fills (one plane)
+struct PzRegs {
+  float u[PZ_PER], v[PZ_PER], w[PZ_PER], p[PZ_PER], f[PZ_PER], ca[PZ_PER][6], cb[PZ_PER][6];
+};
+
+template <bool WL>
+__device__ __forceinline__ void pz_fetch(const Grid& g, int i0, int j0, int kk, const float* __restrict__ U,
+                                         const float* __restrict__ V, const float* __restrict__ W,
+                                         const float* __restrict__ P, const float* __restrict__ C,
+                                         const float* __restrict__ fC, PzRegs& r) {
+  const long plane = (long)g.nx * g.ny;
+  const int kz = wrapi(kk, g.nz);
+#pragma unroll
+  for (int e = 0; e < PZ_PER; e++) {
+    const int pt = threadIdx.x + PZ_THREADS * e;
+    if (pt < PZ_PTS) {
+      const int hx = pt % PZ_HX, hy = pt / PZ_HX;
+      const int ii = wrapi(i0 + hx - 1, g.nx);
+      int jj = j0 + hy - 1;
+      bool ghost = false;
+      int j2 = jj;
+      if (WL) {
+        const bool lo = jj < 0, hi = jj >= g.ny;
+        ghost = lo || hi;
+        jj = lo ? 0 : (hi ? g.ny - 1 : jj);
+        j2 = lo ? 1 : (hi ? g.ny - 2 : jj);
+      } else {
+        jj = wrapi(jj, g.ny);
+      }
+      const long b = ii + (long)g.nx * jj + plane * kz;
+      r.u[e] = U[b];
+      r.v[e] = V[b];
+      r.w[e] = W[b];
+      r.p[e] = P[b];
+      r.f[e] = fC[b];
+#pragma unroll
+      for (int q = 0; q < 6; q++) r.ca[e][q] = C[q * g.n + b];
+      if (WL && ghost) {
+        const long b2 = ii + (long)g.nx * j2 + plane * kz;
+#pragma unroll
+        for (int q = 0; q < 6; q++) r.cb[e][q] = C[q * g.n + b2];
+      }
+    }
+  }
+}
+
+// the fetched plane into an LDS slot, with the rules of the boundary-rule
+// accessor (ldbw, c0_atf) applied
+template <bool WL>
+__device__ __forceinline__ void pz_put(const Grid& g, int j0, const PzRegs& r, float* __restrict__ slot) {
+#pragma unroll
+  for (int e = 0; e < PZ_PER; e++) {
+    const int pt = threadIdx.x + PZ_THREADS * e;
+    if (pt < PZ_PTS) {
+      const int jj = j0 + pt / PZ_HX - 1;
+      float u = r.u[e], v = r.v[e], w = r.w[e], pv = r.p[e];
+      float c0[6];
+      if (WL && (jj < 0 || jj >= g.ny)) {
+        const bool lo = jj < 0;
+        u = lo ? -u : 2.0f * 0.0f - u;
+        w = lo ? -w : 2.0f * 0.0f - w;
+        v = 0.0f;
+        float cg[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) cg[q] = 2.0f * r.ca[e][q] - r.cb[e][q];
+        const float f = peterlin_f((float)(g.lp * g.lp), cg[0] + cg[3] + cg[5]);
+#pragma unroll
+        for (int q = 0; q < 6; q++) c0[q] = f * cg[q] - kron6(q);
+      } else {
+        if (WL && jj == g.ny - 1) v = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 6; q++) c0[q] = r.f[e] * r.ca[e][q] - kron6(q);
+      }
+      slot[0 * PZ_PTS + pt] = u;
+      slot[1 * PZ_PTS + pt] = v;
+      slot[2 * PZ_PTS + pt] = w;
+      slot[3 * PZ_PTS + pt] = pv;
+#pragma unroll
+      for (int q = 0; q < 6; q++) slot[(4 + q) * PZ_PTS + pt] = c0[q];
+    }
+  }
+}
+
+// grid: (nx / PZ_TX) (ny / PZ_TY) ceil(nz / kc) blocks of PZ_THREADS
+template <bool WL>
+__global__ __launch_bounds__(PZ_THREADS) void m_predictor_z(Grid g, int kc, int method, int first, int polymer,
+                                                            const float* __restrict__ U, const float* __restrict__ V,
+                                                            const float* __restrict__ W, const float* __restrict__ P,
+                                                            const float* __restrict__ C, const float* __restrict__ fC,
+                                                            float* __restrict__ A0, float* __restrict__ T0,
+                                                            float* __restrict__ Rp, const float* __restrict__ F,
+                                                            float bf0, float bf1, float bf2, float* __restrict__ Q) {
+  __shared__ float lds[3][PZ_NF * PZ_PTS];
+  const int ntx = g.nx / PZ_TX, nty = g.ny / PZ_TY;
+  const long b = xcd_block();
+  const int bx = (int)(b % ntx), by = (int)((b / ntx) % nty), bz = (int)(b / ((long)ntx * nty));
+  const int i0 = bx * PZ_TX, j0 = by * PZ_TY, k0 = bz * kc;
+  const int k1 = min(k0 + kc, g.nz);
+  const int tx = threadIdx.x % PZ_TX, ty = threadIdx.x / PZ_TX;
+  const int i = i0 + tx, j = j0 + ty;
+  PzRegs r;
+  // planes k0-1, k0, k0+1 into slots 0, 1, 2; plane kk lives in slot
+  // (kk - k0 + 1) mod 3
+#pragma unroll 1
+  for (int s = 0; s < 3; s++) {
+    pz_fetch<WL>(g, i0, j0, k0 - 1 + s, U, V, W, P, C, fC, r);
+    pz_put<WL>(g, j0, r, lds[s]);
+  }
+  if (k0 + 2 <= k1) pz_fetch<WL>(g, i0, j0, k0 + 2, U, V, W, P, C, fC, r);
+  int sm = 0, s0 = 1, sp = 2;
+#pragma unroll 1
+  for (int kk = k0; kk < k1; kk++) {
+    __syncthreads();
+    const long m = cidx(g, i, j, kk);
+    PredCell in;
+    pred_cell_load(g, m, first, A0, T0, F, bf0, bf1, bf2, in);
+    const PredLds S{lds[sm], lds[s0], lds[sp], (ty + 1) * PZ_HX + tx + 1};
+    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Rp, Q);
+    __syncthreads();
+    // plane kk+2 replaces plane kk-1; fetch plane kk+3
+    if (kk + 2 <= k1) pz_put<WL>(g, j0, r, lds[sm]);
+    if (kk + 3 <= k1) pz_fetch<WL>(g, i0, j0, kk + 3, U, V, W, P, C, fC, r);
+    const int t = sm; sm = s0; s0 = sp; sp = t;
+  }
 }
 
 // ================================================================== FFT ===
@@ -2383,13 +2591,21 @@ void mixed_step(Ctx& c) {
     X.cur = 1 - X.cur;
   }
   float* Rp = c.p.method == FP_METHOD_IBM ? X.R : nullptr;
-  {
+  if (g.nx % PZ_TX == 0 && g.ny % PZ_TY == 0) {
+    // streamed along z, wall rows included
+    MTimer t(c, "predictor");
+    const long nbk = (g.nz + PZ_KC - 1) / PZ_KC;
+    const long grid = (long)(g.nx / PZ_TX) * (g.ny / PZ_TY) * nbk;
+    hipLaunchKernelGGL(g.walls ? m_predictor_z<true> : m_predictor_z<false>, dim3(grid), dim3(PZ_THREADS), 0, s, g, PZ_KC,
+                       c.p.method, c.has_history ? 0 : 1, polymer, X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp,
+                       X.F, (float)c.p.body_force[0], (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
+  } else {
     MTimer t(c, "predictor");
     hipLaunchKernelGGL(m_predictor<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
                        X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   }
-  if (g.walls) {
+  if (g.walls && !(g.nx % PZ_TX == 0 && g.ny % PZ_TY == 0)) {
     MTimer t(c, "predictor_walls");
     hipLaunchKernelGGL(m_predictor<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
                        X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
