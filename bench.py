@@ -62,15 +62,15 @@ def cells_per_launch(p, prec, phase, phases):
 def mixed_step_bytes(n_dc):
     """Algorithmic HBM bytes per cell of the mixed IBM step with y-walls at
     the throughput size (4 B words, FP64 where stored so): select 25 (C read,
-    selection byte written), CStar/Newton 69, predictor 116, two Helmholtz
-    sweeps 60 (24 + 36), correction 40; Poisson: divergence fused into the
-    forward x transform 16 (u* read, half spectrum written), the kx-slab solve
-    8 (half spectrum read and written), inverse x transform with the FP64
-    accumulation of Phi 12; the FP64 defect check every 8 steps 3.5 on
-    average (b written, residual 20).  Each further defect-correction sweep
-    adds 36.  The sparse IBM passes and the FP64 re-decision of selections
-    are not counted."""
-    return 346.0 + 3.5 + 36.0 * max(0, n_dc - 1)
+    selection byte written), CStar/Newton 69, predictor 116, the two
+    Helmholtz sweeps in one pass 24 (q read, x written), correction 40;
+    Poisson: divergence fused into the forward x transform 16 (u* read, half
+    spectrum written), the kx-slab solve 8 (half spectrum read and written),
+    inverse x transform with the FP64 accumulation of Phi 12; the FP64 defect
+    check every 8 steps 3.5 on average (b written, residual 20).  Each
+    further defect-correction sweep adds 36.  The sparse IBM passes and the
+    FP64 re-decision of selections are not counted."""
+    return 310.0 + 3.5 + 36.0 * max(0, n_dc - 1)
 
 
 def run_gpu(case, steps, warmup):

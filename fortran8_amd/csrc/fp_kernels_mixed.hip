@@ -1841,6 +1841,160 @@ __global__ __launch_bounds__(256) void m_helm_sweep(Grid g, float eps, const flo
   }
 }
 
+// Two sweeps in one pass, streamed along z: x1 = q + eps L q on the tile and
+// a one-cell halo, one plane ahead, then x2 = q + eps L x1 on the tile.  The
+// LDS holds four planes of q with a two-cell halo (the plane being filled
+// among them) and three planes of x1 with a one-cell halo, the three
+// components each.  Values follow the boundary-rule accessor: q's wall
+// ghosts and the wall face of V on filling, and x1's ghost rows by the same
+// rules from x1 at the nearest row, which the ghost's thread computes itself.
+// HBM traffic: q read once and x2 written once (24 B a cell for both sweeps
+// instead of 60).
+#define HZ_TX 64
+#define HZ_TY 4
+#define HZ_THREADS (HZ_TX * HZ_TY)
+#define HZ_QX (HZ_TX + 4)
+#define HZ_QP (HZ_QX * (HZ_TY + 4))
+#define HZ_XX (HZ_TX + 2)
+#define HZ_XP (HZ_XX * (HZ_TY + 2))
+#define HZ_QPER ((HZ_QP + HZ_THREADS - 1) / HZ_THREADS)
+#define HZ_KC 32
+
+struct HzRegs {
+  float v[HZ_QPER][3];
+};
+
+template <bool WL>
+__device__ __forceinline__ void hz_fetch(const Grid& g, int i0, int j0, int kk, const float* __restrict__ q, HzRegs& r) {
+  const long plane = (long)g.nx * g.ny;
+  const int kz = wrapi(kk, g.nz);
+#pragma unroll
+  for (int e = 0; e < HZ_QPER; e++) {
+    const int pt = threadIdx.x + HZ_THREADS * e;
+    if (pt < HZ_QP) {
+      const int ii = wrapi(i0 + pt % HZ_QX - 2, g.nx);
+      int jj = j0 + pt / HZ_QX - 2;
+      jj = WL ? (jj < 0 ? 0 : (jj >= g.ny ? g.ny - 1 : jj)) : wrapi(jj, g.ny);
+      const long b = ii + (long)g.nx * jj + plane * kz;
+#pragma unroll
+      for (int c = 0; c < 3; c++) r.v[e][c] = q[c * g.n + b];
+    }
+  }
+}
+
+// q values as the boundary-rule accessor returns them (U_kabe, V_kabe, W_kabe)
+template <bool WL>
+__device__ __forceinline__ float hz_rule(const Grid& g, int c, int jj, float v) {
+  if (!WL) return v;
+  const bool lo = jj < 0, hi = jj >= g.ny;
+  if (c == 1) return (lo || hi || jj == g.ny - 1) ? 0.0f : v;
+  return lo ? -v : (hi ? 2.0f * 0.0f - v : v);
+}
+
+template <bool WL>
+__device__ __forceinline__ void hz_put(const Grid& g, int j0, const HzRegs& r, float* __restrict__ slot) {
+#pragma unroll
+  for (int e = 0; e < HZ_QPER; e++) {
+    const int pt = threadIdx.x + HZ_THREADS * e;
+    if (pt < HZ_QP) {
+      const int jj = j0 + pt / HZ_QX - 2;
+#pragma unroll
+      for (int c = 0; c < 3; c++) slot[c * HZ_QP + pt] = hz_rule<WL>(g, c, jj, r.v[e][c]);
+    }
+  }
+}
+
+// one sweep at the x1-halo point (hx, hy) of plane p from the q planes
+// p-1, p, p+1: q + eps L q, the wall face of V set to zero, ghost rows by the
+// rules from the nearest row
+template <bool WL>
+__device__ __forceinline__ void hz_x1(const Grid& g, int j0, float eps, float ix2, float iy2, float iz2,
+                                      const float* __restrict__ qm, const float* __restrict__ q0,
+                                      const float* __restrict__ qp, float* __restrict__ xs) {
+  for (int pt = threadIdx.x; pt < HZ_XP; pt += HZ_THREADS) {
+    const int hx = pt % HZ_XX, hy = pt / HZ_XX;
+    const int jj = j0 + hy - 1;
+    const bool lo = WL && jj < 0, hi = WL && jj >= g.ny;
+    // the row whose sweep value this point takes (itself, or the nearest
+    // interior row of a ghost)
+    const int hyc = lo ? hy + 1 : (hi ? hy - 1 : hy);
+    const int jc = j0 + hyc - 1;
+    const int qc = (hyc + 1) * HZ_QX + hx + 1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float* Qm = qm + c * HZ_QP;
+      const float* Q0 = q0 + c * HZ_QP;
+      const float* Qp = qp + c * HZ_QP;
+      const float x0 = Q0[qc];
+      const float lap = (Q0[qc - 1] - 2.0f * x0 + Q0[qc + 1]) * ix2 + (Q0[qc - HZ_QX] - 2.0f * x0 + Q0[qc + HZ_QX]) * iy2 +
+                        (Qm[qc] - 2.0f * x0 + Qp[qc]) * iz2;
+      float v = x0 + eps * lap;
+      if (c == 1 && WL && jc == g.ny - 1) v = 0.0f;   // wall face of V
+      xs[c * HZ_XP + pt] = (lo || hi) ? hz_rule<WL>(g, c, jj, v) : v;
+    }
+  }
+}
+
+template <bool WL>
+__global__ __launch_bounds__(HZ_THREADS) void m_helm_2sweeps_z(Grid g, float eps, const float* __restrict__ q,
+                                                              float* __restrict__ x) {
+  __shared__ float qs[4][3 * HZ_QP];
+  __shared__ float xs[3][3 * HZ_XP];
+  const int ntx = g.nx / HZ_TX, nty = g.ny / HZ_TY;
+  const long b = xcd_block();
+  const int bx = (int)(b % ntx), by = (int)((b / ntx) % nty), bz = (int)(b / ((long)ntx * nty));
+  const int i0 = bx * HZ_TX, j0 = by * HZ_TY, k0 = bz * HZ_KC;
+  const int k1 = min(k0 + HZ_KC, g.nz);
+  const int tx = threadIdx.x % HZ_TX, ty = threadIdx.x / HZ_TX;
+  const float ix2 = (float)(1.0 / (g.dx * g.dx)), iy2 = (float)(1.0 / (g.dy * g.dy)), iz2 = (float)(1.0 / (g.dz * g.dz));
+  // q plane p in slot (p - k0 + 2) mod 4, x1 plane p in slot (p - k0 + 1) mod 3
+  HzRegs r;
+#pragma unroll 1
+  for (int s = 0; s < 4; s++) {
+    hz_fetch<WL>(g, i0, j0, k0 - 2 + s, q, r);
+    hz_put<WL>(g, j0, r, qs[s]);
+  }
+  if (k0 + 2 <= k1 + 1) hz_fetch<WL>(g, i0, j0, k0 + 2, q, r);
+  __syncthreads();
+  hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[0], qs[1], qs[2], xs[0]);   // x1 at k0-1
+  hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[1], qs[2], qs[3], xs[1]);   // x1 at k0
+  __syncthreads();
+  if (k0 + 2 <= k1 + 1) hz_put<WL>(g, j0, r, qs[0]);                  // q at k0+2
+  if (k0 + 3 <= k1 + 1) hz_fetch<WL>(g, i0, j0, k0 + 3, q, r);
+  __syncthreads();
+#pragma unroll 1
+  for (int kk = k0; kk < k1; kk++) {
+    const int sq = kk - k0 + 2, sx = kk - k0 + 1;
+    // x1 at kk+1 from q at kk, kk+1, kk+2 (into the slot of x1 at kk-2)
+    hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[sq & 3], qs[(sq + 1) & 3], qs[(sq + 2) & 3], xs[(sx + 1) % 3]);
+    __syncthreads();
+    {
+      // x2 at kk from x1 at kk-1, kk, kk+1 and q at kk
+      const float* Xm = xs[(sx + 2) % 3];
+      const float* X0 = xs[sx % 3];
+      const float* Xp = xs[(sx + 1) % 3];
+      const int xc = (ty + 1) * HZ_XX + tx + 1, qc = (ty + 2) * HZ_QX + tx + 2;
+      const int j = j0 + ty;
+      const long m = cidx(g, i0 + tx, j, kk);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const float* A = X0 + c * HZ_XP;
+        const float x0 = A[xc];
+        const float lap = (A[xc - 1] - 2.0f * x0 + A[xc + 1]) * ix2 + (A[xc - HZ_XX] - 2.0f * x0 + A[xc + HZ_XX]) * iy2 +
+                          (Xm[c * HZ_XP + xc] - 2.0f * x0 + Xp[c * HZ_XP + xc]) * iz2;
+        float v = qs[sq & 3][c * HZ_QP + qc] + eps * lap;
+        if (c == 1 && WL && j == g.ny - 1) v = 0.0f;   // wall face of V
+        x[c * g.n + m] = v;
+      }
+    }
+    // q at kk+3 into the fourth slot (read from the next plane on), then
+    // fetch q at kk+4
+    if (kk + 3 <= k1 + 1) hz_put<WL>(g, j0, r, qs[(sq + 3) & 3]);
+    if (kk + 4 <= k1 + 1) hz_fetch<WL>(g, i0, j0, kk + 4, q, r);
+    __syncthreads();
+  }
+}
+
 // Helmholtz solves of the three predictor components: fixed-point sweeps when
 // eps*lambda_max is small, the spectral (+ line) solve otherwise.  q and the
 // result x are 3-plane buffers; x may not alias q.
@@ -1853,6 +2007,12 @@ static void helmholtz32(Ctx& c, const float* q, float* x) {
     return;
   }
   const float eps = (float)(((g.dt * g.beta) / g.re) / 2.0);
+  if (X.helm_iters == 2 && g.nx % HZ_TX == 0 && g.ny % HZ_TY == 0 && g.nz >= 3) {
+    const long grid = (long)(g.nx / HZ_TX) * (g.ny / HZ_TY) * ((g.nz + HZ_KC - 1) / HZ_KC);
+    hipLaunchKernelGGL(g.walls ? m_helm_2sweeps_z<true> : m_helm_2sweeps_z<false>, dim3(grid), dim3(HZ_THREADS), 0,
+                       c.stream, g, eps, q, x);
+    return;
+  }
   const float* in = q;
   for (int it = 0; it < X.helm_iters; it++) {
     // the last sweep writes x; earlier ones alternate so that it does
