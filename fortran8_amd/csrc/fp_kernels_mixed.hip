@@ -2243,8 +2243,10 @@ void mixed_create(Ctx& c) {
   const Grid& g = c.g;
   const long n = g.n;
   FP_HIP_CHECK(hipStreamCreateWithFlags(&X->side, hipStreamNonBlocking));
-  FP_HIP_CHECK(hipEventCreateWithFlags(&X->ev_cstar, hipEventDisableTiming));
-  FP_HIP_CHECK(hipEventCreateWithFlags(&X->sel_done, hipEventDisableTiming));
+  // the two streams' handoffs stay on the device: a device-scope release
+  // instead of the system-scope one
+  FP_HIP_CHECK(hipEventCreateWithFlags(&X->ev_cstar, hipEventDisableTiming | hipEventReleaseToDevice));
+  FP_HIP_CHECK(hipEventCreateWithFlags(&X->sel_done, hipEventDisableTiming | hipEventReleaseToDevice));
   dalloc(&X->U, n); dalloc(&X->V, n); dalloc(&X->W, n); dalloc(&X->P, n);
   dalloc(&X->C[0], 6 * n); dalloc(&X->C[1], 6 * n); dalloc(&X->f, n);
   dalloc(&X->A0, 3 * n); dalloc(&X->T0, 3 * n); dalloc(&X->B, 3 * n);
