@@ -2047,6 +2047,93 @@ __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__
   P[m] = (float)(((double)P[m] + ph) - coef * lap);
 }
 
+// The same correction streamed along z: a 64 x 4 tile walks 8 planes with
+// three planes of Phi (FP64, one-cell halo, the boundary rule of P applied)
+// in LDS and the next plane prefetched; u*, P and the outputs are the cell's
+// own, read and written once.
+#define MZ_TX 64
+#define MZ_TY 4
+#define MZ_THREADS (MZ_TX * MZ_TY)
+#define MZ_HX (MZ_TX + 2)
+#define MZ_PTS (MZ_HX * (MZ_TY + 2))
+#define MZ_PER ((MZ_PTS + MZ_THREADS - 1) / MZ_THREADS)
+#define MZ_KC 8                   // planes per block: a light kernel wants many blocks (32 measured slower)
+
+template <bool WL>
+__device__ __forceinline__ void mz_fetch(const Grid& g, int i0, int j0, int kk, const double* __restrict__ phi,
+                                         double (&r)[MZ_PER]) {
+  const long plane = (long)g.nx * g.ny;
+  const int kz = wrapi(kk, g.nz);
+#pragma unroll
+  for (int e = 0; e < MZ_PER; e++) {
+    const int pt = threadIdx.x + MZ_THREADS * e;
+    if (pt < MZ_PTS) {
+      const int ii = wrapi(i0 + pt % MZ_HX - 1, g.nx);
+      int jj = j0 + pt / MZ_HX - 1;
+      // P_kabe: a ghost row takes the nearest row (ldbw, BC_P)
+      jj = WL ? (jj < 0 ? 0 : (jj >= g.ny ? g.ny - 1 : jj)) : wrapi(jj, g.ny);
+      r[e] = phi[ii + (long)g.nx * jj + plane * kz];
+    }
+  }
+}
+
+__device__ __forceinline__ void mz_put(const double (&r)[MZ_PER], double* __restrict__ slot) {
+#pragma unroll
+  for (int e = 0; e < MZ_PER; e++) {
+    const int pt = threadIdx.x + MZ_THREADS * e;
+    if (pt < MZ_PTS) slot[pt] = r[e];
+  }
+}
+
+// grid: (nx / MZ_TX) (ny / MZ_TY) ceil(nz / MZ_KC) blocks of MZ_THREADS
+template <bool WL>
+__global__ __launch_bounds__(MZ_THREADS) void m_march_z(Grid g, const float* __restrict__ up,
+                                                        const double* __restrict__ phi, float* __restrict__ U,
+                                                        float* __restrict__ V, float* __restrict__ W,
+                                                        float* __restrict__ P) {
+  __shared__ double ps[3][MZ_PTS];
+  const int ntx = g.nx / MZ_TX, nty = g.ny / MZ_TY;
+  const long b = xcd_block();
+  const int bx = (int)(b % ntx), by = (int)((b / ntx) % nty), bz = (int)(b / ((long)ntx * nty));
+  const int i0 = bx * MZ_TX, j0 = by * MZ_TY, k0 = bz * MZ_KC;
+  const int k1 = min(k0 + MZ_KC, g.nz);
+  const int tx = threadIdx.x % MZ_TX, ty = threadIdx.x / MZ_TX;
+  const int i = i0 + tx, j = j0 + ty;
+  const double dt = g.dt;
+  const double coef = ((dt * g.beta) / g.re) / 2.0;
+  const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+  double r[MZ_PER];
+  // plane p in slot (p - k0 + 1) mod 3
+#pragma unroll 1
+  for (int s = 0; s < 3; s++) {
+    mz_fetch<WL>(g, i0, j0, k0 - 1 + s, phi, r);
+    mz_put(r, ps[s]);
+  }
+  if (k0 + 2 <= k1) mz_fetch<WL>(g, i0, j0, k0 + 2, phi, r);
+  const int c = (ty + 1) * MZ_HX + tx + 1;
+  int sm = 0, s0 = 1, sp = 2;
+#pragma unroll 1
+  for (int kk = k0; kk < k1; kk++) {
+    __syncthreads();
+    const long m = cidx(g, i, j, kk);
+    const float u = Uu[m], v = (WL && j == g.ny - 1) ? 0.0f : Vv[m], w = Ww[m];   // V_kabe: the wall face reads 0
+    const float pold = P[m];
+    const double* A = ps[s0];
+    const double ph = A[c];
+    U[m] = (float)((double)u - dt * (A[c + 1] - ph) / g.dx);
+    V[m] = (float)((double)v - dt * (A[c + MZ_HX] - ph) / g.dy);
+    W[m] = (float)((double)w - dt * (ps[sp][c] - ph) / g.dz);
+    const double lap = (A[c - 1] - 2.0 * ph + A[c + 1]) / (g.dx * g.dx) +
+                       (A[c - MZ_HX] - 2.0 * ph + A[c + MZ_HX]) / (g.dy * g.dy) +
+                       (ps[sm][c] - 2.0 * ph + ps[sp][c]) / (g.dz * g.dz);
+    P[m] = (float)(((double)pold + ph) - coef * lap);
+    __syncthreads();
+    if (kk + 2 <= k1) mz_put(r, ps[sm]);
+    if (kk + 3 <= k1) mz_fetch<WL>(g, i0, j0, kk + 3, phi, r);
+    const int t = sm; sm = s0; s0 = sp; sp = t;
+  }
+}
+
 // ================================================================== IBM ===
 __device__ __forceinline__ double roma64(double r) {
   double a = fabs(r);
@@ -2836,8 +2923,14 @@ void mixed_step(Ctx& c) {
   }
   {
     MTimer t(c, "march");
-    hipLaunchKernelGGL(g.walls ? m_march<true> : m_march<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.phi, X.U,
-                       X.V, X.W, X.P);
+    if (g.nx % MZ_TX == 0 && g.ny % MZ_TY == 0) {
+      const long grid = (long)(g.nx / MZ_TX) * (g.ny / MZ_TY) * ((g.nz + MZ_KC - 1) / MZ_KC);
+      hipLaunchKernelGGL(g.walls ? m_march_z<true> : m_march_z<false>, dim3(grid), dim3(MZ_THREADS), 0, s, g, X.Up,
+                         X.phi, X.U, X.V, X.W, X.P);
+    } else {
+      hipLaunchKernelGGL(g.walls ? m_march<true> : m_march<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, X.Up, X.phi,
+                         X.U, X.V, X.W, X.P);
+    }
   }
   c.has_history = 1;
   c.step++;
