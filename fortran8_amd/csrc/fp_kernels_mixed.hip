@@ -58,6 +58,7 @@ struct Mixed {
   unsigned* amb_list = nullptr;   // per-block regions of m * 3 + axis entries
   unsigned* amb_cnt = nullptr;    // entries per select block
   unsigned* amb_off = nullptr;    // exclusive offsets of amb_cnt
+  unsigned* amb_wstart = nullptr; // per 64 entries: the block holding the first
   float* amb_data = nullptr;      // staged cell values, 18 * SEL_STAGE per select block
   unsigned* chk = nullptr;        // selection check: differing decisions
   // IBM
@@ -390,7 +391,8 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
 // and a 16-entry LDS combine, so every access is coalesced or a broadcast.
 #define SCAN_ROWS 16
 __global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ cnt, long nb,
-                                                   unsigned* __restrict__ off, unsigned* __restrict__ total) {
+                                                   unsigned* __restrict__ off, unsigned* __restrict__ total,
+                                                   unsigned* __restrict__ wstart) {
   __shared__ unsigned wsum[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   unsigned carry = 0;
@@ -419,7 +421,12 @@ __global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ 
         tot += sq;
       }
       const long b = p0 + 1024L * r + t;
-      if (b < nb) off[b] = carry + pre + v - x[r];
+      if (b < nb) {
+        const unsigned o0 = carry + pre + v - x[r], o1 = o0 + x[r];
+        off[b] = o0;
+        // the groups of 64 entries whose first entry this block holds
+        for (unsigned long long w = (o0 + 63ull) >> 6; (w << 6) < o1; w++) wstart[w] = (unsigned)b;
+      }
       carry += tot;
       __syncthreads();
     }
@@ -558,11 +565,14 @@ __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restri
                                                     const unsigned* __restrict__ total,
                                                     const unsigned* __restrict__ off, long nb,
                                                     const unsigned* __restrict__ amb_list,
-                                                    const float* __restrict__ amb_data) {
+                                                    const float* __restrict__ amb_data,
+                                                    const unsigned* __restrict__ wstart) {
   const long cnt = (long)*total;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < cnt; t += (long)gridDim.x * blockDim.x) {
-    // largest b with off[b] <= t (that block holds entry t)
-    long lo = 0, hi = nb - 1;
+    // largest b with off[b] <= t (that block holds entry t), searched between
+    // the blocks holding the first entries of t's group of 64 and of the next
+    const long w = t >> 6;
+    long lo = (long)wstart[w], hi = ((w + 1) << 6) < cnt ? (long)wstart[w + 1] : nb - 1;
     while (lo < hi) {
       const long mid = (lo + hi + 1) >> 1;
       if ((long)off[mid] <= t) lo = mid;
@@ -2381,6 +2391,7 @@ void mixed_create(Ctx& c) {
     const long nb = nblk(n, SEL_BLOCK);
     dalloc(&X->amb, 1); dalloc(&X->amb_list, nb * SEL_REGION); dalloc(&X->chk, 1);
     dalloc(&X->amb_cnt, nb); dalloc(&X->amb_off, nb);
+    dalloc(&X->amb_wstart, nb * (SEL_REGION / 64) + 2);
     dalloc(&X->amb_data, nb * 18 * SEL_STAGE);
   }
   // FP32 twiddles and wavenumbers from the FP64 host tables
@@ -2421,7 +2432,7 @@ void mixed_destroy(Ctx& c) {
   if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
   if (X->dc_host) (void)hipHostFree(X->dc_host);
   dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
-  dfree(X->amb_cnt); dfree(X->amb_off); dfree(X->amb_data);
+  dfree(X->amb_cnt); dfree(X->amb_off); dfree(X->amb_wstart); dfree(X->amb_data);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
   dfree(X->fr_row); dfree(X->fr_len); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
@@ -2796,11 +2807,11 @@ static void select_fenep(Ctx& c, int tm, hipStream_t s) {
   }
   {
     MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr, s);
-    hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb);
+    hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb, X.amb_wstart);
     // one wavefront per select block: 16 per SIMD at 256x128x128, enough to
     // hide the search and FP64 latencies; n / 4 entries per pass
     hipLaunchKernelGGL(g.walls ? m_select_fix<true> : m_select_fix<false>, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel,
-                       X.amb, X.amb_off, nb, X.amb_list, X.amb_data);
+                       X.amb, X.amb_off, nb, X.amb_list, X.amb_data, X.amb_wstart);
   }
 }
 
