@@ -1191,32 +1191,66 @@ __global__ __launch_bounds__(PZ_THREADS) void m_predictor_z(Grid g, int kc, int 
 }
 
 // ================================================================== FFT ===
-// Radix-2 Stockham autosort (forward e^{-2 pi i jk/n}, unnormalised; the
-// inverse uses the conjugate twiddles) over nl lines held in LDS with line
-// pitch `pitch` (>= n, padded where lines are staged column-wise so that the
-// staging is free of bank conflicts).  tws: the n/2 twiddles (cos, -sin)(2 pi
-// j/n), already in LDS.  Returns the buffer holding the result.
+// Stockham autosort (forward e^{-2 pi i jk/n}, unnormalised; the inverse
+// uses the conjugate twiddles) over nl lines held in LDS with line pitch
+// `pitch` (>= n, padded where lines are staged column-wise so that the
+// staging is free of bank conflicts): radix-4 stages (decimation in
+// frequency), after one radix-2 stage when log2 n is odd, so a 256-point line
+// takes four barriers instead of eight.  tws: the n/2 twiddles (cos, -sin)(2
+// pi j/n), already in LDS; W^(j + n/2) = -W^j gives the rest.  Returns the
+// buffer holding the result.
+__device__ __forceinline__ float2 tw_at(const float2* tws, int half, int j, bool inv) {
+  const float2 w = j < half ? tws[j] : make_float2(-tws[j - half].x, -tws[j - half].y);
+  return inv ? make_float2(w.x, -w.y) : w;
+}
+__device__ __forceinline__ float2 cmul_f(float2 a, float2 w) {
+  return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+
 __device__ float2* stockham_f(float2* x, float2* y, int n, int logn, int nl, int pitch, const float2* tws, bool inv) {
   int len = n, s = 1, lgs = 0;
   const int half = n >> 1;
-  while (len > 1) {
+  if (logn & 1) {
     const int m = len >> 1;
     for (int b = threadIdx.x; b < nl * half; b += blockDim.x) {
       const int line = b >> (logn - 1);
       const int t = b & (half - 1);
       const int p = t >> lgs, q = t & (s - 1);
-      const float2 w = tws[p * s];
-      const float wr = w.x, wi = inv ? -w.y : w.y;
+      const float2 w = tw_at(tws, half, p * s, inv);
       const float2* xl = x + line * pitch;
       float2* yl = y + line * pitch;
       const float2 a = xl[q + s * p], bb = xl[q + s * (p + m)];
-      const float dr = a.x - bb.x, di = a.y - bb.y;
       yl[q + s * (2 * p)] = make_float2(a.x + bb.x, a.y + bb.y);
-      yl[q + s * (2 * p + 1)] = make_float2(dr * wr - di * wi, dr * wi + di * wr);
+      yl[q + s * (2 * p + 1)] = cmul_f(make_float2(a.x - bb.x, a.y - bb.y), w);
     }
     __syncthreads();
     float2* tt = x; x = y; y = tt;
-    len = m; s <<= 1; lgs++;
+    len = m; s = 2; lgs = 1;
+  }
+  const int quarter = n >> 2;
+  while (len > 1) {
+    const int m = len >> 2;
+    for (int b = threadIdx.x; b < nl * quarter; b += blockDim.x) {
+      const int line = b >> (logn - 2);
+      const int t = b & (quarter - 1);
+      const int p = t >> lgs, q = t & (s - 1);
+      const float2* xl = x + line * pitch;
+      float2* yl = y + line * pitch;
+      const float2 a0 = xl[q + s * p], a1 = xl[q + s * (p + m)], a2 = xl[q + s * (p + 2 * m)],
+                   a3 = xl[q + s * (p + 3 * m)];
+      const float2 b0 = make_float2(a0.x + a2.x, a0.y + a2.y), b1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+      const float2 b2 = make_float2(a1.x + a3.x, a1.y + a3.y), d = make_float2(a1.x - a3.x, a1.y - a3.y);
+      // (a1 - a3) times -i (forward) or +i (inverse)
+      const float2 b3 = inv ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+      const int j1 = p * s;
+      yl[q + s * (4 * p)] = make_float2(b0.x + b2.x, b0.y + b2.y);
+      yl[q + s * (4 * p + 1)] = cmul_f(make_float2(b1.x + b3.x, b1.y + b3.y), tw_at(tws, half, j1, inv));
+      yl[q + s * (4 * p + 2)] = cmul_f(make_float2(b0.x - b2.x, b0.y - b2.y), tw_at(tws, half, 2 * j1, inv));
+      yl[q + s * (4 * p + 3)] = cmul_f(make_float2(b1.x - b3.x, b1.y - b3.y), tw_at(tws, half, 3 * j1, inv));
+    }
+    __syncthreads();
+    float2* tt = x; x = y; y = tt;
+    len = m; s <<= 2; lgs += 2;
   }
   return x;
 }
@@ -1489,19 +1523,41 @@ __device__ void fft_rows_inplace(float2* S, int R, int P, int n, int logn, const
     }
   }
   __syncthreads();
-  for (int st = 1; st <= logn; st++) {
+  // two stages a pass: the four elements i, i+h, i+2h, i+3h of stages st and
+  // st+1 (h = 2^(st-1)) go through both butterflies in registers
+  int st = 1;
+  for (; st + 1 <= logn; st += 2) {
+    const int h = 1 << (st - 1), str = n >> st, str2 = n >> (st + 1);
+    for (int b = threadIdx.x; b < R * (n >> 2); b += blockDim.x) {
+      const int row = b >> (logn - 2), t = b & ((n >> 2) - 1);
+      const int k = t & (h - 1);
+      const int i0 = ((t >> (st - 1)) << (st + 1)) + k;
+      float2* x = S + (size_t)row * P;
+      const float2 w = tw_at(tws, n >> 1, k * str, inv), wa = tw_at(tws, n >> 1, k * str2, inv),
+                   wb = tw_at(tws, n >> 1, (k + h) * str2, inv);
+      const float2 a0 = x[i0], a1 = x[i0 + h], a2 = x[i0 + 2 * h], a3 = x[i0 + 3 * h];
+      const float2 t1 = cmul_f(a1, w), t3 = cmul_f(a3, w);
+      const float2 c0 = make_float2(a0.x + t1.x, a0.y + t1.y), c1 = make_float2(a0.x - t1.x, a0.y - t1.y);
+      const float2 c2 = make_float2(a2.x + t3.x, a2.y + t3.y), c3 = make_float2(a2.x - t3.x, a2.y - t3.y);
+      const float2 u = cmul_f(c2, wa), v = cmul_f(c3, wb);
+      x[i0] = make_float2(c0.x + u.x, c0.y + u.y);
+      x[i0 + 2 * h] = make_float2(c0.x - u.x, c0.y - u.y);
+      x[i0 + h] = make_float2(c1.x + v.x, c1.y + v.y);
+      x[i0 + 3 * h] = make_float2(c1.x - v.x, c1.y - v.y);
+    }
+    __syncthreads();
+  }
+  if (st == logn) {
     const int half = 1 << (st - 1), str = n >> st;
     for (int b = threadIdx.x; b < R * (n >> 1); b += blockDim.x) {
       const int row = b >> (logn - 1), t = b & ((n >> 1) - 1);
       const int k = t & (half - 1);
       const int i = ((t >> (st - 1)) << st) + k, j = i + half;
       float2* x = S + (size_t)row * P;
-      const float2 w = tws[k * str];
-      const float wr = w.x, wi = inv ? -w.y : w.y;
-      const float2 a = x[i], bb = x[j];
-      const float tr = bb.x * wr - bb.y * wi, ti = bb.x * wi + bb.y * wr;
-      x[i] = make_float2(a.x + tr, a.y + ti);
-      x[j] = make_float2(a.x - tr, a.y - ti);
+      const float2 tv = cmul_f(x[j], tw_at(tws, n >> 1, k * str, inv));
+      const float2 a = x[i];
+      x[i] = make_float2(a.x + tv.x, a.y + tv.y);
+      x[j] = make_float2(a.x - tv.x, a.y - tv.y);
     }
     __syncthreads();
   }
