@@ -134,7 +134,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the whole domain runs on a single MI355X "
-              "(no inter-GPU path in this build); run with --gpus 1", file=sys.stderr)
+              "(the library has no inter-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     prec = 1 if a.precision == "mixed" else 0
     if a.small:

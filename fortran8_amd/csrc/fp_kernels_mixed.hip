@@ -559,7 +559,9 @@ __device__ __forceinline__ int select_pruned(const float (&fm)[6], const float (
 
 // FP64 re-decision of the uncertified selections: grid-stride over the
 // total, each entry found in its block's region by a binary search of the
-// exclusive offsets
+// exclusive offsets (between the blocks the scan recorded for its group of
+// 64 entries); its 18 input values come from the block's staging area for
+// the first SEL_STAGE entries of a block and are gathered for the rest
 template <bool WL>
 __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                     const unsigned* __restrict__ total,
