@@ -782,6 +782,13 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
 #pragma unroll
   for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
   int it;
+  // Chord iterations: the Jacobian of the first iterate (the linear operator
+  // plus the rank-one Peterlin term) is factored once, its multipliers kept
+  // below the diagonal; later iterations take fresh residuals through the
+  // same factors.  The Jacobian moves with x only through f and f', by about
+  // 1e-5 relative between iterates, so the chord converges like Newton here
+  // (two iterations in the throughput case) at half the cost of the second.
+  T J[6][6];
   for (it = 1; it <= itrmax; it++) {
     const T s = x[0] + x[3] + x[5];
     const T f = peterlin_t<T>(L2, s);
@@ -795,33 +802,41 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     l[3] = T(2) * (a21 * x[1] + A22 * x[3] + a23 * x[4]) - b[3];
     l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (A33 + A22) * x[4] + a23 * x[5] - b[4];
     l[5] = T(2) * (a31 * x[2] + a32 * x[4] + A33 * x[5]) - b[5];
-    // Jacobian: linear operator + rank one Peterlin term
-    const T gp = dtw * dpeterlin_t<T>(L2, s);
-    T J[6][6] = {
-        {T(2) * A11, T(2) * a12, T(2) * a13, T(0), T(0), T(0)},
-        {a21, A22 + A11, a23, a12, a13, T(0)},
-        {a31, a32, A33 + A11, T(0), a12, a13},
-        {T(0), T(2) * a21, T(0), T(2) * A22, T(2) * a23, T(0)},
-        {T(0), a31, a21, a32, A33 + A22, a23},
-        {T(0), T(0), T(2) * a31, T(0), T(2) * a32, T(2) * A33}};
+    if (it == 1) {
+      // Jacobian: linear operator + rank one Peterlin term
+      const T gp = dtw * dpeterlin_t<T>(L2, s);
+      const T J0[6][6] = {
+          {T(2) * A11, T(2) * a12, T(2) * a13, T(0), T(0), T(0)},
+          {a21, A22 + A11, a23, a12, a13, T(0)},
+          {a31, a32, A33 + A11, T(0), a12, a13},
+          {T(0), T(2) * a21, T(0), T(2) * A22, T(2) * a23, T(0)},
+          {T(0), a31, a21, a32, A33 + A22, a23},
+          {T(0), T(0), T(2) * a31, T(0), T(2) * a32, T(2) * A33}};
 #pragma unroll
-    for (int r = 0; r < 6; r++) {
-      const T gx = gp * x[r];
-      J[r][0] += gx; J[r][3] += gx; J[r][5] += gx;
-    }
-    // Gaussian elimination (diagonally dominant: no pivoting needed)
+      for (int r = 0; r < 6; r++) {
+        const T gx = gp * x[r];
 #pragma unroll
-    for (int kk = 0; kk < 6; kk++) {
-      const T inv = rcp_t(J[kk][kk]);
-#pragma unroll
-      for (int r = kk + 1; r < 6; r++) {
-        const T fct = J[r][kk] * inv;
-#pragma unroll
-        for (int cc = kk + 1; cc < 6; cc++) J[r][cc] -= fct * J[kk][cc];
-        l[r] -= fct * l[kk];
+        for (int cc = 0; cc < 6; cc++) J[r][cc] = J0[r][cc] + ((cc == 0 || cc == 3 || cc == 5) ? gx : T(0));
       }
-      J[kk][kk] = inv;
+      // LU without pivoting (diagonally dominant): multipliers below the
+      // diagonal, reciprocal pivots on it
+#pragma unroll
+      for (int kk = 0; kk < 6; kk++) {
+        const T inv = rcp_t(J[kk][kk]);
+#pragma unroll
+        for (int r = kk + 1; r < 6; r++) {
+          const T fct = J[r][kk] * inv;
+#pragma unroll
+          for (int cc = kk + 1; cc < 6; cc++) J[r][cc] -= fct * J[kk][cc];
+          J[r][kk] = fct;
+        }
+        J[kk][kk] = inv;
+      }
     }
+#pragma unroll
+    for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+      for (int r = kk + 1; r < 6; r++) l[r] -= J[r][kk] * l[kk];
     T rr[6];
 #pragma unroll
     for (int r = 5; r >= 0; r--) {
