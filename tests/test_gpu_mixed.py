@@ -84,6 +84,21 @@ def test_mixed_channel_cylinder_markers_not_repeated_along_z():
     assert g.diagnostics()["drag"] == pytest.approx(o.diagnostics()["drag"], rel=TOL)
 
 
+def test_mixed_channel_cylinder_per_cell_kernels():
+    """NX = 32 is not a multiple of the 64-cell tiles: the predictor, the
+    Helmholtz sweeps and the correction run as per-cell kernels (wall rows
+    in their own instances) beside the IBM passes."""
+    case = mixed(configs.channel_cylinder(nx=32, ny=16, nz=16))
+    print("mixed channel 32x16x16, per-cell kernels, 40 steps", flush=True)
+    o, g = po.load_case(case), api.load_case(case)
+    o.step(40)
+    g.step(40)
+    e = errors(o, g)
+    print(e, flush=True)
+    assert e["u"] < TOL and e["C"] < TOL and e["P"] < TOL
+    assert g.diagnostics()["drag"] == pytest.approx(o.diagnostics()["drag"], rel=TOL)
+
+
 def test_mixed_throughput_layout_small():
     case = mixed(configs.throughput(nx=64, ny=64, nz=16, ncyl=2))
     o, g = po.load_case(case), api.load_case(case)
