@@ -1053,8 +1053,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 // the tile with a one-cell halo for the ten stencil fields (u, v, w, p and
 // c0 of the six components); every neighbour the predictor reads lies there.
 // Plane k+2 is loaded into registers while plane k is computed, then takes
-// the place of plane k-1.  Each value is fetched from L2 about 1.7 times at
-// 64 x 4 tiles and 16 planes (halo rows and chunk ends), in whole rows,
+// the place of plane k-1.  Each value is fetched from L2 about 1.6 times at
+// 64 x 4 tiles and 32 planes (halo rows and chunk ends), in whole rows,
 // where the per-cell stencil of m_predictor fetches it about 6 times.  The
 // values in LDS are those of the boundary-rule accessor (wall ghosts and the
 // wall face of V included), so one kernel covers the wall rows too.
