@@ -44,18 +44,13 @@ KERNEL_WORDS = {
 
 def cells_per_launch(p, prec, phase, phases):
     """Cells one launch of the phase's kernel processes.  Each timed phase is
-    one kernel.  With y-walls the mixed bulk CStar/Newton kernel leaves the
-    4 wall rows to its own instance (timed as fenep_cstar_newton_walls); so
-    does the per-cell mixed predictor (2 rows, predictor_walls), which runs
-    where the z-streamed predictor's tiling does not fit the grid.  The mixed
-    selection, the z-streamed predictor and every FP64 kernel cover all
-    cells."""
+    one kernel.  With y-walls the per-cell mixed predictor, which runs where
+    the z-streamed predictor's tiling does not fit the grid, leaves the 2
+    wall rows to its own instance (timed as predictor_walls); every other
+    kernel covers all cells."""
     cells = p.nx * p.ny * p.nz
-    if prec == 1 and p.walls_y:
-        if phase == "predictor" and "predictor_walls" in phases:
-            return p.nx * (p.ny - 2) * p.nz
-        if phase == "fenep_cstar_newton":
-            return p.nx * (p.ny - 4) * p.nz
+    if prec == 1 and p.walls_y and phase == "predictor" and "predictor_walls" in phases:
+        return p.nx * (p.ny - 2) * p.nz
     return cells
 
 
@@ -161,7 +156,7 @@ def main():
     walls = "true" if case.params.walls_y else "false"
     mixed_pred = "fp::m_predictor<false>" if "predictor_walls" in kt else "fp::m_predictor_z<%s>" % walls
     kernel_names = {"fenep_select": "fp::m_select" if prec else "fp::k_select",
-                    "fenep_cstar_newton": "fp::m_cstar_newton<false>" if prec else "fp::k_cstar_newton<%s>" % walls,
+                    "fenep_cstar_newton": "fp::m_cstar_newton" if prec else "fp::k_cstar_newton<%s>" % walls,
                     "predictor": mixed_pred if prec else "fp::k_predictor<%s>" % walls}
     out = {
         "metric": "Mcell-steps/s 256x128x128 FENE-P+IBM",

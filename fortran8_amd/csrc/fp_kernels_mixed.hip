@@ -238,24 +238,14 @@ __device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
   return fmaf(cp, 0.25f, c) - cm * 0.25f;
 }
 
-// C at cells a-r..a+r along axis d (r = 1 or 2) for the 6 components; wall
-// ghosts by linear extrapolation (C_kabe); points beyond one ghost layer
-// only feed faces whose upwind weight is exactly zero and get the centre.
-template <int R, bool GEN>
+// C at cells a-r..a+r along axis d (r = 1 or 2) for the 6 components,
+// through wrapped offsets.  With y-walls a point of the line on the ghost row
+// takes the linear extrapolation 2 C_0 - C_1 of the two nearest rows
+// (C_kabe), and a point beyond it the centre (it only feeds faces whose
+// upwind weight is exactly zero); the wrapped rows loaded there are replaced.
+template <int R>
 __device__ __forceinline__ void load_line(const float* __restrict__ C, const Grid& g, long m, int i, int j, int k, int d,
                                           float (&v)[2 * R + 1][6]) {
-  if (GEN && d == 1 && g.walls && (j < R || j > g.ny - 1 - R)) {
-#pragma unroll
-    for (int o = -R; o <= R; o++) {
-      const int jj = j + o;
-#pragma unroll
-      for (int q = 0; q < 6; q++) {
-        const float* Cq = C + q * g.n;
-        v[o + R][q] = (jj >= -1 && jj <= g.ny) ? ldbw<true>(Cq, g, i, jj, k, BC_C) : Cq[m];
-      }
-    }
-    return;
-  }
   long off[2 * R + 1];
 #pragma unroll
   for (int o = -R; o <= R; o++) {
@@ -268,13 +258,23 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
 #pragma unroll
     for (int q = 0; q < 6; q++) v[o][q] = o == R ? C[q * g.n + m] : C[q * g.n + m + off[o]];
   if (R == 1 && d == 1 && g.walls) {
-    // the one ghost row of a wall (C_kabe): linear extrapolation 2 C_0 - C_1,
-    // in place of the wrapped row loaded above
     const bool lo = j == 0, hi = j == g.ny - 1;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
       v[0][q] = lo ? 2.0f * v[1][q] - v[2][q] : v[0][q];
       v[2][q] = hi ? 2.0f * v[1][q] - v[0][q] : v[2][q];
+    }
+  }
+  if (R == 2 && d == 1 && g.walls) {
+    // rows 0, 1 at the low wall and ny-2, ny-1 at the high wall; each
+    // replacement reads only entries that no earlier one changes
+    const bool l0 = j == 0, l1 = j == 1, h1 = j == g.ny - 2, h0 = j == g.ny - 1;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      v[1][q] = l0 ? 2.0f * v[2][q] - v[3][q] : v[1][q];
+      v[0][q] = l0 ? v[2][q] : (l1 ? 2.0f * v[1][q] - v[2][q] : v[0][q]);
+      v[3][q] = h0 ? 2.0f * v[2][q] - v[1][q] : v[3][q];
+      v[4][q] = h0 ? v[2][q] : (h1 ? 2.0f * v[3][q] - v[2][q] : v[4][q]);
     }
   }
 }
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
     int idx = 0;
     float v[3][6];
     if (valid) {
-      load_line<1, false>(C, g, m, i, j, k, d, v);
+      load_line<1>(C, g, m, i, j, k, d, v);
       const float ein = SEL_KCE * fmaxf(fmaxf(entry_bound(v[0]), entry_bound(v[1])), entry_bound(v[2]));
       float vals[4], errs[4];
 #pragma unroll
@@ -691,15 +691,17 @@ __device__ __forceinline__ T dpeterlin_t(T L2, T s) {
 // for one cell, in the arithmetic type T (storage stays FP32; the step uses
 // FP32: an FP64 evaluation of the strongly stretched cells was measured to
 // leave the error against the FP64 step unchanged).
-template <bool GEN, typename T>
+template <typename T>
 __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __restrict__ C,
                                                   const uint8_t* __restrict__ sel, const float* __restrict__ U,
                                                   const float* __restrict__ V, const float* __restrict__ W,
                                                   float* __restrict__ Cn, float* __restrict__ fC, int* __restrict__ nitr,
                                                   T eps, int itrmax, long m, int i, int j, int k) {
   const T dt = (T)g.dt;
-  const bool fast = !GEN || !g.walls || (j > 0 && j < g.ny - 1);
   const Nbr o = nbr(g, i, j, k);
+  // the two wall rows: the values the boundary rules give replace the
+  // wrapped rows in the few reads that reach a ghost row or the wall face of V
+  const bool wlo = g.walls && j == 0, whi = g.walls && j == g.ny - 1;
   T cx[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) cx[q] = T(0);
@@ -708,7 +710,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     const T h = (T)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
     T v1, v2;
     if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
-    else if (d == 1) { v1 = fast ? V[m + o.y[0]] : ldbw<true>(V, g, i, j - 1, k, BC_V); v2 = fast ? V[m] : ldbw<true>(V, g, i, j, k, BC_V); }
+    else if (d == 1) { v1 = wlo ? T(0) : (T)V[m + o.y[0]]; v2 = whi ? T(0) : (T)V[m]; }   // V_kabe
     else { v1 = W[m + o.z[0]]; v2 = W[m]; }
     const T r2h = dt / (T(2) * h);
     const T s1 = (-v2 + fabs(v2)) * r2h;
@@ -734,7 +736,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     const T w3 = s3 * gp_ + s2 * gn + s1 * bq;
     const T w4 = s1 * gq;
     float v[5][6];
-    load_line<2, GEN>(C, g, m, i, j, k, d, v);
+    load_line<2>(C, g, m, i, j, k, d, v);
 #pragma unroll
     for (int q = 0; q < 6; q++)
       cx[q] += w0 * (T)v[0][q] + w1 * (T)v[1][q] + w2 * (T)v[2][q] + w3 * (T)v[3][q] + w4 * (T)v[4][q];
@@ -744,10 +746,15 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   const T dx = (T)g.dx, dy = (T)g.dy, dz = (T)g.dz;
   // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
   T a11, a12, a13, a21, a22, a23, a31, a32, a33;
-  if (fast) {
-#define UF(a, bb, c) ((T)U[m + o.x[a] + o.y[bb] + o.z[c]])
-#define VF(a, bb, c) ((T)V[m + o.x[a] + o.y[bb] + o.z[c]])
-#define WF(a, bb, c) ((T)W[m + o.x[a] + o.y[bb] + o.z[c]])
+  {
+#define UR(a, bb, c) ((T)U[m + o.x[a] + o.y[bb] + o.z[c]])
+#define VR(a, bb, c) ((T)V[m + o.x[a] + o.y[bb] + o.z[c]])
+#define WR(a, bb, c) ((T)W[m + o.x[a] + o.y[bb] + o.z[c]])
+    // the boundary rules (U_kabe, V_kabe, W_kabe): U, W ghosts -v below and
+    // 2*0 - v above; V zero at a ghost row and at the wall face j = ny-1
+#define UF(a, bb, c) ((bb) == 0 && wlo ? -UR(a, 1, c) : ((bb) == 2 && whi ? T(2) * T(0) - UR(a, 1, c) : UR(a, bb, c)))
+#define WF(a, bb, c) ((bb) == 0 && wlo ? -WR(a, 1, c) : ((bb) == 2 && whi ? T(2) * T(0) - WR(a, 1, c) : WR(a, bb, c)))
+#define VF(a, bb, c) (((bb) == 0 && wlo) || ((bb) == 1 && whi) ? T(0) : VR(a, bb, c))
     a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) / dx;
     a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) / (T(4) * dx);
     a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) / (T(4) * dx);
@@ -760,22 +767,9 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
 #undef UF
 #undef VF
 #undef WF
-  } else {
-#define UL(a, bb, c) ((T)ldbw<true>(U, g, a, bb, c, BC_U))
-#define VL(a, bb, c) ((T)ldbw<true>(V, g, a, bb, c, BC_V))
-#define WL(a, bb, c) ((T)ldbw<true>(W, g, a, bb, c, BC_W))
-    a11 = -dt * (UL(i, j, k) - UL(i - 1, j, k)) / dx;
-    a21 = -dt * ((VL(i + 1, j - 1, k) - VL(i - 1, j - 1, k)) + (VL(i + 1, j, k) - VL(i - 1, j, k))) / (T(4) * dx);
-    a31 = -dt * ((WL(i + 1, j, k - 1) - WL(i - 1, j, k - 1)) + (WL(i + 1, j, k) - WL(i - 1, j, k))) / (T(4) * dx);
-    a12 = -dt * ((UL(i - 1, j + 1, k) + UL(i, j + 1, k)) - (UL(i - 1, j - 1, k) + UL(i, j - 1, k))) / (T(4) * dy);
-    a22 = -dt * (VL(i, j, k) - VL(i, j - 1, k)) / dy;
-    a32 = -dt * ((WL(i, j + 1, k - 1) - WL(i, j - 1, k - 1)) + (WL(i, j + 1, k) - WL(i, j - 1, k))) / (T(4) * dy);
-    a13 = -dt * ((UL(i - 1, j, k + 1) + UL(i, j, k + 1)) - (UL(i - 1, j, k - 1) + UL(i, j, k - 1))) / (T(4) * dz);
-    a23 = -dt * ((VL(i, j - 1, k + 1) + VL(i, j, k + 1)) - (VL(i, j - 1, k - 1) + VL(i, j, k - 1))) / (T(4) * dz);
-    a33 = -dt * (WL(i, j, k) - WL(i, j, k - 1)) / dz;
-#undef UL
-#undef VL
-#undef WL
+#undef UR
+#undef VR
+#undef WR
   }
   const T L2 = (T)(g.lp * g.lp);
   T x[6];
@@ -857,20 +851,19 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   nitr[m] = it;
 }
 
-// The wall-row instance (GEN) reads every neighbour through the boundary-rule
-// accessor and would take all 256 registers (one wave per SIMD, its 2048
-// waves in two latency-bound rounds); two waves per SIMD, with a few spills,
-// run it in 22 us instead of 31 at 256x128x128 (four spill too much).
-template <bool GEN>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(GEN ? 2 : 1, 8))) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
+// one instance over every cell (the wall rows through the boundary-rule
+// patches in cstar_newton_cell and load_line), in band order; held to 128
+// registers (four waves per SIMD), where the patches would take it to 130
+// and three waves (180 us instead of 163 at 256x128x128)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
                                                       const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, float* __restrict__ Cn,
                                                       float* __restrict__ fC, int* __restrict__ nitr, float eps,
                                                       int itrmax) {
   long m;
   int i, j, k;
-  if (!row_cell<GEN, 2>(g, m, i, j, k)) return;
-  cstar_newton_cell<GEN, float>(g, C, sel, U, V, W, Cn, fC, nitr, eps, itrmax, m, i, j, k);
+  if (!row_cell<false, 0>(g, m, i, j, k)) return;
+  cstar_newton_cell<float>(g, C, sel, U, V, W, Cn, fC, nitr, eps, itrmax, m, i, j, k);
 }
 
 // ============================================================= predictor ===
@@ -2913,16 +2906,10 @@ void mixed_step(Ctx& c) {
     // the selection of this C: made ahead beside the previous step, or now
     if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(s, X.sel_done, 0));
     else select_fenep(c, 1, s);
-    // one kernel per timed phase: the bulk instances and the wall rows apart
     const int itmax = std::min(c.p.newton_itrmax, 16);
     {
       MTimer t(c, "fenep_cstar_newton");
-      hipLaunchKernelGGL(m_cstar_newton<false>, dim3(nblk(n, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
-                         X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
-    }
-    if (g.walls) {
-      MTimer t(c, "fenep_cstar_newton_walls");
-      hipLaunchKernelGGL(m_cstar_newton<true>, dim3(nblk(wall_rows(g, 2), 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
+      hipLaunchKernelGGL(m_cstar_newton, dim3(nblk(n, 128)), dim3(128), 0, s, g, X.C[X.cur], c.sel, X.U, X.V, X.W,
                          X.C[1 - X.cur], X.f, c.nitr, (float)c.p.newton_eps, itmax);
     }
     X.cur = 1 - X.cur;
