@@ -672,10 +672,13 @@ __device__ __forceinline__ void cand_coef_n(int l, T& a, T& b, T& c) {
 }
 __device__ __forceinline__ float rcp_t(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ __forceinline__ double rcp_t(double v) { return 1.0 / v; }
+// Peterlin factor and its derivative: in FP64 within 1 % of L^2, otherwise
+// through the hardware reciprocal (within an ulp; the IEEE division costs
+// about ten instructions in this VALU-bound kernel)
 template <typename T>
 __device__ __forceinline__ T peterlin_t(T L2, T s) {
   if (s > T(0.99) * L2) return (T)(((double)L2 - 3.0) / ((double)L2 - (double)s));
-  return (L2 - T(3)) / (L2 - s);
+  return (L2 - T(3)) * rcp_t(L2 - s);
 }
 template <typename T>
 __device__ __forceinline__ T dpeterlin_t(T L2, T s) {
@@ -683,8 +686,8 @@ __device__ __forceinline__ T dpeterlin_t(T L2, T s) {
     const double den = (double)L2 - (double)s;
     return (T)(((double)L2 - 3.0) / (den * den));
   }
-  const T den = L2 - s;
-  return (L2 - T(3)) / (den * den);
+  const T r = rcp_t(L2 - s);
+  return (L2 - T(3)) * (r * r);
 }
 
 // Cstar + Lyapunov Newton (analytic Jacobian) + Peterlin factor of the new C
@@ -712,7 +715,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
     else if (d == 1) { v1 = wlo ? T(0) : (T)V[m + o.y[0]]; v2 = whi ? T(0) : (T)V[m]; }   // V_kabe
     else { v1 = W[m + o.z[0]]; v2 = W[m]; }
-    const T r2h = dt / (T(2) * h);
+    const T r2h = dt * (T(0.5) * rcp_t(h));
     const T s1 = (-v2 + fabs(v2)) * r2h;
     const T s2 = T(1) / T(6) + (-v2 - fabs(v2)) * r2h;
     const T s3 = T(1) / T(6) + (v1 - fabs(v1)) * r2h;
@@ -741,7 +744,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     for (int q = 0; q < 6; q++)
       cx[q] += w0 * (T)v[0][q] + w1 * (T)v[1][q] + w2 * (T)v[2][q] + w3 * (T)v[3][q] + w4 * (T)v[4][q];
   }
-  const T dtw = dt / (T)g.wi;
+  const T dtw = dt * rcp_t((T)g.wi);
   T b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
   const T dx = (T)g.dx, dy = (T)g.dy, dz = (T)g.dz;
   // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
@@ -755,15 +758,18 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
 #define UF(a, bb, c) ((bb) == 0 && wlo ? -UR(a, 1, c) : ((bb) == 2 && whi ? T(2) * T(0) - UR(a, 1, c) : UR(a, bb, c)))
 #define WF(a, bb, c) ((bb) == 0 && wlo ? -WR(a, 1, c) : ((bb) == 2 && whi ? T(2) * T(0) - WR(a, 1, c) : WR(a, bb, c)))
 #define VF(a, bb, c) (((bb) == 0 && wlo) || ((bb) == 1 && whi) ? T(0) : VR(a, bb, c))
-    a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) / dx;
-    a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) / (T(4) * dx);
-    a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) / (T(4) * dx);
-    a12 = -dt * ((UF(0, 2, 1) + UF(1, 2, 1)) - (UF(0, 0, 1) + UF(1, 0, 1))) / (T(4) * dy);
-    a22 = -dt * (VF(1, 1, 1) - VF(1, 0, 1)) / dy;
-    a32 = -dt * ((WF(1, 2, 0) - WF(1, 0, 0)) + (WF(1, 2, 1) - WF(1, 0, 1))) / (T(4) * dy);
-    a13 = -dt * ((UF(0, 1, 2) + UF(1, 1, 2)) - (UF(0, 1, 0) + UF(1, 1, 0))) / (T(4) * dz);
-    a23 = -dt * ((VF(1, 0, 2) + VF(1, 1, 2)) - (VF(1, 0, 0) + VF(1, 1, 0))) / (T(4) * dz);
-    a33 = -dt * (WF(1, 1, 1) - WF(1, 1, 0)) / dz;
+    // the grid spacings through reciprocals (no division per term)
+    const T idx = rcp_t(dx), idy = rcp_t(dy), idz = rcp_t(dz);
+    const T qdx = T(0.25) * idx, qdy = T(0.25) * idy, qdz = T(0.25) * idz;
+    a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) * idx;
+    a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) * qdx;
+    a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) * qdx;
+    a12 = -dt * ((UF(0, 2, 1) + UF(1, 2, 1)) - (UF(0, 0, 1) + UF(1, 0, 1))) * qdy;
+    a22 = -dt * (VF(1, 1, 1) - VF(1, 0, 1)) * idy;
+    a32 = -dt * ((WF(1, 2, 0) - WF(1, 0, 0)) + (WF(1, 2, 1) - WF(1, 0, 1))) * qdy;
+    a13 = -dt * ((UF(0, 1, 2) + UF(1, 1, 2)) - (UF(0, 1, 0) + UF(1, 1, 0))) * qdz;
+    a23 = -dt * ((VF(1, 0, 2) + VF(1, 1, 2)) - (VF(1, 0, 0) + VF(1, 1, 0))) * qdz;
+    a33 = -dt * (WF(1, 1, 1) - WF(1, 1, 0)) * idz;
 #undef UF
 #undef VF
 #undef WF
@@ -1378,6 +1384,9 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
   double ss = 0.0;
+  // reciprocals of the spacings and of dt (an FP64 division is some twenty
+  // instructions; the products differ from the quotients by an FP64 ulp)
+  const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, idz = 1.0 / g.dz, idt = 1.0 / g.dt;
 #pragma unroll 2
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
     const int c = e / n, i = e - c * n;
@@ -1387,9 +1396,9 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
       const long L = l0 + 2 * c + h;
       if (L >= nlines) continue;
       const int j = (int)(L % g.ny), k = (int)(L / g.ny);
-      const double v = (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) / g.dx +
-                        ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) / g.dy +
-                        ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) / g.dz) / g.dt;
+      const double v = (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) * idx +
+                        ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) * idy +
+                        ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) * idz) * idt;
       if (want_b) {
         b[L * n + i] = v;
         ss += v * v;
@@ -2177,6 +2186,9 @@ __global__ __launch_bounds__(MZ_THREADS) void m_march_z(Grid g, const float* __r
   const int i = i0 + tx, j = j0 + ty;
   const double dt = g.dt;
   const double coef = ((dt * g.beta) / g.re) / 2.0;
+  // reciprocals of the spacings (FP64 divisions cost some twenty instructions)
+  const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, idz = 1.0 / g.dz;
+  const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy), idz2 = 1.0 / (g.dz * g.dz);
   const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
   double r[MZ_PER];
   // plane p in slot (p - k0 + 1) mod 3
@@ -2196,12 +2208,11 @@ __global__ __launch_bounds__(MZ_THREADS) void m_march_z(Grid g, const float* __r
     const float pold = P[m];
     const double* A = ps[s0];
     const double ph = A[c];
-    U[m] = (float)((double)u - dt * (A[c + 1] - ph) / g.dx);
-    V[m] = (float)((double)v - dt * (A[c + MZ_HX] - ph) / g.dy);
-    W[m] = (float)((double)w - dt * (ps[sp][c] - ph) / g.dz);
-    const double lap = (A[c - 1] - 2.0 * ph + A[c + 1]) / (g.dx * g.dx) +
-                       (A[c - MZ_HX] - 2.0 * ph + A[c + MZ_HX]) / (g.dy * g.dy) +
-                       (ps[sm][c] - 2.0 * ph + ps[sp][c]) / (g.dz * g.dz);
+    U[m] = (float)((double)u - dt * (A[c + 1] - ph) * idx);
+    V[m] = (float)((double)v - dt * (A[c + MZ_HX] - ph) * idy);
+    W[m] = (float)((double)w - dt * (ps[sp][c] - ph) * idz);
+    const double lap = (A[c - 1] - 2.0 * ph + A[c + 1]) * idx2 + (A[c - MZ_HX] - 2.0 * ph + A[c + MZ_HX]) * idy2 +
+                       (ps[sm][c] - 2.0 * ph + ps[sp][c]) * idz2;
     P[m] = (float)(((double)pold + ph) - coef * lap);
     __syncthreads();
     if (kk + 2 <= k1) mz_put(r, ps[sm]);
