@@ -190,7 +190,9 @@ int fp_create(const fp_params* p, void** out) {
       p->wi <= 0 || p->lx <= 0 || p->ly <= 0 || p->lz <= 0)
     return FP_E_ARG;
   if (p->walls_y && p->ny < 4) return FP_E_ARG;
-  if (p->precision == FP_PREC_MIXED && (p->method == FP_METHOD_EXPLICIT || p->poisson_variant == FP_POISSON_SOR))
+  // the mixed kernels address each field plane with 32-bit byte offsets
+  if (p->precision == FP_PREC_MIXED &&
+      (p->method == FP_METHOD_EXPLICIT || p->poisson_variant == FP_POISSON_SOR || (long)p->nx * p->ny * p->nz >= (1L << 30)))
     return FP_E_ARG;
   Ctx* c = new (std::nothrow) Ctx();
   if (!c) return FP_E_NOMEM;

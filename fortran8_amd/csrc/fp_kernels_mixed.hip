@@ -120,6 +120,34 @@ __device__ __forceinline__ long woff(int a, int d, int n, long s) {
   t = t < 0 ? t + n : (t >= n ? t - n : t);
   return (long)(t - a) * s;
 }
+// Element e of an array of n elements through a raw buffer resource: the
+// base and the size sit in scalar registers and each load or store takes one
+// 32-bit byte offset per lane instead of a 64-bit address (n <= 2^30 cells
+// keeps 4 n below 2^32).  An offset beyond the size would read zero and
+// drop the write; every index here is a cell index below n.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+}
+template <typename V>
+__device__ __forceinline__ V ldu(const V* base, long n, unsigned e) {
+  static_assert(sizeof(V) == 4 || sizeof(V) == 1, "4-byte or 1-byte elements");
+  const __amdgpu_buffer_rsrc_t r = brsrc(base, (unsigned)n * (unsigned)sizeof(V));
+  if constexpr (sizeof(V) == 4)
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0));
+  else
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b8(r, (int)e, 0, 0));
+}
+template <typename V>
+__device__ __forceinline__ void stu(V* base, long n, unsigned e, V v) {
+  static_assert(sizeof(V) == 4, "4-byte elements");
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), brsrc(base, (unsigned)n * 4u), (int)(e * 4u), 0, 0);
+}
+// wrapped neighbour offset along an axis in 32 bits
+__device__ __forceinline__ int woff32(int a, int d, int n, int s) {
+  int t = a + d;
+  t = t < 0 ? t + n : (t >= n ? t - n : t);
+  return (t - a) * s;
+}
 
 // ---- C+- selection in FP32 with certified decisions ----------------------
 // The selection of fenep.f90:186-221 takes, per cell and axis, the maxloc over
@@ -246,17 +274,20 @@ __device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
 template <int R>
 __device__ __forceinline__ void load_line(const float* __restrict__ C, const Grid& g, long m, int i, int j, int k, int d,
                                           float (&v)[2 * R + 1][6]) {
-  long off[2 * R + 1];
+  // 32-bit cell indices from the plane bases (n <= 2^30 cells)
+  unsigned e[2 * R + 1];
 #pragma unroll
   for (int o = -R; o <= R; o++) {
-    if (d == 0) off[o + R] = woff(i, o, g.nx, 1);
-    else if (d == 1) off[o + R] = woff(j, o, g.ny, g.nx);
-    else off[o + R] = woff(k, o, g.nz, (long)g.nx * g.ny);
+    int off;
+    if (d == 0) off = woff32(i, o, g.nx, 1);
+    else if (d == 1) off = woff32(j, o, g.ny, g.nx);
+    else off = woff32(k, o, g.nz, g.nx * g.ny);
+    e[o + R] = (unsigned)m + (unsigned)off;
   }
 #pragma unroll
   for (int o = 0; o < 2 * R + 1; o++)
 #pragma unroll
-    for (int q = 0; q < 6; q++) v[o][q] = o == R ? C[q * g.n + m] : C[q * g.n + m + off[o]];
+    for (int q = 0; q < 6; q++) v[o][q] = ldu(C + q * g.n, g.n, e[o]);
   if (R == 1 && d == 1 && g.walls) {
     const bool lo = j == 0, hi = j == g.ny - 1;
 #pragma unroll
@@ -701,7 +732,17 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
                                                   float* __restrict__ Cn, float* __restrict__ fC, int* __restrict__ nitr,
                                                   T eps, int itrmax, long m, int i, int j, int k) {
   const T dt = (T)g.dt;
-  const Nbr o = nbr(g, i, j, k);
+  // 32-bit neighbour offsets and cell index (n <= 2^30 cells)
+  struct {
+    int x[3], y[3], z[3];
+  } o;
+#pragma unroll
+  for (int d = -1; d <= 1; d++) {
+    o.x[d + 1] = woff32(i, d, g.nx, 1);
+    o.y[d + 1] = woff32(j, d, g.ny, g.nx);
+    o.z[d + 1] = woff32(k, d, g.nz, g.nx * g.ny);
+  }
+  const unsigned mu = (unsigned)m;
   // the two wall rows: the values the boundary rules give replace the
   // wrapped rows in the few reads that reach a ghost row or the wall face of V
   const bool wlo = g.walls && j == 0, whi = g.walls && j == g.ny - 1;
@@ -712,17 +753,18 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   for (int d = 0; d < 3; d++) {
     const T h = (T)(d == 0 ? g.dx : (d == 1 ? g.dy : g.dz));
     T v1, v2;
-    if (d == 0) { v1 = U[m + o.x[0]]; v2 = U[m]; }
-    else if (d == 1) { v1 = wlo ? T(0) : (T)V[m + o.y[0]]; v2 = whi ? T(0) : (T)V[m]; }   // V_kabe
-    else { v1 = W[m + o.z[0]]; v2 = W[m]; }
+    if (d == 0) { v1 = ldu(U, g.n, mu + o.x[0]); v2 = ldu(U, g.n, mu); }
+    else if (d == 1) { v1 = wlo ? T(0) : (T)ldu(V, g.n, mu + o.y[0]); v2 = whi ? T(0) : (T)ldu(V, g.n, mu); }   // V_kabe
+    else { v1 = ldu(W, g.n, mu + o.z[0]); v2 = ldu(W, g.n, mu); }
     const T r2h = dt * (T(0.5) * rcp_t(h));
     const T s1 = (-v2 + fabs(v2)) * r2h;
     const T s2 = T(1) / T(6) + (-v2 - fabs(v2)) * r2h;
     const T s3 = T(1) / T(6) + (v1 - fabs(v1)) * r2h;
     const T s4 = (v1 + fabs(v1)) * r2h;
-    const long om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
-    const long op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
-    const int lm = (sel[m + om] >> (2 * d)) & 3, l0 = (sel[m] >> (2 * d)) & 3, lp = (sel[m + op] >> (2 * d)) & 3;
+    const int om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
+    const int op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
+    const int lm = (ldu(sel, g.n, mu + om) >> (2 * d)) & 3, l0 = (ldu(sel, g.n, mu) >> (2 * d)) & 3,
+              lp = (ldu(sel, g.n, mu + op) >> (2 * d)) & 3;
     // The four face values are linear in the line c0..c4:
     //   pl = Cp_l0(c1,c2,c3)  own left face      nr = Cn_l0(c1,c2,c3) own right face
     //   pr = Cp_lp(c2,c3,c4)  left face of a+1   nl = Cn_lm(c0,c1,c2) right face of a-1
@@ -750,9 +792,9 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
   T a11, a12, a13, a21, a22, a23, a31, a32, a33;
   {
-#define UR(a, bb, c) ((T)U[m + o.x[a] + o.y[bb] + o.z[c]])
-#define VR(a, bb, c) ((T)V[m + o.x[a] + o.y[bb] + o.z[c]])
-#define WR(a, bb, c) ((T)W[m + o.x[a] + o.y[bb] + o.z[c]])
+#define UR(a, bb, c) ((T)ldu(U, g.n, mu + (unsigned)(o.x[a] + o.y[bb] + o.z[c])))
+#define VR(a, bb, c) ((T)ldu(V, g.n, mu + (unsigned)(o.x[a] + o.y[bb] + o.z[c])))
+#define WR(a, bb, c) ((T)ldu(W, g.n, mu + (unsigned)(o.x[a] + o.y[bb] + o.z[c])))
     // the boundary rules (U_kabe, V_kabe, W_kabe): U, W ghosts -v below and
     // 2*0 - v above; V zero at a ghost row and at the wall face j = ny-1
 #define UF(a, bb, c) ((bb) == 0 && wlo ? -UR(a, 1, c) : ((bb) == 2 && whi ? T(2) * T(0) - UR(a, 1, c) : UR(a, bb, c)))
@@ -780,7 +822,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   const T L2 = (T)(g.lp * g.lp);
   T x[6];
 #pragma unroll
-  for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
+  for (int q = 0; q < 6; q++) x[q] = ldu(C + q * g.n, g.n, mu);
   int it;
   // Chord iterations: the Jacobian of the first iterate (the linear operator
   // plus the rank-one Peterlin term) is factored once, its multipliers kept
@@ -852,15 +894,16 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   }
   if (it > itrmax) it = itrmax;
 #pragma unroll
-  for (int q = 0; q < 6; q++) Cn[q * g.n + m] = (float)x[q];
-  fC[m] = (float)peterlin_t<T>(L2, x[0] + x[3] + x[5]);
-  nitr[m] = it;
+  for (int q = 0; q < 6; q++) stu(Cn + q * g.n, g.n, mu, (float)x[q]);
+  stu(fC, g.n, mu, (float)peterlin_t<T>(L2, x[0] + x[3] + x[5]));
+  stu(nitr, g.n, mu, it);
 }
 
 // one instance over every cell (the wall rows through the boundary-rule
-// patches in cstar_newton_cell and load_line), in band order; held to 128
-// registers (four waves per SIMD), where the patches would take it to 130
-// and three waves (180 us instead of 163 at 256x128x128)
+// patches in cstar_newton_cell and load_line), in band order; held to at
+// least four waves per SIMD (128 registers): with 64-bit addresses the
+// patches took it to 130 registers and three waves (180 us instead of 163 at
+// 256x128x128); the buffer-resource loads leave it at 98
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
                                                       const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, float* __restrict__ Cn,

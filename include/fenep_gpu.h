@@ -92,7 +92,9 @@ typedef struct fp_diag {
 int fp_default_params(fp_params* p);
 
 /* Replaces the module-level state and plan set-up: fft_init (fenep.f90:1374),
- * the allocations of program main (fenep.f90:2424-2446). */
+ * the allocations of program main (fenep.f90:2424-2446).  FP_E_ARG for axes
+ * that are not powers of two up to 1024, and for the mixed precision with
+ * the explicit method, SOR, or 2^30 cells. */
 int fp_create(const fp_params* p, void** ctx);
 int fp_destroy(void* ctx);
 const char* fp_last_error(void* ctx);

@@ -140,6 +140,10 @@ def test_mixed_rejects_unsupported_combinations():
         p = default_params(nx=8, ny=8, nz=8, precision=PREC_MIXED, **kw)
         with pytest.raises(api.FenepError, match="FP_E_ARG"):
             api.FenepGPU(p)
+    # 2^30 cells: beyond the 32-bit plane offsets (rejected before allocating)
+    p = default_params(nx=1024, ny=1024, nz=1024, precision=PREC_MIXED)
+    with pytest.raises(api.FenepError, match="FP_E_ARG"):
+        api.FenepGPU(p)
 
 
 def test_mixed_full_size_short_run_is_finite_and_close():
