@@ -1323,6 +1323,9 @@ __device__ __forceinline__ void load_twiddles(float2* tws, const float2* __restr
 // Half-spectrum element (kx, row, kz) in the x-major layout (kx fastest) or
 // the transposed layout of the slab solve (row fastest, then kz, then kx: one
 // kx slab is contiguous).
+// x-line L = j + ny k -> its row j and plane k (ny is a power of two)
+__device__ __forceinline__ long line_row(const Grid& g, long L) { return L & (long)(g.ny - 1); }
+__device__ __forceinline__ long line_plane(const Grid& g, long L) { return L >> g.lny; }
 __device__ __forceinline__ long hs_index(const Grid& g, int tr, long k, long row, long kk) {
   return tr ? row + (long)g.sy * (kk + (long)g.nz * k) : k + (long)g.nxh * (row + (long)g.sy * kk);
 }
@@ -1342,9 +1345,9 @@ __device__ __forceinline__ void store_half_spectra(const Grid& g, const float2* 
     const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
     const long La = l0 + 2 * c, Lb = La + 1;
     if (La < nlines)
-      spec[hs_index(g, tr, k, La % g.ny + g.jo, La / g.ny)] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
+      spec[hs_index(g, tr, k, line_row(g, La) + g.jo, line_plane(g, La))] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
     if (Lb < nlines)
-      spec[hs_index(g, tr, k, Lb % g.ny + g.jo, Lb / g.ny)] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
+      spec[hs_index(g, tr, k, line_row(g, Lb) + g.jo, line_plane(g, Lb))] = make_float2(0.5f * (Z.y + W.y), 0.5f * (W.x - Z.x));
   }
 }
 
@@ -1364,7 +1367,7 @@ __global__ __launch_bounds__(256) void m_fft_x_fwd(Grid g, const float* __restri
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
-    const int c = e / n, i = e - c * n;
+    const int c = e >> g.lnx, i = e & (n - 1);   // n = nx, a power of two
     const long La = l0 + 2 * c, Lb = La + 1;
     x[e] = make_float2(La < nlines ? q[La * n + i] : 0.0f, Lb < nlines ? q[Lb * n + i] : 0.0f);
   }
@@ -1387,12 +1390,12 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
-    const int c = e / n, i = e - c * n;
+    const int c = e >> g.lnx, i = e & (n - 1);   // n = nx, a power of two
     const long La = l0 + 2 * c, Lb = La + 1;
     const int src = i <= h ? i : n - i;
     float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
-    if (La < nlines) A = spec[src + (long)nxh * ((La % g.ny + g.jo) + g.sy * (La / g.ny))];
-    if (Lb < nlines) B = spec[src + (long)nxh * ((Lb % g.ny + g.jo) + g.sy * (Lb / g.ny))];
+    if (La < nlines) A = spec[src + (long)nxh * ((line_row(g, La) + g.jo) + g.sy * (line_plane(g, La)))];
+    if (Lb < nlines) B = spec[src + (long)nxh * ((line_row(g, Lb) + g.jo) + g.sy * (line_plane(g, Lb)))];
     if (src == 0 || src == h) { A.y = 0.0f; B.y = 0.0f; }
     if (i > h) { A.y = -A.y; B.y = -B.y; }
     x[e] = make_float2(A.x - B.y, A.y + B.x);
@@ -1400,7 +1403,7 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
   __syncthreads();
   const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, true);
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
-    const int c = e / n, i = e - c * n;
+    const int c = e >> g.lnx, i = e & (n - 1);   // n = nx, a power of two
     const long La = l0 + 2 * c, Lb = La + 1;
     const float2 z = r[e];
     if (La < nlines) out[La * n + i] = z.x;
@@ -1432,13 +1435,13 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
   const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, idz = 1.0 / g.dz, idt = 1.0 / g.dt;
 #pragma unroll 2
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
-    const int c = e / n, i = e - c * n;
+    const int c = e >> g.lnx, i = e & (n - 1);   // n = nx, a power of two
     float2 v2 = make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const long L = l0 + 2 * c + h;
       if (L >= nlines) continue;
-      const int j = (int)(L % g.ny), k = (int)(L / g.ny);
+      const int j = (int)(line_row(g, L)), k = (int)(line_plane(g, L));
       const double v = (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) * idx +
                         ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) * idy +
                         ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) * idz) * idt;
@@ -1484,12 +1487,12 @@ __global__ __launch_bounds__(256) void m_fft_x_inv_accum(Grid g, const float2* _
     for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
       const int k = e / nl, jl = e - k * nl;
       const long L = l0 + jl;
-      hs[jl * nxh + k] = L < nlines ? spec[hs_index(g, 1, k, L % g.ny + g.jo, L / g.ny)] : make_float2(0.0f, 0.0f);
+      hs[jl * nxh + k] = L < nlines ? spec[hs_index(g, 1, k, line_row(g, L) + g.jo, line_plane(g, L))] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
   }
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
-    const int c = e / n, i = e - c * n;
+    const int c = e >> g.lnx, i = e & (n - 1);   // n = nx, a power of two
     const long La = l0 + 2 * c, Lb = La + 1;
     const int src = i <= h ? i : n - i;
     float2 A = make_float2(0.0f, 0.0f), B = make_float2(0.0f, 0.0f);
@@ -1497,8 +1500,8 @@ __global__ __launch_bounds__(256) void m_fft_x_inv_accum(Grid g, const float2* _
       A = hs[(2 * c) * nxh + src];
       B = hs[(2 * c + 1) * nxh + src];
     } else {
-      if (La < nlines) A = spec[src + (long)nxh * ((La % g.ny + g.jo) + g.sy * (La / g.ny))];
-      if (Lb < nlines) B = spec[src + (long)nxh * ((Lb % g.ny + g.jo) + g.sy * (Lb / g.ny))];
+      if (La < nlines) A = spec[src + (long)nxh * ((line_row(g, La) + g.jo) + g.sy * (line_plane(g, La)))];
+      if (Lb < nlines) B = spec[src + (long)nxh * ((line_row(g, Lb) + g.jo) + g.sy * (line_plane(g, Lb)))];
     }
     if (src == 0 || src == h) { A.y = 0.0f; B.y = 0.0f; }
     if (i > h) { A.y = -A.y; B.y = -B.y; }
@@ -1507,7 +1510,7 @@ __global__ __launch_bounds__(256) void m_fft_x_inv_accum(Grid g, const float2* _
   __syncthreads();
   const float2* r = stockham_f(x, y, n, ilog2f(n), nc, n, tws, true);
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
-    const int c = e / n, i = e - c * n;
+    const int c = e >> g.lnx, i = e & (n - 1);   // n = nx, a power of two
     const long La = l0 + 2 * c, Lb = La + 1;
     const float2 z = r[e];
     if (La < nlines) phi[La * n + i] = first ? (double)z.x : phi[La * n + i] + (double)z.x;
@@ -1652,7 +1655,7 @@ __global__ __launch_bounds__(1024) void m_poisson_slab(Grid g, float2* __restric
     for (int u = 0; u < LU; u++) {
       const int e = e0 + 1024 * u + (int)threadIdx.x;
       if (e < ny * nz) {
-        const int kk = e / ny, j = e - kk * ny;
+        const int kk = e >> g.lny, j = e & (ny - 1);
         v[u] = slab[(size_t)sy * kk + j + 1];
       }
     }
@@ -1660,7 +1663,7 @@ __global__ __launch_bounds__(1024) void m_poisson_slab(Grid g, float2* __restric
     for (int u = 0; u < LU; u++) {
       const int e = e0 + 1024 * u + (int)threadIdx.x;
       if (e < ny * nz) {
-        const int kk = e / ny, j = e - kk * ny;
+        const int kk = e >> g.lny, j = e & (ny - 1);
         S[(j + 1) * pitch + kk] = v[u];
       }
     }
@@ -1748,7 +1751,7 @@ __global__ __launch_bounds__(1024) void m_poisson_slab(Grid g, float2* __restric
     if (pass == 1) break;
   }
   for (int e = threadIdx.x; e < ny * nz; e += blockDim.x) {
-    const int kk = e / ny, j = e - kk * ny;
+    const int kk = e >> g.lny, j = e & (ny - 1);
     slab[(size_t)sy * kk + j + 1] = S[(j + 1) * pitch + kk];
   }
 }
