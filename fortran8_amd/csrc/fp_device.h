@@ -15,23 +15,24 @@ __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >
 // NB=NY, Uwall=0).
 template <bool WALLS, typename T>
 __device__ __forceinline__ T ldbw(const T* __restrict__ A, const Grid& g, int i, int j, int k, int kind) {
+  // cell indices by shifts (power-of-two axes, n <= 2^30 cells)
   i = wrapi(i, g.nx);
   k = wrapi(k, g.nz);
-  const long plane = (long)g.nx * g.ny;
+  const int lp = g.lnx + g.lny;
   if constexpr (!WALLS) {
     j = wrapi(j, g.ny);
-    return A[i + (long)g.nx * j + plane * k];
+    return A[(unsigned)(i + (j << g.lnx) + (k << lp))];
   } else {
     // y-walls (U_kabe .. C_kabe, fenep_2d.f90:1098-1143), without branches
     // so that a caller's loads can be issued together: every rule is one
     // load of the nearest interior row, negated or replaced by zero, or (C)
     // the linear extrapolation from the two nearest rows
-    const T* col = A + i + plane * k;
-    const int nx = g.nx, ny = g.ny;
+    const T* col = A + (unsigned)(i + (k << lp));
+    const int ny = g.ny;
     const bool lo = j < 0, hi = j >= ny, ghost = lo || hi;
     const int jc = lo ? 0 : (hi ? ny - 1 : j);
     if (kind == BC_U || kind == BC_V || kind == BC_W || kind == BC_P) {
-      const T v = col[(long)nx * jc];
+      const T v = col[(unsigned)(jc << g.lnx)];
       // U, W ghosts: -U_1 below, 2*0 - U_NY above (the reference's forms,
       // which differ for a zero in the sign of the result)
       if (kind == BC_U || kind == BC_W) return lo ? -v : (hi ? T(2) * T(0) - v : v);
@@ -39,7 +40,7 @@ __device__ __forceinline__ T ldbw(const T* __restrict__ A, const Grid& g, int i,
       return zero ? T(0) : v;
     }
     const int j2 = lo ? 1 : (hi ? ny - 2 : jc);
-    const T a = col[(long)nx * jc], b = col[(long)nx * j2];
+    const T a = col[(unsigned)(jc << g.lnx)], b = col[(unsigned)(j2 << g.lnx)];
     return ghost ? T(2) * a - b : a;
   }
 }
@@ -80,7 +81,7 @@ __device__ __forceinline__ void cell_ijk(const Grid& g, long m, int& i, int& j, 
 }
 
 __device__ __forceinline__ long cidx(const Grid& g, int i, int j, int k) {
-  return i + (long)g.nx * (j + (long)g.ny * k);
+  return (long)(unsigned)(i + (j << g.lnx) + (k << (g.lnx + g.lny)));
 }
 
 // Band order of the cells of a stencil kernel: x fastest, then the 16
