@@ -1340,7 +1340,7 @@ __device__ __forceinline__ void store_half_spectra(const Grid& g, const float2* 
   const long nlines = (long)g.ny * g.nz;
   for (int e = threadIdx.x; e < nc * nxh; e += blockDim.x) {
     int c, k;
-    if (tr) { k = e / nc; c = e - k * nc; }
+    if (tr) { k = e >> __builtin_ctz(nc); c = e & (nc - 1); }   // nl: 4, 8 or 16 (lines_for32)
     else { c = e / nxh; k = e - c * nxh; }
     const float2 Z = r[c * n + k], W = r[c * n + ((n - k) & (n - 1))];
     const long La = l0 + 2 * c, Lb = La + 1;
@@ -1485,7 +1485,7 @@ __global__ __launch_bounds__(256) void m_fft_x_inv_accum(Grid g, const float2* _
     // unrolled loop keeps several loads of a thread in flight
 #pragma unroll 4
     for (int e = threadIdx.x; e < nl * nxh; e += blockDim.x) {
-      const int k = e / nl, jl = e - k * nl;
+      const int k = e >> __builtin_ctz(nl), jl = e & (nl - 1);   // nl: 4, 8 or 16 (lines_for32)
       const long L = l0 + jl;
       hs[jl * nxh + k] = L < nlines ? spec[hs_index(g, 1, k, line_row(g, L) + g.jo, line_plane(g, L))] : make_float2(0.0f, 0.0f);
     }
@@ -1837,7 +1837,8 @@ __global__ __launch_bounds__(256) void m_thomas(Grid g, float2* __restrict__ spe
 }
 
 // real x-lines per block (even: the x transforms pack two real lines per
-// complex line); LDS 8 B * n * lines + n/2 twiddles
+// complex line); LDS 8 B * n * lines + n/2 twiddles.  A power of two for the
+// power-of-two n (4, 8 or 16): the spectrum stages index lines by shifts.
 static int lines_for32(int n) { return std::max(2, std::min(16, 4096 / n)) & ~1; }
 // kx columns per block of the y/z transforms (pitch n + 1)
 static int cols_for32(int n) { return std::max(1, std::min(16, 3968 / (n + 1))); }
