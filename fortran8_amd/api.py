@@ -34,12 +34,25 @@ def header_exports():
     return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fp_\w+)\s*\(", txt, re.M)))
 
 
+def build_id():
+    """Source hash compiled into the loaded library (equal to
+    fortran8_amd.build.source_hash() of this tree, or lib() refuses it)."""
+    return lib().fp_build_id().decode()
+
+
 def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise FenepError("HIP library %s is not built; run fortran8_amd/build.py" % LIB_PATH)
         L = C.CDLL(LIB_PATH)
+        L.fp_build_id.restype = C.c_char_p
+        L.fp_build_id.argtypes = []
+        from .build import source_hash
+        built, tree = L.fp_build_id().decode(), source_hash()
+        if built != tree:
+            raise FenepError("%s was built from other sources (build id %s, sources %s); rebuild it with "
+                             "fortran8_amd/build.py" % (LIB_PATH, built, tree))
         L.fp_default_params.argtypes = [C.POINTER(FpParams)]
         L.fp_create.argtypes = [C.POINTER(FpParams), C.POINTER(_vp)]
         L.fp_destroy.argtypes = [_vp]

@@ -7,6 +7,7 @@ mixed-precision and adjoint units contract freely except inside functions
 that say `#pragma clang fp contract(off)` (the reference formulas they share
 with the parity path): plain -ffp-contract=fast would fuse those as well.
 """
+import hashlib
 import os
 import subprocess
 import sys
@@ -28,6 +29,22 @@ UNITS = [
 ]
 
 
+def source_hash():
+    """SHA-256 (first 16 hex digits) over the HIP sources, the C-ABI header and
+    the compile flags.  It is compiled into the library (fp_build_id) and
+    compared by the loader, so a library built from other sources is refused."""
+    h = hashlib.sha256()
+    files = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".h")))
+    paths = [os.path.join(CSRC, f) for f in files]
+    paths.append(os.path.join(os.path.dirname(HERE), "include", "fenep_gpu.h"))
+    for path in paths:
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as fh:
+            h.update(fh.read())
+    h.update(repr((COMMON, UNITS)).encode())
+    return h.hexdigest()[:16]
+
+
 def hipcc():
     for c in ("/opt/rocm/bin/hipcc", "hipcc"):
         if os.path.exists(c) or c == "hipcc":
@@ -36,12 +53,13 @@ def hipcc():
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
+    tag = ["-DFP_SOURCE_HASH=\"%s\"" % source_hash()]
     procs = []
     objs = []
     for src, extra in UNITS:
         o = os.path.join(OBJ, src.replace(".hip", ".o"))
         objs.append(o)
-        cmd = [hipcc()] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", o]
+        cmd = [hipcc()] + COMMON + extra + tag + ["-c", os.path.join(CSRC, src), "-o", o]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

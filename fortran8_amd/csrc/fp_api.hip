@@ -50,6 +50,13 @@ std::vector<double> modwav(int n, double h, int count) {
   return k2;
 }
 
+// Setters replace state that an fp_step(sync = 0) still in flight reads:
+// join the selection stream and drain the context's stream first.
+void drain(Ctx& c) {
+  if (c.mixed) mixed_join(c);
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
 int fail(Ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -273,6 +280,7 @@ int fp_set_state(void* vc, const double* U, const double* V, const double* W, co
   Ctx* c = (Ctx*)vc;
   if (!c || !U || !V || !W || !P || !C6) return fail(c, FP_E_ARG, "null argument");
   try {
+    drain(*c);
     const Grid& g = c->g;
     const long n = g.n;
     std::vector<double> buf(6 * n);
@@ -365,6 +373,7 @@ int fp_set_body_force(void* vc, const double* fx, const double* fy, const double
   Ctx* c = (Ctx*)vc;
   if (!c) return FP_E_ARG;
   try {
+    drain(*c);
     const long n = c->g.n;
     if (!fx && !fy && !fz) { dfree(c->F); if (c->mixed) mixed_set_force(*c); return FP_OK; }
     if (!c->F) dalloc(&c->F, 3 * n);
@@ -385,6 +394,7 @@ int fp_set_markers(void* vc, int nc, int nl, const double* xc, const double* yc,
   Ctx* c = (Ctx*)vc;
   if (!c || nc <= 0 || nl <= 0 || !xc || !yc || !zc || !uc || !vc_ || !wc) return fail(c, FP_E_ARG, "bad markers");
   try {
+    drain(*c);
     const Grid& g = c->g;
     long nmk = (long)nc * nl * g.nz;
     // every stencil must stay within one ghost layer (and off the walls)
@@ -478,6 +488,7 @@ int fp_set_state_device(void* vc, const double* U, const double* V, const double
   Ctx* c = (Ctx*)vc;
   if (!c || !U || !V || !W || !P || !C6) return FP_E_ARG;
   try {
+    drain(*c);
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const long n = c->g.n;
     FP_HIP_CHECK(hipMemcpyAsync(c->U, U, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -543,6 +554,11 @@ int fp_kernel_times(void* vc, double* ms, int max_n, char* names, int names_len)
   }
   return k;
 }
+
+#ifndef FP_SOURCE_HASH
+#define FP_SOURCE_HASH "unknown"
+#endif
+const char* fp_build_id(void) { return FP_SOURCE_HASH; }
 
 int fp_seeded_uniform(uint64_t seed, uint64_t offset, size_t n, double* out) {
   if (!out) return FP_E_ARG;
@@ -610,6 +626,11 @@ int fp_set_physics(void* vc, double wi, double beta, double lp) {
   if (c->mixed) return fail(c, FP_E_ARG, "fp_set_physics runs on the FP64 path");
   if (!(wi > 0.0) || !(beta > 0.0 && beta <= 1.0) || !(lp * lp > 3.0))
     return fail(c, FP_E_ARG, "need Wi > 0, 0 < beta <= 1 and Lp^2 > 3");
+  try {
+    drain(*c);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
   c->p.wi = wi; c->p.beta = beta; c->p.lp = lp;
   c->g.wi = wi; c->g.beta = beta; c->g.lp = lp;
   return FP_OK;

@@ -100,6 +100,9 @@ int fp_destroy(void* ctx);
 const char* fp_last_error(void* ctx);
 
 /* Replaces init/input (fenep.f90:52-145, 1186-1213): copy a full state in.
+ * Every setter (fp_set_state, fp_set_state_device, fp_set_body_force,
+ * fp_set_markers, fp_set_physics) first waits for steps still in flight
+ * after fp_step(..., sync = 0).
  * hist may be NULL; otherwise it holds 9 interior arrays A0x,A0y,A0z,T0x,T0y,
  * T0z,B0x,B0y,B0z.  has_history=0 reproduces the step-1 rule of fft_navier /
  * ibm_preliminary / navier (fenep.f90:1445, 1985, 787): the first step copies
@@ -207,6 +210,13 @@ int fp_adj_record(void* ctx, int nsteps, int every, int replay, double* drag_out
  * state after the recording. */
 int fp_adj_backward(void* ctx, const double* weights, const double* bar_final,
                     double* bar_initial, double* bar_params, double* bar_force);
+
+/* Identity of the sources the library was built from: 16 hex digits of a
+ * SHA-256 over the HIP sources, this header and the compile flags.  No
+ * counterpart in the reference (it is compiled in one piece with the program);
+ * host bindings compare it with the tree they run from and refuse a library
+ * built from other sources. */
+const char* fp_build_id(void);
 
 /* Deterministic seeded uniform numbers in [0,1): SplitMix64 of
  * (seed, offset+i), top 53 bits.  Host-side utility so that the Fortran

@@ -27,7 +27,9 @@ def build(force=False):
 def lib():
     global _LIB
     if _LIB is None:
-        so = os.path.join(_HERE, "liboracle.so")
+        # FENEP_ORACLE_LIB: a library built from a modified copy of the
+        # restatement (tests/test_oracle_mutations.py)
+        so = os.environ.get("FENEP_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(so):
             build()
         L = C.CDLL(so)

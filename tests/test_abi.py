@@ -68,3 +68,9 @@ def test_oracle_is_not_linked_by_the_product():
     assert "oracle" not in out
     syms = subprocess.run(["nm", "-D", api.LIB_PATH], capture_output=True, text=True).stdout
     assert "orc_" not in syms
+
+
+def test_library_matches_the_sources_of_this_tree():
+    from fortran8_amd import build
+    assert api.build_id() == build.source_hash()
+    assert len(api.build_id()) == 16
