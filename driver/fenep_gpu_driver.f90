@@ -13,6 +13,16 @@
 !                   (fenep_mpi.f90:2200-2348, read by _energy.py)
 !   Log/energy_same_<step>.d  shell spectrum averaged over the Tstep records
 !                   of each Gstep period (fft_energy_dif, fenep_mpi.f90:2566-2665)
+!   Log/time_same_mask.d, Map/same_<step>.bin  masked time series and the
+!                   18-column (x, y) map of all_time_dif_map (fenep_mpi.f90:2350-2564)
+!   Log/energy_same_mask_<step>.d, Log/energy_same_z_<step>.d  masked z
+!                   spectra of fft_energy_dif_map (fenep_mpi.f90:2670-2797)
+!   the same records with 'diff' in place of 'same' on the second pass of
+!                   input_type = 2, the time mean subtracted (fenep_mpi.f90:2999-3067)
+!   ibm/<cylinder>.d  marker outline of each cylinder at k = NZ/2 (ibm_get,
+!                   fenep.f90:1902-1917, read by ibm/ibm12/ibm3.py)
+!   {z,y,all}_<step>.d  text maps, format (18e12.4) (get_data_xy, get_data_xz,
+!                   get_data, fenep.f90:982-1106; _map.py form 'd'), map_text = 1
 !
 ! Usage: fenep_gpu [namelist-file]      (default: fenep.nml)
 module fenep_gpu_api
@@ -60,6 +70,33 @@ module fenep_gpu_api
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: e(*)
       integer(c_int), value :: nbins
+    end function
+    integer(c_int) function fp_time_stats_map(ctx, ncyl, xy, diameter, map18, out13) &
+        bind(c, name='fp_time_stats_map')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncyl
+      real(c_double), intent(in) :: xy(*)
+      real(c_double), value :: diameter
+      real(c_double), intent(out) :: map18(*), out13(13)
+    end function
+    integer(c_int) function fp_energy_spectrum_z(ctx, ncyl, xy, diameter, emask, efirst) &
+        bind(c, name='fp_energy_spectrum_z')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncyl
+      real(c_double), intent(in) :: xy(*)
+      real(c_double), value :: diameter
+      real(c_double), intent(out) :: emask(*), efirst(*)
+    end function
+    integer(c_int) function fp_mean_accumulate(ctx) bind(c, name='fp_mean_accumulate')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function fp_mean_apply(ctx, mode) bind(c, name='fp_mean_apply')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
     end function
     integer(c_int) function fp_destroy(ctx) bind(c, name='fp_destroy')
       import :: c_int, c_ptr
@@ -154,11 +191,12 @@ program fenep_gpu
   real(8) :: DC = 0.5d0, xcyl(64) = 0.0d0, ycyl(64) = 0.0d0
   integer :: newton_itrmax = 10000, sor_itrmax = 10000, n_dc_max = 2
   real(8) :: newton_eps = 1.0d-6, sor_eps = 1.0d-5
+  integer :: map_text = 0
   character(256) :: dir = './'
   namelist /run/ NX, NY, NZ, Lx, Ly, Lz, dt, Re, beta, Wi, Lp, method, walls_y, poisson_variant, &
                  precision, Nstep, Lstep, Gstep, Ostep, koide_step, Tstep, flow_type, input_type, seed, &
                  force_amp, body_force, ncyl, DC, xcyl, ycyl, newton_itrmax, sor_itrmax, n_dc_max, &
-                 newton_eps, sor_eps, dir
+                 newton_eps, sor_eps, map_text, dir
 
   type(fp_params), target :: prm
   type(fp_diag) :: dg
@@ -171,8 +209,10 @@ program fenep_gpu
   real(8) :: dX, dY, dZ
   integer(8) :: t0, t1, trate
   integer(c_int) :: has_hist
-  real(8), allocatable :: Energy(:), Etmp(:)
-  integer :: n_energy = 0
+  real(8), allocatable :: Energy(:), Etmp(:), Ezm(:), Ezf(:), Ezm_t(:), Ezf_t(:)
+  integer :: n_energy = 0, n_energy_z = 0, pass
+  character(4) :: key = 'same'
+  real(c_double), allocatable :: cxy(:)
 
   nml_file = 'fenep.nml'
   if (command_argument_count() >= 1) call get_command_argument(1, nml_file)
@@ -216,34 +256,25 @@ program fenep_gpu
   if (method == 2) call set_markers()
 
   write(*, '(a, 3i6, a, i2, a, i2)') 'grid:', NX, NY, NZ, '  method:', method, '  precision:', precision
+  allocate(cxy(2*max(ncyl, 1)))
+  cxy = 0.0d0
+  do i = 1, ncyl
+    cxy(2*i - 1) = xcyl(i); cxy(2*i) = ycyl(i)
+  enddo
+  if (method == 2) call ibm_get()
   call system_clock(t0, trate)
-  step = 0
-  do while (step < Nstep)
-    chunk = min(Nstep - step, next_event(step) - step)
-    call check(fp_step(ctx, chunk, 1), ctx, 'fp_step')
-    step = step + chunk
-    ! Fortran does not short-circuit .and.: test the period before mod()
-    if (Lstep > 0) then
-      if (mod(step, Lstep) == 0) call log_progress(step)
+  ! input_type = 2 (fenep_mpi.f90:2999-3067): a first pass accumulates the
+  ! time mean of the velocity, the second repeats the run from the same state
+  ! with the statistics of the velocity less that mean (the 'diff' records)
+  do pass = 1, merge(2, 1, input_type == 2)
+    if (pass == 2) then
+      call check(fp_mean_apply(ctx, 1_c_int), ctx, 'fp_mean_apply')
+      key = 'diff'
+      call init_fields()
+      call read_restart()
+      call check(fp_set_state(ctx, U, V, W, P, C, c_loc(hist), 1_c_int), ctx, 'fp_set_state')
     endif
-    if (koide_step > 0) then
-      if (mod(step, koide_step) == 0) call write_koide(step)
-    endif
-    if (Gstep > 0) then
-      if (mod(step, Gstep) == 0) call get_data_binary(step)
-    endif
-    if (Ostep > 0) then
-      if (mod(step, Ostep) == 0) call write_restart()
-    endif
-    if (Tstep > 0) then
-      if (mod(step, Tstep) == 0) then
-        call write_time_stats()
-        if (walls_y == 0) call add_energy()
-      endif
-    endif
-    if (Gstep > 0 .and. Tstep > 0) then
-      if (mod(step, Gstep) == 0 .and. n_energy > 0) call write_energy(step)
-    endif
+    call run_pass()
   enddo
   call system_clock(t1)
   rc = fp_diagnostics(ctx, dg)
@@ -253,6 +284,129 @@ program fenep_gpu
   rc = fp_destroy(ctx)
 
 contains
+
+  subroutine run_pass()
+    integer :: ev
+    step = 0
+    do while (step < Nstep)
+      ev = next_event(step)
+      if (input_type == 2 .and. pass == 1) ev = step + 1   ! the mean takes every step
+      chunk = min(Nstep - step, ev - step)
+      call check(fp_step(ctx, chunk, 1), ctx, 'fp_step')
+      step = step + chunk
+      if (input_type == 2 .and. pass == 1) call check(fp_mean_accumulate(ctx), ctx, 'fp_mean_accumulate')
+      ! Fortran does not short-circuit .and.: test the period before mod()
+      if (Lstep > 0) then
+        if (mod(step, Lstep) == 0) call log_progress(step)
+      endif
+      if (koide_step > 0 .and. pass == 1) then
+        if (mod(step, koide_step) == 0) call write_koide(step)
+      endif
+      if (Gstep > 0 .and. pass == 1) then
+        if (mod(step, Gstep) == 0) then
+          call get_data_binary(step)
+          if (map_text > 0) call get_data_text(step)
+        endif
+      endif
+      if (Ostep > 0 .and. pass == 1) then
+        if (mod(step, Ostep) == 0) call write_restart()
+      endif
+      if (Tstep > 0) then
+        if (mod(step, Tstep) == 0) then
+          call write_time_stats(step)
+          call write_time_stats_map(step)
+          if (walls_y == 0) call add_energy()
+          call add_energy_z()
+        endif
+      endif
+      if (Gstep > 0 .and. Tstep > 0) then
+        if (mod(step, Gstep) == 0 .and. n_energy > 0) call write_energy(step)
+        if (mod(step, Gstep) == 0 .and. n_energy_z > 0) call write_energy_z(step)
+      endif
+    enddo
+  end subroutine run_pass
+
+  subroutine write_time_stats_map(s)
+    ! all_time_dif_map (fenep_mpi.f90:2350-2564): Log/time_<key>_mask.d every
+    ! record, Map/<key>_<step>.bin every Gstep
+    integer, intent(in) :: s
+    integer :: iu, r
+    real(c_double) :: ts(13)
+    real(c_double), allocatable :: mp(:)
+    character(8) :: str
+    allocate(mp(18*NX*NY))
+    call check(fp_time_stats_map(ctx, int(ncyl, c_int), cxy, DC, mp, ts), ctx, 'fp_time_stats_map')
+    call execute_command_line('mkdir -p '//trim(dir)//'Log')
+    open(newunit=iu, file=trim(dir)//'Log/time_'//trim(key)//'_mask.d', position='append')
+    write(iu, '(i10, 12es25.16e3)') s, s*dt, ts(3:13)
+    close(iu)
+    if (Gstep > 0) then
+      if (mod(s, Gstep) == 0) then
+        write(str, '(I8.8)') s
+        open(newunit=iu, file=trim(dir)//'Map/'//trim(key)//'_'//str//'.bin', form='unformatted', &
+             status='replace', access='stream')
+        do r = 0, NX*NY - 1
+          write(iu) mp(18*r + 1:18*r + 18)
+        enddo
+        close(iu)
+      endif
+    endif
+    deallocate(mp)
+  end subroutine write_time_stats_map
+
+  subroutine add_energy_z()
+    ! column z spectra of fft_energy_dif_map, accumulated between Gstep writes
+    if (.not. allocated(Ezm)) then
+      allocate(Ezm(0:NZ-1), Ezf(0:NZ-1), Ezm_t(0:NZ-1), Ezf_t(0:NZ-1))
+      Ezm = 0.0d0; Ezf = 0.0d0
+    endif
+    call check(fp_energy_spectrum_z(ctx, int(ncyl, c_int), cxy, DC, Ezm_t, Ezf_t), ctx, 'fp_energy_spectrum_z')
+    Ezm = Ezm + Ezm_t
+    Ezf = Ezf + Ezf_t
+    n_energy_z = n_energy_z + 1
+  end subroutine add_energy_z
+
+  subroutine write_energy_z(s)
+    ! Log/energy_<key>_mask_<step>.d and Log/energy_<key>_z_<step>.d, rows
+    ! kz = 0..NZ-1, averaged over the accumulated records (fenep_mpi.f90:2768-2794)
+    integer, intent(in) :: s
+    integer :: iu, k
+    character(8) :: str
+    write(str, '(I8.8)') s
+    call execute_command_line('mkdir -p '//trim(dir)//'Log')
+    open(newunit=iu, file=trim(dir)//'Log/energy_'//trim(key)//'_z_'//str//'.d', status='replace')
+    do k = 0, NZ - 1
+      write(iu, '(i8, es25.16e3)') k, Ezf(k)/dble(n_energy_z)
+    enddo
+    close(iu)
+    open(newunit=iu, file=trim(dir)//'Log/energy_'//trim(key)//'_mask_'//str//'.d', status='replace')
+    do k = 0, NZ - 1
+      write(iu, '(i8, es25.16e3)') k, Ezm(k)/dble(n_energy_z)
+    enddo
+    close(iu)
+    Ezm = 0.0d0; Ezf = 0.0d0
+    n_energy_z = 0
+  end subroutine write_energy_z
+
+  subroutine ibm_get()
+    ! the markers of each cylinder in the plane k = NZ/2, closed by the first
+    ! marker (fenep.f90:1902-1917)
+    integer :: iu, l, m, idx, nc
+    character(8) :: str
+    nc = max(ncyl, 1)
+    call execute_command_line('mkdir -p '//trim(dir)//'ibm')
+    do l = 1, nc
+      write(str, '(I8.8)') l
+      open(newunit=iu, file=trim(dir)//'ibm/'//str//'.d', status='replace')
+      do m = 1, nl
+        idx = l + nc*((m - 1) + nl*(NZ/2 - 1))
+        write(iu, '(2e12.4)') Xc(idx), Yc(idx)
+      enddo
+      idx = l + nc*(nl*(NZ/2 - 1))
+      write(iu, '(2e12.4)') Xc(idx), Yc(idx)
+      close(iu)
+    enddo
+  end subroutine ibm_get
 
   integer function next_event(s)
     integer, intent(in) :: s
@@ -370,16 +524,18 @@ contains
     close(iu)
   end subroutine write_koide
 
-  subroutine write_time_stats()
-    ! one record of Log/time_same.d (all_time_dif, fenep_mpi.f90:2342-2345),
-    ! statistics reduced on the device
+  subroutine write_time_stats(s)
+    ! one record of Log/time_<key>.d (all_time_dif, fenep_mpi.f90:2342-2345),
+    ! statistics reduced on the device; step of the pass (it restarts on the
+    ! second pass of input_type = 2, as the reference's loop variable)
+    integer, intent(in) :: s
     integer :: iu
     real(c_double) :: ts(13)
     call check(fp_time_stats(ctx, ts), ctx, 'fp_time_stats')
     call execute_command_line('mkdir -p '//trim(dir)//'Log')
-    open(newunit=iu, file=trim(dir)//'Log/time_same.d', position='append')
+    open(newunit=iu, file=trim(dir)//'Log/time_'//trim(key)//'.d', position='append')
     ! one line per record (list-directed output may wrap long records)
-    write(iu, '(i10, 12es25.16e3)') nint(ts(1)), ts(2:13)
+    write(iu, '(i10, 12es25.16e3)') s, s*dt, ts(3:13)
     close(iu)
   end subroutine write_time_stats
 
@@ -403,7 +559,7 @@ contains
     character(8) :: str
     write(str, '(I8.8)') s
     call execute_command_line('mkdir -p '//trim(dir)//'Log')
-    open(newunit=iu, file=trim(dir)//'Log/energy_same_'//str//'.d', status='replace')
+    open(newunit=iu, file=trim(dir)//'Log/energy_'//trim(key)//'_'//str//'.d', status='replace')
     do i = 0, NX - 1
       write(iu, '(i8, es25.16e3)') i, Energy(i)/dble(n_energy)
     enddo
@@ -412,10 +568,14 @@ contains
     n_energy = 0
   end subroutine write_energy
 
-  subroutine plane_row(iu, i, j, k)
-    ! one 18-column record of get_data_binary (fenep.f90:1121-1146)
-    integer, intent(in) :: iu, i, j, k
-    real(8) :: D(3, 3), Om(3), E(3), trC
+  subroutine cell_values(i, j, k, qti, r)
+    ! the 18 values of a map record at cell (i, j, k) (fenep.f90:996-1017,
+    ! 1121-1146); column 7 holds Qti (get_data, get_data_xy, get_data_xz)
+    ! or |u|^2/2 (get_data_binary)
+    integer, intent(in) :: i, j, k
+    logical, intent(in) :: qti
+    real(8), intent(out) :: r(18)
+    real(8) :: D(3, 3), Om(3), E(3)
     D(1, 1) = (U(i, j, k) - U(i-1, j, k))/dX
     D(1, 2) = (U(i, j+1, k) - U(i, j-1, k) + U(i-1, j+1, k) - U(i-1, j-1, k))/(4*dY)
     D(1, 3) = (U(i, j, k+1) - U(i, j, k-1) + U(i-1, j, k+1) - U(i-1, j, k-1))/(4*dZ)
@@ -431,11 +591,66 @@ contains
     Om(1) = D(3, 2) - D(2, 3)
     Om(2) = D(1, 3) - D(3, 1)
     Om(3) = D(2, 1) - D(1, 2)
-    trC = C(1, i, j, k) + C(4, i, j, k) + C(6, i, j, k)
-    write(iu) (i - 0.5d0)*dX, (j - 0.5d0)*dY, (k - 0.5d0)*dZ, E(1), E(2), E(3), sum(E**2)/2, &
-             Om(1), Om(2), Om(3), sum(Om**2)/2, trC, C(1, i, j, k), C(2, i, j, k), C(3, i, j, k), &
-             C(4, i, j, k), C(5, i, j, k), C(6, i, j, k)
+    r(1) = (i - 0.5d0)*dX; r(2) = (j - 0.5d0)*dY; r(3) = (k - 0.5d0)*dZ
+    r(4:6) = E
+    if (qti) then
+      r(7) = D(2, 2)*D(3, 3) - D(3, 2)*D(2, 3) + D(1, 1)*D(2, 2) - D(2, 1)*D(1, 2) + D(1, 1)*D(3, 3) - D(3, 1)*D(1, 3)
+    else
+      r(7) = sum(E**2)/2
+    endif
+    r(8:10) = Om
+    r(11) = sum(Om**2)/2
+    r(12) = C(1, i, j, k) + C(4, i, j, k) + C(6, i, j, k)
+    r(13:18) = C(1:6, i, j, k)
+  end subroutine cell_values
+
+  subroutine plane_row(iu, i, j, k)
+    ! one 18-column record of get_data_binary (fenep.f90:1121-1146)
+    integer, intent(in) :: iu, i, j, k
+    real(8) :: r(18)
+    call cell_values(i, j, k, .false., r)
+    write(iu) r
   end subroutine plane_row
+
+  subroutine get_data_text(s)
+    ! text maps of get_data_xy (k = NZ/2), get_data_xz (j = NY/2) and, for
+    ! map_text = 2, get_data (every cell), format (18e12.4) (fenep.f90:982-1106)
+    integer, intent(in) :: s
+    character(8) :: str
+    integer :: iu
+    real(8) :: r(18)
+    write(str, '(I8.8)') s
+    open(newunit=iu, file=trim(dir)//'z_'//str//'.d', status='replace')
+    k = NZ/2
+    do j = 1, NY
+      do i = 1, NX
+        call cell_values(i, j, k, .true., r)
+        write(iu, '(18e12.4)') r
+      enddo
+    enddo
+    close(iu)
+    open(newunit=iu, file=trim(dir)//'y_'//str//'.d', status='replace')
+    j = NY/2
+    do k = 1, NZ
+      do i = 1, NX
+        call cell_values(i, j, k, .true., r)
+        write(iu, '(18e12.4)') r
+      enddo
+    enddo
+    close(iu)
+    if (map_text >= 2) then
+      open(newunit=iu, file=trim(dir)//'all_'//str//'.d', status='replace')
+      do k = 1, NZ
+        do j = 1, NY
+          do i = 1, NX
+            call cell_values(i, j, k, .true., r)
+            write(iu, '(18e12.4)') r
+          enddo
+        enddo
+      enddo
+      close(iu)
+    endif
+  end subroutine get_data_text
 
   subroutine get_data_binary(s)
     integer, intent(in) :: s

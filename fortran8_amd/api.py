@@ -75,6 +75,10 @@ def lib():
         L.fp_time_stats.argtypes = [_vp, _dp]
         L.fp_energy_spectrum.argtypes = [_vp, _dp, C.c_int]
         L.fp_check_selection.argtypes = [_vp, _dp]
+        L.fp_time_stats_map.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp, _dp]
+        L.fp_energy_spectrum_z.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp, _dp]
+        L.fp_mean_accumulate.argtypes = [_vp]
+        L.fp_mean_apply.argtypes = [_vp, C.c_int]
         L.fp_adj_record.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
         L.fp_adj_backward.argtypes = [_vp, _dp, _vp, _vp, _dp, _vp]
         _LIB = L
@@ -190,6 +194,33 @@ class FenepGPU:
         out = np.zeros(nb)
         self._check(lib().fp_energy_spectrum(self.h, _ptr(out), nb), "fp_energy_spectrum")
         return out
+
+    def time_stats_map(self, centres=(), diameter=0.0):
+        """all_time_dif_map: (map (NY*NX, 18) in record order, masked 13-value
+        record); centres: (x, y) pairs of the masked cylinders."""
+        xy = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1))
+        m = np.zeros((self.ny * self.nx, 18))
+        out = np.zeros(13)
+        self._check(lib().fp_time_stats_map(self.h, len(xy) // 2, _ptr(xy) if len(xy) else None, float(diameter),
+                                            _ptr(m), _ptr(out)), "fp_time_stats_map")
+        return m, out
+
+    def energy_spectrum_z(self, centres=(), diameter=0.0):
+        """fft_energy_dif_map of the current state: (masked plane average of
+        the column z spectra, spectrum of column (1, 1)), NZ entries each."""
+        xy = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1))
+        em, ef = np.zeros(self.nz), np.zeros(self.nz)
+        self._check(lib().fp_energy_spectrum_z(self.h, len(xy) // 2, _ptr(xy) if len(xy) else None, float(diameter),
+                                               _ptr(em), _ptr(ef)), "fp_energy_spectrum_z")
+        return em, ef
+
+    def mean_accumulate(self):
+        self._check(lib().fp_mean_accumulate(self.h), "fp_mean_accumulate")
+
+    def mean_apply(self, mode):
+        """0: no subtracted mean; 1: the accumulated mean averaged over z;
+        2: the accumulated mean as it is."""
+        self._check(lib().fp_mean_apply(self.h, int(mode)), "fp_mean_apply")
 
     def check_selection(self):
         """Mixed precision: (re-decided in FP64, differing from the reference's

@@ -264,6 +264,7 @@ int fp_destroy(void* vc) {
   dfree(c->spec); dfree(c->thp); dfree(c->red); dfree(c->part);
   dfree(c->twx); dfree(c->twy); dfree(c->twz); dfree(c->kx2); dfree(c->ky2); dfree(c->kz2);
   dfree(c->mk); dfree(c->mf); dfree(c->mb); dfree(c->ftmp);
+  dfree(c->mean_acc); dfree(c->mean_u);
   for (int k = 0; k < 3; k++) { dfree(c->sp_row[k]); dfree(c->sp_tgt[k]); dfree(c->sp_ent[k]); }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -593,6 +594,58 @@ int fp_energy_spectrum(void* vc, double* E, int nbins) {
   try {
     if (c->mixed) mixed_store_to_f64(*c);
     f64_energy_spectrum(*c, E, nbins);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
+int fp_time_stats_map(void* vc, int ncyl, const double* xy, double diameter, double* map18, double* out13) {
+  Ctx* c = (Ctx*)vc;
+  if (!c || ncyl < 0 || (ncyl > 0 && !xy) || !(diameter >= 0.0) || (!map18 && !out13)) return fail(c, FP_E_ARG, "bad arguments");
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  try {
+    if (c->mixed) mixed_store_to_f64(*c);
+    if (f64_time_stats_map(*c, ncyl, xy, diameter, map18, out13)) return fail(c, FP_E_ARG, "the mask covers the whole plane");
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
+int fp_energy_spectrum_z(void* vc, int ncyl, const double* xy, double diameter, double* Emask, double* Efirst) {
+  Ctx* c = (Ctx*)vc;
+  if (!c || ncyl < 0 || (ncyl > 0 && !xy) || !(diameter >= 0.0) || (!Emask && !Efirst)) return fail(c, FP_E_ARG, "bad arguments");
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  try {
+    if (c->mixed) mixed_store_to_f64(*c);
+    if (f64_energy_spectrum_z(*c, ncyl, xy, diameter, Emask, Efirst)) return fail(c, FP_E_ARG, "the mask covers the whole plane");
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
+int fp_mean_accumulate(void* vc) {
+  Ctx* c = (Ctx*)vc;
+  if (!c) return FP_E_ARG;
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  try {
+    const float *u = nullptr, *v = nullptr, *w = nullptr;
+    if (c->mixed) mixed_velocity(*c, &u, &v, &w);
+    f64_mean_accumulate(*c, u, v, w);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
+int fp_mean_apply(void* vc, int mode) {
+  Ctx* c = (Ctx*)vc;
+  if (!c) return FP_E_ARG;
+  if (mode < 0 || mode > 2) return fail(c, FP_E_ARG, "mode is 0, 1 or 2");
+  try {
+    if (f64_mean_apply(*c, mode)) return fail(c, FP_E_STATE, "no accumulated velocity to average");
   } catch (const std::string& e) {
     return fail(c, FP_E_HIP, e);
   }

@@ -79,6 +79,11 @@ struct Ctx {
   const uint8_t* sel_replay = nullptr;
   // recording and adjoint state (fp_adjoint.hip)
   void* adj = nullptr;
+  // statistics: time-mean accumulator of U, V, W and the mean subtracted by
+  // the statistics (3 planes each; nullptr: none)
+  double* mean_acc = nullptr;
+  double* mean_u = nullptr;
+  long mean_count = 0;
 };
 
 // ---- launchers (fp_kernels_f64.hip)
@@ -95,6 +100,10 @@ void f64_zero_c(Ctx& c);
 void f64_diag(Ctx& c, fp_diag* d);
 void f64_time_stats(Ctx& c, double* out13);  // all_time_dif statistics
 void f64_energy_spectrum(Ctx& c, double* E, int nbins);  // shell spectrum (periodic)
+int f64_time_stats_map(Ctx& c, int ncyl, const double* xy, double diameter, double* map18, double* out13);
+int f64_energy_spectrum_z(Ctx& c, int ncyl, const double* xy, double diameter, double* Emask, double* Efirst);
+void f64_mean_accumulate(Ctx& c, const float* u32, const float* v32, const float* w32);
+int f64_mean_apply(Ctx& c, int mode);
 void f64_fill_f(Ctx& c);     // Peterlin factor of the current C
 void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp);  // spread + ghost folds
 int thomas_lines(int ny, int bytes_per_row);  // lines per block of the LDS line solve

@@ -851,48 +851,63 @@ void f64_zero_c(Ctx& c) {
 }
 
 // =============================================== time-series statistics ===
-// Partial sums of all_time_dif (fenep_mpi.f90:2200-2348, no subtracted mean):
-// 0-2 regular-grid velocity squares, 3-5 squared staggered gradients, 6 S:S
-// with S = D + D^T of the 4-point velocity gradient, 7 |u|^2 on the staggered
-// grid, 8 log f(C), 9 tr C.
+// Per-cell terms of all_time_dif / all_time_dif_map (fenep_mpi.f90:2200-2564)
+// for the velocity fluctuation u' = u - u_mean (u_mean = nullptr: none, the
+// 'same' records; otherwise the 'diff' records): 0-2 regular-grid u'^2,
+// 3-5 squared staggered gradients of u', 6 S:S with S = D + D^T of the
+// 4-point gradient of u', 7 |u|^2 on the staggered grid (raw u, the
+// K_staggered term), 8 log f(C), 9 tr C.  The ghosts of u' are those of u
+// minus those of the mean, each by the boundary rules.
 constexpr int TS_N = 10;
+__device__ __forceinline__ void stat_cell(const Grid& g, const double* __restrict__ U, const double* __restrict__ V,
+                                          const double* __restrict__ W, const double* __restrict__ Um,
+                                          const double* __restrict__ C, int i, int j, int k, long m, double* s) {
+  const double dx = g.dx, dy = g.dy, dz = g.dz;
+  const double* Vm = Um ? Um + g.n : nullptr;
+  const double* Wm = Um ? Um + 2 * g.n : nullptr;
+  auto UG = [&](int a, int b, int c) {
+    return ldb(U, g, i + a, j + b, k + c, BC_U) - (Um ? ldb(Um, g, i + a, j + b, k + c, BC_U) : 0.0);
+  };
+  auto VG = [&](int a, int b, int c) {
+    return ldb(V, g, i + a, j + b, k + c, BC_V) - (Vm ? ldb(Vm, g, i + a, j + b, k + c, BC_V) : 0.0);
+  };
+  auto WG = [&](int a, int b, int c) {
+    return ldb(W, g, i + a, j + b, k + c, BC_W) - (Wm ? ldb(Wm, g, i + a, j + b, k + c, BC_W) : 0.0);
+  };
+  const double u0 = UG(0, 0, 0), v0 = VG(0, 0, 0), w0 = WG(0, 0, 0);
+  const double ur = (u0 + UG(-1, 0, 0)) / 2.0, vr = (v0 + VG(0, -1, 0)) / 2.0, wr = (w0 + WG(0, 0, -1)) / 2.0;
+  s[0] = ur * ur; s[1] = vr * vr; s[2] = wr * wr;
+  double D[3][3];
+  D[0][0] = (u0 - UG(-1, 0, 0)) / dx;
+  D[0][1] = (UG(0, 1, 0) - UG(0, -1, 0) + UG(-1, 1, 0) - UG(-1, -1, 0)) / (4 * dy);
+  D[0][2] = (UG(0, 0, 1) - UG(0, 0, -1) + UG(-1, 0, 1) - UG(-1, 0, -1)) / (4 * dz);
+  D[1][0] = (VG(1, 0, 0) - VG(-1, 0, 0) + VG(1, -1, 0) - VG(-1, -1, 0)) / (4 * dx);
+  D[1][1] = (v0 - VG(0, -1, 0)) / dy;
+  D[1][2] = (VG(0, 0, 1) - VG(0, 0, -1) + VG(0, -1, 1) - VG(0, -1, -1)) / (4 * dz);
+  D[2][0] = (WG(1, 0, 0) - WG(-1, 0, 0) + WG(1, 0, -1) - WG(-1, 0, -1)) / (4 * dx);
+  D[2][1] = (WG(0, 1, 0) - WG(0, -1, 0) + WG(0, 1, -1) - WG(0, -1, -1)) / (4 * dy);
+  D[2][2] = (w0 - WG(0, 0, -1)) / dz;
+  s[3] = D[0][0] * D[0][0]; s[4] = D[1][1] * D[1][1]; s[5] = D[2][2] * D[2][2];
+  s[6] = 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) { const double sab = D[a][b] + D[b][a]; s[6] += sab * sab; }
+  s[7] = U[m] * U[m] + V[m] * V[m] + W[m] * W[m];
+  const double t = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m];
+  const double L2 = g.lp * g.lp;
+  s[8] = log((L2 - 3.0) / (L2 - t));
+  s[9] = t;
+}
+
 __global__ __launch_bounds__(256) void k_time_stats(Grid g, const double* __restrict__ U, const double* __restrict__ V,
-                                                    const double* __restrict__ W, const double* __restrict__ C,
-                                                    double* __restrict__ part, int nb) {
+                                                    const double* __restrict__ W, const double* __restrict__ Um,
+                                                    const double* __restrict__ C, double* __restrict__ part, int nb) {
   const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double s[TS_N];
   for (int q = 0; q < TS_N; q++) s[q] = 0.0;
   if (m < g.n) {
     int i, j, k;
     cell_ijk(g, m, i, j, k);
-    const double dx = g.dx, dy = g.dy, dz = g.dz;
-#define UG(a, b, c) ldb(U, g, i + (a), j + (b), k + (c), BC_U)
-#define VG(a, b, c) ldb(V, g, i + (a), j + (b), k + (c), BC_V)
-#define WG(a, b, c) ldb(W, g, i + (a), j + (b), k + (c), BC_W)
-    const double u0 = UG(0, 0, 0), v0 = VG(0, 0, 0), w0 = WG(0, 0, 0);
-    const double ur = (u0 + UG(-1, 0, 0)) / 2.0, vr = (v0 + VG(0, -1, 0)) / 2.0, wr = (w0 + WG(0, 0, -1)) / 2.0;
-    s[0] = ur * ur; s[1] = vr * vr; s[2] = wr * wr;
-    double D[3][3];
-    D[0][0] = (u0 - UG(-1, 0, 0)) / dx;
-    D[0][1] = (UG(0, 1, 0) - UG(0, -1, 0) + UG(-1, 1, 0) - UG(-1, -1, 0)) / (4 * dy);
-    D[0][2] = (UG(0, 0, 1) - UG(0, 0, -1) + UG(-1, 0, 1) - UG(-1, 0, -1)) / (4 * dz);
-    D[1][0] = (VG(1, 0, 0) - VG(-1, 0, 0) + VG(1, -1, 0) - VG(-1, -1, 0)) / (4 * dx);
-    D[1][1] = (v0 - VG(0, -1, 0)) / dy;
-    D[1][2] = (VG(0, 0, 1) - VG(0, 0, -1) + VG(0, -1, 1) - VG(0, -1, -1)) / (4 * dz);
-    D[2][0] = (WG(1, 0, 0) - WG(-1, 0, 0) + WG(1, 0, -1) - WG(-1, 0, -1)) / (4 * dx);
-    D[2][1] = (WG(0, 1, 0) - WG(0, -1, 0) + WG(0, 1, -1) - WG(0, -1, -1)) / (4 * dy);
-    D[2][2] = (w0 - WG(0, 0, -1)) / dz;
-#undef UG
-#undef VG
-#undef WG
-    s[3] = D[0][0] * D[0][0]; s[4] = D[1][1] * D[1][1]; s[5] = D[2][2] * D[2][2];
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) { const double sab = D[a][b] + D[b][a]; s[6] += sab * sab; }
-    s[7] = U[m] * U[m] + V[m] * V[m] + W[m] * W[m];
-    const double t = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m];
-    const double L2 = g.lp * g.lp;
-    s[8] = log((L2 - 3.0) / (L2 - t));
-    s[9] = t;
+    stat_cell(g, U, V, W, Um, C, i, j, k, m, s);
   }
   __shared__ double sh[256];
   for (int q = 0; q < TS_N; q++) {
@@ -900,6 +915,119 @@ __global__ __launch_bounds__(256) void k_time_stats(Grid g, const double* __rest
     if (threadIdx.x == 0) part[(long)q * nb + blockIdx.x] = v;
     __syncthreads();
   }
+}
+
+// all_time_dif_map (fenep_mpi.f90:2350-2500): the per-cell terms summed over
+// z for each (i, j) column, in k order, then the z variance of tr C about the
+// column mean.  col[q * nx*ny + i + nx*j]: q = 0..9 as stat_cell (7 unused),
+// q = 10 the sum of (tr C - column mean)^2.
+constexpr int COL_N = 11;
+__global__ __launch_bounds__(256) void k_column_stats(Grid g, const double* __restrict__ U, const double* __restrict__ V,
+                                                      const double* __restrict__ W, const double* __restrict__ Um,
+                                                      const double* __restrict__ C, double* __restrict__ col) {
+  const long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long nxy = (long)g.nx * g.ny;
+  if (ij >= nxy) return;
+  const int i = (int)(ij & (g.nx - 1)), j = (int)(ij >> g.lnx);
+  double acc[TS_N], s[TS_N];
+  for (int q = 0; q < TS_N; q++) acc[q] = 0.0;
+  for (int k = 0; k < g.nz; k++) {
+    const long m = ij + nxy * k;
+    stat_cell(g, U, V, W, Um, C, i, j, k, m, s);
+    for (int q = 0; q < TS_N; q++) acc[q] += s[q];
+  }
+  const double mean = acc[9] / (double)g.nz;
+  double var = 0.0;
+  for (int k = 0; k < g.nz; k++) {
+    const long m = ij + nxy * k;
+    const double t = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m] - mean;
+    var += t * t;
+  }
+  for (int q = 0; q < TS_N; q++) col[q * nxy + ij] = acc[q];
+  col[TS_N * nxy + ij] = var;
+}
+
+// fft_energy_dif_map (fenep_mpi.f90:2670-2797): the z transform of the
+// regular-grid fluctuation velocity of every (i, j) column, normalised by
+// 1/NZ (fft_z, fenep_mpi.f90:1442-1463), as a direct sum over z; energy
+// (|u^|^2 + |v^|^2 + |w^|^2)/2 per |kz| = 0..NZ/2, doubled for 0 < kz < NZ/2
+// (the +-kz pair).  E[kz * nx*ny + i + nx*j].
+__global__ __launch_bounds__(256) void k_zspec_columns(Grid g, const double* __restrict__ U, const double* __restrict__ V,
+                                                       const double* __restrict__ W, const double* __restrict__ Um,
+                                                       double* __restrict__ E) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long nxy = (long)g.nx * g.ny;
+  const int nk = g.nz / 2 + 1;
+  if (t >= nxy * nk) return;
+  const long ij = t % nxy;
+  const int kz = (int)(t / nxy);
+  const int i = (int)(ij & (g.nx - 1)), j = (int)(ij >> g.lnx);
+  const double* Vm = Um ? Um + g.n : nullptr;
+  const double* Wm = Um ? Um + 2 * g.n : nullptr;
+  double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+  for (int k = 0; k < g.nz; k++) {
+    const double ur = ((ldb(U, g, i, j, k, BC_U) - (Um ? ldb(Um, g, i, j, k, BC_U) : 0.0))
+                       + (ldb(U, g, i - 1, j, k, BC_U) - (Um ? ldb(Um, g, i - 1, j, k, BC_U) : 0.0))) / 2.0;
+    const double vr = ((ldb(V, g, i, j, k, BC_V) - (Vm ? ldb(Vm, g, i, j, k, BC_V) : 0.0))
+                       + (ldb(V, g, i, j - 1, k, BC_V) - (Vm ? ldb(Vm, g, i, j - 1, k, BC_V) : 0.0))) / 2.0;
+    const double wr = ((ldb(W, g, i, j, k, BC_W) - (Wm ? ldb(Wm, g, i, j, k, BC_W) : 0.0))
+                       + (ldb(W, g, i, j, k - 1, BC_W) - (Wm ? ldb(Wm, g, i, j, k - 1, BC_W) : 0.0))) / 2.0;
+    // e^{-2 pi i kz k / NZ}; the product kz*k is reduced mod NZ first
+    const double a = 2.0 * (double)((long)kz * k & (g.nz - 1)) / (double)g.nz;
+    double sn, cs;
+    sincospi(a, &sn, &cs);
+    re[0] += ur * cs; im[0] -= ur * sn;
+    re[1] += vr * cs; im[1] -= vr * sn;
+    re[2] += wr * cs; im[2] -= wr * sn;
+  }
+  const double nz = (double)g.nz;
+  double e = 0.0;
+  for (int q = 0; q < 3; q++) e += (re[q] / nz) * (re[q] / nz) + (im[q] / nz) * (im[q] / nz);
+  E[t] = (kz == 0 || kz == g.nz / 2) ? e / 2.0 : e;
+}
+
+// masked average over the (i, j) plane of each kz row: sum(E * mask)/sum(mask)
+__global__ __launch_bounds__(256) void k_masked_rows(long nxy, const double* __restrict__ E,
+                                                     const double* __restrict__ mask, double msum,
+                                                     double* __restrict__ out) {
+  const double* row = E + nxy * blockIdx.x;
+  double v = 0.0;
+  for (long ij = threadIdx.x; ij < nxy; ij += blockDim.x) v += row[ij] * mask[ij];
+  __shared__ double sh[256];
+  v = block_sum256(v, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = v / msum;
+}
+
+// time-mean accumulation of the velocity (fenep_mpi.f90:2969-2971)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mean_acc(long n, const T* __restrict__ U, const T* __restrict__ V,
+                                                  const T* __restrict__ W, double* __restrict__ acc) {
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  acc[m] += (double)U[m];
+  acc[n + m] += (double)V[m];
+  acc[2 * n + m] += (double)W[m];
+}
+
+// mean = acc / count, then (zavg) its z average for every (i, j) (ave_mean,
+// fenep_mpi.f90:2010-2033), in k order
+__global__ __launch_bounds__(256) void k_mean_finish(Grid g, const double* __restrict__ acc, double count, int zavg,
+                                                     double* __restrict__ mean) {
+  const long nxy = (long)g.nx * g.ny;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 3 * nxy) return;
+  const int comp = (int)(t / nxy);
+  const long ij = t % nxy;
+  const double* a = acc + comp * g.n;
+  double* o = mean + comp * g.n;
+  if (!zavg) {
+    for (int k = 0; k < g.nz; k++) o[ij + nxy * k] = a[ij + nxy * k] / count;
+    return;
+  }
+  double s = 0.0;
+  for (int k = 0; k < g.nz; k++) s += a[ij + nxy * k] / count;
+  s = s / (double)g.nz;
+  for (int k = 0; k < g.nz; k++) o[ij + nxy * k] = s;
 }
 
 // sum of (tr C - mean)^2, per block
@@ -926,7 +1054,7 @@ void f64_time_stats(Ctx& c, double* out) {
   FP_HIP_CHECK(hipMalloc((void**)&red, sizeof(double) * (TS_N + 2)));
   double r[TS_N + 2];
   try {
-    hipLaunchKernelGGL(k_time_stats, dim3(nb), dim3(256), 0, s, g, c.U, c.V, c.W, c.C[c.cur], part, nb);
+    hipLaunchKernelGGL(k_time_stats, dim3(nb), dim3(256), 0, s, g, c.U, c.V, c.W, c.mean_u, c.C[c.cur], part, nb);
     for (int q = 0; q < TS_N; q++)
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, part + (long)q * nb, nb, red + q, 0);
     FP_HIP_CHECK(hipMemcpyAsync(r, red, sizeof(double) * TS_N, hipMemcpyDeviceToHost, s));
@@ -963,13 +1091,169 @@ void f64_time_stats(Ctx& c, double* out) {
   out[12] = std::sqrt(r[TS_N + 1] / N);
 }
 
+// all_time_dif_map (fenep_mpi.f90:2350-2564): column sums on the device, the
+// per-(i, j) statistics and their masked plane averages here.  mask(i, j) = 0
+// within 1.5 radii of a cylinder centre (fenep_mpi.f90:2530-2537, with the
+// centres given instead of the reference's four fixed ones, and dY for y).
+static std::vector<double> plane_mask(const Grid& g, int ncyl, const double* xy, double diameter) {
+  std::vector<double> mask((size_t)g.nx * g.ny, 1.0);
+  const double r = diameter * 1.5 / 2.0;
+  for (int j = 0; j < g.ny; j++)
+    for (int i = 0; i < g.nx; i++)
+      for (int c = 0; c < ncyl; c++) {
+        const double dxc = (i + 0.5) * g.dx - xy[2 * c], dyc = (j + 0.5) * g.dy - xy[2 * c + 1];
+        if (dxc * dxc + dyc * dyc < r * r) mask[i + (size_t)g.nx * j] = 0.0;
+      }
+  return mask;
+}
+
+int f64_time_stats_map(Ctx& c, int ncyl, const double* xy, double diameter, double* map18, double* out13) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const long nxy = (long)g.nx * g.ny;
+  std::vector<double> col((size_t)COL_N * nxy);
+  double* dcol = nullptr;
+  FP_HIP_CHECK(hipMalloc((void**)&dcol, sizeof(double) * col.size()));
+  try {
+    hipLaunchKernelGGL(k_column_stats, dim3(nblk(nxy, 256)), dim3(256), 0, s, g, c.U, c.V, c.W, c.mean_u, c.C[c.cur], dcol);
+    FP_HIP_CHECK(hipMemcpyAsync(col.data(), dcol, sizeof(double) * col.size(), hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(dcol);
+    throw;
+  }
+  (void)hipFree(dcol);
+  const std::vector<double> mask = plane_mask(g, ncyl, xy, diameter);
+  double msum = 0.0;
+  for (double m : mask) msum += m;
+  if (msum == 0.0) return 1;
+  const double NZ = (double)g.nz, nu = 1.0 / c.p.re, L2 = c.p.lp * c.p.lp;
+  const double epc = 0.5 * (1.0 - c.p.beta) / c.p.re / c.p.wi * (L2 - 3.0);
+  double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long ij = 0; ij < nxy; ij++) {
+    auto C_ = [&](int q) { return col[(size_t)q * nxy + ij]; };
+    const double urms = C_(0) / NZ, vrms = C_(1) / NZ, wrms = C_(2) / NZ;
+    const double all_rms = (urms + vrms + wrms) / 3.0;
+    const double all_grad = (C_(3) / NZ + C_(4) / NZ + C_(5) / NZ) / 3.0;
+    const double eps = nu / 2.0 * C_(6) / NZ;
+    const double Lk = std::pow(nu * nu * nu / eps, 0.25), Tk = std::sqrt(nu / eps), Uk = std::pow(nu * eps, 0.25);
+    const double lam = std::sqrt(all_rms / all_grad);
+    const double relam = std::sqrt(all_rms) * lam / nu;
+    const double K = (urms + vrms + wrms) / 2.0;
+    const double Ep = epc * (C_(8) / NZ);
+    const double trm = C_(9) / NZ, trs = std::sqrt(C_(10) / NZ);
+    const double vals[11] = {K, lam, relam, eps, Lk, Tk, Uk, Ep, trm, trs, mask[ij]};
+    const int i = (int)(ij % g.nx), j = (int)(ij / g.nx);
+    double* row = map18 ? map18 + 18 * ij : nullptr;
+    if (row) {
+      row[0] = (i + 0.5) * g.dx;
+      row[1] = (j + 0.5) * g.dy;
+      for (int q = 0; q < 11; q++) row[2 + q] = vals[q];
+      for (int q = 13; q < 18; q++) row[q] = 0.0;
+    }
+    for (int q = 0; q < 10; q++) acc[q] += vals[q] * mask[ij];
+  }
+  if (out13) {
+    out13[0] = (double)c.step;
+    out13[1] = c.step * c.p.dt;
+    out13[2] = acc[0] / msum;   // K
+    out13[3] = 0.0;             // no K_staggered in the masked record
+    out13[4] = acc[1] / msum;   // lambda
+    out13[5] = acc[2] / msum;   // Re_lambda
+    out13[6] = acc[3] / msum;   // epsilon
+    out13[7] = acc[4] / msum;   // Kolmogorov length, time, velocity
+    out13[8] = acc[5] / msum;
+    out13[9] = acc[6] / msum;
+    out13[10] = acc[7] / msum;  // Ep
+    out13[11] = acc[8] / msum;  // tr C mean
+    out13[12] = acc[9] / msum;  // tr C std
+  }
+  return 0;
+}
+
+// fft_energy_dif_map (fenep_mpi.f90:2670-2797) of the current state:
+// Emask[kz] = masked plane average of the column z spectra, Efirst[kz] = the
+// spectrum of column (1, 1); kz = 0..NZ-1 (zero above NZ/2).
+int f64_energy_spectrum_z(Ctx& c, int ncyl, const double* xy, double diameter, double* Emask, double* Efirst) {
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const long nxy = (long)g.nx * g.ny;
+  const int nk = g.nz / 2 + 1;
+  const std::vector<double> mask = plane_mask(g, ncyl, xy, diameter);
+  double msum = 0.0;
+  for (double m : mask) msum += m;
+  if (msum == 0.0) return 1;
+  double *dE = nullptr, *dmask = nullptr, *dout = nullptr;
+  std::vector<double> em(nk), ef(nk);
+  try {
+    FP_HIP_CHECK(hipMalloc((void**)&dE, sizeof(double) * nxy * nk));
+    FP_HIP_CHECK(hipMalloc((void**)&dmask, sizeof(double) * nxy));
+    FP_HIP_CHECK(hipMalloc((void**)&dout, sizeof(double) * nk));
+    FP_HIP_CHECK(hipMemcpyAsync(dmask, mask.data(), sizeof(double) * nxy, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_zspec_columns, dim3(nblk(nxy * nk, 256)), dim3(256), 0, s, g, c.U, c.V, c.W, c.mean_u, dE);
+    hipLaunchKernelGGL(k_masked_rows, dim3(nk), dim3(256), 0, s, nxy, dE, dmask, msum, dout);
+    FP_HIP_CHECK(hipMemcpyAsync(em.data(), dout, sizeof(double) * nk, hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipMemcpy2DAsync(ef.data(), sizeof(double), dE, sizeof(double) * nxy, sizeof(double), nk,
+                                  hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(dE); (void)hipFree(dmask); (void)hipFree(dout);
+    throw;
+  }
+  (void)hipFree(dE); (void)hipFree(dmask); (void)hipFree(dout);
+  for (int k = 0; k < g.nz; k++) {
+    if (Emask) Emask[k] = k < nk ? em[k] : 0.0;
+    if (Efirst) Efirst[k] = k < nk ? ef[k] : 0.0;
+  }
+  return 0;
+}
+
+// time mean of the velocity for the mean-subtracted statistics
+void f64_mean_accumulate(Ctx& c, const float* u32, const float* v32, const float* w32) {
+  const Grid& g = c.g;
+  if (!c.mean_acc) {
+    FP_HIP_CHECK(hipMalloc((void**)&c.mean_acc, sizeof(double) * 3 * g.n));
+    zero_words(c.mean_acc, 6 * g.n, c.stream);
+    c.mean_count = 0;
+  }
+  if (u32)
+    hipLaunchKernelGGL(k_mean_acc<float>, dim3(nblk(g.n, 256)), dim3(256), 0, c.stream, g.n, u32, v32, w32, c.mean_acc);
+  else
+    hipLaunchKernelGGL(k_mean_acc<double>, dim3(nblk(g.n, 256)), dim3(256), 0, c.stream, g.n, (const double*)c.U,
+                       (const double*)c.V, (const double*)c.W, c.mean_acc);
+  c.mean_count++;
+}
+
+int f64_mean_apply(Ctx& c, int mode) {
+  const Grid& g = c.g;
+  if (mode == 0) {
+    if (c.mean_u) (void)hipFree(c.mean_u);
+    c.mean_u = nullptr;
+  } else {
+    if (!c.mean_acc || c.mean_count == 0) return 1;
+    if (!c.mean_u) FP_HIP_CHECK(hipMalloc((void**)&c.mean_u, sizeof(double) * 3 * g.n));
+    const long nxy = (long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_mean_finish, dim3(nblk(3 * nxy, 256)), dim3(256), 0, c.stream, g, c.mean_acc,
+                       (double)c.mean_count, mode == 1 ? 1 : 0, c.mean_u);
+  }
+  if (c.mean_acc) {
+    FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+    (void)hipFree(c.mean_acc);
+    c.mean_acc = nullptr;
+  }
+  c.mean_count = 0;
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
 // ===================================================== energy spectrum =====
 // fft_energy_dif (fenep_mpi.f90:2566-2665) without a subtracted mean: the
 // regular-grid velocity, its forward transform / N, |u^|^2 summed over the
 // three components into integer shells nint(|k|), half weight on the kx = 0
 // and Nyquist planes of the half spectrum.
 __global__ __launch_bounds__(256) void k_reg_velocity(Grid g, const double* __restrict__ U, const double* __restrict__ V,
-                                                      const double* __restrict__ W, double* __restrict__ out) {
+                                                      const double* __restrict__ W, const double* __restrict__ Um,
+                                                      double* __restrict__ out) {
   const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (m >= g.n) return;
   int i, j, k;
@@ -977,6 +1261,12 @@ __global__ __launch_bounds__(256) void k_reg_velocity(Grid g, const double* __re
   out[m] = (U[m] + ldb(U, g, i - 1, j, k, BC_U)) / 2.0;
   out[g.n + m] = (V[m] + ldb(V, g, i, j - 1, k, BC_V)) / 2.0;
   out[2 * g.n + m] = (W[m] + ldb(W, g, i, j, k - 1, BC_W)) / 2.0;
+  if (Um) {  // the fluctuation about the subtracted mean
+    const double *Vm = Um + g.n, *Wm = Um + 2 * g.n;
+    out[m] = ((U[m] - Um[m]) + (ldb(U, g, i - 1, j, k, BC_U) - ldb(Um, g, i - 1, j, k, BC_U))) / 2.0;
+    out[g.n + m] = ((V[m] - Vm[m]) + (ldb(V, g, i, j - 1, k, BC_V) - ldb(Vm, g, i, j - 1, k, BC_V))) / 2.0;
+    out[2 * g.n + m] = ((W[m] - Wm[m]) + (ldb(W, g, i, j, k - 1, BC_W) - ldb(Wm, g, i, j, k - 1, BC_W))) / 2.0;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_spec_shells(Grid g, const double2* __restrict__ spec, int nbins,
@@ -1011,7 +1301,7 @@ void f64_energy_spectrum(Ctx& c, double* E, int nbins) {
   FP_HIP_CHECK(hipMalloc((void**)&dE, sizeof(double) * nbins));
   try {
     zero_words(dE, 2L * nbins, s);
-    hipLaunchKernelGGL(k_reg_velocity, dim3(nblk(g.n, 256)), dim3(256), 0, s, g, c.U, c.V, c.W, c.Q);
+    hipLaunchKernelGGL(k_reg_velocity, dim3(nblk(g.n, 256)), dim3(256), 0, s, g, c.U, c.V, c.W, c.mean_u, c.Q);
     const int lx = lines_for(g.nx);
     const long xlines = (long)g.ny * g.nz;
     const size_t shx = sizeof(double2) * 2 * lx * g.nx;

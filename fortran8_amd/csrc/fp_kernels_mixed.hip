@@ -2858,6 +2858,11 @@ int mixed_check_selection(Ctx& c, double* out3) {
 // correction, [2] guarded selections re-decided in FP64
 // The context's stream waits for work made ahead on the side stream, so that
 // synchronizing the context's stream covers it.
+void mixed_velocity(Ctx& c, const float** u, const float** v, const float** w) {
+  Mixed& X = MX(c);
+  *u = X.U; *v = X.V; *w = X.W;
+}
+
 void mixed_join(Ctx& c) {
   Mixed& X = MX(c);
   if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(c.stream, X.sel_done, 0));

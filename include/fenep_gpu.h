@@ -159,20 +159,56 @@ int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
                     int names_len);
 
 /* Time-series statistics of all_time_dif (fenep_mpi.f90:2200-2348) of the
- * current state, without a subtracted mean (the Log/time_same.d record) and
+ * current state, of the velocity less the mean set by fp_mean_apply (none by
+ * default: the Log/time_same.d record; otherwise Log/time_diff.d), and
  * with unit reference scales (U_C = L_C = 1, nu = 1/Re).  out[13] = step,
  * step*dt, K, K_staggered, lambda, Re_lambda, epsilon, eta, tau_eta, v_eta,
  * Ep, trC mean, trC std (the column order of fenep_mpi.f90:2343-2344).
  * Reductions in FP64 on the device. */
 int fp_time_stats(void* ctx, double* out);
 
-/* Shell-summed kinetic-energy spectrum of the regular-grid velocity
- * (fft_energy_dif, fenep_mpi.f90:2566-2665, no subtracted mean): forward FFT
+/* Shell-summed kinetic-energy spectrum of the regular-grid velocity less the
+ * mean set by fp_mean_apply (fft_energy_dif, fenep_mpi.f90:2566-2665): forward FFT
  * / (NX NY NZ), |u^|^2 of the three components summed into shells
  * nint(|k|) with half weight on the kx = 0 and Nyquist planes.  E[0..nbins)
  * receives shells 0..nbins-1 (the reference writes 0..NX-1); periodic grids
  * only.  sum(E) over all shells equals K of fp_time_stats (Parseval). */
 int fp_energy_spectrum(void* ctx, double* E, int nbins);
+
+/* all_time_dif_map (fenep_mpi.f90:2350-2564) of the current state: the
+ * statistics of fp_time_stats per (i, j) column, averaged over z, and their
+ * plane averages over the cells outside 1.5 cylinder radii of the given
+ * centres (the reference masks four fixed centres, fenep_mpi.f90:2530-2537).
+ * xy: ncyl (x, y) pairs; diameter: the cylinders' diameter DC.
+ * map18 (NX*NY*18, may be NULL): the Map/{same,diff}_<step>.bin records, j
+ * outer and i inner: x, y, K, lambda, Re_lambda, epsilon, eta, tau_eta,
+ * v_eta, Ep, trC mean, trC std, mask, 0 x 5.  out13 (may be NULL): the
+ * Log/time_{same,diff}_mask.d record, the column order of fp_time_stats
+ * with K_staggered = 0.  FP_E_ARG if the mask covers every column. */
+int fp_time_stats_map(void* ctx, int ncyl, const double* xy, double diameter,
+                      double* map18, double* out13);
+
+/* fft_energy_dif_map (fenep_mpi.f90:2670-2797) of the current state: each
+ * (i, j) column's regular-grid velocity transformed along z (/ NZ), energy
+ * (|u^|^2 + |v^|^2 + |w^|^2)/2 per |kz|, doubled for 0 < kz < NZ/2.
+ * Emask[NZ]: the plane average over the unmasked columns (mask as in
+ * fp_time_stats_map); Efirst[NZ]: the column (1, 1).  Entries above NZ/2 are
+ * zero.  The reference writes their averages over the records of a Gstep
+ * period (Log/energy_*_mask_<step>.d, Log/energy_*_z_<step>.d). */
+int fp_energy_spectrum_z(void* ctx, int ncyl, const double* xy, double diameter,
+                         double* Emask, double* Efirst);
+
+/* Time mean of the velocity for the mean-subtracted statistics (the second
+ * pass of input_type = 2, fenep_mpi.f90:2969-2971, 2999-3067).
+ * fp_mean_accumulate adds the current U, V, W to a device accumulator.
+ * fp_mean_apply(mode): 0 clears the subtracted mean (the 'same' records);
+ * 1 makes the accumulated mean, averaged over z (ave_mean,
+ * fenep_mpi.f90:2010-2033), the mean that fp_time_stats, fp_time_stats_map,
+ * fp_energy_spectrum and fp_energy_spectrum_z subtract (the 'diff' records);
+ * 2 the same without the z average.  Modes 1 and 2 need at least one
+ * accumulated state; every mode empties the accumulator. */
+int fp_mean_accumulate(void* ctx);
+int fp_mean_apply(void* ctx, int mode);
 
 /* Mixed precision: the C+- selection of the current stored (FP32) field as
  * the next step makes it (FP32 decisions that carry an error certificate, the

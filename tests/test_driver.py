@@ -118,3 +118,89 @@ def test_driver_periodic_time_series_and_spectrum(tmp_path):
         assert e[:, 1].sum() == pytest.approx(rows[:, 2].mean(), rel=1e-10)
         # the 2D Taylor-Green field (k = (1, 1, 0)) lives in the shell nint(sqrt 2) = 1
         assert np.argmax(e[:, 1]) == 1 and e[1, 1] > 0.999 * e[:, 1].sum()
+
+
+@pytest.mark.gpu
+def test_driver_masked_records_maps_outlines_and_text_maps(tmp_path):
+    nml = ("NX=64, NY=32, NZ=16, Lx=4.0, Ly=2.0, Lz=1.0, dt=0.002, Re=200, beta=0.8, Wi=1, Lp=10, method=2, "
+           "walls_y=1, precision=0, body_force=0.015,0,0, ncyl=1, DC=0.5, xcyl(1)=1.0, ycyl(1)=1.0, "
+           "Nstep=20, Lstep=10, koide_step=0, Tstep=5, Gstep=10, map_text=1, dir='./chan/'")
+    r = run_driver(tmp_path, nml)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    run = os.path.join(str(tmp_path), "chan")
+    tm = np.loadtxt(os.path.join(run, "Log", "time_same_mask.d"))
+    assert tm.shape == (4, 13) and np.array_equal(tm[:, 0], [5, 10, 15, 20]) and np.all(tm[:, 3] == 0.0)
+    h = 4.0 / 64
+    for step in (10, 20):
+        m = np.fromfile(os.path.join(run, "Map", "same_%08d.bin" % step), dtype=np.float64).reshape(-1, 18)
+        assert m.shape == (64 * 32, 18)
+        X, Y = m[:, 0], m[:, 1]
+        inside = (X - 1.0) ** 2 + (Y - 1.0) ** 2 < 0.375 ** 2
+        assert np.array_equal(m[:, 12] == 0.0, inside)
+        # the masked record is the map's average over the unmasked columns
+        row = tm[tm[:, 0] == step][0]
+        assert row[2] == pytest.approx(m[~inside, 2].mean(), rel=1e-12)
+        assert row[11] == pytest.approx(m[~inside, 10].mean(), rel=1e-12)
+        ez = np.loadtxt(os.path.join(run, "Log", "energy_same_mask_%08d.d" % step))
+        assert ez.shape == (16, 2) and np.all(ez[9:, 1] == 0.0)
+        # Parseval over the records of the Gstep period (fenep_mpi.f90:2775-2781)
+        rows = tm[(tm[:, 0] > step - 10) & (tm[:, 0] <= step)]
+        assert ez[:, 1].sum() == pytest.approx(rows[:, 2].mean(), rel=1e-10)
+        assert np.loadtxt(os.path.join(run, "Log", "energy_same_z_%08d.d" % step)).shape == (16, 2)
+        # text maps as _map.py reads them (form 'd'): 18 columns, Qti in column 7
+        zt = np.loadtxt(os.path.join(run, "z_%08d.d" % step))
+        yt = np.loadtxt(os.path.join(run, "y_%08d.d" % step))
+        assert zt.shape == (64 * 32, 18) and yt.shape == (64 * 16, 18)
+        zb = np.fromfile(os.path.join(run, "Map", "z_%08d.bin" % step), dtype=np.float64).reshape(-1, 18)
+        cols = [0, 1, 2, 3, 4, 5] + list(range(7, 18))
+        assert np.allclose(zt[:, cols], zb[:, cols], rtol=1e-3, atol=1e-4 * np.abs(zb[:, cols]).max())
+    # cylinder outline (ibm_get): the markers of plane NZ/2, closed
+    ol = np.loadtxt(os.path.join(run, "ibm", "00000001.d"))
+    assert np.allclose(ol[0], ol[-1])
+    rad = np.hypot(ol[:, 0] - 1.0, ol[:, 1] - 1.0)
+    assert rad.max() == pytest.approx(0.25, rel=1e-3) and ol.shape[0] == 64 + 1
+
+
+@pytest.mark.gpu
+def test_driver_second_pass_subtracts_the_time_mean_like_the_oracle(tmp_path):
+    """input_type = 2: the first pass accumulates the mean velocity, the
+    second restarts from Restart.d and writes the 'diff' records; checked
+    against the oracle run through the same two passes."""
+    case = configs.kolmogorov(n=16)
+    o = po.load_case(case)
+    o.step(2)
+    st = o.get_state()
+    st.hist[6:] = 0.0        # Restart.d carries A0 and T0 only
+    os.makedirs(os.path.join(str(tmp_path), "run"))
+    write_restart(os.path.join(str(tmp_path), "run", "Restart.d"), st)
+    p = case.params
+    nml = ("NX=16, NY=16, NZ=16, Lx=%r, Ly=%r, Lz=%r, dt=%r, Re=%r, beta=%r, Wi=%r, Lp=%r, method=1, "
+           "Nstep=6, Lstep=6, Tstep=3, Gstep=6, Ostep=0, koide_step=0, input_type=2, flow_type=0, dir='./run/'"
+           % (p.lx, p.ly, p.lz, p.dt, p.re, p.beta, p.wi, p.lp))
+    r = run_driver(tmp_path, nml)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    log = os.path.join(str(tmp_path), "run", "Log")
+    same, diff = np.loadtxt(os.path.join(log, "time_same.d")), np.loadtxt(os.path.join(log, "time_diff.d"))
+    diff_mask = np.loadtxt(os.path.join(log, "time_diff_mask.d"))
+    assert np.array_equal(diff[:, 0], [3, 6]) and np.array_equal(same[:, 0], [3, 6])
+    # the oracle through the same passes (no body force: flow_type = 0)
+    st.has_history = 1
+    a = po.Oracle(p)
+    a.set_state(st)
+    for _ in range(6):
+        a.step(1)
+        a.mean_accumulate()
+    a.mean_apply(1)
+    b = po.Oracle(p)
+    b.set_state(st)
+    b._mean = a._mean
+    for row, rowm in zip(diff, diff_mask):
+        b.step(3)
+        ref = b.time_stats_np(step=row[0])
+        np.testing.assert_allclose(row, ref, rtol=1e-10)
+        np.testing.assert_allclose(rowm, b.time_stats_map(step=row[0])[1], rtol=1e-10)
+    assert np.all(diff[:, 2] < same[:, 2])
+    em = np.loadtxt(os.path.join(log, "energy_diff_mask_%08d.d" % 6))
+    assert em[:, 1].sum() == pytest.approx(diff_mask[:, 2].mean(), rel=1e-10)

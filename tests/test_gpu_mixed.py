@@ -160,36 +160,31 @@ def test_mixed_full_size_short_run_is_finite_and_close():
     assert d["nan_flag"] == 0
 
 
+@pytest.mark.timeout(300)
 def test_mixed_throughput_1000_steps_against_fp64():
     """The chosen precision form at 256x128x128 with 8 cylinders after 1000
-    steps against the FP64 GPU step (bit-identical to the oracle): relative
-    L2 of u, the dynamic pressure, the drag and the kinetic energy within
-    1e-4; C within its FP32-storage bound."""
+    steps against the FP64 GPU step.  u, the drag, the kinetic energy and
+    the dynamic pressure stay within 1e-4.  Each field is also compared with
+    the FP32-storage floor: the FP64 step with its state rounded to FP32 after
+    every step.  The wakes of the 8 cylinders at Re_D = 200 amplify that
+    per-step rounding alone to about 9e-4 in C (tests/_storage_floor.py); the
+    mixed step must stay within twice that floor in every field."""
+    import os
+    import subprocess
+    import sys
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_storage_floor.py")
+    print("mixed vs FP64, 256x128x128, 1000 steps, with the FP32 floors", flush=True)
+    r = subprocess.run([sys.executable, "-u", script, "throughput", "1000"], capture_output=True, text=True,
+                       timeout=280)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
     import json
-    steps = 1000
-    print("mixed vs FP64, 256x128x128, %d steps" % steps, flush=True)
-    g = api.load_case(mixed(configs.throughput()))
-    g.step(steps)
-    h = api.load_case(configs.throughput())
-    h.step(steps)
-    e = errors(h, g)
-    dg, dh = g.diagnostics(), h.diagnostics()
-    e["drag"] = abs(dg["drag"] - dh["drag"]) / abs(dh["drag"])
-    e["ke"] = abs(dg["ke"] - dh["ke"]) / abs(dh["ke"])
-    e["div_max_mixed"] = dg["div_max"]
-    e["div_max_f64"] = dh["div_max"]
-    sel = g.check_selection()
-    e["sel_fp64_fraction"] = sel["fp64"] / sel["total"]
-    e["sel_differ"] = sel["differ"]
-    print(json.dumps({k: float(v) for k, v in e.items()}), flush=True)
-    # The C+- selection is the reference's selection of the stored field at
-    # every cell (certified FP32 decisions, FP64 re-decision of the rest), so
-    # u, p, the drag and the kinetic energy stay within 1e-4.  C does not:
-    # the wakes of the 8 cylinders are unstable at Re_D = 200, and the FP64
-    # step started from the FP32-rounded initial state alone ends 3.3e-4 away
-    # from the FP64 run in C after 1000 steps; an FP32-storage step rounds
-    # every step (measured 1.2e-3).
-    assert sel["differ"] == 0
-    assert e["ke"] < TOL and e["u"] < TOL and e["P_dyn"] < TOL and e["drag"] < TOL
-    assert e["C"] < 3e-3
-    assert dg["nan_flag"] == 0
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    m, fl = res["mixed"], res["floor_storage"]
+    for run in ("mixed", "floor_ic", "floor_storage"):
+        print(run, {k: "%.2e" % v for k, v in res[run].items()}, flush=True)
+    assert m["sel_differ"] == 0
+    for k in ("u", "drag", "ke", "P_dyn"):
+        assert m[k] < TOL, k
+    for k in ("U", "V", "W", "P", "C", "u", "drag"):
+        assert m[k] <= 2.0 * fl[k], (k, m[k], fl[k])

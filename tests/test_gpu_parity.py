@@ -208,3 +208,77 @@ def test_energy_spectrum_matches_oracle():
     np.testing.assert_allclose(Eg, Eo, rtol=1e-11, atol=1e-14 * Eo.sum())
     g.close()
     o.close()
+
+
+def _close_with_nan(a, b, rtol):
+    a, b = np.asarray(a), np.asarray(b)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    ok = ~np.isnan(a)
+    np.testing.assert_allclose(a[ok], b[ok], rtol=rtol, atol=1e-300)
+
+
+def test_map_statistics_and_z_spectra_match_oracle_with_and_without_mean():
+    """all_time_dif_map and fft_energy_dif_map on the device against the
+    oracle, for the raw velocity ('same') and after the z-averaged time mean
+    of three states is subtracted ('diff'), periodic and with walls."""
+    for case, centres in ((configs.kolmogorov(n=32, nz=16), [(1.0, 2.0), (4.0, 4.5)]),
+                          (configs.channel_cylinder(nx=64, ny=32, nz=16), [(2.0, 2.0)])):
+        print("map statistics", case.name, flush=True)
+        o, g = po.load_case(case), api.load_case(case)
+        for _ in range(3):
+            o.step(1)
+            g.step(1)
+            o.mean_accumulate()
+            g.mean_accumulate()
+        for mode in (0, 1):
+            if mode:
+                o.mean_apply(1)
+                g.mean_apply(1)
+            rows_o, rec_o = o.time_stats_map(centres, 0.5)
+            rows_g, rec_g = g.time_stats_map(centres, 0.5)
+            _close_with_nan(rows_g, rows_o, 1e-11)
+            _close_with_nan(rec_g, rec_o, 1e-11)
+            np.testing.assert_allclose(g.time_stats(), o.time_stats_np(), rtol=1e-11, atol=1e-300)
+            em_o, ef_o = o.energy_spectrum_z(centres, 0.5)
+            em_g, ef_g = g.energy_spectrum_z(centres, 0.5)
+            np.testing.assert_allclose(em_g, em_o, rtol=1e-9, atol=1e-13 * em_o.sum())
+            np.testing.assert_allclose(ef_g, ef_o, rtol=1e-9, atol=1e-13 * max(ef_o.sum(), 1e-300))
+            if not case.params.walls_y:
+                Eo, Eg = o.energy_spectrum(), g.energy_spectrum()
+                if mode:
+                    # the oracle's shell spectrum of the fluctuation
+                    Eo = _shell_spectrum_of_fluct(o)
+                np.testing.assert_allclose(Eg, Eo, rtol=1e-10, atol=1e-14 * Eo.sum())
+        g.close()
+        o.close()
+
+
+def _shell_spectrum_of_fluct(o):
+    st, (u, v, w) = o._fluct()
+    nx, ny, nz = o.nx, o.ny, o.nz
+    N = nx * ny * nz
+    ureg = (u[1:-1, 1:-1, 1:-1] + u[1:-1, 1:-1, :-2]) / 2.0
+    vreg = (v[1:-1, 1:-1, 1:-1] + v[1:-1, :-2, 1:-1]) / 2.0
+    wreg = (w[1:-1, 1:-1, 1:-1] + w[:-2, 1:-1, 1:-1]) / 2.0
+    e = sum(np.abs(np.fft.rfftn(a) / N) ** 2 for a in (ureg, vreg, wreg))
+    kx = np.arange(nx // 2 + 1, dtype=float)[None, None, :]
+    ky = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny).astype(float)[None, :, None]
+    kz = np.where(np.arange(nz) <= nz // 2, np.arange(nz), np.arange(nz) - nz).astype(float)[:, None, None]
+    idx = np.floor(np.sqrt(kx ** 2 + ky ** 2 + kz ** 2) + 0.5).astype(int)
+    wgt = np.ones(nx // 2 + 1)
+    wgt[0] = wgt[-1] = 0.5
+    E = np.bincount(idx.ravel(), weights=(e * wgt[None, None, :]).ravel(), minlength=idx.max() + 1)
+    return E[:nx] if len(E) >= nx else np.pad(E, (0, nx - len(E)))
+
+
+def test_mean_accumulation_of_the_mixed_state():
+    case = configs.kolmogorov(n=32, nz=16)
+    case.params.precision = 1
+    g = api.load_case(case)
+    g.step(2)
+    g.mean_accumulate()
+    g.mean_apply(2)
+    s = g.time_stats()
+    assert s[2] == 0.0 and s[3] > 0.0
+    with pytest.raises(api.FenepError, match="FP_E_STATE"):
+        g.mean_apply(1)

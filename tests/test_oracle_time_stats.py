@@ -96,3 +96,107 @@ def test_energy_spectrum_parseval_and_single_mode():
     o.close()
     assert E[3] == pytest.approx(math.cos(1.5 * h) ** 2 / 4, rel=1e-13)
     assert np.abs(np.delete(E, 3)).max() < 1e-28
+
+
+# ---------------------------------------------- maps, masks, subtracted mean
+def _state_of(U, V, W, C=None):
+    from fortran8_amd.fields import State
+    nz, ny, nx = U.shape
+    if C is None:
+        C = np.zeros((nz, ny, nx, 6))
+        C[..., 0] = C[..., 3] = C[..., 5] = 1.0
+    return State(pad_periodic(U), pad_periodic(V), pad_periodic(W), pad_periodic(np.zeros_like(U)),
+                 np.pad(C, ((1, 1), (1, 1), (1, 1), (0, 0)), mode="wrap"))
+
+
+def _oracle_at(case, steps=3):
+    o = po.load_case(case)
+    o.step(steps)
+    return o
+
+
+def test_numpy_restatement_matches_the_c_restatement():
+    """Two restatements of all_time_dif: the C loop and the numpy arrays."""
+    for case in (configs.kolmogorov(n=16), configs.channel_cylinder(nx=32, ny=16, nz=8)):
+        o = _oracle_at(case)
+        np.testing.assert_allclose(o.time_stats_np(), o.time_stats(), rtol=1e-12)
+
+
+def test_unmasked_map_averages_equal_the_volume_statistics():
+    """The quantities linear in the column sums (K, epsilon, Ep, tr C mean)
+    averaged over the plane without a mask are the volume statistics."""
+    o = _oracle_at(configs.kolmogorov(n=16))
+    rows, rec = o.time_stats_map()
+    vol = o.time_stats()
+    for q in (2, 6, 10, 11):
+        assert rec[q] == pytest.approx(vol[q], rel=1e-12), q
+    assert rec[3] == 0.0
+    assert rows.shape == (16 * 16, 18) and np.all(rows[:, 12] == 1.0)
+    # record order: j outer, i inner, x and y at the cell centres
+    h = 2 * math.pi / 16
+    assert rows[1, 0] == pytest.approx(1.5 * h) and rows[1, 1] == pytest.approx(0.5 * h)
+    assert rows[16, 1] == pytest.approx(1.5 * h)
+
+
+def test_mask_removes_the_columns_within_one_and_a_half_radii():
+    """A velocity confined to a disk around a cylinder centre (zero on every
+    cell a statistic's stencil reaches outside it) has zero masked energy."""
+    n = 32
+    L = 2 * math.pi
+    h = L / n
+    xc, yc, D = L / 2, L / 2, 1.6
+    X, Y = np.meshgrid((np.arange(n) + 0.5) * h, (np.arange(n) + 0.5) * h)
+    inner = (X - xc) ** 2 + (Y - yc) ** 2 < (0.75 * D - 2.5 * h) ** 2
+    U = np.broadcast_to(np.where(inner, 0.3, 0.0)[None], (8, n, n)).copy()
+    case = configs.kolmogorov(n=n, nz=8)
+    o = po.Oracle(case.params)
+    o.set_state(_state_of(U, np.zeros_like(U), np.zeros_like(U)))
+    rows, rec = o.time_stats_map(centres=[(xc, yc)], diameter=D)
+    mask = rows[:, 12].reshape(n, n)
+    r2 = (X - xc) ** 2 + (Y - yc) ** 2
+    assert np.array_equal(mask == 0, r2 < (0.75 * D) ** 2)
+    assert rec[2] == 0.0
+    assert o.time_stats_map()[1][2] > 0.0
+    em, _ = o.energy_spectrum_z(centres=[(xc, yc)], diameter=D)
+    assert np.all(em == 0.0)
+
+
+def test_subtracted_mean_of_a_steady_field_leaves_no_fluctuation():
+    o = _oracle_at(configs.kolmogorov(n=16))
+    o.mean_accumulate()
+    o.mean_accumulate()
+    o.mean_apply(2)                 # the mean of two equal states: the state
+    s = o.time_stats_np()
+    assert s[2] == 0.0 and s[6] == 0.0
+    assert s[3] > 0.0               # K_staggered is of the raw velocity
+    assert o.time_stats_map()[1][2] == 0.0
+    assert np.all(o.energy_spectrum_z()[0] == 0.0)
+    o.mean_apply(0)
+    assert o.time_stats_np()[2] > 0.0
+
+
+def test_z_averaged_mean_removes_the_z_mean_of_each_column():
+    """mode 1 (ave_mean): a field a(x, y) + b(x, y) cos(3 z) keeps only the
+    cos(3 z) part; its energy sits in |kz| = 3 with the closed-form value
+    <b_reg^2>/4 per column."""
+    n, nz = 16, 16
+    L = 2 * math.pi
+    h = L / n
+    xf, xc = np.arange(1, n + 1) * h, (np.arange(n) + 0.5) * h
+    zc = (np.arange(nz) + 0.5) * (L / nz)
+    Z, Y, X = np.meshgrid(zc, xc, xf, indexing="ij")
+    a, b = 0.4 * np.sin(Y), 0.1 + 0.05 * np.cos(X)
+    U = a + b * np.cos(3 * Z)
+    case = configs.kolmogorov(n=n, nz=nz)
+    o = po.Oracle(case.params)
+    o.set_state(_state_of(U, np.zeros_like(U), np.zeros_like(U)))
+    o.mean_accumulate()
+    o.mean_apply(1)
+    em, ef = o.energy_spectrum_z()
+    b_reg = (b[0] + np.roll(b[0], 1, axis=1)) / 2.0      # regular-grid average in x
+    assert em[3] == pytest.approx((b_reg ** 2).mean() / 4.0, rel=1e-12)
+    assert ef[3] == pytest.approx(b_reg[0, 0] ** 2 / 4.0, rel=1e-12)
+    assert np.max(np.abs(np.delete(em, 3))) < 1e-25
+    # Parseval, the reference's own check (fenep_mpi.f90:2775-2781)
+    rows, rec = o.time_stats_map()
+    assert em.sum() == pytest.approx(rec[2], rel=1e-12)
