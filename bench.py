@@ -5,20 +5,29 @@ Poisson (the throughput configuration).  A "step" is one full time step of
 the state resident in HBM.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--precision mixed|f64]
-                  [--no-cpu] [--cpu-steps S] [--small]
+                  [--no-cpu] [--cpu-steps S] [--no-f64] [--no-adjoint] [--small]
 
 The roofline kernel's duration comes from HIP events recorded around it on
 its stream inside the timed region (the only events there: every phase
 timed would add ~7 % to the step); the per-phase breakdown comes from a few
 steps run afterwards with every phase timed.  HBM traffic per launch of the
 roofline kernel is the rocprofv3 FETCH_SIZE/WRITE_SIZE reading kept in
-profiles/traffic.txt (already scaled by the copy-kernel calibration).
+profiles/traffic.txt (already scaled by the copy-kernel calibration), and the
+VALU instruction count of the selection kernel its SQ_INSTS_VALU reading.
+
+Beside the mixed-precision step the same line carries the FP64 step timed
+over the same steps (the comparator of the mixed/FP64 speed), the accuracy of
+the mixed state against the FP64 one at the end of those runs, and the cost
+of the discrete adjoint on the 64x64x32 sensitivity configuration.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
@@ -68,10 +77,11 @@ def mixed_step_bytes(n_dc):
     return 310.0 + 3.5 + 36.0 * max(0, n_dc - 1)
 
 
-def run_gpu(case, steps, warmup):
+def run_gpu(case, steps, warmup, keep_state=False):
     """Timed region: `steps` steps with HIP events only around the predictor
-    (the roofline kernel; two events per step).  Then, outside the timed
-    region, a few steps with every phase timed for the breakdown."""
+    (the roofline kernel; two events per step).  The state after the timed
+    region is kept (for the accuracy figures) before, outside the timed
+    region, a few steps run with every phase timed for the breakdown."""
     g = api.load_case(case)
     g.enable_timing(2)
     g.step(warmup, sync=True)           # warm-up (also fills the event pool)
@@ -81,6 +91,7 @@ def run_gpu(case, steps, warmup):
     t1 = time.perf_counter()
     kt_timed = {k: v / steps for k, v in g.kernel_times().items()}
     d = g.diagnostics()
+    st = g.get_state() if keep_state else None
     nb = max(1, min(steps, 10))
     g.enable_timing(1)
     g.step(nb, sync=True)
@@ -88,21 +99,85 @@ def run_gpu(case, steps, warmup):
     g.enable_timing(0)
     g.close()
     wall = (t1 - t0) / steps
-    return wall, kt_timed, kt, d
+    return wall, kt_timed, kt, d, st
 
 
-def read_traffic(precision):
-    """Bytes per launch of each kernel from profiles/traffic.txt (lines
-    `traffic <phase> <precision> <bytes>`), or {} if absent."""
+def accuracy(ref, dref, got, dgot):
+    """Relative L2 of u (stacked), U, V, W, P and C, and the relative drag and
+    kinetic-energy differences, of `got` against `ref`."""
+    from fortran8_amd.fields import interior, interior_c, rel_l2
+    e = {n: rel_l2(interior(getattr(got, n)), interior(getattr(ref, n))) for n in ("U", "V", "W", "P")}
+    e["C"] = rel_l2(interior_c(got.C), interior_c(ref.C))
+    e["u"] = rel_l2(np.concatenate([interior(getattr(got, n)).ravel() for n in "UVW"]),
+                    np.concatenate([interior(getattr(ref, n)).ravel() for n in "UVW"]))
+    e["drag"] = abs(dgot["drag"] - dref["drag"]) / abs(dref["drag"])
+    e["ke"] = abs(dgot["ke"] - dref["ke"]) / abs(dref["ke"])
+    return {k: float("%.3e" % v) for k, v in e.items()}
+
+
+def adjoint_cost():
+    """Cost of the discrete adjoint on the sensitivity configuration (64x64x32,
+    one cylinder, 400 FP64 steps, checkpoints every 20, drag averaged over the
+    last 200): wall time of a plain forward run, of the recording run, and of
+    the reverse sweep (which recomputes each checkpoint segment)."""
+    case = configs.adjoint_case()
+    nsteps, every, last = 400, 20, 200
+    g = api.load_case(case)
+    w = np.zeros(nsteps)
+    w[nsteps - last:] = 1.0 / last
+    g.step(5, sync=True)                 # warm-up
+    g.set_state(case.state)
+    t0 = time.perf_counter()
+    g.step(nsteps, sync=True)
+    t1 = time.perf_counter()
+    g.set_state(case.state)
+    t2 = time.perf_counter()
+    g.adj_record(nsteps, every)
+    t3 = time.perf_counter()
+    grads = g.adj_backward(weights=w)
+    t4 = time.perf_counter()
+    g.close()
+    fwd = t1 - t0
+    return {"config": "64x64x32, 400 steps, checkpoints every 20", "forward_s": round(fwd, 4),
+            "record_s": round(t3 - t2, 4), "reverse_s": round(t4 - t3, 4),
+            "reverse_over_forward": round((t4 - t3) / fwd, 3),
+            "record_and_reverse_over_forward": round((t4 - t2) / fwd, 3),
+            "dJ_dWi": grads[0], "dJ_dbeta": grads[1]}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def read_profile_counts(kind, precision):
+    """Per-launch readings of profiles/traffic.txt: lines `traffic <phase>
+    <precision> <bytes>` (calibrated HBM bytes) or `valu <phase> <precision>
+    <instructions>` (SQ_INSTS_VALU, wave instructions per dispatch)."""
     out = {}
     path = os.path.join(HERE, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return out
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and f[0] == "traffic" and f[2] == precision:
+        if len(f) == 4 and f[0] == kind and f[2] == precision:
             out[f[1]] = float(f[3])
     return out
+
+
+# VALU issue: a wave64 VALU instruction occupies its SIMD for 4 cycles (16
+# lanes per cycle, non-packed FP32/FP64); MI355X has 256 CUs x 4 SIMDs and a
+# 2.4 GHz peak engine clock.
+SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4
+
+
+def valu_issue_ms(valu_insts):
+    return valu_insts * CYCLES_PER_VALU / SIMDS / CLOCK_HZ * 1e3
 
 
 def cpu_baseline(case, steps=1):
@@ -120,12 +195,14 @@ def cpu_baseline(case, steps=1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", choices=["mixed", "f64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--small", action="store_true", help="64x32x32 smoke size")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparator and the accuracy figures")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost figures")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the whole domain runs on a single MI355X "
@@ -136,7 +213,8 @@ def main():
         case = configs.throughput(nx=64, ny=32, nz=32, precision=prec, ncyl=1)
     else:
         case = configs.throughput(precision=prec)
-    wall, kt_timed, kt, d = run_gpu(case, a.steps, a.warmup)
+    compare = prec == 1 and not a.no_f64
+    wall, kt_timed, kt, d, st = run_gpu(case, a.steps, a.warmup, keep_state=compare)
     cells = case.cells
     value = cells / wall / 1e6
     width = 8 if prec == 0 else 4
@@ -151,7 +229,7 @@ def main():
     dom_bytes = (words * width + extra) * units
     dom_ms = kt_timed.get(dom, 0.0)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
-    traffic = read_traffic(a.precision).get(dom)
+    traffic = read_profile_counts("traffic", a.precision).get(dom)
     step_bytes = STEP_WORDS * 8 * cells if prec == 0 else mixed_step_bytes(d.get("n_dc", 0)) * cells
     walls = "true" if case.params.walls_y else "false"
     mixed_pred = "fp::m_predictor<false>" if "predictor_walls" in kt else "fp::m_predictor_z<%s>" % walls
@@ -187,9 +265,11 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
+            # the kernel with the largest share of the step: bound by VALU
+            # issue, not by HBM; its fraction is the VALU issue time of its
+            # instructions (SQ_INSTS_VALU) over its duration
+            "dominant": dominant_valu(kt, kernel_names, a.precision),
         },
-        "compute_bound": {"bound": "valu", "phases_ms": {k: round(kt[k], 4) for k in ("fenep_select", "fenep_cstar_newton")
-                                                          if k in kt}},
         "step_roofline": {
             "algorithmic_bytes_per_cell_step": step_bytes / cells,
             "achieved": round(step_bytes / wall / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -198,12 +278,48 @@ def main():
         "diag": {k: d[k] for k in ("ke", "cfl", "trc_min", "trc_max", "drag", "div_max", "newton_itr_mean", "n_dc",
                                     "dc_residual", "sel_guarded", "nan_flag")},
     }
+    if compare:
+        # the FP64 step over the same steps from the same state: its speed is
+        # the comparator of the mixed step, its state the accuracy reference
+        ref = configs.throughput(precision=0)
+        wall64, _, kt64, d64, st64 = run_gpu(ref, a.steps, a.warmup, keep_state=True)
+        out["fp64_comparator"] = {"ms_per_step": round(wall64 * 1e3, 4),
+                                  "value": round(cells / wall64 / 1e6, 3), "unit": "Mcell-steps/s",
+                                  "mixed_speedup": round(wall64 / wall, 3)}
+        # floor: the FP64 step from the FP32-rounded initial state
+        fl = configs.throughput(precision=0)
+        for nm in ("U", "V", "W", "P", "C"):
+            setattr(fl.state, nm, getattr(fl.state, nm).astype(np.float32).astype(np.float64))
+        gf = api.load_case(fl)
+        gf.step(a.warmup + a.steps, sync=True)
+        stf, dfl = gf.get_state(), gf.diagnostics()
+        gf.close()
+        out["accuracy"] = {"steps": a.warmup + a.steps, "reference": "FP64 GPU step (bit-identical to the oracle)",
+                           "mixed": accuracy(st64, d64, st, d),
+                           "floor_fp32_initial_state": accuracy(st64, d64, stf, dfl)}
+    if not a.no_adjoint and not a.small:
+        out["adjoint"] = adjoint_cost()
     if not a.no_cpu:
         cv, cores, sec = cpu_baseline(configs.throughput(precision=0), a.cpu_steps)
         out["cpu_baseline"] = {"value": round(cv, 4), "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+                               "cpu": cpu_model(),
                                "sample": "%d step(s) of the full 256x128x128 case on the FP64 CPU restatement "
                                          "(OpenMP, %d threads), %.1f s" % (a.cpu_steps, cores, sec)}
     print(json.dumps(out), flush=True)
+
+
+def dominant_valu(kt, kernel_names, precision):
+    """The selection kernel (the largest share of the mixed step's kernel
+    time) against its VALU issue time."""
+    phase = "fenep_select"
+    insts = read_profile_counts("valu", precision).get(phase)
+    ms = kt.get(phase)
+    if not insts or not ms:
+        return None
+    floor = valu_issue_ms(insts)
+    return {"kernel": kernel_names[phase], "bound": "valu", "valu_insts": insts,
+            "issue_ms": round(floor, 4), "launch_ms": round(ms, 4), "frac": round(floor / ms, 4),
+            "note": "launch_ms from the phase-timed steps; the kernel shares the GPU with the Poisson solve"}
 
 
 if __name__ == "__main__":

@@ -41,12 +41,13 @@ module fenep_gpu_api
     real(c_double) :: body_force(3)
     integer(c_int32_t) :: n_dc_max
     real(c_double) :: dc_tol
-    integer(c_int32_t) :: reserved(8)
+    integer(c_int32_t) :: helm_sweeps
+    integer(c_int32_t) :: reserved(7)
   end type fp_params
 
   type, bind(c) :: fp_diag
     real(c_double) :: ke, cfl, trc_min, trc_max, trc_mean, trc_std, div_max, drag, lift
-    real(c_double) :: newton_itr_mean, sor_itr, n_dc, dc_residual, sel_guarded
+    real(c_double) :: newton_itr_mean, sor_itr, n_dc, dc_residual, sel_guarded, newton_unconverged
     integer(c_int32_t) :: nan_flag, step
   end type fp_diag
 

@@ -1,12 +1,15 @@
 """Drag sensitivities of the FENE-P / IBM step behind torch.autograd.
 
-``DragObjective.apply(wi, beta, L2, force)`` runs ``nsteps`` FP64 steps of a
-case on the GPU from its initial state and returns the time-averaged drag
+``DragObjective.apply(wi, beta, L2, force)`` runs ``nsteps`` steps of a case
+(FP64, or the mixed-precision step when the case asks for it) on the GPU from
+its initial state and returns the time-averaged drag
 J = sum_s w_s D_s with D_s = -sum_markers Fxc dX^3 of step s (the forces of
 ibm_Helmholtz, fenep_mpi.f90:1845-1853).  Its backward runs the hand-written
 discrete adjoint of every kernel of the step (fp_adj_backward) with
 checkpointed recomputation and returns dJ/dWi, dJ/dbeta, dJ/d(L^2) and dJ/dF
-for the body-force field F (3, NZ, NY, NX).
+for the body-force field F (3, NZ, NY, NX).  For a mixed-precision forward the
+reverse sweep is the FP64 adjoint along FP64 recomputations of each segment
+from the mixed step's checkpoints.
 
 Import torch before this module's library is first used in a process (torch
 ships its own HIP runtime).
@@ -23,8 +26,6 @@ class DragRun:
     """A case bound to a device context, with the objective's step weights."""
 
     def __init__(self, case, nsteps, every, avg_last):
-        if case.params.precision != 0:
-            raise api.FenepError("the adjoint runs on the FP64 path")
         self.case = case
         self.nsteps = int(nsteps)
         self.every = int(every)

@@ -26,6 +26,7 @@
 // Gradients: d/dWi, d/dbeta, d/d(L^2) of sum_s w_s D_s (+ <bar_final, state>),
 // the body-force field and the initial state.
 #include "fp_device.h"
+#include "fp_mixed.h"
 #include <cmath>
 #include <vector>
 
@@ -590,6 +591,7 @@ struct Adj {
   double* ck = nullptr;            // (nck + 1) checkpoints; the last is the final state
   std::vector<int> ck_hist;
   uint8_t* selh = nullptr;         // sel_steps * n
+  int mixed_rec = 0;               // recorded by the mixed step
   double* drag = nullptr;          // nsteps
   double* tape = nullptr;          // every * TAPE_WORDS * n + 6n (C after the segment)
   std::vector<int> tape_first;
@@ -645,7 +647,6 @@ void load_state(Ctx& c, const double* src) {
 }
 
 void check_supported(Ctx& c) {
-  if (c.mixed) throw std::string("E_ARG:recording and the adjoint run on the FP64 path");
   if (c.g.walls) throw std::string("E_ARG:the adjoint supports periodic grids only");
   if (c.p.poisson_variant != FP_POISSON_FFT) throw std::string("E_ARG:the adjoint needs the FFT Poisson solve");
   if (c.p.method != FP_METHOD_FFT && c.p.method != FP_METHOD_IBM)
@@ -730,7 +731,8 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
   Adj& a = adj_of(c);
   const long n = c.g.n;
   const int polymer = c.p.beta != 1.0;
-  if (replay && (!a.selh || a.sel_steps < nsteps || a.n != n))
+  if (replay && c.mixed) throw std::string("E_ARG:replay of recorded selections runs on the FP64 path");
+  if (replay && (!a.selh || a.sel_steps < nsteps || a.n != n || a.mixed_rec))
     throw std::string("E_STATE:replay needs a recorded run of at least as many steps");
   a.n = n;
   const int nck = (nsteps + every - 1) / every;
@@ -749,10 +751,21 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
   a.ck_hist.assign(nck + 1, 0);
   const double dx3 = (c.g.dx * c.g.dx) * c.g.dx;
   const int step0 = c.step;
+  // Mixed precision: the checkpoints and the drag are those of the mixed
+  // step (its FP32 state widened to FP64); the reverse sweep recomputes each
+  // segment from its checkpoint with the FP64 step and its own selections.
+  a.mixed_rec = c.mixed ? 1 : 0;
   for (int s = 0; s < nsteps; s++) {
     if (s % every == 0) {
+      if (c.mixed) mixed_store_to_f64(c);
       save_state(c, a.ck + (size_t)(s / every) * CK_WORDS * n);
       a.ck_hist[s / every] = c.has_history;
+    }
+    if (c.mixed) {
+      mixed_step(c);
+      if (c.p.method == FP_METHOD_IBM)
+        hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, mixed_marker_forces(c), c.nmk, dx3, a.drag + s);
+      continue;
     }
     c.sel_replay = (replay && polymer) ? a.selh + (size_t)s * n : nullptr;
     step_f64(c);
@@ -760,6 +773,10 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
     if (!replay && polymer) d2d(c, a.selh + (size_t)s * n, c.sel, n);
     if (c.p.method == FP_METHOD_IBM)
       hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, c.mf, c.nmk, dx3, a.drag + s);
+  }
+  if (c.mixed) {
+    mixed_join(c);
+    mixed_store_to_f64(c);
   }
   save_state(c, a.ck + (size_t)nck * CK_WORDS * n);
   a.ck_hist[nck] = c.has_history;
@@ -793,6 +810,7 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
     alloc0(&a.acc, NACC); alloc0(&a.part, (size_t)NACC * nblk(n, 256));
   }
   hipStream_t st = c.stream;
+  if (c.mixed) mixed_join(c);   // no selection made ahead still writes c.sel
   if (bar_final) d2d(c, a.bar, bar_final, sizeof(double) * BAR_WORDS * n);
   else zero_words(a.bar, 2 * BAR_WORDS * n, st);
   zero_words(a.Fb, 6 * n, st);
@@ -811,9 +829,10 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
       d2d(c, E + 3 * n, c.P, sizeof(double) * n);
       d2d(c, E + 4 * n, c.C[c.cur], sizeof(double) * 6 * n);
       a.tape_first[s - s0] = c.has_history ? 0 : 1;
-      c.sel_replay = c.p.beta != 1.0 ? a.selh + (size_t)s * n : nullptr;
+      c.sel_replay = (c.p.beta != 1.0 && !a.mixed_rec) ? a.selh + (size_t)s * n : nullptr;
       step_f64(c);
       c.sel_replay = nullptr;
+      if (a.mixed_rec && c.p.beta != 1.0) d2d(c, a.selh + (size_t)s * n, c.sel, n);
       d2d(c, E + 10 * n, c.Up, sizeof(double) * 3 * n);
       d2d(c, E + 13 * n, c.Phi, sizeof(double) * n);
     }
@@ -827,6 +846,7 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
   load_state(c, a.ck + (size_t)a.nck * CK_WORDS * n);
   c.has_history = a.ck_hist[a.nck];
   c.step = step0;
+  if (c.mixed) mixed_load_from_f64(c);
   if (bar_initial) d2d(c, bar_initial, a.bar, sizeof(double) * BAR_WORDS * n);
   if (bar_force) d2d(c, bar_force, a.Fb, sizeof(double) * 3 * n);
   double acc[NACC];

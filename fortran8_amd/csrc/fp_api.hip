@@ -452,35 +452,37 @@ int fp_step(void* vc, int nsteps, int sync) {
   return FP_OK;
 }
 
+// Body of fp_diagnostics: the reductions run in `scratch`
+// (diag_scratch_words(c.g) doubles of device memory).
+static void diagnostics_into(Ctx& c, fp_diag* d, double* scratch) {
+  std::memset(d, 0, sizeof(*d));
+  if (c.mixed) mixed_store_to_f64(c);
+  const double* mf = c.nmk ? (c.mixed ? mixed_marker_forces(c) : c.mf) : nullptr;
+  f64_diag(c, d, scratch, mf);
+  d->sor_itr = c.sor_itr;
+  d->n_dc = c.mixed ? mixed_last_ndc(c) : 0.0;
+  if (c.mixed) {
+    double st[3];
+    mixed_stats(c, st);
+    d->dc_residual = st[0] > 0 ? std::sqrt(st[1] / st[0]) : 0.0;
+    d->sel_guarded = st[2];
+  }
+  d->step = c.step;
+}
+
 int fp_diagnostics(void* vc, fp_diag* d) {
   Ctx* c = (Ctx*)vc;
   if (!c || !d) return FP_E_ARG;
   if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  double* scratch = nullptr;
   try {
-    std::memset(d, 0, sizeof(*d));
-    if (c->mixed) mixed_store_to_f64(*c);
-    f64_diag(*c, d);
-    if (c->nmk) {
-      std::vector<double> f(2 * c->nmk);
-      const double* mf = c->mixed ? mixed_marker_forces(*c) : c->mf;
-      FP_HIP_CHECK(hipMemcpy(f.data(), mf, sizeof(double) * 2 * c->nmk, hipMemcpyDeviceToHost));
-      double dx3 = (c->g.dx * c->g.dx) * c->g.dx, sx = 0, sy = 0;
-      for (long id = 0; id < c->nmk; id++) { sx += f[id]; sy += f[c->nmk + id]; }
-      d->drag = -(sx * dx3);
-      d->lift = -(sy * dx3);
-    }
-    d->sor_itr = c->sor_itr;
-    d->n_dc = c->mixed ? mixed_last_ndc(*c) : 0.0;
-    if (c->mixed) {
-      double st[3];
-      mixed_stats(*c, st);
-      d->dc_residual = st[0] > 0 ? std::sqrt(st[1] / st[0]) : 0.0;
-      d->sel_guarded = st[2];
-    }
-    d->step = c->step;
+    FP_HIP_CHECK(hipMalloc((void**)&scratch, sizeof(double) * diag_scratch_words(c->g)));
+    diagnostics_into(*c, d, scratch);
   } catch (const std::string& e) {
+    if (scratch) (void)hipFree(scratch);
     return fail(c, FP_E_HIP, e);
   }
+  (void)hipFree(scratch);
   return d->nan_flag ? FP_E_NAN : FP_OK;
 }
 
@@ -676,7 +678,6 @@ static int adj_fail(Ctx* c, const std::string& e) {
 int fp_set_physics(void* vc, double wi, double beta, double lp) {
   Ctx* c = (Ctx*)vc;
   if (!c) return FP_E_ARG;
-  if (c->mixed) return fail(c, FP_E_ARG, "fp_set_physics runs on the FP64 path");
   if (!(wi > 0.0) || !(beta > 0.0 && beta <= 1.0) || !(lp * lp > 3.0))
     return fail(c, FP_E_ARG, "need Wi > 0, 0 < beta <= 1 and Lp^2 > 3");
   try {
@@ -686,6 +687,13 @@ int fp_set_physics(void* vc, double wi, double beta, double lp) {
   }
   c->p.wi = wi; c->p.beta = beta; c->p.lp = lp;
   c->g.wi = wi; c->g.beta = beta; c->g.lp = lp;
+  if (c->mixed) {
+    try {
+      mixed_physics(*c);
+    } catch (const std::string& e) {
+      return fail(c, FP_E_HIP, e);
+    }
+  }
   return FP_OK;
 }
 

@@ -227,6 +227,7 @@ __device__ __forceinline__ void gauss6_d(double (&Ah)[6][7], double* x) {
 }
 
 // Newton of Lyapunov (fenep.f90:554-578) with the central-difference Jacobian.
+// Returns the iterations taken, itrmax + 1 when the tolerance was not met.
 __device__ __forceinline__ int newton_d(const double* A0, const double* b, double* x, double dt, double wi,
                                         double lp, double dc, double eps, int itrmax) {
   int itr;
@@ -258,7 +259,7 @@ __device__ __forceinline__ int newton_d(const double* A0, const double* b, doubl
     for (int m = 0; m < 6; m++) sx = sx + x[m] * x[m];
     if (sqrt(sr / sx) < eps) break;
   }
-  return itr > itrmax ? itrmax : itr;
+  return itr;
 }
 
 __device__ __forceinline__ double peterlin_d(double lp, double c0, double c3, double c5) {

@@ -97,7 +97,8 @@ void f64_ibm(Ctx& c);        // interp + forces + spread + predictor RHS into Q
 void f64_ibm_setup(Ctx& c);  // CSR build (host) after markers are set
 int  f64_sor(Ctx& c, const double* q, double* phi);
 void f64_zero_c(Ctx& c);
-void f64_diag(Ctx& c, fp_diag* d);
+void f64_diag(Ctx& c, fp_diag* d, double* scratch, const double* mf);  // scratch: diag_scratch_words
+size_t diag_scratch_words(const Grid& g);
 void f64_time_stats(Ctx& c, double* out13);  // all_time_dif statistics
 void f64_energy_spectrum(Ctx& c, double* E, int nbins);  // shell spectrum (periodic)
 int f64_time_stats_map(Ctx& c, int ncyl, const double* xy, double diameter, double* map18, double* out13);

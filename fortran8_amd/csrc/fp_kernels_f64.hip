@@ -124,11 +124,11 @@ __global__ __launch_bounds__(128) void k_cstar_newton(Grid g, const double* __re
   double x[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) x[q] = C[q * g.n + m];
-  int it = newton_d(A, b, x, dt, g.wi, g.lp, dc, eps, itrmax);
+  const int it = newton_d(A, b, x, dt, g.wi, g.lp, dc, eps, itrmax);
 #pragma unroll
   for (int q = 0; q < 6; q++) Cn[q * g.n + m] = x[q];
   fC[m] = peterlin_d(g.lp, x[0], x[3], x[5]);
-  nitr[m] = it;
+  nitr[m] = it > itrmax ? -itrmax : it;   // negative: stopped at the cap
 }
 
 // ========================================================= predictor ======
@@ -1332,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__
                                               const double* __restrict__ W, const double* __restrict__ C,
                                               const int* __restrict__ nitr, int polymer, double* __restrict__ red) {
   long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  double s[5] = {0, 0, 0, 0, 0};  // ke, trsum, trsq, nitr, nan
+  double s[6] = {0, 0, 0, 0, 0, 0};  // ke, trsum, trsq, nitr, nan, Newton stopped at the cap
   double mx[6] = {-1e300, -1e300, -1e300, -1e300, 1e300, 0.0};  // umax vmax wmax trmax -trmin divmax
   if (m < g.n) {
     int i, j, k;
@@ -1344,7 +1344,8 @@ __global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__
     s[0] = ur * ur + vr * vr + wr * wr;
     double tr = (C[m] + C[3 * g.n + m]) + C[5 * g.n + m];
     s[1] = tr; s[2] = tr * tr;
-    s[3] = polymer ? (double)nitr[m] : 0.0;
+    s[3] = polymer ? (double)abs(nitr[m]) : 0.0;
+    s[5] = (polymer && nitr[m] < 0) ? 1.0 : 0.0;
     s[4] = isnan(U[m]) ? 1.0 : 0.0;
     mx[0] = u; mx[1] = v; mx[2] = w; mx[3] = tr; mx[4] = tr;
     double dv = ((u - ldb(U, g, i - 1, j, k, BC_U)) / g.dx + (v - ldb(V, g, i, j - 1, k, BC_V)) / g.dy)
@@ -1352,15 +1353,15 @@ __global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__
     mx[5] = fabs(dv);
   }
   __shared__ double sh[256];
-  for (int a = 0; a < 11; a++) {
-    double v = a < 5 ? s[a] : mx[a - 5];
+  for (int a = 0; a < 12; a++) {
+    double v = a < 6 ? s[a] : mx[a - 6];
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
       if (threadIdx.x < o) {
         double x = sh[threadIdx.x], y = sh[threadIdx.x + o];
-        if (a < 5) sh[threadIdx.x] = x + y;
-        else if (a == 9) sh[threadIdx.x] = fmin(x, y);
+        if (a < 6) sh[threadIdx.x] = x + y;
+        else if (a == 10) sh[threadIdx.x] = fmin(x, y);
         else sh[threadIdx.x] = fmax(x, y);
       }
       __syncthreads();
@@ -1370,34 +1371,79 @@ __global__ __launch_bounds__(256) void k_diag(Grid g, const double* __restrict__
   }
 }
 
-void f64_diag(Ctx& c, fp_diag* d) {
-  const Grid& g = c.g;
-  int nb = nblk(g.n, 256);
-  double* buf;
-  FP_HIP_CHECK(hipMalloc(&buf, sizeof(double) * 11 * nb));
-  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c.stream, g, c.U, c.V, c.W, c.C[c.cur], c.nitr, c.p.beta != 1.0 ? 1 : 0, buf);
-  std::vector<double> h(11 * (size_t)nb);
-  FP_HIP_CHECK(hipMemcpyAsync(h.data(), buf, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
-  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
-  (void)hipFree(buf);
-  double s[5] = {0, 0, 0, 0, 0}, mx[6] = {-1e300, -1e300, -1e300, -1e300, 1e300, 0};
-  for (int b = 0; b < nb; b++) {
-    for (int a = 0; a < 5; a++) s[a] += h[a * nb + b];
-    for (int a = 0; a < 6; a++) {
-      double v = h[(5 + a) * nb + b];
-      mx[a] = a == 4 ? std::min(mx[a], v) : std::max(mx[a], v);
-    }
+// the 12 per-block partials of k_diag reduced to one value each: sums for
+// a < 6, the minimum of tr C for a = 10, maxima otherwise (one block per a)
+__global__ __launch_bounds__(1024) void k_diag_reduce(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  const int a = blockIdx.x;
+  const double* row = part + (long)a * nb;
+  __shared__ double sh[1024];
+  const double init = a < 6 ? 0.0 : (a == 10 ? 1e300 : -1e300);
+  double v = init;
+  for (int b = threadIdx.x; b < nb; b += 1024) {
+    const double x = row[b];
+    v = a < 6 ? v + x : (a == 10 ? fmin(v, x) : fmax(v, x));
   }
-  double N = (double)g.n;
-  d->ke = (s[0] / 2.0) / N;
-  d->cfl = std::max(mx[0] * g.dt / g.dx, std::max(mx[1] * g.dt / g.dy, mx[2] * g.dt / g.dz)) * 6.0;
-  d->trc_mean = s[1] / N;
-  d->trc_std = std::sqrt(std::max(0.0, s[2] / N - d->trc_mean * d->trc_mean));
-  d->trc_max = mx[3];
-  d->trc_min = mx[4];
-  d->div_max = mx[5];
-  d->newton_itr_mean = s[3] / N;
-  d->nan_flag = s[4] > 0 ? 1 : 0;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double x = sh[threadIdx.x], y = sh[threadIdx.x + o];
+      sh[threadIdx.x] = a < 6 ? x + y : (a == 10 ? fmin(x, y) : fmax(x, y));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[a] = sh[0];
+}
+
+// sums of the x and y marker forces (drag and lift before the -dX^3 factor)
+__global__ __launch_bounds__(1024) void k_marker_sums(const double* __restrict__ mf, long nmk, double* __restrict__ out) {
+  __shared__ double sh[1024];
+  for (int comp = 0; comp < 2; comp++) {
+    double v = 0.0;
+    for (long id = threadIdx.x; id < nmk; id += 1024) v += mf[comp * nmk + id];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[comp] = sh[0];
+    __syncthreads();
+  }
+}
+
+size_t diag_scratch_words(const Grid& g) { return (size_t)12 * nblk(g.n, 256) + 16; }
+
+// log_progress quantities (fenep.f90:2279-2311) and the drag and lift of the
+// marker forces mf (nullptr: none), reduced on the device in `scratch`
+// (diag_scratch_words doubles, owned by the caller); 14 doubles come back.
+void f64_diag(Ctx& c, fp_diag* d, double* scratch, const double* mf) {
+  const Grid& g = c.g;
+  const int nb = nblk(g.n, 256);
+  double* out = scratch + (size_t)12 * nb;
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, c.stream, g, c.U, c.V, c.W, c.C[c.cur], c.nitr,
+                     c.p.beta != 1.0 ? 1 : 0, scratch);
+  hipLaunchKernelGGL(k_diag_reduce, dim3(12), dim3(1024), 0, c.stream, scratch, nb, out);
+  if (mf && c.nmk) hipLaunchKernelGGL(k_marker_sums, dim3(1), dim3(1024), 0, c.stream, mf, c.nmk, out + 12);
+  double h[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  FP_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(double) * ((mf && c.nmk) ? 14 : 12), hipMemcpyDeviceToHost, c.stream));
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
+  const double N = (double)g.n;
+  d->ke = (h[0] / 2.0) / N;
+  d->cfl = std::max(h[6] * g.dt / g.dx, std::max(h[7] * g.dt / g.dy, h[8] * g.dt / g.dz)) * 6.0;
+  d->trc_mean = h[1] / N;
+  d->trc_std = std::sqrt(std::max(0.0, h[2] / N - d->trc_mean * d->trc_mean));
+  d->trc_max = h[9];
+  d->trc_min = h[10];
+  d->div_max = h[11];
+  d->newton_itr_mean = h[3] / N;
+  d->newton_unconverged = h[5];
+  d->nan_flag = h[4] > 0 ? 1 : 0;
+  if (mf && c.nmk) {
+    const double dx3 = (g.dx * g.dx) * g.dx;
+    d->drag = -(h[12] * dx3);
+    d->lift = -(h[13] * dx3);
+  }
 }
 
 }  // namespace fp

@@ -826,10 +826,22 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   int it;
   // Chord iterations: the Jacobian of the first iterate (the linear operator
   // plus the rank-one Peterlin term) is factored once, its multipliers kept
-  // below the diagonal; later iterations take fresh residuals through the
-  // same factors.  The Jacobian moves with x only through f and f', by about
-  // 1e-5 relative between iterates, so the chord converges like Newton here
-  // (two iterations in the throughput case) at half the cost of the second.
+  // below the diagonal; the second iteration takes a fresh residual through
+  // the same factors.  The Jacobian moves with x only through f and f', by
+  // about 1e-5 relative between iterates, so the chord converges like Newton
+  // here (two iterations in the throughput case) at half the cost of the
+  // second.  Near the pole (tr C -> L^2) f varies fast with tr C and the
+  // chord can stall: from the third iteration on the Jacobian is factored
+  // again at the current iterate (full Newton).
+  //
+  // The LU takes no pivots.  J = J0 + gp x e^T: J0, the linear operator, has
+  // diagonal entries 1 + f dt/Wi + O(dt |grad u|) and off-diagonal entries
+  // O(dt |grad u|); the Peterlin term adds gp x_r (gp = dt/Wi f' > 0) to
+  // columns 0, 3, 5 of row r, which on the diagonal of the rows 0, 3, 5 is
+  // gp times a diagonal entry of the SPD iterate, > 0.  Eliminating a column
+  // subtracts from a later diagonal pivot at most the part of its own
+  // rank-one entry that the earlier rows carry, so every pivot stays above
+  // 1 + f dt/Wi - O(dt |grad u|).
   T J[6][6];
   for (it = 1; it <= itrmax; it++) {
     const T s = x[0] + x[3] + x[5];
@@ -844,7 +856,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     l[3] = T(2) * (a21 * x[1] + A22 * x[3] + a23 * x[4]) - b[3];
     l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (A33 + A22) * x[4] + a23 * x[5] - b[4];
     l[5] = T(2) * (a31 * x[2] + a32 * x[4] + A33 * x[5]) - b[5];
-    if (it == 1) {
+    if (it == 1 || it >= 3) {
       // Jacobian: linear operator + rank one Peterlin term
       const T gp = dtw * dpeterlin_t<T>(L2, s);
       const T J0[6][6] = {
@@ -892,7 +904,7 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     for (int q = 0; q < 6; q++) { x[q] -= rr[q]; sr += rr[q] * rr[q]; sx += x[q] * x[q]; }
     if (sr < eps * eps * sx) break;
   }
-  if (it > itrmax) it = itrmax;
+  if (it > itrmax) it = -itrmax;   // stopped at the cap without meeting eps
 #pragma unroll
   for (int q = 0; q < 6; q++) stu(Cn + q * g.n, g.n, mu, (float)x[q]);
   stu(fC, g.n, mu, (float)peterlin_t<T>(L2, x[0] + x[3] + x[5]));
@@ -2456,6 +2468,21 @@ __global__ void m_to64(long n, const float* __restrict__ a, double* __restrict__
 }
 
 // ================================================================= host ===
+// Fixed-point Helmholtz sweeps x <- q + eps Lap x from x = q: the error
+// starts at eps Lap x* (<= rho |x*|, rho = eps lambda_max) and contracts by
+// rho per sweep, so k sweeps leave rho^(k+1) |x*|; enough sweeps to bring that
+// below the FP32 unit roundoff 2^-24 of the stored result, 0 (the spectral
+// solve) if more than 6 would be needed.
+static int helm_sweeps(const Grid& g, int requested) {
+  if (requested > 0) return std::min(requested, 16);
+  if (requested < 0) return 0;
+  const double eps = ((g.dt * g.beta) / g.re) / 2.0;
+  const double lam = 4.0 / (g.dx * g.dx) + 4.0 / (g.dy * g.dy) + 4.0 / (g.dz * g.dz);
+  const double rho = eps * lam;
+  int m = rho > 0.0 ? (int)std::ceil(std::log(std::ldexp(1.0, -24)) / std::log(rho)) - 1 : 1;
+  return (rho < 0.25 && m <= 6) ? std::max(1, m) : 0;
+}
+
 void mixed_create(Ctx& c) {
   if (c.p.method == FP_METHOD_EXPLICIT || c.p.poisson_variant == FP_POISSON_SOR)
     throw std::string("mixed precision supports the FFT and IBM methods with the spectral Poisson solve");
@@ -2496,18 +2523,7 @@ void mixed_create(Ctx& c) {
   FP_HIP_CHECK(hipEventCreateWithFlags(&X->dc_ev, hipEventDisableTiming));
   dalloc(&X->skip, 1);
   dalloc(&X->R, 3 * n); dalloc(&X->X2, 3 * n);
-  {
-    // fixed-point Helmholtz sweeps x <- q + eps Lap x from x = q: the error
-    // starts at eps Lap x* (<= rho |x*|, rho = eps lambda_max) and contracts
-    // by rho per sweep, so k sweeps leave rho^(k+1) |x*|; enough sweeps to
-    // bring that below the FP32 unit roundoff 2^-24 of the stored result,
-    // spectral solve if more than 6
-    const double eps = ((g.dt * g.beta) / g.re) / 2.0;
-    const double lam = 4.0 / (g.dx * g.dx) + 4.0 / (g.dy * g.dy) + 4.0 / (g.dz * g.dz);
-    const double rho = eps * lam;
-    int m = rho > 0.0 ? (int)std::ceil(std::log(std::ldexp(1.0, -24)) / std::log(rho)) - 1 : 1;
-    X->helm_iters = (rho < 0.25 && m <= 6) ? std::max(1, m) : 0;
-  }
+  X->helm_iters = helm_sweeps(g, c.p.helm_sweeps);
   // every (cell, axis) may need the FP64 decision: a region of 3 * 256
   // entries per select block (3 n for n >= 256; m * 3 + axis < 2^32 as every
   // axis has at most 1024 cells)
@@ -2858,6 +2874,11 @@ int mixed_check_selection(Ctx& c, double* out3) {
 // correction, [2] guarded selections re-decided in FP64
 // The context's stream waits for work made ahead on the side stream, so that
 // synchronizing the context's stream covers it.
+void mixed_physics(Ctx& c) {
+  mixed_join(c);
+  MX(c).helm_iters = helm_sweeps(c.g, c.p.helm_sweeps);
+}
+
 void mixed_velocity(Ctx& c, const float** u, const float** v, const float** w) {
   Mixed& X = MX(c);
   *u = X.U; *v = X.V; *w = X.W;

@@ -17,5 +17,6 @@ int mixed_stats(Ctx& c, double* out3);
 int mixed_check_selection(Ctx& c, double* out3);
 void mixed_step(Ctx& c);
 void mixed_join(Ctx& c);
+void mixed_physics(Ctx& c);   // after Wi, beta or Lp changed
 void mixed_velocity(Ctx& c, const float** u, const float** v, const float** w);  // FP32 state
 }  // namespace fp

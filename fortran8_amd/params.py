@@ -23,7 +23,8 @@ class FpParams(C.Structure):
         ("sor_omega", C.c_double),
         ("body_force", C.c_double * 3),
         ("n_dc_max", C.c_int32), ("dc_tol", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("helm_sweeps", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
     def copy(self):
@@ -49,6 +50,7 @@ class FpDiag(C.Structure):
         ("div_max", C.c_double), ("drag", C.c_double), ("lift", C.c_double),
         ("newton_itr_mean", C.c_double), ("sor_itr", C.c_double),
         ("n_dc", C.c_double), ("dc_residual", C.c_double), ("sel_guarded", C.c_double),
+        ("newton_unconverged", C.c_double),
         ("nan_flag", C.c_int32), ("step", C.c_int32),
     ]
 

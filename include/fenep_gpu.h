@@ -68,7 +68,10 @@ typedef struct fp_params {
   double  body_force[3];       /* constant body force (mean pressure gradient)*/
   int32_t n_dc_max;            /* mixed: max FP64 defect-correction sweeps     */
   double  dc_tol;              /* mixed: relative defect tolerance             */
-  int32_t reserved[8];
+  int32_t helm_sweeps;         /* mixed: fixed-point sweeps of the CN Helmholtz
+                                * solve; 0 = as many as bring the error below
+                                * FP32 rounding, -1 = the spectral solve       */
+  int32_t reserved[7];
 } fp_params;
 
 /* Diagnostics (log_progress, fenep.f90:2266-2342, plus drag and divergence). */
@@ -84,6 +87,8 @@ typedef struct fp_diag {
   double n_dc;        /* defect-correction sweeps of the last mixed solve    */
   double dc_residual; /* mixed: |b - L phi| / |b| before the last correction */
   double sel_guarded; /* mixed: C+- selections re-decided in FP64, last step */
+  double newton_unconverged; /* cells whose Lyapunov Newton stopped at its
+                              * iteration cap without meeting the tolerance */
   int32_t nan_flag;   /* nonzero: NaN in U (driver prints 'NaN value')        */
   int32_t step;       /* steps taken by this context                         */
 } fp_diag;
@@ -219,13 +224,17 @@ int fp_mean_apply(void* ctx, int mode);
  * in total (3 NX NY NZ).  The state is not changed. */
 int fp_check_selection(void* ctx, double* out);
 
-/* ---- Sensitivities (FP64 path, periodic grids, FFT Poisson, methods 1 and
- * 2).  The reference has no derivative code: these entry points serve the
- * discrete adjoint of the step it advances (fenep.f90:2477-2527) with respect
- * to Wi, beta, L^2 = Lp^2 (fenep.f90:23-26 module constants) and the body-force
- * field F of init (fenep.f90:96-120). */
+/* ---- Sensitivities (periodic grids, FFT Poisson, methods 1 and 2).  The
+ * reference has no derivative code: these entry points serve the discrete
+ * adjoint of the step it advances (fenep.f90:2477-2527) with respect to Wi,
+ * beta, L^2 = Lp^2 (fenep.f90:23-26 module constants) and the body-force field
+ * F of init (fenep.f90:96-120).  On an FP64 context the adjoint is that of the
+ * recorded FP64 steps.  On a mixed-precision context the recording runs the
+ * mixed step (its drag and its FP32 state, widened, as checkpoints) and the
+ * reverse sweep is the FP64 adjoint along FP64 recomputations of each
+ * checkpoint segment, with their own C+- selections. */
 
-/* Change Wi, beta and Lp of an FP64 context between runs. */
+/* Change Wi, beta and Lp of a context between runs. */
 int fp_set_physics(void* ctx, double wi, double beta, double lp);
 
 /* Advance nsteps steps from the current state while recording: the state is
@@ -233,7 +242,7 @@ int fp_set_physics(void* ctx, double wi, double beta, double lp);
  * drag D_s = -sum_markers Fxc dX^3 of every step (IBM method; 0 otherwise)
  * are kept; drag_out (host, nsteps, may be NULL) receives D_s.  replay != 0
  * reuses the selections of the previous recording instead of computing them
- * (the same discrete branch; for finite-difference checks). */
+ * (the same discrete branch; for finite-difference checks; FP64 contexts). */
 int fp_adj_record(void* ctx, int nsteps, int every, int replay, double* drag_out);
 
 /* Reverse sweep over the last recording for the objective

@@ -29,6 +29,18 @@
 #include <string.h>
 #include <stdio.h>
 
+/* The state scalar: double, or in the C++ complex-step build (ORC_CS) the
+ * complex-step type of cstep.h, whose real parts are computed by the same
+ * double operations.  Geometry (spacings, twiddles, modified wavenumbers,
+ * marker positions) stays double. */
+#ifdef ORC_CS
+#include "cstep.h"
+typedef cs_t real_t;
+#else
+typedef double real_t;
+static inline double rv(double x) { return x; }
+#endif
+
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
 #endif
@@ -42,25 +54,27 @@ typedef struct {
   long n;               /* interior count */
   double dx, dy, dz;
   int walls;
-  double *U, *V, *W, *P, *Phi;
-  double *C[6], *C0[6];
-  double *A[3], *B[3], *T[3], *A0[3], *B0[3], *T0[3];
-  double *Up[3], *Ua[3];
-  double *F[3];
-  double *fint[3];
-  double *ftmp;
-  double *Cp[3][6], *Cn[3][6];
-  double *Cx[6];
-  double *q;            /* interior scratch */
-  double *e;            /* SOR residual */
+  real_t wi, beta, lp;  /* Wi, beta, Lp of the step (p holds their real values) */
+  real_t *U, *V, *W, *P, *Phi;
+  real_t *C[6], *C0[6];
+  real_t *A[3], *B[3], *T[3], *A0[3], *B0[3], *T0[3];
+  real_t *Up[3], *Ua[3];
+  real_t *F[3];
+  real_t *fint[3];
+  real_t *ftmp;
+  real_t *Cp[3][6], *Cn[3][6];
+  real_t *Cx[6];
+  real_t *q;            /* interior scratch */
+  real_t *e;            /* SOR residual */
   /* spectral */
   double *twx, *twy, *twz, *kx2, *ky2, *kz2;
-  double *spec;         /* complex (nxh)(sy)(nz) */
+  real_t *spec;         /* complex (nxh)(sy)(nz) */
   long sy;              /* spectral rows in y: ny (periodic) or ny+2 (walls) */
   int jo;               /* row offset of interior j=1 in spec: 0 or 1 */
   /* markers */
   int nc, nl, nmk;
-  double *xc, *yc, *zc, *uc, *vc, *wc, *fxc, *fyc, *fzc, *ubm, *vbm, *wbm;
+  double *xc, *yc, *zc, *uc, *vc, *wc;
+  real_t *fxc, *fyc, *fzc, *ubm, *vbm, *wbm;
   int has_history, step;
   double newton_itr_total;
   double sor_itr;
@@ -70,13 +84,14 @@ typedef struct {
 #define GI(o, i, j, k) ((long)(i) + (o)->gx * ((long)(j) + (o)->gy * (long)(k)))
 #define II(o, i, j, k) ((long)((i) - 1) + (long)(o)->nx * ((long)((j) - 1) + (long)(o)->ny * (long)((k) - 1)))
 
-static double* zalloc(long n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+static real_t* zalloc(long n) { return (real_t*)calloc((size_t)(n > 0 ? n : 1), sizeof(real_t)); }
+static double* dzalloc(long n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
 
 /* ------------------------------------------------------------------ ghosts */
 /* PBM (fenep.f90:147-155) in x and z; in y either periodic or the wall rules
  * of fenep_2d.f90:1098-1143 with NA=0, NB=NY (stationary walls, Uwall=0).
  * The order x, then y, then z makes edge/corner ghosts consistent. */
-static void fill_ghosts(orc_t* o, double* A, int kind) {
+static void fill_ghosts(orc_t* o, real_t* A, int kind) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
   for (long k = 0; k < o->gz; k++)
     for (long j = 0; j < o->gy; j++) {
@@ -121,22 +136,22 @@ static void fill_ghosts(orc_t* o, double* A, int kind) {
 /* Vieta, fenep.f90:425-444: smallest eigenvalue of the symmetric matrix a
  * (xx,xy,xz,yy,yz,zz) by the trigonometric root with +2pi/3; -100 when the
  * discriminant t >= 0. */
-double orc_vieta(const double* a) {
-  double a2 = -((a[0] + a[3]) + a[5]);
-  double a1 = (((((a[0] * a[3]) + (a[3] * a[5])) + (a[5] * a[0])) - (a[4] * a[4])) - (a[2] * a[2])) - (a[1] * a[1]);
-  double a0 = -((((((a[0] * a[3]) * a[5] + (a[1] * a[4]) * a[2]) + (a[1] * a[4]) * a[2]) - (a[0] * a[4]) * a[4]) - (a[1] * a[1]) * a[5]) - (a[2] * a[3]) * a[2]);
-  double p = a1 - (a2 * a2) / 3.0;
-  double q = (a0 - (a1 * a2) / 3.0) + (2.0 * ((a2 * a2) * a2)) / 27.0;
-  double q2 = q / 2.0, p3 = p / 3.0;
-  double t = q2 * q2 + (p3 * p3) * p3;
+static real_t vieta_r(const real_t* a) {
+  real_t a2 = -((a[0] + a[3]) + a[5]);
+  real_t a1 = (((((a[0] * a[3]) + (a[3] * a[5])) + (a[5] * a[0])) - (a[4] * a[4])) - (a[2] * a[2])) - (a[1] * a[1]);
+  real_t a0 = -((((((a[0] * a[3]) * a[5] + (a[1] * a[4]) * a[2]) + (a[1] * a[4]) * a[2]) - (a[0] * a[4]) * a[4]) - (a[1] * a[1]) * a[5]) - (a[2] * a[3]) * a[2]);
+  real_t p = a1 - (a2 * a2) / 3.0;
+  real_t q = (a0 - (a1 * a2) / 3.0) + (2.0 * ((a2 * a2) * a2)) / 27.0;
+  real_t q2 = q / 2.0, p3 = p / 3.0;
+  real_t t = q2 * q2 + (p3 * p3) * p3;
   if (t >= 0.0) return -100.0;
-  double arg = ((3.0 * q) / (2.0 * p)) * sqrt(-(3.0 / p));
-  double ang = acos(arg) / 3.0 + (2.0 * M_PI) / 3.0;
+  real_t arg = ((3.0 * q) / (2.0 * p)) * sqrt(-(3.0 / p));
+  real_t ang = acos(arg) / 3.0 + (2.0 * M_PI) / 3.0;
   return (2.0 * sqrt(-(p / 3.0))) * cos(ang) - a2 / 3.0;
 }
 
 /* minval of two values with NaN ignored unless both are NaN (gfortran MINVAL) */
-static inline double minval2(double a, double b) {
+static inline real_t minval2(real_t a, real_t b) {
   if (isnan(a)) return b;
   if (isnan(b)) return a;
   return (b < a) ? b : a;
@@ -144,8 +159,8 @@ static inline double minval2(double a, double b) {
 
 /* Candidate face values and selection of one cell along one axis,
  * fenep.f90:179-230.  Returns the 1-based index (4 = zero slope). */
-int orc_select(const double* cm, const double* c, const double* cp, double* cpo, double* cno) {
-  double Cpt[3][6], Cnt[3][6], ml[4];
+static int select_r(const real_t* cm, const real_t* c, const real_t* cp, real_t* cpo, real_t* cno) {
+  real_t Cpt[3][6], Cnt[3][6], ml[4];
   for (int m = 0; m < 6; m++) {
     Cpt[0][m] = (c[m] * 3.0) / 2.0 - cp[m] / 2.0;
     Cnt[0][m] = c[m] / 2.0 + cp[m] / 2.0;
@@ -154,9 +169,9 @@ int orc_select(const double* cm, const double* c, const double* cp, double* cpo,
     Cpt[2][m] = (cm[m] / 4.0 + c[m]) - cp[m] / 4.0;
     Cnt[2][m] = (-(cm[m] / 4.0) + c[m]) + cp[m] / 4.0;
   }
-  for (int l = 0; l < 3; l++) ml[l] = minval2(orc_vieta(Cpt[l]), orc_vieta(Cnt[l]));
+  for (int l = 0; l < 3; l++) ml[l] = minval2(vieta_r(Cpt[l]), vieta_r(Cnt[l]));
   ml[3] = 0.0;
-  int idx = -1; double best = 0.0;
+  int idx = -1; real_t best = 0.0;
   for (int l = 0; l < 4; l++) {   /* maxloc: first maximum, NaN skipped */
     if (isnan(ml[l])) continue;
     if (idx < 0 || ml[l] > best) { idx = l; best = ml[l]; }
@@ -171,23 +186,23 @@ int orc_select(const double* cm, const double* c, const double* cp, double* cpo,
 
 /* gauss, fenep.f90:608-643: 6x6 Gaussian elimination with partial pivoting,
  * Gauss-Jordan form; x = 0 when a zero pivot is met.  a is column-major. */
-void orc_gauss6(const double* a, const double* b, double* x) {
-  double Ah[6][7];
+static void gauss6_r(const real_t* a, const real_t* b, real_t* x) {
+  real_t Ah[6][7];
   for (int r = 0; r < 6; r++) {
     for (int c = 0; c < 6; c++) Ah[r][c] = a[r + 6 * c];
     Ah[r][6] = b[r];
   }
   for (int k = 0; k < 6; k++) {
-    int mi = k; double ma = fabs(Ah[k][k]);
+    int mi = k; real_t ma = fabs(Ah[k][k]);
     for (int r = k + 1; r < 6; r++) if (fabs(Ah[r][k]) > ma) { ma = fabs(Ah[r][k]); mi = r; }
     if (ma == 0.0) { for (int r = 0; r < 6; r++) Ah[r][6] = 0.0; break; }
-    if (mi != k) for (int c = k; c < 7; c++) { double t = Ah[k][c]; Ah[k][c] = Ah[mi][c]; Ah[mi][c] = t; }
-    double piv = Ah[k][k];
+    if (mi != k) for (int c = k; c < 7; c++) { real_t t = Ah[k][c]; Ah[k][c] = Ah[mi][c]; Ah[mi][c] = t; }
+    real_t piv = Ah[k][k];
     for (int c = k + 1; c < 7; c++) Ah[k][c] = Ah[k][c] / piv;
     Ah[k][k] = 1.0;
     for (int r = 0; r < 6; r++) {
       if (r == k) continue;
-      double f = Ah[r][k];
+      real_t f = Ah[r][k];
       for (int c = k + 1; c < 7; c++) Ah[r][c] = Ah[r][c] - f * Ah[k][c];
       Ah[r][k] = 0.0;
     }
@@ -197,12 +212,12 @@ void orc_gauss6(const double* a, const double* b, double* x) {
 
 /* Lyapunov_func, fenep.f90:588-605: l = M(x) x - b with
  * A = A0 + diag((1 + f(x) dt/Wi)/2) and f the Peterlin factor (fenep.f90:513). */
-static inline void lyap_func(const double* A0, const double* b, const double* x,
-                             double dt, double wi, double lp, double* l) {
-  double A[9];
+static inline void lyap_func(const real_t* A0, const real_t* b, const real_t* x,
+                             double dt, real_t wi, real_t lp, real_t* l) {
+  real_t A[9];
   for (int m = 0; m < 9; m++) A[m] = A0[m];
-  double f = (lp * lp - 3.0) / (((lp * lp - x[0]) - x[3]) - x[5]);
-  double dg = (1.0 + (f * dt) / wi) / 2.0;
+  real_t f = (lp * lp - 3.0) / (((lp * lp - x[0]) - x[3]) - x[5]);
+  real_t dg = (1.0 + (f * dt) / wi) / 2.0;
 #define AM(r, c) A[((r) - 1) + 3 * ((c) - 1)]
   AM(1, 1) = AM(1, 1) + dg;
   AM(2, 2) = AM(2, 2) + dg;
@@ -218,9 +233,9 @@ static inline void lyap_func(const double* A0, const double* b, const double* x,
 
 /* One cell of Lyapunov (fenep.f90:554-578): Newton from x0 with a central
  * finite-difference Jacobian; returns the iteration count. */
-int orc_newton_cell(const double* A0, const double* b, const double* x0, double dt,
-                    double wi, double lp, double dc, double eps, int itrmax, double* xo) {
-  double x[6], x1[6], x2[6], l[6], l1[6], l2[6], J[36], r[6];
+static int newton_cell_r(const real_t* A0, const real_t* b, const real_t* x0, double dt,
+                         real_t wi, real_t lp, double dc, double eps, int itrmax, real_t* xo) {
+  real_t x[6], x1[6], x2[6], l[6], l1[6], l2[6], J[36], r[6];
   for (int m = 0; m < 6; m++) x[m] = x0[m];
   int itr;
   for (itr = 1; itr <= itrmax; itr++) {
@@ -233,9 +248,9 @@ int orc_newton_cell(const double* A0, const double* b, const double* x0, double 
       for (int row = 0; row < 6; row++) J[row + 6 * col] = (l1[row] - l2[row]) / (2.0 * dc);
     }
     lyap_func(A0, b, x, dt, wi, lp, l);
-    orc_gauss6(J, l, r);
+    gauss6_r(J, l, r);
     for (int m = 0; m < 6; m++) x[m] = x[m] - r[m];
-    double sr = 0.0, sx = 0.0;
+    real_t sr = 0.0, sx = 0.0;
     for (int m = 0; m < 6; m++) sr = sr + r[m] * r[m];
     for (int m = 0; m < 6; m++) sx = sx + x[m] * x[m];
     if (sqrt(sr / sx) < eps) break;
@@ -249,7 +264,7 @@ int orc_newton_cell(const double* A0, const double* b, const double* x0, double 
 /* Stockham radix-2 autosort DFT, forward e^{-2 pi i jk/n}, unnormalised;
  * inverse with the conjugate twiddles.  tw holds (cos, -sin)(2 pi j/n),
  * j < n/2, computed once on the host. Returns the buffer with the result. */
-static double* stockham(int n, double* x, double* y, const double* tw, int inv) {
+static real_t* stockham(int n, real_t* x, real_t* y, const double* tw, int inv) {
   int len = n, s = 1;
   while (len > 1) {
     int m = len / 2;
@@ -257,26 +272,26 @@ static double* stockham(int n, double* x, double* y, const double* tw, int inv) 
       double wr = tw[2 * (p * s)], wi = tw[2 * (p * s) + 1];
       if (inv) wi = -wi;
       for (int q = 0; q < s; q++) {
-        const double* a = x + 2 * (q + s * p);
-        const double* b = x + 2 * (q + s * (p + m));
-        double* y0 = y + 2 * (q + s * (2 * p));
-        double* y1 = y + 2 * (q + s * (2 * p + 1));
-        double ar = a[0], ai = a[1], br = b[0], bi = b[1];
+        const real_t* a = x + 2 * (q + s * p);
+        const real_t* b = x + 2 * (q + s * (p + m));
+        real_t* y0 = y + 2 * (q + s * (2 * p));
+        real_t* y1 = y + 2 * (q + s * (2 * p + 1));
+        real_t ar = a[0], ai = a[1], br = b[0], bi = b[1];
         y0[0] = ar + br;
         y0[1] = ai + bi;
-        double dr = ar - br, di = ai - bi;
+        real_t dr = ar - br, di = ai - bi;
         y1[0] = dr * wr - di * wi;
         y1[1] = dr * wi + di * wr;
       }
     }
-    double* t = x; x = y; y = t;
+    real_t* t = x; x = y; y = t;
     len = m; s *= 2;
   }
   return x;
 }
 
 static double* twiddles(int n) {
-  double* tw = zalloc(n > 1 ? n : 2);
+  double* tw = dzalloc(n > 1 ? n : 2);
   for (int j = 0; j < n / 2; j++) {
     double ang = (2.0 * M_PI * (double)j) / (double)n;
     tw[2 * j] = cos(ang);
@@ -289,7 +304,7 @@ static double* twiddles(int n) {
  * th = 2 pi m/N (DECISION: fenep.f90:1458 uses (i-1)*dX, equal only for
  * L = 2 pi) */
 static double* modwav(int n, double h, int count) {
-  double* k2 = zalloc(count);
+  double* k2 = dzalloc(count);
   for (int m = 0; m < count; m++) {
     double th = (2.0 * M_PI * (double)m) / (double)n;
     k2[m] = (2.0 * (1.0 - cos(th))) / (h * h);
@@ -297,31 +312,33 @@ static double* modwav(int n, double h, int count) {
   return k2;
 }
 
-void orc_fft(int n, double* data, int inverse) {
+#ifndef ORC_CS
+static void fft_r(int n, real_t* data, int inverse) {
   double* tw = twiddles(n);
-  double* w = zalloc(2 * n);
-  double* r = stockham(n, data, w, tw, inverse);
-  if (r != data) memcpy(data, r, sizeof(double) * 2 * n);
+  real_t* w = zalloc(2 * n);
+  real_t* r = stockham(n, data, w, tw, inverse);
+  if (r != data) memcpy(data, r, sizeof(real_t) * 2 * n);
   free(w); free(tw);
 }
+#endif
 
 /* real line (n) -> half spectrum (n/2+1) */
-static void rfft_line(orc_t* o, const double* in, long in_stride, double* out, long out_stride,
-                      double* b0, double* b1) {
+static void rfft_line(orc_t* o, const real_t* in, long in_stride, real_t* out, long out_stride,
+                      real_t* b0, real_t* b1) {
   int n = o->nx;
   for (int i = 0; i < n; i++) { b0[2 * i] = in[i * in_stride]; b0[2 * i + 1] = 0.0; }
-  double* r = stockham(n, b0, b1, o->twx, 0);
+  real_t* r = stockham(n, b0, b1, o->twx, 0);
   for (int i = 0; i <= n / 2; i++) { out[2 * i * out_stride] = r[2 * i]; out[2 * i * out_stride + 1] = r[2 * i + 1]; }
 }
 /* half spectrum -> real line (c2r: imaginary parts of DC and Nyquist ignored) */
-static void irfft_line(orc_t* o, const double* in, long in_stride, double* out, long out_stride,
-                       double* b0, double* b1) {
+static void irfft_line(orc_t* o, const real_t* in, long in_stride, real_t* out, long out_stride,
+                       real_t* b0, real_t* b1) {
   int n = o->nx, h = n / 2;
   for (int i = 0; i <= h; i++) { b0[2 * i] = in[2 * i * in_stride]; b0[2 * i + 1] = in[2 * i * in_stride + 1]; }
   b0[1] = 0.0;
   b0[2 * h + 1] = 0.0;
   for (int i = 1; i < h; i++) { b0[2 * (n - i)] = b0[2 * i]; b0[2 * (n - i) + 1] = -b0[2 * i + 1]; }
-  double* r = stockham(n, b0, b1, o->twx, 1);
+  real_t* r = stockham(n, b0, b1, o->twx, 1);
   for (int i = 0; i < n; i++) out[i * out_stride] = r[2 * i];
 }
 /* complex lines along an axis of the spectral array */
@@ -329,22 +346,22 @@ static void cfft_axis(orc_t* o, int axis, int inv, long j0, long j1) {
   long nxh = o->nxh, sy = o->sy, nz = o->nz;
   int n = axis == 1 ? o->ny : o->nz;
   const double* tw = axis == 1 ? o->twy : o->twz;
-  double* b0 = zalloc(2 * n); double* b1 = zalloc(2 * n);
+  real_t* b0 = zalloc(2 * n); real_t* b1 = zalloc(2 * n);
   if (axis == 1) {
     for (long k = 0; k < nz; k++)
       for (long i = 0; i < nxh; i++) {
-        double* base = o->spec + 2 * (i + nxh * (o->jo + sy * k));
+        real_t* base = o->spec + 2 * (i + nxh * (o->jo + sy * k));
         for (int j = 0; j < n; j++) { b0[2 * j] = base[2 * nxh * j]; b0[2 * j + 1] = base[2 * nxh * j + 1]; }
-        double* r = stockham(n, b0, b1, tw, inv);
+        real_t* r = stockham(n, b0, b1, tw, inv);
         for (int j = 0; j < n; j++) { base[2 * nxh * j] = r[2 * j]; base[2 * nxh * j + 1] = r[2 * j + 1]; }
       }
   } else {
     for (long j = j0; j < j1; j++)
       for (long i = 0; i < nxh; i++) {
-        double* base = o->spec + 2 * (i + nxh * j);
+        real_t* base = o->spec + 2 * (i + nxh * j);
         long st = 2 * nxh * sy;
         for (int k = 0; k < n; k++) { b0[2 * k] = base[st * k]; b0[2 * k + 1] = base[st * k + 1]; }
-        double* r = stockham(n, b0, b1, tw, inv);
+        real_t* r = stockham(n, b0, b1, tw, inv);
         for (int k = 0; k < n; k++) { base[st * k] = r[2 * k]; base[st * k + 1] = r[2 * k + 1]; }
       }
   }
@@ -353,14 +370,14 @@ static void cfft_axis(orc_t* o, int axis, int inv, long j0, long j1) {
 
 /* tdma, fenep_2d.f90:1145-1177 with real coefficients (a diagonal, b upper,
  * c lower) and complex right-hand side; pivots below 1e-16 give P=Q=0. */
-void orc_tdma(int n, const double* a, const double* b, const double* c, const double* dre,
-              const double* dim, double* xre, double* xim) {
-  double* P = zalloc(n); double* Qr = zalloc(n); double* Qi = zalloc(n);
+static void tdma_r(int n, const real_t* a, const real_t* b, const real_t* c, const real_t* dre,
+                   const real_t* dim, real_t* xre, real_t* xim) {
+  real_t* P = zalloc(n); real_t* Qr = zalloc(n); real_t* Qi = zalloc(n);
   P[0] = -b[0] / a[0];
   Qr[0] = dre[0] / a[0];
   Qi[0] = dim[0] / a[0];
   for (int i = 1; i < n; i++) {
-    double den = a[i] + c[i] * P[i - 1];
+    real_t den = a[i] + c[i] * P[i - 1];
     if (fabs(den) < 1.0e-16) { P[i] = 0.0; Qr[i] = 0.0; Qi[i] = 0.0; }
     else {
       P[i] = -b[i] / den;
@@ -381,15 +398,15 @@ void orc_tdma(int n, const double* a, const double* b, const double* c, const do
  * (fenep.f90:1400-1428) with the LHS of fenep.f90:1455-1463 / 1536-1544.
  * Walls: FFT in x,z, then the tridiagonal line solve with the boundary rows
  * of fenep_2d.f90:1222-1237 (U,W), 1262-1276 (V), 1318-1331 (Phi). */
-static void solve_grid(orc_t* o, int kind, const double* q, double* x) {
+static void solve_grid(orc_t* o, int kind, const real_t* q, real_t* x) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
   const long nxh = o->nxh, sy = o->sy;
-  double* b0 = zalloc(2 * nx); double* b1 = zalloc(2 * nx);
+  real_t* b0 = zalloc(2 * nx); real_t* b1 = zalloc(2 * nx);
   /* forward x */
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       rfft_line(o, q + II(o, 1, j, k), 1, o->spec + 2 * (nxh * ((j - 1 + o->jo) + sy * (k - 1))), 1, b0, b1);
-  const double coef = ((o->p.dt * o->p.beta) / o->p.re) / 2.0;
+  const real_t coef = ((o->p.dt * o->beta) / o->p.re) / 2.0;
   if (!o->walls) {
     cfft_axis(o, 1, 0, 0, 0);
     cfft_axis(o, 2, 0, 0, sy);
@@ -397,11 +414,11 @@ static void solve_grid(orc_t* o, int kind, const double* q, double* x) {
     for (long k = 0; k < nz; k++)
       for (long j = 0; j < ny; j++)
         for (long i = 0; i < nxh; i++) {
-          double lhs;
+          real_t lhs;
           if (kind == 3) lhs = -((o->kx2[i] + o->ky2[j]) + o->kz2[k]);
           else lhs = 1.0 + coef * ((o->kx2[i] + o->ky2[j]) + o->kz2[k]);
-          double* s = o->spec + 2 * (i + nxh * (j + sy * k));
-          double re = s[0] / ntot, im = s[1] / ntot;
+          real_t* s = o->spec + 2 * (i + nxh * (j + sy * k));
+          real_t re = s[0] / ntot, im = s[1] / ntot;
           if (fabs(lhs) < 1.0e-16) { s[0] = 0.0; s[1] = 0.0; }
           else { s[0] = re / lhs; s[1] = im / lhs; }
         }
@@ -411,14 +428,14 @@ static void solve_grid(orc_t* o, int kind, const double* q, double* x) {
     cfft_axis(o, 2, 0, 1, ny + 1);
     const double nxz = (double)((long)nx * nz);
     const int size = ny + 2;
-    double *a = zalloc(size), *b = zalloc(size), *c = zalloc(size);
-    double *dr = zalloc(size), *di = zalloc(size), *xr = zalloc(size), *xi = zalloc(size);
+    real_t *a = zalloc(size), *b = zalloc(size), *c = zalloc(size);
+    real_t *dr = zalloc(size), *di = zalloc(size), *xr = zalloc(size), *xi = zalloc(size);
     const double idy2 = o->dy * o->dy;
     for (long k = 0; k < nz; k++)
       for (long i = 0; i < nxh; i++) {
-        double kk = o->kx2[i] + o->kz2[k];
+        real_t kk = o->kx2[i] + o->kz2[k];
         for (int r = 1; r <= ny; r++) {
-          double* s = o->spec + 2 * (i + nxh * (r + sy * k));
+          real_t* s = o->spec + 2 * (i + nxh * (r + sy * k));
           dr[r] = s[0] / nxz; di[r] = s[1] / nxz;
         }
         if (kind == 3) {
@@ -434,9 +451,9 @@ static void solve_grid(orc_t* o, int kind, const double* q, double* x) {
           a[0] = 1.0; b[0] = 1.0; c[0] = 0.0; dr[0] = 0.0; di[0] = 0.0;
           a[size - 1] = 1.0; c[size - 1] = 1.0; b[size - 1] = 0.0; dr[size - 1] = 0.0; di[size - 1] = 0.0;
         }
-        orc_tdma(size, a, b, c, dr, di, xr, xi);
+        tdma_r(size, a, b, c, dr, di, xr, xi);
         for (int r = 0; r < size; r++) {
-          double* s = o->spec + 2 * (i + nxh * (r + sy * k));
+          real_t* s = o->spec + 2 * (i + nxh * (r + sy * k));
           s[0] = xr[r]; s[1] = xi[r];
         }
       }
@@ -461,9 +478,9 @@ static void cpm_axis(orc_t* o, int d) {
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
         long m = GI(o, i, j, k);
-        double cm[6], c[6], cp[6], po[6], no[6];
+        real_t cm[6], c[6], cp[6], po[6], no[6];
         for (int q = 0; q < 6; q++) { cm[q] = o->C[q][m - s]; c[q] = o->C[q][m]; cp[q] = o->C[q][m + s]; }
-        int idx = orc_select(cm, c, cp, po, no);
+        int idx = select_r(cm, c, cp, po, no);
         cnt[idx - 1] += 1.0;
         long fl, fr;  /* faces a-1 (Cp) and a (Cn) along the axis */
         if (d == 0) { fl = (i - 1) + fn[0] * ((j - 1) + (long)ny * (k - 1)); fr = fl + 1; }
@@ -474,7 +491,7 @@ static void cpm_axis(orc_t* o, int d) {
   for (int l = 0; l < 4; l++) o->sel_count[l] += cnt[l];
   /* periodic wrap of the end faces (fenep.f90:240-241) */
   for (int q = 0; q < 6; q++) {
-    double* Cp = o->Cp[d][q]; double* Cn = o->Cn[d][q];
+    real_t* Cp = o->Cp[d][q]; real_t* Cn = o->Cn[d][q];
     if (d == 0) {
       for (long r = 0; r < (long)ny * nz; r++) { Cp[nx + fn[0] * r] = Cp[0 + fn[0] * r]; Cn[0 + fn[0] * r] = Cn[nx + fn[0] * r]; }
     } else if (d == 1) {
@@ -492,27 +509,27 @@ static void cpm_axis(orc_t* o, int d) {
 /* Cstar, fenep.f90:464-507 */
 static void cstar(orc_t* o) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dt = o->p.dt, dx = o->dx, dy = o->dy, dz = o->dz;
+  const real_t dt = o->p.dt, dx = o->dx, dy = o->dy, dz = o->dz;
   const long fx = nx + 1, fy = ny + 1;
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
-        double u1 = o->U[GI(o, i - 1, j, k)], u2 = o->U[GI(o, i, j, k)];
-        double v1 = o->V[GI(o, i, j - 1, k)], v2 = o->V[GI(o, i, j, k)];
-        double w1 = o->W[GI(o, i, j, k - 1)], w2 = o->W[GI(o, i, j, k)];
-        double s1 = ((-u2 + fabs(u2)) * dt) / (2.0 * dx);
-        double s2 = 1.0 / 6.0 + ((-u2 - fabs(u2)) * dt) / (2.0 * dx);
-        double s3 = 1.0 / 6.0 + ((u1 - fabs(u1)) * dt) / (2.0 * dx);
-        double s4 = ((u1 + fabs(u1)) * dt) / (2.0 * dx);
-        double s5 = ((-v2 + fabs(v2)) * dt) / (2.0 * dy);
-        double s6 = 1.0 / 6.0 + ((-v2 - fabs(v2)) * dt) / (2.0 * dy);
-        double s7 = 1.0 / 6.0 + ((v1 - fabs(v1)) * dt) / (2.0 * dy);
-        double s8 = ((v1 + fabs(v1)) * dt) / (2.0 * dy);
-        double s9 = ((-w2 + fabs(w2)) * dt) / (2.0 * dz);
-        double s10 = 1.0 / 6.0 + ((-w2 - fabs(w2)) * dt) / (2.0 * dz);
-        double s11 = 1.0 / 6.0 + ((w1 - fabs(w1)) * dt) / (2.0 * dz);
-        double s12 = ((w1 + fabs(w1)) * dt) / (2.0 * dz);
+        real_t u1 = o->U[GI(o, i - 1, j, k)], u2 = o->U[GI(o, i, j, k)];
+        real_t v1 = o->V[GI(o, i, j - 1, k)], v2 = o->V[GI(o, i, j, k)];
+        real_t w1 = o->W[GI(o, i, j, k - 1)], w2 = o->W[GI(o, i, j, k)];
+        real_t s1 = ((-u2 + fabs(u2)) * dt) / (2.0 * dx);
+        real_t s2 = 1.0 / 6.0 + ((-u2 - fabs(u2)) * dt) / (2.0 * dx);
+        real_t s3 = 1.0 / 6.0 + ((u1 - fabs(u1)) * dt) / (2.0 * dx);
+        real_t s4 = ((u1 + fabs(u1)) * dt) / (2.0 * dx);
+        real_t s5 = ((-v2 + fabs(v2)) * dt) / (2.0 * dy);
+        real_t s6 = 1.0 / 6.0 + ((-v2 - fabs(v2)) * dt) / (2.0 * dy);
+        real_t s7 = 1.0 / 6.0 + ((v1 - fabs(v1)) * dt) / (2.0 * dy);
+        real_t s8 = ((v1 + fabs(v1)) * dt) / (2.0 * dy);
+        real_t s9 = ((-w2 + fabs(w2)) * dt) / (2.0 * dz);
+        real_t s10 = 1.0 / 6.0 + ((-w2 - fabs(w2)) * dt) / (2.0 * dz);
+        real_t s11 = 1.0 / 6.0 + ((w1 - fabs(w1)) * dt) / (2.0 * dz);
+        real_t s12 = ((w1 + fabs(w1)) * dt) / (2.0 * dz);
         long xl = (i - 1) + fx * ((j - 1) + (long)ny * (k - 1)), xr = xl + 1;
         long yl = (i - 1) + (long)nx * ((j - 1) + fy * (k - 1)), yr = yl + nx;
         long zl = (i - 1) + (long)nx * ((j - 1) + (long)ny * (k - 1)), zr = zl + (long)nx * ny;
@@ -528,32 +545,33 @@ static void cstar(orc_t* o) {
 /* Lyapunov, fenep.f90:518-585 */
 static void lyapunov(orc_t* o) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dt = o->p.dt, dx = o->dx, dy = o->dy, dz = o->dz, wi = o->p.wi;
+  const double dt = o->p.dt, dx = o->dx, dy = o->dy, dz = o->dz;
+  const real_t wi = o->wi;
   double itr_total = 0.0;
 #pragma omp parallel for reduction(+ : itr_total) schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
-        const double *U = o->U, *V = o->V, *W = o->W;
+        const real_t *U = o->U, *V = o->V, *W = o->W;
 #define Ug(a, b, c) U[GI(o, a, b, c)]
 #define Vg(a, b, c) V[GI(o, a, b, c)]
 #define Wg(a, b, c) W[GI(o, a, b, c)]
         long m = II(o, i, j, k);
-        double bb[6];
+        real_t bb[6];
         for (int q = 0; q < 6; q++) bb[q] = o->Cx[q][m];
         bb[0] = bb[0] + dt / wi;
         bb[3] = bb[3] + dt / wi;
         bb[5] = bb[5] + dt / wi;
-        double A[9];  /* A0 built then transposed: A[r + 3c] = A0(c, r) */
-        double a11 = -((dt * (-Ug(i - 1, j, k) + Ug(i, j, k))) / dx);
-        double a12 = -((dt * (((-Vg(i - 1, j - 1, k) + Vg(i + 1, j - 1, k)) - Vg(i - 1, j, k)) + Vg(i + 1, j, k))) / (4.0 * dx));
-        double a13 = -((dt * (((-Wg(i - 1, j, k - 1) + Wg(i + 1, j, k - 1)) - Wg(i - 1, j, k)) + Wg(i + 1, j, k))) / (4.0 * dx));
-        double a21 = -((dt * (((-Ug(i - 1, j - 1, k) - Ug(i, j - 1, k)) + Ug(i - 1, j + 1, k)) + Ug(i, j + 1, k))) / (4.0 * dy));
-        double a22 = -((dt * (-Vg(i, j - 1, k) + Vg(i, j, k))) / dy);
-        double a23 = -((dt * (((-Wg(i, j - 1, k - 1) + Wg(i, j + 1, k - 1)) - Wg(i, j - 1, k)) + Wg(i, j + 1, k))) / (4.0 * dy));
-        double a31 = -((dt * (((-Ug(i - 1, j, k - 1) - Ug(i, j, k - 1)) + Ug(i - 1, j, k + 1)) + Ug(i, j, k + 1))) / (4.0 * dz));
-        double a32 = -((dt * (((-Vg(i, j - 1, k - 1) - Vg(i, j, k - 1)) + Vg(i, j - 1, k + 1)) + Vg(i, j, k + 1))) / (4.0 * dz));
-        double a33 = -((dt * (-Wg(i, j, k - 1) + Wg(i, j, k))) / dz);
+        real_t A[9];  /* A0 built then transposed: A[r + 3c] = A0(c, r) */
+        real_t a11 = -((dt * (-Ug(i - 1, j, k) + Ug(i, j, k))) / dx);
+        real_t a12 = -((dt * (((-Vg(i - 1, j - 1, k) + Vg(i + 1, j - 1, k)) - Vg(i - 1, j, k)) + Vg(i + 1, j, k))) / (4.0 * dx));
+        real_t a13 = -((dt * (((-Wg(i - 1, j, k - 1) + Wg(i + 1, j, k - 1)) - Wg(i - 1, j, k)) + Wg(i + 1, j, k))) / (4.0 * dx));
+        real_t a21 = -((dt * (((-Ug(i - 1, j - 1, k) - Ug(i, j - 1, k)) + Ug(i - 1, j + 1, k)) + Ug(i, j + 1, k))) / (4.0 * dy));
+        real_t a22 = -((dt * (-Vg(i, j - 1, k) + Vg(i, j, k))) / dy);
+        real_t a23 = -((dt * (((-Wg(i, j - 1, k - 1) + Wg(i, j + 1, k - 1)) - Wg(i, j - 1, k)) + Wg(i, j + 1, k))) / (4.0 * dy));
+        real_t a31 = -((dt * (((-Ug(i - 1, j, k - 1) - Ug(i, j, k - 1)) + Ug(i - 1, j, k + 1)) + Ug(i, j, k + 1))) / (4.0 * dz));
+        real_t a32 = -((dt * (((-Vg(i, j - 1, k - 1) - Vg(i, j, k - 1)) + Vg(i, j - 1, k + 1)) + Vg(i, j, k + 1))) / (4.0 * dz));
+        real_t a33 = -((dt * (-Wg(i, j, k - 1) + Wg(i, j, k))) / dz);
 #undef Ug
 #undef Vg
 #undef Wg
@@ -561,11 +579,11 @@ static void lyapunov(orc_t* o) {
         A[0] = a11; A[3] = a21; A[6] = a31;
         A[1] = a12; A[4] = a22; A[7] = a32;
         A[2] = a13; A[5] = a23; A[8] = a33;
-        double x0[6], x[6];
+        real_t x0[6], x[6];
         long g = GI(o, i, j, k);
         for (int q = 0; q < 6; q++) x0[q] = o->C[q][g];
-        int itr = orc_newton_cell(A, bb, x0, dt, wi, o->p.lp, o->p.newton_dc, o->p.newton_eps,
-                                  o->p.newton_itrmax, x);
+        int itr = newton_cell_r(A, bb, x0, dt, wi, o->lp, o->p.newton_dc, o->p.newton_eps,
+                                o->p.newton_itrmax, x);
         itr_total += (double)itr;
         for (int q = 0; q < 6; q++) o->Cx[q][m] = x[q];  /* stage; C is read by neighbours */
       }
@@ -580,26 +598,27 @@ static void lyapunov(orc_t* o) {
 /* polymer_stress, fenep.f90:645-691 (C0 pointwise on the ghosted array) */
 static void polymer_stress(orc_t* o) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dx = o->dx, dy = o->dy, dz = o->dz, lp = o->p.lp;
+  const double dx = o->dx, dy = o->dy, dz = o->dz;
+  const real_t lp = o->lp;
   static const double E[6] = {1.0, 0.0, 0.0, 1.0, 0.0, 1.0};
   for (long g = 0; g < o->gn; g++) {
-    double f = (lp * lp - 3.0) / (((lp * lp - o->C[0][g]) - o->C[3][g]) - o->C[5][g]);
+    real_t f = (lp * lp - 3.0) / (((lp * lp - o->C[0][g]) - o->C[3][g]) - o->C[5][g]);
     for (int q = 0; q < 6; q++) o->C0[q][g] = f * o->C[q][g] - E[q];
   }
-  const double coef = (1.0 - o->p.beta) / (o->p.re * o->p.wi);
+  const real_t coef = (1.0 - o->beta) / (o->p.re * o->wi);
 #define C0g(q, a, b, c) o->C0[q][GI(o, a, b, c)]
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
         long m = II(o, i, j, k);
-        double tx = ((-C0g(0, i, j, k) + C0g(0, i + 1, j, k)) / dx
+        real_t tx = ((-C0g(0, i, j, k) + C0g(0, i + 1, j, k)) / dx
                      + (((-C0g(1, i, j - 1, k) - C0g(1, i + 1, j - 1, k)) + C0g(1, i, j + 1, k)) + C0g(1, i + 1, j + 1, k)) / (4.0 * dy))
                     + (((-C0g(2, i, j, k - 1) - C0g(2, i + 1, j, k - 1)) + C0g(2, i, j, k + 1)) + C0g(2, i + 1, j, k + 1)) / (4.0 * dz);
-        double ty = ((((-C0g(1, i - 1, j, k) + C0g(1, i + 1, j, k)) - C0g(1, i - 1, j + 1, k)) + C0g(1, i + 1, j + 1, k)) / (4.0 * dx)
+        real_t ty = ((((-C0g(1, i - 1, j, k) + C0g(1, i + 1, j, k)) - C0g(1, i - 1, j + 1, k)) + C0g(1, i + 1, j + 1, k)) / (4.0 * dx)
                      + (-C0g(3, i, j, k) + C0g(3, i, j + 1, k)) / dy)
                     + (((-C0g(4, i, j, k - 1) - C0g(4, i, j + 1, k - 1)) + C0g(4, i, j, k + 1)) + C0g(4, i, j + 1, k + 1)) / (4.0 * dz);
-        double tz = ((((-C0g(2, i - 1, j, k) + C0g(2, i + 1, j, k)) - C0g(2, i - 1, j, k + 1)) + C0g(2, i + 1, j, k + 1)) / (4.0 * dx)
+        real_t tz = ((((-C0g(2, i - 1, j, k) + C0g(2, i + 1, j, k)) - C0g(2, i - 1, j, k + 1)) + C0g(2, i + 1, j, k + 1)) / (4.0 * dx)
                      + (((-C0g(4, i, j - 1, k) + C0g(4, i, j + 1, k)) - C0g(4, i, j - 1, k + 1)) + C0g(4, i, j + 1, k + 1)) / (4.0 * dy))
                     + (-C0g(5, i, j, k) + C0g(5, i, j, k + 1)) / dz;
         o->T[0][m] = coef * tx;
@@ -612,7 +631,7 @@ static void polymer_stress(orc_t* o) {
 /* convection, fenep.f90:694-734 */
 static void convection(orc_t* o) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dx = o->dx, dy = o->dy, dz = o->dz;
+  const real_t dx = o->dx, dy = o->dy, dz = o->dz;
 #define Ug(a, b, c) o->U[GI(o, a, b, c)]
 #define Vg(a, b, c) o->V[GI(o, a, b, c)]
 #define Wg(a, b, c) o->W[GI(o, a, b, c)]
@@ -621,20 +640,20 @@ static void convection(orc_t* o) {
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
         long m = II(o, i, j, k);
-        double h1 = (Ug(i - 1, j, k) + Ug(i, j, k)) / 2.0, h2 = (Ug(i, j, k) + Ug(i + 1, j, k)) / 2.0;
-        double ax = ((-(h1 * h1) + h2 * h2) / dx
+        real_t h1 = (Ug(i - 1, j, k) + Ug(i, j, k)) / 2.0, h2 = (Ug(i, j, k) + Ug(i + 1, j, k)) / 2.0;
+        real_t ax = ((-(h1 * h1) + h2 * h2) / dx
                      + (-((((Vg(i, j - 1, k) + Vg(i + 1, j - 1, k)) / 2.0) * (Ug(i, j - 1, k) + Ug(i, j, k))) / 2.0)
                         + (((Vg(i, j, k) + Vg(i + 1, j, k)) / 2.0) * (Ug(i, j, k) + Ug(i, j + 1, k))) / 2.0) / dy)
                     + (-((((Wg(i, j, k - 1) + Wg(i + 1, j, k - 1)) / 2.0) * (Ug(i, j, k - 1) + Ug(i, j, k))) / 2.0)
                        + (((Wg(i, j, k) + Wg(i + 1, j, k)) / 2.0) * (Ug(i, j, k) + Ug(i, j, k + 1))) / 2.0) / dz;
-        double g1 = (Vg(i, j - 1, k) + Vg(i, j, k)) / 2.0, g2 = (Vg(i, j, k) + Vg(i, j + 1, k)) / 2.0;
-        double ay = ((-((((Ug(i - 1, j, k) + Ug(i - 1, j + 1, k)) / 2.0) * (Vg(i - 1, j, k) + Vg(i, j, k))) / 2.0)
+        real_t g1 = (Vg(i, j - 1, k) + Vg(i, j, k)) / 2.0, g2 = (Vg(i, j, k) + Vg(i, j + 1, k)) / 2.0;
+        real_t ay = ((-((((Ug(i - 1, j, k) + Ug(i - 1, j + 1, k)) / 2.0) * (Vg(i - 1, j, k) + Vg(i, j, k))) / 2.0)
                       + (((Ug(i, j, k) + Ug(i, j + 1, k)) / 2.0) * (Vg(i, j, k) + Vg(i + 1, j, k))) / 2.0) / dx
                      + (-(g1 * g1) + g2 * g2) / dy)
                     + (-((((Wg(i, j, k - 1) + Wg(i, j + 1, k - 1)) / 2.0) * (Vg(i, j, k - 1) + Vg(i, j, k))) / 2.0)
                        + (((Wg(i, j, k) + Wg(i, j + 1, k)) / 2.0) * (Vg(i, j, k) + Vg(i, j, k + 1))) / 2.0) / dz;
-        double e1 = (Wg(i, j, k - 1) + Wg(i, j, k)) / 2.0, e2 = (Wg(i, j, k) + Wg(i, j, k + 1)) / 2.0;
-        double az = ((-((((Ug(i - 1, j, k) + Ug(i - 1, j, k + 1)) / 2.0) * (Wg(i - 1, j, k) + Wg(i, j, k))) / 2.0)
+        real_t e1 = (Wg(i, j, k - 1) + Wg(i, j, k)) / 2.0, e2 = (Wg(i, j, k) + Wg(i, j, k + 1)) / 2.0;
+        real_t az = ((-((((Ug(i - 1, j, k) + Ug(i - 1, j, k + 1)) / 2.0) * (Wg(i - 1, j, k) + Wg(i, j, k))) / 2.0)
                       + (((Ug(i, j, k) + Ug(i, j, k + 1)) / 2.0) * (Wg(i, j, k) + Wg(i + 1, j, k))) / 2.0) / dx
                      + (-((((Vg(i, j - 1, k) + Vg(i, j - 1, k + 1)) / 2.0) * (Wg(i, j - 1, k) + Wg(i, j, k))) / 2.0)
                         + (((Vg(i, j, k) + Vg(i, j, k + 1)) / 2.0) * (Wg(i, j, k) + Wg(i, j + 1, k))) / 2.0) / dy)
@@ -648,18 +667,18 @@ static void convection(orc_t* o) {
 /* viscous, fenep.f90:736-770 */
 static void viscous(orc_t* o) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dx = o->dx, dy = o->dy, dz = o->dz;
-  const double cb = o->p.beta / o->p.re;
-  double* F[3] = {o->U, o->V, o->W};
+  const real_t dx = o->dx, dy = o->dy, dz = o->dz;
+  const real_t cb = o->beta / o->p.re;
+  real_t* F[3] = {o->U, o->V, o->W};
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
         long m = II(o, i, j, k);
         for (int c = 0; c < 3; c++) {
-          const double* X = F[c];
-          double x0 = X[GI(o, i, j, k)];
-          double b = (((X[GI(o, i - 1, j, k)] - 2.0 * x0) + X[GI(o, i + 1, j, k)]) / (dx * dx)
+          const real_t* X = F[c];
+          real_t x0 = X[GI(o, i, j, k)];
+          real_t b = (((X[GI(o, i - 1, j, k)] - 2.0 * x0) + X[GI(o, i + 1, j, k)]) / (dx * dx)
                       + ((X[GI(o, i, j - 1, k)] - 2.0 * x0) + X[GI(o, i, j + 1, k)]) / (dy * dy))
                      + ((X[GI(o, i, j, k - 1)] - 2.0 * x0) + X[GI(o, i, j, k + 1)]) / (dz * dz);
           o->B[c][m] = cb * b;
@@ -670,31 +689,31 @@ static void viscous(orc_t* o) {
 #undef Wg
 }
 
-static inline double fval(orc_t* o, int c, long m) {
+static inline real_t fval(orc_t* o, int c, long m) {
   return (o->F[c] ? o->F[c][m] : 0.0) + o->p.body_force[c];
 }
 
 /* pressure gradient term dt*(-P(i)+P(i+1))/h for component c at cell (i,j,k) */
-static inline double dpdx(orc_t* o, const double* P, int c, int i, int j, int k) {
-  double h = c == 0 ? o->dx : (c == 1 ? o->dy : o->dz);
-  double p0 = P[GI(o, i, j, k)];
-  double p1 = c == 0 ? P[GI(o, i + 1, j, k)] : (c == 1 ? P[GI(o, i, j + 1, k)] : P[GI(o, i, j, k + 1)]);
+static inline real_t dpdx(orc_t* o, const real_t* P, int c, int i, int j, int k) {
+  real_t h = c == 0 ? o->dx : (c == 1 ? o->dy : o->dz);
+  real_t p0 = P[GI(o, i, j, k)];
+  real_t p1 = c == 0 ? P[GI(o, i + 1, j, k)] : (c == 1 ? P[GI(o, i, j + 1, k)] : P[GI(o, i, j, k + 1)]);
   return (o->p.dt * (-p0 + p1)) / h;
 }
 
 static const int BCK[3] = {BC_U, BC_V, BC_W};
 
 /* right-hand side of the predictor, per method */
-static void predictor_rhs(orc_t* o, int c, double* Q) {
+static void predictor_rhs(orc_t* o, int c, real_t* Q) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dt = o->p.dt;
-  double* Uc = c == 0 ? o->U : (c == 1 ? o->V : o->W);
+  const real_t dt = o->p.dt;
+  real_t* Uc = c == 0 ? o->U : (c == 1 ? o->V : o->W);
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
         long m = II(o, i, j, k);
-        double u = Uc[GI(o, i, j, k)];
-        double r;
+        real_t u = Uc[GI(o, i, j, k)];
+        real_t r;
         if (o->p.method == FP_METHOD_EXPLICIT) {
           /* navier, fenep.f90:803-806 */
           r = (((u - dpdx(o, o->P, c, i, j, k))
@@ -719,7 +738,7 @@ static void predictor_rhs(orc_t* o, int c, double* Q) {
       }
 }
 
-static void interior_to_ghosted(orc_t* o, const double* in, double* out) {
+static void interior_to_ghosted(orc_t* o, const real_t* in, real_t* out) {
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++) out[GI(o, i, j, k)] = in[II(o, i, j, k)];
@@ -727,7 +746,7 @@ static void interior_to_ghosted(orc_t* o, const double* in, double* out) {
 
 /* ------------------------------------------------------------------- IBM */
 /* weight, fenep.f90:2045-2055: 3-point kernel of Roma, Peskin & Berger */
-double orc_roma_weight(double r) {
+static double roma_weight(double r) {
   double a = fabs(r);
   if (a < 1.5 && a >= 0.5) return ((5.0 - 3.0 * a) - sqrt(-(3.0 * ((1.0 - a) * (1.0 - a))) + 1.0)) / 6.0;
   if (a < 0.5) return (1.0 + sqrt(-(3.0 * (r * r)) + 1.0)) / 3.0;
@@ -735,7 +754,7 @@ double orc_roma_weight(double r) {
 }
 /* delta, fenep.f90:2039-2043 */
 static inline double delta3(orc_t* o, double x, double y, double z, double xc, double yc, double zc) {
-  return ((orc_roma_weight((x - xc) / o->dx) * orc_roma_weight((y - yc) / o->dy)) * orc_roma_weight((z - zc) / o->dz))
+  return ((roma_weight((x - xc) / o->dx) * roma_weight((y - yc) / o->dy)) * roma_weight((z - zc) / o->dz))
          / ((o->dx * o->dy) * o->dz);
 }
 /* stencil origin and sample position of component c for marker (xc,yc) at
@@ -759,9 +778,9 @@ static inline void ibm_pos(orc_t* o, int c, int i, int j, int k, double* x, doub
 static void ibm_helmholtz(orc_t* o) {
   const int nc = o->nc, nl = o->nl, nz = o->nz;
   const double dx3 = (o->dx * o->dx) * o->dx, dt = o->p.dt;
-  double* Fm[3] = {o->fxc, o->fyc, o->fzc};
-  double* Bm[3] = {o->ubm, o->vbm, o->wbm};
-  double* Cm[3] = {o->uc, o->vc, o->wc};
+  real_t* Fm[3] = {o->fxc, o->fyc, o->fzc};
+  real_t* Bm[3] = {o->ubm, o->vbm, o->wbm};
+  const double* Cm[3] = {o->uc, o->vc, o->wc};
   for (int n = 1; n <= nz; n++)
     for (int mm = 1; mm <= nl; mm++)
       for (int l = 1; l <= nc; l++) {
@@ -769,7 +788,7 @@ static void ibm_helmholtz(orc_t* o) {
         double xc = o->xc[id], yc = o->yc[id], zc = o->zc[id];
         for (int c = 0; c < 3; c++) {
           int i0, j0; ibm_origin(o, c, xc, yc, &i0, &j0);
-          double acc = 0.0;
+          real_t acc = 0.0;
           for (int k = n - 1; k <= n + 1; k++)
             for (int j = j0; j <= j0 + 2; j++)
               for (int i = i0; i <= i0 + 2; i++) {
@@ -783,8 +802,8 @@ static void ibm_helmholtz(orc_t* o) {
     for (int c = 0; c < 3; c++) Fm[c][id] = Fm[c][id] + (Cm[c][id] - Bm[c][id]) / dt;
   const int nx = o->nx, ny = o->ny;
   for (int c = 0; c < 3; c++) {
-    double* ft = o->ftmp;
-    memset(ft, 0, sizeof(double) * o->gn);
+    real_t* ft = o->ftmp;
+    memset((void*)ft, 0, sizeof(real_t) * o->gn);
     for (int n = 1; n <= nz; n++)
       for (int mm = 1; mm <= nl; mm++)
         for (int l = 1; l <= nc; l++) {
@@ -823,16 +842,16 @@ static void ibm_helmholtz(orc_t* o) {
 
 /* ------------------------------------------------------------------- SOR */
 /* poisson, fenep.f90:848-905: lexicographic SOR, odd then even i per row */
-static int sor_solve(orc_t* o, const double* Q, double* Phi) {
+static int sor_solve(orc_t* o, const real_t* Q, real_t* Phi) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
   const double dx = o->dx, dy = o->dy, dz = o->dz;
   double om = o->p.sor_omega > 0.0 ? o->p.sor_omega : 2.0 / (1.0 + sin(M_PI * dx));
   double BXM = 1.0 / (dx * dx), BXP = 1.0 / (dx * dx), BYM = 1.0 / (dy * dy), BYP = 1.0 / (dy * dy);
   double BZM = 1.0 / (dz * dz), BZP = 1.0 / (dz * dz);
   double B0 = ((((BXM + BXP) + BYM) + BYP) + BZM) + BZP;
-  double er0 = 0.0;
+  real_t er0 = 0.0;
   for (long m = 0; m < o->n; m++) er0 = er0 + Q[m] * Q[m];
-  double* E = o->e;
+  real_t* E = o->e;
   int itr;
   for (itr = 1; itr <= o->p.sor_itrmax; itr++) {
     for (int k = 1; k <= nz; k++)
@@ -840,14 +859,14 @@ static int sor_solve(orc_t* o, const double* Q, double* Phi) {
         for (int l = 1; l <= 2; l++)
           for (int i = l; i <= nx; i += 2) {
             long g = GI(o, i, j, k), m = II(o, i, j, k);
-            double e = (((((BZM * Phi[GI(o, i, j, k - 1)] + BYM * Phi[GI(o, i, j - 1, k)]) + BXM * Phi[GI(o, i - 1, j, k)])
+            real_t e = (((((BZM * Phi[GI(o, i, j, k - 1)] + BYM * Phi[GI(o, i, j - 1, k)]) + BXM * Phi[GI(o, i - 1, j, k)])
                           - B0 * Phi[g]) + BXP * Phi[GI(o, i + 1, j, k)]) + BYP * Phi[GI(o, i, j + 1, k)]
                         + BZP * Phi[GI(o, i, j, k + 1)]) - Q[m];
             E[m] = e;
             Phi[g] = Phi[g] + (om * e) / B0;
           }
     fill_ghosts(o, Phi, BC_P);
-    double er = 0.0;
+    real_t er = 0.0;
     for (long m = 0; m < o->n; m++) er = er + E[m] * E[m];
     if (sqrt(er / er0) < o->p.sor_eps) break;
   }
@@ -856,8 +875,8 @@ static int sor_solve(orc_t* o, const double* Q, double* Phi) {
 }
 
 /* divergence RHS of fft_poisson / poisson, fenep.f90:1550-1552 */
-static void div_rhs(orc_t* o, double* Q) {
-  const double dt = o->p.dt;
+static void div_rhs(orc_t* o, real_t* Q) {
+  const real_t dt = o->p.dt;
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++)
@@ -869,12 +888,12 @@ static void div_rhs(orc_t* o, double* Q) {
 /* ------------------------------------------------------------------ step */
 static void one_step(orc_t* o) {
   const int nx = o->nx, ny = o->ny, nz = o->nz;
-  const double dt = o->p.dt;
+  const real_t dt = o->p.dt;
   const int first = !o->has_history;
   /* fenep.f90:2479-2497 */
   if (o->p.beta == 1.0) {
-    for (int q = 0; q < 6; q++) memset(o->C[q], 0, sizeof(double) * o->gn);
-    for (int c = 0; c < 3; c++) memset(o->T[c], 0, sizeof(double) * o->n);
+    for (int q = 0; q < 6; q++) memset((void*)o->C[q], 0, sizeof(real_t) * o->gn);
+    for (int c = 0; c < 3; c++) memset((void*)o->T[c], 0, sizeof(real_t) * o->n);
   } else {
     cpm_axis(o, 0); cpm_axis(o, 1); cpm_axis(o, 2);
     cstar(o);
@@ -885,17 +904,17 @@ static void one_step(orc_t* o) {
   viscous(o);
   if (first) {
     for (int c = 0; c < 3; c++) {
-      memcpy(o->A0[c], o->A[c], sizeof(double) * o->n);
-      memcpy(o->T0[c], o->T[c], sizeof(double) * o->n);
-      memcpy(o->B0[c], o->B[c], sizeof(double) * o->n);
+      memcpy(o->A0[c], o->A[c], sizeof(real_t) * o->n);
+      memcpy(o->T0[c], o->T[c], sizeof(real_t) * o->n);
+      memcpy(o->B0[c], o->B[c], sizeof(real_t) * o->n);
     }
   }
-  double* Q = o->q;
+  real_t* Q = o->q;
   if (o->p.method == FP_METHOD_EXPLICIT || o->p.method == FP_METHOD_FFT) {
     for (int c = 0; c < 3; c++) {
       predictor_rhs(o, c, Q);
       if (o->p.method == FP_METHOD_FFT) {
-        double* tmp = zalloc(o->n);
+        real_t* tmp = zalloc(o->n);
         solve_grid(o, c, Q, tmp);
         interior_to_ghosted(o, tmp, o->Up[c]);
         free(tmp);
@@ -909,12 +928,12 @@ static void one_step(orc_t* o) {
       predictor_rhs(o, c, Q);
       interior_to_ghosted(o, Q, o->Ua[c]);
       fill_ghosts(o, o->Ua[c], BCK[c]);
-      memcpy(o->Up[c], o->Ua[c], sizeof(double) * o->gn);
+      memcpy(o->Up[c], o->Ua[c], sizeof(real_t) * o->gn);
     }
     for (long id = 0; id < o->nmk; id++) { o->fxc[id] = 0.0; o->fyc[id] = 0.0; o->fzc[id] = 0.0; }
     ibm_helmholtz(o);
     /* ibm_predict, fenep.f90:2226-2256 */
-    double* tmp = zalloc(o->n);
+    real_t* tmp = zalloc(o->n);
     for (int c = 0; c < 3; c++) {
       for (int k = 1; k <= nz; k++)
         for (int j = 1; j <= ny; j++)
@@ -930,16 +949,16 @@ static void one_step(orc_t* o) {
   }
   /* histories (fenep.f90:837-845, 1521-1526, 2020-2025) */
   for (int c = 0; c < 3; c++) {
-    memcpy(o->A0[c], o->A[c], sizeof(double) * o->n);
-    memcpy(o->T0[c], o->T[c], sizeof(double) * o->n);
-    memcpy(o->B0[c], o->B[c], sizeof(double) * o->n);
+    memcpy(o->A0[c], o->A[c], sizeof(real_t) * o->n);
+    memcpy(o->T0[c], o->T[c], sizeof(real_t) * o->n);
+    memcpy(o->B0[c], o->B[c], sizeof(real_t) * o->n);
   }
   /* Poisson */
   div_rhs(o, Q);
   if (o->p.poisson_variant == FP_POISSON_SOR) {
     o->sor_itr = (double)sor_solve(o, Q, o->Phi);
   } else {
-    double* tmp = zalloc(o->n);
+    real_t* tmp = zalloc(o->n);
     solve_grid(o, 3, Q, tmp);
     interior_to_ghosted(o, tmp, o->Phi);
     free(tmp);
@@ -947,9 +966,9 @@ static void one_step(orc_t* o) {
     o->sor_itr = -1.0;
   }
   /* march / fft_march (fenep.f90:907-946, 1562-1604) */
-  const double coef = ((dt * o->p.beta) / o->p.re) / 2.0;
-  const double dx = o->dx, dy = o->dy, dz = o->dz;
-  double* Ph = o->Phi;
+  const real_t coef = ((dt * o->beta) / o->p.re) / 2.0;
+  const real_t dx = o->dx, dy = o->dy, dz = o->dz;
+  real_t* Ph = o->Phi;
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -960,7 +979,7 @@ static void one_step(orc_t* o) {
         if (o->p.method == FP_METHOD_EXPLICIT) {
           o->P[g] = o->P[g] + Ph[g];
         } else {
-          double lap = ((((Ph[GI(o, i - 1, j, k)] - 2.0 * Ph[g]) + Ph[GI(o, i + 1, j, k)]) / (dx * dx)
+          real_t lap = ((((Ph[GI(o, i - 1, j, k)] - 2.0 * Ph[g]) + Ph[GI(o, i + 1, j, k)]) / (dx * dx)
                          + ((Ph[GI(o, i, j - 1, k)] - 2.0 * Ph[g]) + Ph[GI(o, i, j + 1, k)]) / (dy * dy))
                         + ((Ph[GI(o, i, j, k - 1)] - 2.0 * Ph[g]) + Ph[GI(o, i, j, k + 1)]) / (dz * dz));
           o->P[g] = (o->P[g] + Ph[g]) - coef * lap;
@@ -987,6 +1006,7 @@ void* orc_create(const fp_params* p) {
   o->n = (long)p->nx * p->ny * p->nz;
   o->dx = p->lx / p->nx; o->dy = p->ly / p->ny; o->dz = p->lz / p->nz;
   o->walls = p->walls_y != 0;
+  o->wi = p->wi; o->beta = p->beta; o->lp = p->lp;
   o->U = zalloc(o->gn); o->V = zalloc(o->gn); o->W = zalloc(o->gn); o->P = zalloc(o->gn); o->Phi = zalloc(o->gn);
   for (int q = 0; q < 6; q++) { o->C[q] = zalloc(o->gn); o->C0[q] = zalloc(o->gn); o->Cx[q] = zalloc(o->n); }
   for (int c = 0; c < 3; c++) {
@@ -1025,13 +1045,17 @@ void orc_destroy(void* vo) {
   free(o);
 }
 
+/* state in and out of the restatement's scalar (identity in the FP64 build) */
+static void load_r(real_t* d, const double* src, long n) { for (long i = 0; i < n; i++) d[i] = src[i]; }
+static void store_r(double* d, const real_t* src, long n) { for (long i = 0; i < n; i++) d[i] = rv(src[i]); }
+
 int orc_set_state(void* vo, const double* U, const double* V, const double* W, const double* P,
                   const double* C6, const double* hist, int has_history) {
   orc_t* o = (orc_t*)vo;
-  memcpy(o->U, U, sizeof(double) * o->gn);
-  memcpy(o->V, V, sizeof(double) * o->gn);
-  memcpy(o->W, W, sizeof(double) * o->gn);
-  memcpy(o->P, P, sizeof(double) * o->gn);
+  load_r(o->U, U, o->gn);
+  load_r(o->V, V, o->gn);
+  load_r(o->W, W, o->gn);
+  load_r(o->P, P, o->gn);
   for (long g = 0; g < o->gn; g++)
     for (int q = 0; q < 6; q++) o->C[q][g] = C6[6 * g + q];
   fill_ghosts(o, o->U, BC_U); fill_ghosts(o, o->V, BC_V); fill_ghosts(o, o->W, BC_W);
@@ -1039,37 +1063,37 @@ int orc_set_state(void* vo, const double* U, const double* V, const double* W, c
   for (int q = 0; q < 6; q++) fill_ghosts(o, o->C[q], BC_C);
   if (hist) {
     for (int c = 0; c < 3; c++) {
-      memcpy(o->A0[c], hist + (long)c * o->n, sizeof(double) * o->n);
-      memcpy(o->T0[c], hist + (long)(3 + c) * o->n, sizeof(double) * o->n);
-      memcpy(o->B0[c], hist + (long)(6 + c) * o->n, sizeof(double) * o->n);
+      load_r(o->A0[c], hist + (long)c * o->n, o->n);
+      load_r(o->T0[c], hist + (long)(3 + c) * o->n, o->n);
+      load_r(o->B0[c], hist + (long)(6 + c) * o->n, o->n);
     }
   }
   o->has_history = has_history && hist;
-  memset(o->Phi, 0, sizeof(double) * o->gn);
+  memset((void*)o->Phi, 0, sizeof(real_t) * o->gn);
   return 0;
 }
 
 int orc_get_state(void* vo, double* U, double* V, double* W, double* P, double* C6, double* hist) {
   orc_t* o = (orc_t*)vo;
-  if (U) memcpy(U, o->U, sizeof(double) * o->gn);
-  if (V) memcpy(V, o->V, sizeof(double) * o->gn);
-  if (W) memcpy(W, o->W, sizeof(double) * o->gn);
-  if (P) memcpy(P, o->P, sizeof(double) * o->gn);
+  if (U) store_r(U, o->U, o->gn);
+  if (V) store_r(V, o->V, o->gn);
+  if (W) store_r(W, o->W, o->gn);
+  if (P) store_r(P, o->P, o->gn);
   if (C6)
     for (long g = 0; g < o->gn; g++)
-      for (int q = 0; q < 6; q++) C6[6 * g + q] = o->C[q][g];
+      for (int q = 0; q < 6; q++) C6[6 * g + q] = rv(o->C[q][g]);
   if (hist)
     for (int c = 0; c < 3; c++) {
-      memcpy(hist + (long)c * o->n, o->A0[c], sizeof(double) * o->n);
-      memcpy(hist + (long)(3 + c) * o->n, o->T0[c], sizeof(double) * o->n);
-      memcpy(hist + (long)(6 + c) * o->n, o->B0[c], sizeof(double) * o->n);
+      store_r(hist + (long)c * o->n, o->A0[c], o->n);
+      store_r(hist + (long)(3 + c) * o->n, o->T0[c], o->n);
+      store_r(hist + (long)(6 + c) * o->n, o->B0[c], o->n);
     }
   return 0;
 }
 
 int orc_get_phi(void* vo, double* Phi) {
   orc_t* o = (orc_t*)vo;
-  memcpy(Phi, o->Phi, sizeof(double) * o->gn);
+  store_r(Phi, o->Phi, o->gn);
   return 0;
 }
 
@@ -1078,7 +1102,7 @@ int orc_set_body_force(void* vo, const double* fx, const double* fy, const doubl
   const double* f[3] = {fx, fy, fz};
   for (int c = 0; c < 3; c++) {
     free(o->F[c]); o->F[c] = NULL;
-    if (f[c]) { o->F[c] = zalloc(o->n); memcpy(o->F[c], f[c], sizeof(double) * o->n); }
+    if (f[c]) { o->F[c] = zalloc(o->n); load_r(o->F[c], f[c], o->n); }
   }
   return 0;
 }
@@ -1088,17 +1112,19 @@ int orc_set_markers(void* vo, int nc, int nl, const double* xc, const double* yc
   orc_t* o = (orc_t*)vo;
   o->nc = nc; o->nl = nl; o->nmk = nc * nl * o->nz;
   long n = o->nmk;
-  double** dst[12] = {&o->xc, &o->yc, &o->zc, &o->uc, &o->vc, &o->wc, &o->fxc, &o->fyc, &o->fzc, &o->ubm, &o->vbm, &o->wbm};
+  double** pos[6] = {&o->xc, &o->yc, &o->zc, &o->uc, &o->vc, &o->wc};
   const double* src[6] = {xc, yc, zc, uc, vc, wc};
-  for (int a = 0; a < 12; a++) { free(*dst[a]); *dst[a] = zalloc(n); if (a < 6) memcpy(*dst[a], src[a], sizeof(double) * n); }
+  for (int a = 0; a < 6; a++) { free(*pos[a]); *pos[a] = dzalloc(n); memcpy(*pos[a], src[a], sizeof(double) * n); }
+  real_t** frc[6] = {&o->fxc, &o->fyc, &o->fzc, &o->ubm, &o->vbm, &o->wbm};
+  for (int a = 0; a < 6; a++) { free(*frc[a]); *frc[a] = zalloc(n); }
   return 0;
 }
 
 int orc_get_marker_forces(void* vo, double* fx, double* fy, double* fz) {
   orc_t* o = (orc_t*)vo;
-  memcpy(fx, o->fxc, sizeof(double) * o->nmk);
-  memcpy(fy, o->fyc, sizeof(double) * o->nmk);
-  memcpy(fz, o->fzc, sizeof(double) * o->nmk);
+  store_r(fx, o->fxc, o->nmk);
+  store_r(fy, o->fyc, o->nmk);
+  store_r(fz, o->fzc, o->nmk);
   return 0;
 }
 
@@ -1109,48 +1135,49 @@ int orc_step(void* vo, int nsteps) {
   return 0;
 }
 
+#ifndef ORC_CS
 int orc_diagnostics(void* vo, fp_diag* d) {
   orc_t* o = (orc_t*)vo;
   memset(d, 0, sizeof(*d));
-  double su = 0, sv = 0, sw = 0, umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY;
-  double tmin = INFINITY, tmax = -INFINITY, tsum = 0, dmax = 0;
+  real_t su = 0, sv = 0, sw = 0, umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY;
+  real_t tmin = INFINITY, tmax = -INFINITY, tsum = 0, dmax = 0;
   int nan = 0;
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++) {
         long g = GI(o, i, j, k);
-        double ur = ((o->U[g] + o->U[GI(o, i - 1, j, k)]) * 1.0) / 2.0;
-        double vr = ((o->V[g] + o->V[GI(o, i, j - 1, k)]) * 1.0) / 2.0;
-        double wr = ((o->W[g] + o->W[GI(o, i, j, k - 1)]) * 1.0) / 2.0;
+        real_t ur = ((o->U[g] + o->U[GI(o, i - 1, j, k)]) * 1.0) / 2.0;
+        real_t vr = ((o->V[g] + o->V[GI(o, i, j - 1, k)]) * 1.0) / 2.0;
+        real_t wr = ((o->W[g] + o->W[GI(o, i, j, k - 1)]) * 1.0) / 2.0;
         su += ur * ur; sv += vr * vr; sw += wr * wr;
         if (isnan(o->U[g])) nan = 1;
         if (o->U[g] > umax) umax = o->U[g];
         if (o->V[g] > vmax) vmax = o->V[g];
         if (o->W[g] > wmax) wmax = o->W[g];
-        double tr = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
+        real_t tr = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
         if (tr < tmin) tmin = tr;
         if (tr > tmax) tmax = tr;
         tsum += tr;
-        double dv = ((o->U[g] - o->U[GI(o, i - 1, j, k)]) / o->dx + (o->V[g] - o->V[GI(o, i, j - 1, k)]) / o->dy)
+        real_t dv = ((o->U[g] - o->U[GI(o, i - 1, j, k)]) / o->dx + (o->V[g] - o->V[GI(o, i, j - 1, k)]) / o->dy)
                     + (o->W[g] - o->W[GI(o, i, j, k - 1)]) / o->dz;
         if (fabs(dv) > dmax) dmax = fabs(dv);
       }
-  double N = (double)o->n;
+  real_t N = (real_t)o->n;
   d->ke = ((su + sv + sw) / 2.0) / N;
-  double c1 = umax * o->p.dt / o->dx, c2 = vmax * o->p.dt / o->dy, c3 = wmax * o->p.dt / o->dz;
+  real_t c1 = umax * o->p.dt / o->dx, c2 = vmax * o->p.dt / o->dy, c3 = wmax * o->p.dt / o->dz;
   d->cfl = fmax(c1, fmax(c2, c3)) * 6.0;
   d->trc_min = tmin; d->trc_max = tmax; d->trc_mean = tsum / N;
-  double ss = 0;
+  real_t ss = 0;
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++) {
         long g = GI(o, i, j, k);
-        double tr = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
+        real_t tr = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
         ss += (tr - d->trc_mean) * (tr - d->trc_mean);
       }
   d->trc_std = sqrt(ss / N);
   d->div_max = dmax;
-  double dx3 = (o->dx * o->dx) * o->dx, dr = 0, lf = 0;
+  real_t dx3 = (o->dx * o->dx) * o->dx, dr = 0, lf = 0;
   for (long id = 0; id < o->nmk; id++) { dr += o->fxc[id]; lf += o->fyc[id]; }
   d->drag = -(dr * dx3); d->lift = -(lf * dx3);
   d->newton_itr_mean = o->newton_itr_total / N;
@@ -1165,27 +1192,27 @@ int orc_diagnostics(void* vo, fp_diag* d) {
  * a subtracted mean (the 'same' file) and with unit reference scales
  * (U_C = L_C = 1, nu = 1/Re): out = step, step*dt, K, K_staggered, lambda,
  * Re_lambda, epsilon, eta, tau_eta, v_eta, Ep, trC mean, trC std. */
-int orc_time_stats(void* vo, double* out) {
+int orc_time_stats(void* vo, real_t* out) {
   orc_t* o = (orc_t*)vo;
-  const double dx = o->dx, dy = o->dy, dz = o->dz;
-  double ur2 = 0, vr2 = 0, wr2 = 0, ug2 = 0, vg2 = 0, wg2 = 0, s2 = 0, kst = 0, ep = 0, tr = 0;
-  const double L2 = o->p.lp * o->p.lp;
+  const real_t dx = o->dx, dy = o->dy, dz = o->dz;
+  real_t ur2 = 0, vr2 = 0, wr2 = 0, ug2 = 0, vg2 = 0, wg2 = 0, s2 = 0, kst = 0, ep = 0, tr = 0;
+  const real_t L2 = o->p.lp * o->p.lp;
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++) {
-        const double* U = o->U;
-        const double* V = o->V;
-        const double* W = o->W;
+        const real_t* U = o->U;
+        const real_t* V = o->V;
+        const real_t* W = o->W;
         long g = GI(o, i, j, k);
-        double ur = (U[g] + U[GI(o, i - 1, j, k)]) / 2.0;
-        double vr = (V[g] + V[GI(o, i, j - 1, k)]) / 2.0;
-        double wr = (W[g] + W[GI(o, i, j, k - 1)]) / 2.0;
+        real_t ur = (U[g] + U[GI(o, i - 1, j, k)]) / 2.0;
+        real_t vr = (V[g] + V[GI(o, i, j - 1, k)]) / 2.0;
+        real_t wr = (W[g] + W[GI(o, i, j, k - 1)]) / 2.0;
         ur2 += ur * ur; vr2 += vr * vr; wr2 += wr * wr;
-        double ug = (U[g] - U[GI(o, i - 1, j, k)]) / dx;
-        double vg = (V[g] - V[GI(o, i, j - 1, k)]) / dy;
-        double wg = (W[g] - W[GI(o, i, j, k - 1)]) / dz;
+        real_t ug = (U[g] - U[GI(o, i - 1, j, k)]) / dx;
+        real_t vg = (V[g] - V[GI(o, i, j - 1, k)]) / dy;
+        real_t wg = (W[g] - W[GI(o, i, j, k - 1)]) / dz;
         ug2 += ug * ug; vg2 += vg * vg; wg2 += wg * wg;
-        double D[3][3];
+        real_t D[3][3];
         D[0][0] = (U[g] - U[GI(o, i - 1, j, k)]) / dx;
         D[0][1] = (U[GI(o, i, j + 1, k)] - U[GI(o, i, j - 1, k)] + U[GI(o, i - 1, j + 1, k)] - U[GI(o, i - 1, j - 1, k)]) / (4 * dy);
         D[0][2] = (U[GI(o, i, j, k + 1)] - U[GI(o, i, j, k - 1)] + U[GI(o, i - 1, j, k + 1)] - U[GI(o, i - 1, j, k - 1)]) / (4 * dz);
@@ -1197,29 +1224,29 @@ int orc_time_stats(void* vo, double* out) {
         D[2][2] = (W[g] - W[GI(o, i, j, k - 1)]) / dz;
         for (int a = 0; a < 3; a++)
           for (int b = 0; b < 3; b++) {
-            double sab = D[a][b] + D[b][a];
+            real_t sab = D[a][b] + D[b][a];
             s2 += sab * sab;
           }
         kst += U[g] * U[g] + V[g] * V[g] + W[g] * W[g];
-        double t = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
+        real_t t = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
         tr += t;
         ep += log((L2 - 3.0) / (L2 - t));
       }
-  const double N = (double)o->n, nu = 1.0 / o->p.re;
-  const double urms = ur2 / N, vrms = vr2 / N, wrms = wr2 / N;
-  const double ugr = ug2 / N, vgr = vg2 / N, wgr = wg2 / N;
-  const double all_rms = (urms + vrms + wrms) / 3.0, all_grad = (ugr + vgr + wgr) / 3.0;
-  const double eps = nu / 2.0 * s2 / N;
-  const double lambda = sqrt(all_rms / all_grad);
-  double mean = tr / N, var = 0;
+  const real_t N = (real_t)o->n, nu = 1.0 / o->p.re;
+  const real_t urms = ur2 / N, vrms = vr2 / N, wrms = wr2 / N;
+  const real_t ugr = ug2 / N, vgr = vg2 / N, wgr = wg2 / N;
+  const real_t all_rms = (urms + vrms + wrms) / 3.0, all_grad = (ugr + vgr + wgr) / 3.0;
+  const real_t eps = nu / 2.0 * s2 / N;
+  const real_t lambda = sqrt(all_rms / all_grad);
+  real_t mean = tr / N, var = 0;
   for (int k = 1; k <= o->nz; k++)
     for (int j = 1; j <= o->ny; j++)
       for (int i = 1; i <= o->nx; i++) {
         long g = GI(o, i, j, k);
-        double t = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
+        real_t t = (o->C[0][g] + o->C[3][g]) + o->C[5][g];
         var += (t - mean) * (t - mean);
       }
-  out[0] = (double)o->step;
+  out[0] = (real_t)o->step;
   out[1] = o->step * o->p.dt;
   out[2] = (urms + vrms + wrms) / 2.0;
   out[3] = kst / (2.0 * N);
@@ -1241,26 +1268,26 @@ int orc_selection_counts(void* vo, double* c4) {
   return 0;
 }
 
-int orc_eval_operator(void* vo, int which, double* ox, double* oy, double* oz) {
+int orc_eval_operator(void* vo, int which, real_t* ox, real_t* oy, real_t* oz) {
   orc_t* o = (orc_t*)vo;
-  double** src;
+  real_t** src;
   if (which == 0) { convection(o); src = o->A; }
   else if (which == 1) { viscous(o); src = o->B; }
   else { polymer_stress(o); src = o->T; }
-  memcpy(ox, src[0], sizeof(double) * o->n);
-  memcpy(oy, src[1], sizeof(double) * o->n);
-  memcpy(oz, src[2], sizeof(double) * o->n);
+  memcpy(ox, src[0], sizeof(real_t) * o->n);
+  memcpy(oy, src[1], sizeof(real_t) * o->n);
+  memcpy(oz, src[2], sizeof(real_t) * o->n);
   return 0;
 }
 
-int orc_solve(void* vo, int kind, const double* q, double* x) {
+int orc_solve(void* vo, int kind, const real_t* q, real_t* x) {
   orc_t* o = (orc_t*)vo;
   if (kind < 0 || kind > 3) return FP_E_ARG;
   solve_grid(o, kind, q, x);
   return 0;
 }
 
-int orc_sor(void* vo, const double* q, double* phi) {
+int orc_sor(void* vo, const real_t* q, real_t* phi) {
   orc_t* o = (orc_t*)vo;
   fill_ghosts(o, phi, BC_P);
   return sor_solve(o, q, phi);
@@ -1269,18 +1296,73 @@ int orc_sor(void* vo, const double* q, double* phi) {
 /* ibm_Helmholtz alone on a given ghosted velocity (u*,v*,w*): returns the
  * spread force (interior), marker forces and interpolated marker velocities
  * (for the adjointness / conservation tests). */
-int orc_ibm_apply(void* vo, const double* up3, double* fint3, double* fm3, double* ub3) {
+int orc_ibm_apply(void* vo, const real_t* up3, real_t* fint3, real_t* fm3, real_t* ub3) {
   orc_t* o = (orc_t*)vo;
   if (!o->nmk) return FP_E_STATE;
-  for (int c = 0; c < 3; c++) memcpy(o->Up[c], up3 + (long)c * o->gn, sizeof(double) * o->gn);
+  for (int c = 0; c < 3; c++) memcpy(o->Up[c], up3 + (long)c * o->gn, sizeof(real_t) * o->gn);
   for (long id = 0; id < o->nmk; id++) { o->fxc[id] = 0.0; o->fyc[id] = 0.0; o->fzc[id] = 0.0; }
   ibm_helmholtz(o);
-  double* Fm[3] = {o->fxc, o->fyc, o->fzc};
-  double* Bm[3] = {o->ubm, o->vbm, o->wbm};
+  real_t* Fm[3] = {o->fxc, o->fyc, o->fzc};
+  real_t* Bm[3] = {o->ubm, o->vbm, o->wbm};
   for (int c = 0; c < 3; c++) {
-    memcpy(fint3 + (long)c * o->n, o->fint[c], sizeof(double) * o->n);
-    memcpy(fm3 + (long)c * o->nmk, Fm[c], sizeof(double) * o->nmk);
-    memcpy(ub3 + (long)c * o->nmk, Bm[c], sizeof(double) * o->nmk);
+    memcpy(fint3 + (long)c * o->n, o->fint[c], sizeof(real_t) * o->n);
+    memcpy(fm3 + (long)c * o->nmk, Fm[c], sizeof(real_t) * o->nmk);
+    memcpy(ub3 + (long)c * o->nmk, Bm[c], sizeof(real_t) * o->nmk);
   }
   return 0;
 }
+
+/* ---- unit-level entry points (pinning tests of the FP64 build) */
+double orc_vieta(const double* a) { return vieta_r(a); }
+int orc_select(const double* cm, const double* c, const double* cp, double* cpo, double* cno) {
+  return select_r(cm, c, cp, cpo, cno);
+}
+void orc_gauss6(const double* a, const double* b, double* x) { gauss6_r(a, b, x); }
+int orc_newton_cell(const double* A0, const double* b, const double* x0, double dt, double wi, double lp,
+                    double dc, double eps, int itrmax, double* x) {
+  return newton_cell_r(A0, b, x0, dt, wi, lp, dc, eps, itrmax, x);
+}
+double orc_roma_weight(double r) { return roma_weight(r); }
+void orc_tdma(int n, const double* a, const double* b, const double* c, const double* dre, const double* dim,
+              double* xre, double* xim) {
+  tdma_r(n, a, b, c, dre, dim, xre, xim);
+}
+void orc_fft(int n, double* data, int inverse) { fft_r(n, data, inverse); }
+
+#else  /* ORC_CS: the complex-step build */
+
+/* Seed one parameter: Wi (which 0), beta (1) or L^2 (2) takes imaginary part
+ * h; the others none.  Lp = sqrt(L^2) carries h / (2 Lp), so Lp*Lp has
+ * imaginary part h. */
+int orc_cs_seed_param(void* vo, int which, double h) {
+  orc_t* o = (orc_t*)vo;
+  o->wi = cs_t(o->p.wi, which == 0 ? h : 0.0);
+  o->beta = cs_t(o->p.beta, which == 1 ? h : 0.0);
+  o->lp = cs_t(o->p.lp, which == 2 ? h / (2.0 * o->p.lp) : 0.0);
+  return 0;
+}
+
+/* Seed the body-force field along a direction dF (3 interior planes): F
+ * takes imaginary part h dF.  The field must have been set. */
+int orc_cs_seed_force(void* vo, const double* dF, double h) {
+  orc_t* o = (orc_t*)vo;
+  for (int c = 0; c < 3; c++) {
+    if (!o->F[c]) return FP_E_STATE;
+    for (long m = 0; m < o->n; m++) o->F[c][m].im = h * dF[(long)c * o->n + m];
+  }
+  return 0;
+}
+
+/* Drag of the last step, D = -sum_markers Fxc dX^3: out[0] = D, out[1] = its
+ * imaginary part. */
+int orc_cs_drag(void* vo, double* out2) {
+  orc_t* o = (orc_t*)vo;
+  const double dx3 = (o->dx * o->dx) * o->dx;
+  real_t dr = 0.0;
+  for (long id = 0; id < o->nmk; id++) dr += o->fxc[id];
+  const real_t d = -(dr * dx3);
+  out2[0] = d.re;
+  out2[1] = d.im;
+  return 0;
+}
+#endif

@@ -62,6 +62,12 @@ void   orc_tdma(int n, const double* a, const double* b, const double* c,
                 const double* dre, const double* dim, double* xre, double* xim);
 void   orc_fft(int n, double* data_interleaved, int inverse);
 
+/* ---- complex-step build only (liboracle_cs.so, oracle/cstep.h): the same
+ * state entry points, and the seeds and drag of a complex-step run */
+int orc_cs_seed_param(void* o, int which, double h);
+int orc_cs_seed_force(void* o, const double* dF3, double h);
+int orc_cs_drag(void* o, double* out2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -380,6 +380,94 @@ def plane_mask(nx, ny, dx, dy, centres, diameter):
     return mask
 
 
+_CS_LIB = None
+
+
+def cs_lib():
+    """The complex-step build of the restatement (liboracle_cs.so)."""
+    global _CS_LIB
+    if _CS_LIB is None:
+        so = os.path.join(_HERE, "liboracle_cs.so")
+        if not os.path.exists(so):
+            build(force=True)
+        L = C.CDLL(so)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(FpParams)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_state.argtypes = [C.c_void_p] + [_dp] * 6 + [C.c_int]
+        L.orc_get_state.argtypes = [C.c_void_p] + [_dp] * 6
+        L.orc_set_body_force.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_set_markers.argtypes = [C.c_void_p, C.c_int, C.c_int] + [_dp] * 6
+        L.orc_get_marker_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_cs_seed_param.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_cs_seed_force.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_cs_drag.argtypes = [C.c_void_p, _dp]
+        _CS_LIB = L
+    return _CS_LIB
+
+
+class ComplexStep:
+    """The restatement evaluated in complex-step arithmetic (oracle/cstep.h):
+    the real parts are the FP64 oracle's bit for bit, the imaginary parts h
+    times the derivative along one seeded direction."""
+
+    def __init__(self, case):
+        self.case = case
+        self.h = cs_lib().orc_create(C.byref(case.params))
+        if not self.h:
+            raise ValueError("oracle rejected parameters")
+        L = cs_lib()
+        st = case.state
+        hist = None if st.hist is None else np.ascontiguousarray(st.hist, dtype=np.float64)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (st.U, st.V, st.W, st.P, st.C)]
+        L.orc_set_state(self.h, *[ptr(a) for a in arrs], ptr(hist), int(st.has_history))
+        if case.force is not None:
+            self._f = [np.ascontiguousarray(a, dtype=np.float64) for a in case.force]
+            L.orc_set_body_force(self.h, *[ptr(a) for a in self._f])
+        if case.markers is not None:
+            mk = case.markers
+            self._mk = [np.ascontiguousarray(a, dtype=np.float64) for a in (mk.xc, mk.yc, mk.zc, mk.uc, mk.vc, mk.wc)]
+            L.orc_set_markers(self.h, mk.nc, mk.nl, *[ptr(a) for a in self._mk])
+
+    def seed_param(self, which, h):
+        """which: 0 Wi, 1 beta, 2 L^2."""
+        cs_lib().orc_cs_seed_param(self.h, int(which), float(h))
+
+    def seed_force(self, dF, h):
+        d = np.ascontiguousarray(dF, dtype=np.float64)
+        if cs_lib().orc_cs_seed_force(self.h, ptr(d), float(h)):
+            raise RuntimeError("the case has no body-force field to seed")
+
+    def drag_series(self, nsteps):
+        """Steps nsteps times; returns (drag, imaginary part of drag) per step."""
+        L = cs_lib()
+        re, im = np.zeros(nsteps), np.zeros(nsteps)
+        out = np.zeros(2)
+        for s in range(nsteps):
+            L.orc_step(self.h, 1)
+            L.orc_cs_drag(self.h, ptr(out))
+            re[s], im[s] = out
+        return re, im
+
+    def get_state(self):
+        nx, ny, nz = self.case.params.nx, self.case.params.ny, self.case.params.nz
+        g = lambda: np.zeros((nz + 2, ny + 2, nx + 2))
+        U, V, W, P = g(), g(), g(), g()
+        Cc = np.zeros((nz + 2, ny + 2, nx + 2, 6))
+        hist = np.zeros((9, nz, ny, nx))
+        cs_lib().orc_get_state(self.h, ptr(U), ptr(V), ptr(W), ptr(P), ptr(Cc), ptr(hist))
+        return State(U, V, W, P, Cc, hist, 1)
+
+    def close(self):
+        if self.h:
+            cs_lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 def load_case(case):
     o = Oracle(case.params)
     o.set_state(case.state)

@@ -155,8 +155,9 @@ def check_params():
     print(json.dumps(out), flush=True)
 
 
-def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000):
+def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000, precision=0, fd=True):
     case = configs.adjoint_case(newton_eps=newton_eps, newton_itrmax=itrmax)   # 64x64x32, Lz = pi
+    case.params.precision = precision
     nsteps, every, last = 400, 20, 200
     p = case.params
     fx = np.stack([np.asarray(a) for a in case.force])
@@ -173,7 +174,7 @@ def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     if timing:
-        # plain forward of the same steps (no recording) for the cost ratio
+        # plain forward of the same steps (no recording) for the cost ratios
         g = run.g
         g.set_physics(p.wi, p.beta, p.lp)
         g.set_state(case.state)
@@ -181,19 +182,28 @@ def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000):
         t3 = time.perf_counter()
         g.step(nsteps)
         t4 = time.perf_counter()
-        out.update({"J": float(J), "t_record_s": t1 - t0, "t_backward_s": t2 - t1, "t_forward_s": t4 - t3,
-                    "ratio_backward_forward": (t2 - t1) / (t4 - t3), "ratio_record_forward": (t1 - t0) / (t4 - t3)})
-    run.replay = True
+        out.update({"%s_J" % tag: float(J), "%s_t_record_s" % tag: t1 - t0, "%s_t_backward_s" % tag: t2 - t1,
+                    "%s_t_forward_s" % tag: t4 - t3,
+                    "%s_ratio_backward_forward" % tag: (t2 - t1) / (t4 - t3),
+                    "%s_ratio_record_forward" % tag: (t1 - t0) / (t4 - t3),
+                    "%s_ratio_record_backward_forward" % tag: (t2 - t0) / (t4 - t3)})
+    grads = [float(wi.grad), float(beta.grad), float(L2.grad)]
     rng = np.random.default_rng(5)
     dF = rng.standard_normal(fx.shape)
+    dirF = float(torch.sum(F.grad.cpu() * torch.from_numpy(dF)))
+    out["%s_dF_dir" % tag] = dirF
+    for q, name in enumerate(("wi", "beta", "L2")):
+        out["%s_d%s" % (tag, name)] = grads[q]
+    if not fd:
+        run.close()
+        return
+    run.replay = True
     base = [p.wi, p.beta, p.lp ** 2]
-    grads = [float(wi.grad), float(beta.grad), float(L2.grad)]
     for q, name in enumerate(("wi", "beta", "L2")):
         def Jq(t):
             x = list(base)
             x[q] += t
             return run.forward(*x, fx)[0]
-        out["%s_d%s" % (tag, name)] = grads[q]
         for hrel in hrels:
             fd = central(Jq, hrel * base[q])
             out["%s_fd_%s_h%g" % (tag, name, hrel)] = fd
@@ -201,9 +211,7 @@ def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000):
         fd = richardson(Jq, 1e-4 * base[q])
         out["%s_fd_%s_rich" % (tag, name)] = fd
         out["%s_rel_%s_rich" % (tag, name)] = abs(fd - grads[q]) / abs(grads[q])
-    dirF = float(torch.sum(F.grad.cpu() * torch.from_numpy(dF)))
     fd = central(lambda t: run.forward(*base, fx + t * dF)[0], 1e-6)
-    out["%s_dF_dir" % tag] = dirF
     out["%s_rel_F" % tag] = abs(fd - dirF) / abs(dirF)
     run.close()
 
@@ -211,10 +219,29 @@ def cfg5_run(newton_eps, hrels, out, tag, timing, itrmax=10000):
 def check_cfg5():
     out = {"check": "cfg5"}
     # the reference's Newton tolerance (1e-6): the stopping rule makes the
-    # step piecewise smooth, so the parameter differences use h = 1e-4
+    # step piecewise smooth in the parameters
     cfg5_run(1e-6, (1e-4, 1e-6), out, "ref", True)
-    # Newton converged to round-off: differences at h = 1e-6
-    cfg5_run(1e-12, (1e-6,), out, "tight", False, itrmax=50)
+    # Newton run to a fixed count of 8 iterations (tolerance 0): converged to
+    # round-off and smooth in the parameters; central differences at h = 1e-6
+    cfg5_run(0.0, (1e-6,), out, "conv", False, itrmax=8)
+    # mixed-precision forward (FP32 bulk), FP64 reverse sweep along the
+    # recomputed segments: its gradients against the FP64 ones
+    cfg5_run(1e-6, (), out, "mixed", True, precision=1, fd=False)
+    for name in ("wi", "beta", "L2"):
+        out["mixed_rel_%s_vs_fp64" % name] = abs(out["mixed_d" + name] - out["ref_d" + name]) / abs(out["ref_d" + name])
+    out["mixed_rel_F_vs_fp64"] = abs(out["mixed_dF_dir"] - out["ref_dF_dir"]) / abs(out["ref_dF_dir"])
+    # complex-step gradients of the FP64 CPU oracle (tests/golden/
+    # make_complex_step.py): the same objective, Newton settings and force
+    # direction
+    with open(os.path.join(ROOT, "tests", "golden", "cfg5_complex_step.json")) as f:
+        cs = json.load(f)
+    for tag in ("conv", "ref"):
+        for name in ("wi", "beta", "L2"):
+            a, c = out["%s_d%s" % (tag, name)], cs[tag]["d" + name]
+            out["%s_rel_%s_cs" % (tag, name)] = abs(a - c) / abs(c)
+        a, c = out["%s_dF_dir" % tag], cs[tag]["dF"]
+        out["%s_rel_F_cs" % tag] = abs(a - c) / abs(c)
+        out["%s_J_cs" % tag] = cs[tag]["J"]
     print(json.dumps(out), flush=True)
 
 

@@ -155,11 +155,12 @@ def test_driver_masked_records_maps_outlines_and_text_maps(tmp_path):
         zb = np.fromfile(os.path.join(run, "Map", "z_%08d.bin" % step), dtype=np.float64).reshape(-1, 18)
         cols = [0, 1, 2, 3, 4, 5] + list(range(7, 18))
         assert np.allclose(zt[:, cols], zb[:, cols], rtol=1e-3, atol=1e-4 * np.abs(zb[:, cols]).max())
-    # cylinder outline (ibm_get): the markers of plane NZ/2, closed
+    # cylinder outline (ibm_get): the markers of plane NZ/2, closed, in the
+    # reference's (2e12.4) format (four significant digits)
     ol = np.loadtxt(os.path.join(run, "ibm", "00000001.d"))
     assert np.allclose(ol[0], ol[-1])
     rad = np.hypot(ol[:, 0] - 1.0, ol[:, 1] - 1.0)
-    assert rad.max() == pytest.approx(0.25, rel=1e-3) and ol.shape[0] == 64 + 1
+    assert rad.max() == pytest.approx(0.25, abs=1e-3) and ol.shape[0] == 64 + 1
 
 
 @pytest.mark.gpu

@@ -39,12 +39,32 @@ def test_adjoint_parameter_and_force_gradients_multi_segment():
 
 def test_adjoint_cfg5_drag_gradients_and_cost():
     out = run_check("cfg5", timeout=900)
-    # configuration 5 as specified (reference Newton tolerance) and with the
-    # Newton converged to round-off; reference derivative: Richardson-
-    # extrapolated central differences.  dJ/d(L^2) vanishes here (tr C ~ 3):
-    # it is checked on the stretched state of the multi-segment test.
-    for tag in ("ref", "tight"):
-        assert out[tag + "_rel_wi_rich"] < 1e-5, tag
-        assert out[tag + "_rel_beta_rich"] < 1e-6, tag
-        assert out[tag + "_rel_F"] < 1e-6, tag
-    assert out["ratio_backward_forward"] < 3.0
+    # configuration 5 with the Newton converged (8 iterations, round-off):
+    # central differences at h = 1e-6 within 1e-6 of the adjoint.  dJ/d(L^2)
+    # is about 4e-14 here (tr C ~ 3) and is checked on the stretched state of
+    # the multi-segment test.
+    for k in ("conv_rel_wi_h1e-06", "conv_rel_beta_h1e-06", "conv_rel_F"):
+        assert out[k] < 1e-6, k
+    # and within 1e-6 of the complex-step gradient of the FP64 CPU oracle
+    # (the same discrete branch: the oracle and the GPU step are bit-identical
+    # on this periodic case)
+    for k in ("conv_rel_wi_cs", "conv_rel_beta_cs", "conv_rel_F_cs"):
+        print(k, out[k])
+        assert out[k] < 1e-6, k
+    print("L^2:", out["conv_dL2"], "complex step rel", out["conv_rel_L2_cs"])
+    # at the reference tolerance the complex step differentiates the Newton
+    # iterate actually stopped, the adjoint the converged root
+    print("reference tolerance vs complex step:", {k: out[k] for k in out if k.startswith("ref_rel") and k.endswith("_cs")})
+    # the reference's tolerance (1e-6) stops the Newton at a point that
+    # moves in steps with the parameters: Richardson-extrapolated differences
+    # at h = 1e-4 within 1e-5
+    assert out["ref_rel_wi_rich"] < 1e-5 and out["ref_rel_beta_rich"] < 1e-6 and out["ref_rel_F"] < 1e-6
+    # the reverse sweep alone (recomputation included) costs below 3 forward
+    # runs; recording plus reverse is reported beside it
+    assert out["ref_ratio_backward_forward"] < 3.0
+    print("record + reverse / forward:", out["ref_ratio_record_backward_forward"])
+    # mixed-precision forward: the gradients agree with the FP64 ones to the
+    # accuracy of the mixed trajectory
+    for k in ("mixed_rel_wi_vs_fp64", "mixed_rel_beta_vs_fp64", "mixed_rel_F_vs_fp64"):
+        print(k, out[k])
+        assert out[k] < 1e-2, k
