@@ -66,6 +66,7 @@ def lib():
         L.fp_get_marker_forces.argtypes = [_vp, _dp, _dp, _dp]
         L.fp_step.argtypes = [_vp, C.c_int, C.c_int]
         L.fp_diagnostics.argtypes = [_vp, C.POINTER(FpDiag)]
+        L.fp_diagnostics_static.argtypes = [_vp, C.POINTER(FpDiag)]
         L.fp_set_state_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]
         L.fp_get_state_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fp_enable_timing.argtypes = [_vp, C.c_int]

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdio>
+#include <mutex>
 #include <new>
 
 using namespace fp;
@@ -452,8 +453,8 @@ int fp_step(void* vc, int nsteps, int sync) {
   return FP_OK;
 }
 
-// Body of fp_diagnostics: the reductions run in `scratch`
-// (diag_scratch_words(c.g) doubles of device memory).
+// Body of fp_diagnostics and fp_diagnostics_static: the reductions run in
+// `scratch` (diag_scratch_words(c.g) doubles of device memory).
 static void diagnostics_into(Ctx& c, fp_diag* d, double* scratch) {
   std::memset(d, 0, sizeof(*d));
   if (c.mixed) mixed_store_to_f64(c);
@@ -483,6 +484,54 @@ int fp_diagnostics(void* vc, fp_diag* d) {
     return fail(c, FP_E_HIP, e);
   }
   (void)hipFree(scratch);
+  return d->nan_flag ? FP_E_NAN : FP_OK;
+}
+
+}  // extern "C"
+
+// ---- fp_diagnostics_static: the diagnostics without any allocation.
+// Its reductions use this file-scope device array instead of a buffer from
+// hipMalloc.  Each GPU holds its own copy (6.3 MB, for grids up to
+// FP_DIAG_STATIC_MAX_CELLS cells) until the process ends.  In host code the
+// name `scratch` is not a device address: the address is taken once per
+// device with hipGetSymbolAddress and only that pointer is used.
+namespace {
+constexpr size_t kScratchWords = (size_t)12 * (FP_DIAG_STATIC_MAX_CELLS / 256) + 16;
+static __device__ __align__(256) double scratch[kScratchWords];
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+// the current device's copy of `scratch`, looked up on its first use
+int scratch_ptr(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return FP_E_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FP_E_HIP;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fp_diagnostics_static(void* vc, fp_diag* d) {
+  Ctx* c = (Ctx*)vc;
+  if (!c || !d) return FP_E_ARG;
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  if (c->g.n > (long)FP_DIAG_STATIC_MAX_CELLS || diag_scratch_words(c->g) > kScratchWords)
+    return fail(c, FP_E_ARG, "fp_diagnostics_static: more than FP_DIAG_STATIC_MAX_CELLS cells");
+  double* ptr = nullptr;
+  if (scratch_ptr(&ptr) != FP_OK) return fail(c, FP_E_HIP, "hipGetSymbolAddress of the static scratch failed");
+  try {
+    diagnostics_into(*c, d, ptr);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
   return d->nan_flag ? FP_E_NAN : FP_OK;
 }
 

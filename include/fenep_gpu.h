@@ -143,6 +143,17 @@ int fp_step(void* ctx, int nsteps, int sync);
 
 int fp_diagnostics(void* ctx, fp_diag* d);
 
+/* fp_diagnostics without any allocation.  No counterpart in the reference
+ * (its diagnostics are plain loops over host arrays); it is for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The same reductions as
+ * fp_diagnostics (bit-identical results) run in a static device array of the
+ * library, one per GPU, held until the process ends.  Grids above
+ * FP_DIAG_STATIC_MAX_CELLS cells return FP_E_ARG.  Calls that overlap on one
+ * GPU (from several host threads) share that array and race. */
+#define FP_DIAG_STATIC_MAX_CELLS (1L << 24)
+int fp_diagnostics_static(void* ctx, fp_diag* d);
+
 /* Device-pointer variants for callers that keep the state resident (the
  * PyTorch wrapper passes tensor.data_ptr()).  Layout: interior only, x
  * fastest, FP64: U,V,W,P are nx*ny*nz each, C6 is 6 consecutive planes
