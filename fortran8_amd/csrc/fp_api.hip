@@ -200,7 +200,7 @@ int fp_create(const fp_params* p, void** out) {
   if (p->walls_y && p->ny < 4) return FP_E_ARG;
   // the mixed kernels address each field plane with 32-bit byte offsets
   if (p->precision == FP_PREC_MIXED &&
-      (p->method == FP_METHOD_EXPLICIT || p->poisson_variant == FP_POISSON_SOR || (long)p->nx * p->ny * p->nz >= (1L << 30)))
+      (p->method == FP_METHOD_EXPLICIT || (long)p->nx * p->ny * p->nz >= (1L << 30)))
     return FP_E_ARG;
   Ctx* c = new (std::nothrow) Ctx();
   if (!c) return FP_E_NOMEM;
@@ -549,8 +549,14 @@ int fp_set_state_device(void* vc, const double* U, const double* V, const double
     FP_HIP_CHECK(hipMemcpyAsync(c->P, P, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     c->cur = 0;
     FP_HIP_CHECK(hipMemcpyAsync(c->C[0], C6, sizeof(double) * 6 * n, hipMemcpyDeviceToDevice, s));
-    FP_HIP_CHECK(hipMemsetAsync(c->Phi, 0, sizeof(double) * n, s));
+    if (has_history == 2) {
+      // the potential stays: the SOR variant's sweeps start from it
+      if (c->mixed) mixed_phi_to_f64(*c);
+    } else {
+      FP_HIP_CHECK(hipMemsetAsync(c->Phi, 0, sizeof(double) * n, s));
+    }
     FP_HIP_CHECK(hipStreamSynchronize(s));
+    FP_HIP_CHECK(hipStreamSynchronize(c->stream));
     c->has_history = has_history ? 1 : 0;
     c->state_set = 1;
     if (c->mixed) mixed_load_from_f64(*c);

@@ -42,7 +42,11 @@ struct Mixed {
   float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
   float *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
   double *phi = nullptr, *b = nullptr;
-  float *r32 = nullptr, *d32 = nullptr;
+  float* r32 = nullptr;
+  float* e32 = nullptr;           // SOR: the FP32 correction e
+  double* sor_pb = nullptr;       // SOR: per-block |b|^2
+  int* sor_flag = nullptr;        // SOR: converged, sweeps, FP64 check (m_sor_decide)
+  int* sor_host = nullptr;        // pinned copy of sor_flag
   double* red = nullptr;          // [0]=|b|^2, [1]=|r|^2 of the last sweep
   double* part = nullptr;         // per-block partial sums
   int* skip = nullptr;            // defect correction converged: skip the sweep
@@ -1423,8 +1427,19 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
   }
 }
 
-// The Poisson right-hand side b = div_h(u*)/dt (fft_poisson,
-// fenep.f90:1550-1552; FP64 from the FP32 u*) computed in the load stage of
+// b = div_h(u*)/dt at cell (i, j, k) in FP64 from the FP32 u* (fft_poisson,
+// fenep.f90:1550-1552; poisson, :869-877).  idx.. are the FP64 reciprocals
+// of the spacings and of dt (an FP64 division is some twenty instructions;
+// the products differ from the quotients by an FP64 ulp).
+template <bool WL>
+__device__ __forceinline__ double div_rhs(const Grid& g, const float* Uu, const float* Vv, const float* Ww, int i, int j,
+                                          int k, double idx, double idy, double idz, double idt) {
+  return (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) * idx +
+          ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) * idy +
+          ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) * idz) * idt;
+}
+
+// The Poisson right-hand side b = div_h(u*)/dt (div_rhs) computed in the load stage of
 // the forward x transform, two real lines per complex transform as in
 // m_fft_x_fwd.  want_b: also store b (FP64) and the per-block |b|^2 for the
 // defect check.
@@ -1442,8 +1457,6 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
   double ss = 0.0;
-  // reciprocals of the spacings and of dt (an FP64 division is some twenty
-  // instructions; the products differ from the quotients by an FP64 ulp)
   const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, idz = 1.0 / g.dz, idt = 1.0 / g.dt;
 #pragma unroll 2
   for (int e = threadIdx.x; e < nc * n; e += blockDim.x) {
@@ -1454,9 +1467,7 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
       const long L = l0 + 2 * c + h;
       if (L >= nlines) continue;
       const int j = (int)(line_row(g, L)), k = (int)(line_plane(g, L));
-      const double v = (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) * idx +
-                        ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) * idy +
-                        ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) * idz) * idt;
+      const double v = div_rhs<WL>(g, Uu, Vv, Ww, i, j, k, idx, idy, idz, idt);
       if (want_b) {
         b[L * n + i] = v;
         ss += v * v;
@@ -1956,6 +1967,125 @@ __global__ __launch_bounds__(1024) void m_sum_decide(const double* __restrict__ 
     red[0] = sb;
     red[1] = sh[0];
     skip[0] = sh[0] <= tol * tol * sb ? 1 : 0;
+  }
+}
+
+// ============================================================ SOR ==========
+// The SOR variant (poisson, fenep.f90:848-905) as FP32 red-black sweeps of a
+// correction e on the FP64 potential: with r = b - L_h phi in FP64, the
+// sweeps solve L_h e = r from e = 0, so that phi + e is the iterate of the
+// sweeps on L_h phi = b, and the residual a sweep takes before updating a
+// cell, E = L_h e - r, is the reference's E = L_h(phi + e) - Q to FP32
+// rounding of the correction.  The stopping test sqrt(sum E^2 / sum Q^2) < eps
+// is the reference's, summed in FP64.  Every few sweeps phi += e in FP64 and
+// the defect is taken again; once the sweeps stop, the FP64 residual of the
+// new phi is held to the same tolerance, and sweeps go on if it is not met.
+//
+// flag[0]: converged (sweeps return at once), flag[1]: sweeps counted as the
+// reference counts them, flag[2]: the FP64 check passed.
+
+// b (FP64), r = b - L_h phi (FP64, stored in FP32), e = 0, per-block |b|^2
+template <bool WL>
+__global__ __launch_bounds__(256) void m_sor_start(Grid g, const float* __restrict__ up, const double* __restrict__ phi,
+                                                   double* __restrict__ b, float* __restrict__ r32,
+                                                   float* __restrict__ e, double* __restrict__ pb) {
+  const long m = xcd_cell();
+  double bb = 0.0;
+  if (m < g.n) {
+    int i, j, k;
+    cell_ijk(g, m, i, j, k);
+    const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, idz = 1.0 / g.dz, idt = 1.0 / g.dt;
+    bb = div_rhs<WL>(g, up, up + g.n, up + 2 * g.n, i, j, k, idx, idy, idz, idt);
+    const double p = phi[m];
+#define PH(a, bq, c) ldbw<WL>(phi, g, a, bq, c, BC_P)
+    const double lap = (PH(i - 1, j, k) - 2.0 * p + PH(i + 1, j, k)) / (g.dx * g.dx) +
+                       (PH(i, j - 1, k) - 2.0 * p + PH(i, j + 1, k)) / (g.dy * g.dy) +
+                       (PH(i, j, k - 1) - 2.0 * p + PH(i, j, k + 1)) / (g.dz * g.dz);
+#undef PH
+    b[m] = bb;
+    r32[m] = (float)(bb - lap);
+    e[m] = 0.0f;
+  }
+  __shared__ double sh[256];
+  const double sum = block_sum256(bb * bb, sh);
+  if (threadIdx.x == 0) pb[blockIdx.x] = sum;
+}
+
+// One colour of a sweep: cells with (i + j + k) % 2 == color, i fastest, as
+// k_sor_color of the FP64 path.  The per-block sum of E^2 is stored by the
+// first colour and added to by the second.
+template <bool WL>
+__global__ __launch_bounds__(256) void m_sor_sweep(Grid g, int color, float om_b0, float bx, float by, float bz, float b0,
+                                                   const float* __restrict__ r32, float* __restrict__ e,
+                                                   double* __restrict__ e2, const int* __restrict__ flag) {
+  if (flag[0]) return;   // converged: the same for every thread of the launch
+  const long h = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  float E = 0.0f;
+  if (h < g.n / 2) {
+    const long jk = h >> (g.lnx - 1);
+    const int j = (int)(jk & (g.ny - 1)), k = (int)(jk >> g.lny);
+    const int i = (int)(2 * (h & ((g.nx >> 1) - 1))) + ((j + k + color) & 1);
+    const long m = cidx(g, i, j, k);
+    const float e0 = e[m];
+#define EE(a, bq, c) ldbw<WL>(e, g, a, bq, c, BC_P)
+    E = (((((bz * EE(i, j, k - 1) + by * EE(i, j - 1, k)) + bx * EE(i - 1, j, k)) - b0 * e0) + bx * EE(i + 1, j, k)) +
+         by * EE(i, j + 1, k)) + bz * EE(i, j, k + 1) - r32[m];
+#undef EE
+    e[m] = e0 + om_b0 * E;
+  }
+  __shared__ double sh[256];
+  const double s = block_sum256((double)E * (double)E, sh);
+  if (threadIdx.x == 0) e2[blockIdx.x] = color == 0 ? s : e2[blockIdx.x] + s;
+}
+
+// sum of the per-block partials of n blocks, in a fixed order
+__device__ __forceinline__ double sum_partials1024(const double* __restrict__ p, int n, double* sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) v += p[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// mode 0 (after m_sor_start): red[0] = sum Q^2, flags cleared (converged at
+// once when Q = 0).  mode 1 (after a sweep): count the sweep and test it.
+// mode 2 (after the FP64 residual of phi + e, only once the sweeps have met
+// the test): red[1] = |b - L_h phi|^2, flag[2] = the FP64 test; the sweeps
+// resume when it fails.
+__global__ __launch_bounds__(1024) void m_sor_decide(const double* __restrict__ part, int nb, double* __restrict__ red,
+                                                     double eps, int mode, int* __restrict__ flag) {
+  if (mode == 2 && !flag[0]) return;   // the same for every thread
+  __shared__ double sh[1024];
+  const double sum = sum_partials1024(part, nb, sh);
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    red[0] = sum;
+    flag[0] = sum == 0.0 ? 1 : 0;
+    flag[1] = 0;
+    flag[2] = 0;
+  } else if (mode == 1) {
+    if (!flag[0]) {
+      flag[1] += 1;
+      if (sqrt(sum / red[0]) < eps) flag[0] = 1;
+    }
+  } else {
+    red[1] = sum;
+    const int ok = red[0] == 0.0 || sqrt(sum / red[0]) < eps;
+    flag[2] = ok;
+    flag[0] = ok;
+  }
+}
+
+// phi += e (FP64), e = 0
+__global__ __launch_bounds__(256) void m_sor_accum(long n, double* __restrict__ phi, float* __restrict__ e) {
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m < n) {
+    phi[m] += (double)e[m];
+    e[m] = 0.0f;
   }
 }
 
@@ -2484,8 +2614,8 @@ static int helm_sweeps(const Grid& g, int requested) {
 }
 
 void mixed_create(Ctx& c) {
-  if (c.p.method == FP_METHOD_EXPLICIT || c.p.poisson_variant == FP_POISSON_SOR)
-    throw std::string("mixed precision supports the FFT and IBM methods with the spectral Poisson solve");
+  if (c.p.method == FP_METHOD_EXPLICIT)
+    throw std::string("mixed precision supports the FFT and IBM methods");
   Mixed* X = new Mixed();
   c.mixed = X;
   const Grid& g = c.g;
@@ -2501,7 +2631,11 @@ void mixed_create(Ctx& c) {
   dalloc(&X->Q, 3 * n); dalloc(&X->Up, 3 * n);
   dalloc(&X->spec, (size_t)g.nxh * g.sy * g.nz);
   if (g.walls) dalloc(&X->thp, (size_t)(g.ny + 2) * g.nxh * g.nz);
-  dalloc(&X->phi, n); dalloc(&X->b, n); dalloc(&X->r32, n); dalloc(&X->d32, n);
+  dalloc(&X->phi, n); dalloc(&X->b, n); dalloc(&X->r32, n);
+  if (c.p.poisson_variant == FP_POISSON_SOR) {
+    dalloc(&X->e32, n); dalloc(&X->sor_pb, (size_t)nblk(n, 256)); dalloc(&X->sor_flag, 4);
+    FP_HIP_CHECK(hipHostMalloc((void**)&X->sor_host, 4 * sizeof(int), hipHostMallocDefault));
+  }
   dalloc(&X->red, 8);
   dalloc(&X->part, (size_t)(n / 256 + 1024));
   dalloc(&X->partb, (size_t)blocks_x32(g) + 1);
@@ -2566,7 +2700,9 @@ void mixed_destroy(Ctx& c) {
   dfree(X->A0); dfree(X->T0); dfree(X->B); dfree(X->Q); dfree(X->Up); dfree(X->F);
   dfree(X->spec); dfree(X->thp); dfree(X->twx); dfree(X->twy); dfree(X->twz);
   dfree(X->kx2); dfree(X->ky2); dfree(X->kz2);
-  dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->d32); dfree(X->red); dfree(X->part); dfree(X->skip);
+  dfree(X->phi); dfree(X->b); dfree(X->r32); dfree(X->red); dfree(X->part); dfree(X->skip);
+  dfree(X->e32); dfree(X->sor_pb); dfree(X->sor_flag);
+  if (X->sor_host) (void)hipHostFree(X->sor_host);
   dfree(X->partb);
   dfree(X->pbuf);
   if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
@@ -2597,8 +2733,13 @@ void mixed_load_from_f64(Ctx& c) {
   hipLaunchKernelGGL(m_to32, dim3(nblk(6 * n, 256)), dim3(256), 0, s, 6 * n, c.C[c.cur], X.C[0]);
   hipLaunchKernelGGL(m_to32, dim3(nblk(3 * n, 256)), dim3(256), 0, s, 3 * n, c.A0, X.A0);
   hipLaunchKernelGGL(m_to32, dim3(nblk(3 * n, 256)), dim3(256), 0, s, 3 * n, c.T0, X.T0);
-  FP_HIP_CHECK(hipMemsetAsync(X.phi, 0, sizeof(double) * n, s));
+  // the potential: the SOR variant starts its sweeps from it
+  FP_HIP_CHECK(hipMemcpyAsync(X.phi, c.Phi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   FP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mixed_phi_to_f64(Ctx& c) {
+  FP_HIP_CHECK(hipMemcpyAsync(c.Phi, MX(c).phi, sizeof(double) * c.g.n, hipMemcpyDeviceToDevice, c.stream));
 }
 
 void mixed_store_to_f64(Ctx& c) {
@@ -2978,6 +3119,51 @@ static void select_ahead(Ctx& c) {
   X.sel_ready = 1;
 }
 
+// The SOR variant's solve of L_h phi = div u*/dt (m_sor_* above).  Every
+// SOR_BATCH sweeps the correction is moved into phi (FP64) and the defect
+// taken again in FP64, so that the FP32 sweeps only ever carry the change of
+// the last few sweeps: a smooth correction that grew over many sweeps would
+// make L_h e cancel in FP32 down to the size of E and stall the stopping test.
+// (SOR is affine in its iterate: the restarted sweeps are the same iteration.)
+// The host reads the flags back after each batch, as the FP64 path reads its
+// residuals; sweeps launched after convergence return at once.
+static void mixed_sor(Ctx& c) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  const long n = g.n;
+  const int nb = nblk(n, 256), nbh = nblk(n / 2, 256);
+  const double om = c.p.sor_omega > 0.0 ? c.p.sor_omega : 2.0 / (1.0 + sin(3.14159265358979323846 * g.dx));
+  const double BX = 1.0 / (g.dx * g.dx), BY = 1.0 / (g.dy * g.dy), BZ = 1.0 / (g.dz * g.dz);
+  const double B0 = ((((BX + BX) + BY) + BY) + BZ) + BZ;
+  const float om_b0 = (float)(om / B0);
+  const int SOR_BATCH = 10;
+  hipLaunchKernelGGL(g.walls ? m_sor_start<true> : m_sor_start<false>, dim3(nb), dim3(256), 0, s, g, X.Up, X.phi, X.b,
+                     X.r32, X.e32, X.sor_pb);
+  hipLaunchKernelGGL(m_sor_decide, dim3(1), dim3(1024), 0, s, X.sor_pb, nb, X.red, c.p.sor_eps, 0, X.sor_flag);
+  int launched = 0;
+  while (launched < c.p.sor_itrmax) {
+    const int batch = std::min(SOR_BATCH, c.p.sor_itrmax - launched);
+    for (int b = 0; b < batch; b++) {
+      for (int color = 0; color < 2; color++)
+        hipLaunchKernelGGL(g.walls ? m_sor_sweep<true> : m_sor_sweep<false>, dim3(nbh), dim3(256), 0, s, g, color, om_b0,
+                           (float)BX, (float)BY, (float)BZ, (float)B0, X.r32, X.e32, X.part, X.sor_flag);
+      hipLaunchKernelGGL(m_sor_decide, dim3(1), dim3(1024), 0, s, X.part, nbh, X.red, c.p.sor_eps, 1, X.sor_flag);
+    }
+    launched += batch;
+    // phi += e, e = 0, the FP64 defect of the new phi; once the sweeps have
+    // met the test, the same test on the FP64 residual (else they go on)
+    hipLaunchKernelGGL(m_sor_accum, dim3(nb), dim3(256), 0, s, n, X.phi, X.e32);
+    hipLaunchKernelGGL(g.walls ? m_residual<true> : m_residual<false>, dim3(nb), dim3(256), 0, s, g, X.b, X.phi, X.r32,
+                       X.part);
+    hipLaunchKernelGGL(m_sor_decide, dim3(1), dim3(1024), 0, s, X.part, nb, X.red, c.p.sor_eps, 2, X.sor_flag);
+    FP_HIP_CHECK(hipMemcpyAsync(X.sor_host, X.sor_flag, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    FP_HIP_CHECK(hipStreamSynchronize(s));
+    if (X.sor_host[0]) break;
+  }
+  c.sor_itr = (double)X.sor_host[1];
+}
+
 void mixed_step(Ctx& c) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
@@ -3049,7 +3235,10 @@ void mixed_step(Ctx& c) {
     MTimer t(c, "helmholtz");
     helmholtz32(c, X.R, X.Up);
   }
-  {
+  if (c.p.poisson_variant == FP_POISSON_SOR) {
+    MTimer t(c, "poisson");
+    mixed_sor(c);
+  } else {
     MTimer t(c, "poisson");
     // first sweep: b = div u*/dt in the load stage of the x transform, the
     // FP32 spectral + line solve, Phi accumulated in FP64 in the store stage

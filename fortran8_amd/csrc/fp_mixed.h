@@ -9,6 +9,7 @@ void mixed_create(Ctx& c);
 void mixed_destroy(Ctx& c);
 void mixed_load_from_f64(Ctx& c);   // FP64 state buffers -> mixed state
 void mixed_store_to_f64(Ctx& c);    // mixed state -> FP64 state buffers
+void mixed_phi_to_f64(Ctx& c);      // the FP64 potential only
 void mixed_set_force(Ctx& c);
 void mixed_set_markers(Ctx& c);
 const double* mixed_marker_forces(Ctx& c);

@@ -99,7 +99,7 @@ int fp_default_params(fp_params* p);
 /* Replaces the module-level state and plan set-up: fft_init (fenep.f90:1374),
  * the allocations of program main (fenep.f90:2424-2446).  FP_E_ARG for axes
  * that are not powers of two up to 1024, and for the mixed precision with
- * the explicit method, SOR, or 2^30 cells. */
+ * the explicit method or 2^30 cells. */
 int fp_create(const fp_params* p, void** ctx);
 int fp_destroy(void* ctx);
 const char* fp_last_error(void* ctx);
@@ -158,7 +158,10 @@ int fp_diagnostics_static(void* ctx, fp_diag* d);
  * PyTorch wrapper passes tensor.data_ptr()).  Layout: interior only, x
  * fastest, FP64: U,V,W,P are nx*ny*nz each, C6 is 6 consecutive planes
  * (structure of arrays).  `stream` is a hipStream_t or NULL for the
- * context's own stream. */
+ * context's own stream.  has_history of fp_set_state_device: 0 and 1 as in
+ * fp_set_state (the potential Phi starts from 0, as after init,
+ * fenep.f90:95); 2 as 1 and Phi kept, for a state that continues the
+ * context's own run (the SOR variant's sweeps start from Phi). */
 int fp_set_state_device(void* ctx, const double* U, const double* V,
                         const double* W, const double* P, const double* C6,
                         int has_history, void* stream);

@@ -7,7 +7,7 @@ first, so the C-ABI library binds to the HIP runtime torch loaded):
   every step (every operation in FP64, FP32 storage only).
 
 Usage: _storage_floor.py <config> <steps> [out.json]; config is one of
-throughput, channel, kolmogorov64.  Prints one JSON object."""
+throughput, channel, channel_sor, kolmogorov64.  Prints one JSON object."""
 import json
 import os
 import sys
@@ -21,10 +21,11 @@ import numpy as np  # noqa: E402
 
 from fortran8_amd import api, configs  # noqa: E402
 from fortran8_amd.fields import interior, interior_c, rel_l2  # noqa: E402
-from fortran8_amd.params import PREC_MIXED  # noqa: E402
+from fortran8_amd.params import POISSON_SOR, PREC_MIXED  # noqa: E402
 
 BUILD = {"throughput": lambda: configs.throughput(),
          "channel": lambda: configs.channel_cylinder(),
+         "channel_sor": lambda: configs.channel_cylinder(poisson_variant=POISSON_SOR),
          "kolmogorov64": lambda: configs.kolmogorov(n=64)}
 
 
@@ -91,7 +92,8 @@ def main():
         for b in bufs:
             b.copy_(b.float().double())
         torch.cuda.synchronize()
-        assert L.fp_set_state_device(g.h, *[b.data_ptr() for b in bufs], 1, None) == 0
+        # 2: continue, keeping the potential (the SOR sweeps start from it)
+        assert L.fp_set_state_device(g.h, *[b.data_ptr() for b in bufs], 2, None) == 0
         if (s + 1) % 250 == 0:
             print("  floor_storage step", s + 1, flush=True)
     out["floor_storage"] = errors(R, dR, g.get_state(), g.diagnostics())

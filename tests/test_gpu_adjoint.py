@@ -45,16 +45,16 @@ def test_adjoint_cfg5_drag_gradients_and_cost():
     # the multi-segment test.
     for k in ("conv_rel_wi_h1e-06", "conv_rel_beta_h1e-06", "conv_rel_F"):
         assert out[k] < 1e-6, k
-    # and within 1e-6 of the complex-step gradient of the FP64 CPU oracle
-    # (the same discrete branch: the oracle and the GPU step are bit-identical
-    # on this periodic case)
-    for k in ("conv_rel_wi_cs", "conv_rel_beta_cs", "conv_rel_F_cs"):
-        print(k, out[k])
-        assert out[k] < 1e-6, k
-    print("L^2:", out["conv_dL2"], "complex step rel", out["conv_rel_L2_cs"])
-    # at the reference tolerance the complex step differentiates the Newton
-    # iterate actually stopped, the adjoint the converged root
-    print("reference tolerance vs complex step:", {k: out[k] for k in out if k.startswith("ref_rel") and k.endswith("_cs")})
+    # and the complex-step gradient of the FP64 CPU oracle (the same discrete
+    # branch: the oracle and the GPU step are bit-identical on this periodic
+    # case), at both Newton settings and for all four directions, L^2
+    # included (dJ/dL^2 ~ 1e-9, below what differences resolve): measured
+    # 4e-16 .. 7e-14
+    for tag in ("conv", "ref"):
+        for k in ("wi", "beta", "L2", "F"):
+            key = "%s_rel_%s_cs" % (tag, k)
+            print(key, out[key])
+            assert out[key] < 1e-10, key
     # the reference's tolerance (1e-6) stops the Newton at a point that
     # moves in steps with the parameters: Richardson-extrapolated differences
     # at h = 1e-4 within 1e-5
