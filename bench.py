@@ -40,14 +40,16 @@ STEP_WORDS = 80
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per cell of the single-kernel phases: (words read +
-# written at the storage width, extra bytes).  select: C 6 read, the
-# selection byte written.  cstar_newton: C 6 + U,V,W read, selection byte
-# read, C 6 + Peterlin f written, Newton count (int32) written.  predictor
-# (IBM form): U,V,W,P, C 6, f, A0 3, T0 3 read; Q 3, A0 3, T0 3, B 3 written.
+# written at the storage width, extra bytes), per precision (0 FP64, 1
+# mixed).  select: C 6 read, the selection byte written.  cstar_newton: C 6 +
+# U,V,W read, selection byte read, C 6 + Peterlin f written, Newton count
+# (int32) written.  predictor: U,V,W,P, C 6, f, A0 3, T0 3 read; A0 3, T0 3
+# written, and the IBM predictor's outputs: FP64 u~ 3 and the ibm_predict
+# right-hand side 3, mixed the increment d_q 3 alone.
 KERNEL_WORDS = {
-    "fenep_select": (6, 1),
-    "fenep_cstar_newton": (16, 5),
-    "predictor": (29, 0),
+    ("fenep_select", 0): (6, 1), ("fenep_select", 1): (6, 1),
+    ("fenep_cstar_newton", 0): (16, 5), ("fenep_cstar_newton", 1): (16, 5),
+    ("predictor", 0): (29, 0), ("predictor", 1): (26, 0),
 }
 
 
@@ -66,15 +68,16 @@ def cells_per_launch(p, prec, phase, phases):
 def mixed_step_bytes(n_dc):
     """Algorithmic HBM bytes per cell of the mixed IBM step with y-walls at
     the throughput size (4 B words, FP64 where stored so): select 25 (C read,
-    selection byte written), CStar/Newton 69, predictor 116, the two
-    Helmholtz sweeps in one pass 24 (q read, x written), correction 40;
-    Poisson: divergence fused into the forward x transform 16 (u* read, half
-    spectrum written), the kx-slab solve 8 (half spectrum read and written),
-    inverse x transform with the FP64 accumulation of Phi 12; the FP64 defect
-    check every 8 steps 3.5 on average (b written, residual 20).  Each
-    further defect-correction sweep adds 36.  The sparse IBM passes and the
-    FP64 re-decision of selections are not counted."""
-    return 310.0 + 3.5 + 36.0 * max(0, n_dc - 1)
+    selection byte written), CStar/Newton 69, predictor 104 (its output is
+    the increment d_q alone), the two Helmholtz sweeps in one pass 24 (d_q
+    read, d written), correction 52 (u^n, d, P, Phi read, u, P written);
+    Poisson: divergence fused into the forward x transform 28 (u^n and d
+    read, half spectrum written), the kx-slab solve 8 (half spectrum read and
+    written), inverse x transform with the FP64 accumulation of Phi 12; the
+    FP64 defect check every 8 steps 3.5 on average (b written, residual 20).
+    Each further defect-correction sweep adds 36.  The sparse IBM passes and
+    the FP64 re-decision of selections are not counted."""
+    return 322.0 + 3.5 + 36.0 * max(0, n_dc - 1)
 
 
 def run_gpu(case, steps, warmup, keep_state=False):
@@ -224,7 +227,7 @@ def main():
     # mixed step the selection also runs beside the Poisson solve on a second
     # stream, so its duration there includes the sharing).
     dom = "predictor"
-    words, extra = KERNEL_WORDS[dom]
+    words, extra = KERNEL_WORDS[(dom, prec)]
     units = cells_per_launch(case.params, prec, dom, kt)
     dom_bytes = (words * width + extra) * units
     dom_ms = kt_timed.get(dom, 0.0)

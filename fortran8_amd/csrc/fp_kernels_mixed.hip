@@ -36,7 +36,9 @@ struct Mixed {
   int cur = 0;
   float* f = nullptr;
   float *A0 = nullptr, *T0 = nullptr, *B = nullptr;
-  float *Q = nullptr, *Up = nullptr, *F = nullptr;
+  float* Q = nullptr;             // predictor increment d_q (+ dt f of the IBM spread)
+  float* Up = nullptr;            // Helmholtz increment d: u* = u^n + d
+  float* F = nullptr;
   float2* spec = nullptr;
   float* thp = nullptr;
   float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
@@ -56,7 +58,7 @@ struct Mixed {
   int* dc_host = nullptr;         // pinned copy of the last periodic decision
   hipEvent_t dc_ev = nullptr;
   bool dc_pending = false, dc_correct = false;
-  float *R = nullptr, *X2 = nullptr;  // Helmholtz right-hand side / iterate
+  float* X2 = nullptr;            // Helmholtz iterate
   int helm_iters = 0;             // fixed-point sweeps (0: spectral solve)
   unsigned* amb = nullptr;        // uncertified selections, re-decided in FP64
   unsigned* amb_list = nullptr;   // per-block regions of m * 3 + axis entries
@@ -995,12 +997,25 @@ __device__ __forceinline__ void pred_cell_load(const Grid& g, long m, int first,
   }
 }
 
-// predictor body over a stencil accessor S and the cell's own inputs
+// predictor body over a stencil accessor S and the cell's own inputs.
+// Increment form: Q receives d_q = u~ - u^n, the predictor's change of the
+// velocity, summed in FP32 before it meets u^n.  The Helmholtz solve then
+// works on the increment d = u* - u^n and the correction adds u^n + d -
+// dt grad Phi in FP64, so the velocity is rounded to FP32 once per step.
+// (Adding each term to u ~ 1 in FP32 rounds every near-uniform term, such as
+// dt times a constant body force, the same way in every cell of a mean flow:
+// a bias that grows linearly in time.)
+//   fft_navier (fenep.f90:1466-1477): [1 - eps L] u* = u^n + r + dt B/2, with
+//     u* = u^n + d:  [1 - eps L] d = r + dt B  (eps L u^n = dt B / 2)
+//   ibm_preliminary + ibm_predict (fenep.f90:1994-2013, 2229):
+//     u~ = u^n + r + dt B,  [1 - eps L] u* = u~ + dt f - dt B/2,
+//     so [1 - eps L] d = r + dt B + dt f
+// with r = -dt grad p + dt (3A - A0)/2 + dt (T + T0)/2 + dt F: the same
+// d_q = r + dt B for both methods; the IBM spread adds dt f to it.
 template <class S_t>
 __device__ __forceinline__ void predictor_body(const Grid& g, const S_t& S, const PredCell& in, long m, int j,
                                                int method, int first, int polymer, float* __restrict__ A0,
-                                               float* __restrict__ T0, float* __restrict__ Rp,
-                                               float* __restrict__ Q) {
+                                               float* __restrict__ T0, float* __restrict__ Q) {
   const float dx = (float)g.dx, dy = (float)g.dy, dz = (float)g.dz, dt = (float)g.dt;
   const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
 #define Ug(a, b, c) S.u(a, b, c)
@@ -1051,28 +1066,21 @@ __device__ __forceinline__ void predictor_body(const Grid& g, const S_t& S, cons
   } else {
     t[0] = t[1] = t[2] = 0.0f;
   }
+  (void)method;
   const float p0 = Pg(0, 0, 0);
   const float pn[3] = {Pg(1, 0, 0), Pg(0, 1, 0), Pg(0, 0, 1)};
-  const float uu[3] = {uc, vc, wc};
   const float ih[3] = {idx_, idy, idz};
-  const float bcoef = method == FP_METHOD_FFT ? 0.5f : 1.0f;   // CN half (fft_navier) or full (ibm_preliminary)
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     long oo = c * g.n + m;
     float a0 = first ? a[c] : in.a0[c];
     float t0 = first ? t[c] : in.t0[c];
-    const float qv = uu[c] - dt * (pn[c] - p0) * ih[c] + 0.5f * dt * (3.0f * a[c] - a0) + bcoef * dt * bv[c] +
+    const float dq = -(dt * (pn[c] - p0) * ih[c]) + 0.5f * dt * (3.0f * a[c] - a0) + dt * bv[c] +
                      0.5f * dt * (t[c] + t0) + dt * in.fv[c];
-    Q[oo] = qv;
+    // the wall face of V stays 0 (V_kabe)
+    Q[oo] = (c == 1 && g.walls && j == g.ny - 1) ? 0.0f : dq;
     A0[oo] = a[c];
     T0[oo] = t[c];
-    if (Rp) {
-      // ibm_predict right-hand side without the forcing (fenep.f90:2229):
-      // Ua - dt B/2, with the wall face of V read as 0 (V_kabe); the spread
-      // adds dt f at the cells the markers reach
-      const float ua = (c == 1 && g.walls && j == g.ny - 1) ? 0.0f : qv;
-      Rp[oo] = ua - 0.5f * dt * bv[c];
-    }
   }
 #undef Ug
 #undef Vg
@@ -1089,7 +1097,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                                                    const float* __restrict__ W, const float* __restrict__ P,
                                                    const float* __restrict__ C, const float* __restrict__ fC,
                                                    float* __restrict__ A0, float* __restrict__ T0,
-                                                   float* __restrict__ Rp, const float* __restrict__ F,
+                                                   const float* __restrict__ F,
                                                    float bf0, float bf1, float bf2, float* __restrict__ Q) {
   long m;
   int i, j, k;
@@ -1098,10 +1106,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
   pred_cell_load(g, m, first, A0, T0, F, bf0, bf1, bf2, in);
   if (GEN) {
     const PredWalls S{g, i, j, k, U, V, W, P, C, fC};
-    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Rp, Q);
+    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Q);
   } else {
     const PredFast S{g, m, nbr(g, i, j, k), U, V, W, P, C, fC};
-    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Rp, Q);
+    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Q);
   }
 }
 
@@ -1229,7 +1237,7 @@ __global__ __launch_bounds__(PZ_THREADS) void m_predictor_z(Grid g, int kc, int 
                                                             const float* __restrict__ W, const float* __restrict__ P,
                                                             const float* __restrict__ C, const float* __restrict__ fC,
                                                             float* __restrict__ A0, float* __restrict__ T0,
-                                                            float* __restrict__ Rp, const float* __restrict__ F,
+                                                            const float* __restrict__ F,
                                                             float bf0, float bf1, float bf2, float* __restrict__ Q) {
   __shared__ float lds[3][PZ_NF * PZ_PTS];
   const int ntx = g.nx / PZ_TX, nty = g.ny / PZ_TY;
@@ -1256,7 +1264,7 @@ __global__ __launch_bounds__(PZ_THREADS) void m_predictor_z(Grid g, int kc, int 
     PredCell in;
     pred_cell_load(g, m, first, A0, T0, F, bf0, bf1, bf2, in);
     const PredLds S{lds[sm], lds[s0], lds[sp], (ty + 1) * PZ_HX + tx + 1};
-    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Rp, Q);
+    predictor_body(g, S, in, m, j, method, first, polymer, A0, T0, Q);
     __syncthreads();
     // plane kk+2 replaces plane kk-1; fetch plane kk+3
     if (kk + 2 <= k1) pz_put<WL>(g, j0, r, lds[sm]);
@@ -1427,16 +1435,25 @@ __global__ __launch_bounds__(256) void m_fft_x_inv(Grid g, const float2* __restr
   }
 }
 
-// b = div_h(u*)/dt at cell (i, j, k) in FP64 from the FP32 u* (fft_poisson,
-// fenep.f90:1550-1552; poisson, :869-877).  idx.. are the FP64 reciprocals
-// of the spacings and of dt (an FP64 division is some twenty instructions;
-// the products differ from the quotients by an FP64 ulp).
+// u* = u^n + d at cell (i, j, k), exact in FP64, each part through the
+// boundary rules (linear and homogeneous, so that the sum takes them too)
 template <bool WL>
-__device__ __forceinline__ double div_rhs(const Grid& g, const float* Uu, const float* Vv, const float* Ww, int i, int j,
-                                          int k, double idx, double idy, double idz, double idt) {
-  return (((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - (double)ldbw<WL>(Uu, g, i - 1, j, k, BC_U)) * idx +
-          ((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - (double)ldbw<WL>(Vv, g, i, j - 1, k, BC_V)) * idy +
-          ((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - (double)ldbw<WL>(Ww, g, i, j, k - 1, BC_W)) * idz) * idt;
+__device__ __forceinline__ double ustar(const Grid& g, const float* un, const float* d, int i, int j, int k, int kind) {
+  return (double)ldbw<WL>(un, g, i, j, k, kind) + (double)ldbw<WL>(d, g, i, j, k, kind);
+}
+
+// b = div_h(u*)/dt at cell (i, j, k) in FP64 from u* = u^n + d (fft_poisson,
+// fenep.f90:1550-1552; poisson, :869-877): U, V, W the state, D the three
+// planes of the increment.  idx.. are the FP64 reciprocals of the spacings
+// and of dt (an FP64 division is some twenty instructions; the products
+// differ from the quotients by an FP64 ulp).
+template <bool WL>
+__device__ __forceinline__ double div_rhs(const Grid& g, const float* U, const float* V, const float* W, const float* D,
+                                          int i, int j, int k, double idx, double idy, double idz, double idt) {
+  const float *Du = D, *Dv = D + g.n, *Dw = D + 2 * g.n;
+  return ((ustar<WL>(g, U, Du, i, j, k, BC_U) - ustar<WL>(g, U, Du, i - 1, j, k, BC_U)) * idx +
+          (ustar<WL>(g, V, Dv, i, j, k, BC_V) - ustar<WL>(g, V, Dv, i, j - 1, k, BC_V)) * idy +
+          (ustar<WL>(g, W, Dw, i, j, k, BC_W) - ustar<WL>(g, W, Dw, i, j, k - 1, BC_W)) * idz) * idt;
 }
 
 // The Poisson right-hand side b = div_h(u*)/dt (div_rhs) computed in the load stage of
@@ -1444,7 +1461,9 @@ __device__ __forceinline__ double div_rhs(const Grid& g, const float* Uu, const 
 // m_fft_x_fwd.  want_b: also store b (FP64) and the per-block |b|^2 for the
 // defect check.
 template <bool WL>
-__global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __restrict__ up, float2* __restrict__ spec,
+__global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __restrict__ U, const float* __restrict__ V,
+                                                       const float* __restrict__ W, const float* __restrict__ up,
+                                                       float2* __restrict__ spec,
                                                        const float2* __restrict__ tw, int nl, double* __restrict__ b,
                                                        double* __restrict__ nrm, int want_b, int tr) {
   extern __shared__ float2 ldsf[];
@@ -1453,7 +1472,6 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
   float2* y = ldsf + nc * n;
   float2* tws = ldsf + 2 * nc * n;
   load_twiddles(tws, tw, n);
-  const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
   const long nlines = (long)g.ny * g.nz;
   const long l0 = (long)blockIdx.x * nl;
   double ss = 0.0;
@@ -1467,7 +1485,7 @@ __global__ __launch_bounds__(256) void m_div_fft_x_fwd(Grid g, const float* __re
       const long L = l0 + 2 * c + h;
       if (L >= nlines) continue;
       const int j = (int)(line_row(g, L)), k = (int)(line_plane(g, L));
-      const double v = div_rhs<WL>(g, Uu, Vv, Ww, i, j, k, idx, idy, idz, idt);
+      const double v = div_rhs<WL>(g, U, V, W, up, i, j, k, idx, idy, idz, idt);
       if (want_b) {
         b[L * n + i] = v;
         ss += v * v;
@@ -1986,7 +2004,9 @@ __global__ __launch_bounds__(1024) void m_sum_decide(const double* __restrict__ 
 
 // b (FP64), r = b - L_h phi (FP64, stored in FP32), e = 0, per-block |b|^2
 template <bool WL>
-__global__ __launch_bounds__(256) void m_sor_start(Grid g, const float* __restrict__ up, const double* __restrict__ phi,
+__global__ __launch_bounds__(256) void m_sor_start(Grid g, const float* __restrict__ U, const float* __restrict__ V,
+                                                   const float* __restrict__ W, const float* __restrict__ up,
+                                                   const double* __restrict__ phi,
                                                    double* __restrict__ b, float* __restrict__ r32,
                                                    float* __restrict__ e, double* __restrict__ pb) {
   const long m = xcd_cell();
@@ -1995,7 +2015,7 @@ __global__ __launch_bounds__(256) void m_sor_start(Grid g, const float* __restri
     int i, j, k;
     cell_ijk(g, m, i, j, k);
     const double idx = 1.0 / g.dx, idy = 1.0 / g.dy, idz = 1.0 / g.dz, idt = 1.0 / g.dt;
-    bb = div_rhs<WL>(g, up, up + g.n, up + 2 * g.n, i, j, k, idx, idy, idz, idt);
+    bb = div_rhs<WL>(g, U, V, W, up, i, j, k, idx, idy, idz, idt);
     const double p = phi[m];
 #define PH(a, bq, c) ldbw<WL>(phi, g, a, bq, c, BC_P)
     const double lap = (PH(i - 1, j, k) - 2.0 * p + PH(i + 1, j, k)) / (g.dx * g.dx) +
@@ -2300,19 +2320,22 @@ static void helmholtz32(Ctx& c, const float* q, float* x) {
 // fft_march (fenep.f90:1562-1604) with FP64 potential, FP32 storage
 template <bool WL>
 __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__ up, const double* __restrict__ phi,
-                                               float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
-                                               float* __restrict__ P) {
+                                               float* U, float* V, float* W, float* __restrict__ P) {
   long m = xcd_cell();
   if (m >= g.n) return;
   int i, j, k;
   cell_ijk(g, m, i, j, k);
   const double dt = g.dt;
-  const float *Uu = up, *Vv = up + g.n, *Ww = up + 2 * g.n;
+  const float *Du = up, *Dv = up + g.n, *Dw = up + 2 * g.n;
 #define PH(a, b, c) ldbw<WL>(phi, g, a, b, c, BC_P)
   double ph = phi[m];
-  U[m] = (float)((double)ldbw<WL>(Uu, g, i, j, k, BC_U) - dt * (PH(i + 1, j, k) - ph) / g.dx);
-  V[m] = (float)((double)ldbw<WL>(Vv, g, i, j, k, BC_V) - dt * (PH(i, j + 1, k) - ph) / g.dy);
-  W[m] = (float)((double)ldbw<WL>(Ww, g, i, j, k, BC_W) - dt * (PH(i, j, k + 1) - ph) / g.dz);
+  // u^{n+1} = (u^n + d) - dt grad Phi in FP64, one rounding; each thread
+  // reads and writes its own cell of U, V, W only
+  const double us = ustar<WL>(g, U, Du, i, j, k, BC_U), vs = ustar<WL>(g, V, Dv, i, j, k, BC_V),
+               ws = ustar<WL>(g, W, Dw, i, j, k, BC_W);
+  U[m] = (float)(us - dt * (PH(i + 1, j, k) - ph) / g.dx);
+  V[m] = (float)(vs - dt * (PH(i, j + 1, k) - ph) / g.dy);
+  W[m] = (float)(ws - dt * (PH(i, j, k + 1) - ph) / g.dz);
   const double coef = ((dt * g.beta) / g.re) / 2.0;
   double lap = (PH(i - 1, j, k) - 2.0 * ph + PH(i + 1, j, k)) / (g.dx * g.dx) +
                (PH(i, j - 1, k) - 2.0 * ph + PH(i, j + 1, k)) / (g.dy * g.dy) +
@@ -2323,8 +2346,8 @@ __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__
 
 // The same correction streamed along z: a 64 x 4 tile walks 8 planes with
 // three planes of Phi (FP64, one-cell halo, the boundary rule of P applied)
-// in LDS and the next plane prefetched; u*, P and the outputs are the cell's
-// own, read and written once.
+// in LDS and the next plane prefetched; u^n, the increment d, P and the
+// outputs are the cell's own, read and written once.
 #define MZ_TX 64
 #define MZ_TY 4
 #define MZ_THREADS (MZ_TX * MZ_TY)
@@ -2393,13 +2416,16 @@ __global__ __launch_bounds__(MZ_THREADS) void m_march_z(Grid g, const float* __r
   for (int kk = k0; kk < k1; kk++) {
     __syncthreads();
     const long m = cidx(g, i, j, kk);
-    const float u = Uu[m], v = (WL && j == g.ny - 1) ? 0.0f : Vv[m], w = Ww[m];   // V_kabe: the wall face reads 0
+    // u* = u^n + d, exact in FP64 (V_kabe: the wall face reads 0)
+    const double u = (double)U[m] + (double)Uu[m];
+    const double v = (WL && j == g.ny - 1) ? 0.0 : (double)V[m] + (double)Vv[m];
+    const double w = (double)W[m] + (double)Ww[m];
     const float pold = P[m];
     const double* A = ps[s0];
     const double ph = A[c];
-    U[m] = (float)((double)u - dt * (A[c + 1] - ph) * idx);
-    V[m] = (float)((double)v - dt * (A[c + MZ_HX] - ph) * idy);
-    W[m] = (float)((double)w - dt * (ps[sp][c] - ph) * idz);
+    U[m] = (float)(u - dt * (A[c + 1] - ph) * idx);
+    V[m] = (float)(v - dt * (A[c + MZ_HX] - ph) * idy);
+    W[m] = (float)(w - dt * (ps[sp][c] - ph) * idz);
     const double lap = (A[c - 1] - 2.0 * ph + A[c + 1]) * idx2 + (A[c - MZ_HX] - 2.0 * ph + A[c + MZ_HX]) * idy2 +
                        (ps[sm][c] - 2.0 * ph + ps[sp][c]) * idz2;
     P[m] = (float)(((double)pold + ph) - coef * lap);
@@ -2462,31 +2488,37 @@ __global__ void m_ibm_setup_spread(Grid g, int comp, int rows, const int* __rest
   }
 }
 
-// interpolation of Ua at the markers and the direct forcing Fc = (Uc - Ub)/dt
-// (ibm_Helmholtz, fenep.f90:2076-2119), one thread per (component, marker)
+// interpolation of Ua = u^n + d_q at the markers (the two summed apart) and
+// the direct forcing Fc = (Uc - Ub)/dt (ibm_Helmholtz, fenep.f90:2076-2119),
+// one thread per (component, marker)
 template <bool WL>
 __global__ __launch_bounds__(256) void m_ibm_interp(Grid g, long nmk, const float* __restrict__ wi, const int4* __restrict__ org,
-                                                    const float* __restrict__ uc, const float* __restrict__ ua,
-                                                    float* __restrict__ mf) {
+                                                    const float* __restrict__ uc, const float* __restrict__ U,
+                                                    const float* __restrict__ V, const float* __restrict__ W,
+                                                    const float* __restrict__ dq, float* __restrict__ mf) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= 3 * nmk) return;
   const int c = (int)(t / nmk);
   const long id = t - c * nmk;
   const float idt = (float)(1.0 / g.dt);
   const int4 o = org[t];
-  const float* X = ua + c * g.n;
+  const float* X = c == 0 ? U : (c == 1 ? V : W);   // three separate planes of the state
+  const float* D = dq + c * g.n;
   const int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
   const float* w = wi + (long)c * 27 * nmk + id;
-  float acc = 0.0f;
+  float acc = 0.0f, accd = 0.0f;
   int s = 0;
 #pragma unroll
   for (int kk = 0; kk < 3; kk++)
 #pragma unroll
     for (int jj = 0; jj < 3; jj++)
 #pragma unroll
-      for (int ii = 0; ii < 3; ii++, s++)
-        acc += w[(long)s * nmk] * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
-  mf[t] = (uc[t] - acc) * idt;
+      for (int ii = 0; ii < 3; ii++, s++) {
+        const float wv = w[(long)s * nmk];
+        acc += wv * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
+        accd += wv * ldbw<WL>(D, g, o.x + ii - 1, o.y + jj - 1, o.z - 1 + kk - 1, kind);
+      }
+  mf[t] = ((uc[t] - acc) - accd) * idt;
 }
 
 // entries of a sliced-ELL slice come in groups of this many
@@ -2529,7 +2561,9 @@ __global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* 
 template <bool WL>
 __global__ __launch_bounds__(256) void m_ibm_interp_z(Grid g, long nmk, long zP, const float* __restrict__ wz,
                                                       const int2* __restrict__ orgz, const float* __restrict__ uc,
-                                                      const float* __restrict__ ua, float* __restrict__ mf) {
+                                                      const float* __restrict__ U, const float* __restrict__ V,
+                                                      const float* __restrict__ W, const float* __restrict__ dq,
+                                                      float* __restrict__ mf) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= 3 * nmk) return;
   const int c = (int)(t / nmk);
@@ -2538,19 +2572,23 @@ __global__ __launch_bounds__(256) void m_ibm_interp_z(Grid g, long nmk, long zP,
   const long p = id - (long)L * zP;
   const float idt = (float)(1.0 / g.dt);
   const int2 o = orgz[c * zP + p];
-  const float* X = ua + c * g.n;
+  const float* X = c == 0 ? U : (c == 1 ? V : W);   // three separate planes of the state
+  const float* D = dq + c * g.n;
   const int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
   const float* w = wz + (long)c * 27 * zP + p;
-  float acc = 0.0f;
+  float acc = 0.0f, accd = 0.0f;
   int s = 0;
 #pragma unroll
   for (int kk = 0; kk < 3; kk++)
 #pragma unroll
     for (int jj = 0; jj < 3; jj++)
 #pragma unroll
-      for (int ii = 0; ii < 3; ii++, s++)
-        acc += w[(long)s * zP] * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, L + kk - 1, kind);
-  mf[t] = (uc[t] - acc) * idt;
+      for (int ii = 0; ii < 3; ii++, s++) {
+        const float wv = w[(long)s * zP];
+        acc += wv * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, L + kk - 1, kind);
+        accd += wv * ldbw<WL>(D, g, o.x + ii - 1, o.y + jj - 1, L + kk - 1, kind);
+      }
+  mf[t] = ((uc[t] - acc) - accd) * idt;
 }
 
 // Spreading: thread (plane k, row r of plane 0); an entry names the layer-0
@@ -2656,7 +2694,7 @@ void mixed_create(Ctx& c) {
   *X->dc_host = 1;
   FP_HIP_CHECK(hipEventCreateWithFlags(&X->dc_ev, hipEventDisableTiming));
   dalloc(&X->skip, 1);
-  dalloc(&X->R, 3 * n); dalloc(&X->X2, 3 * n);
+  dalloc(&X->X2, 3 * n);
   X->helm_iters = helm_sweeps(g, c.p.helm_sweeps);
   // every (cell, axis) may need the FP64 decision: a region of 3 * 256
   // entries per select block (3 n for n >= 256; m * 3 + axis < 2^32 as every
@@ -2707,7 +2745,7 @@ void mixed_destroy(Ctx& c) {
   dfree(X->pbuf);
   if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
   if (X->dc_host) (void)hipHostFree(X->dc_host);
-  dfree(X->R); dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
+  dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
   dfree(X->amb_cnt); dfree(X->amb_off); dfree(X->amb_wstart); dfree(X->amb_data);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
@@ -3138,7 +3176,7 @@ static void mixed_sor(Ctx& c) {
   const double B0 = ((((BX + BX) + BY) + BY) + BZ) + BZ;
   const float om_b0 = (float)(om / B0);
   const int SOR_BATCH = 10;
-  hipLaunchKernelGGL(g.walls ? m_sor_start<true> : m_sor_start<false>, dim3(nb), dim3(256), 0, s, g, X.Up, X.phi, X.b,
+  hipLaunchKernelGGL(g.walls ? m_sor_start<true> : m_sor_start<false>, dim3(nb), dim3(256), 0, s, g, X.U, X.V, X.W, X.Up, X.phi, X.b,
                      X.r32, X.e32, X.sor_pb);
   hipLaunchKernelGGL(m_sor_decide, dim3(1), dim3(1024), 0, s, X.sor_pb, nb, X.red, c.p.sor_eps, 0, X.sor_flag);
   int launched = 0;
@@ -3184,25 +3222,24 @@ void mixed_step(Ctx& c) {
     }
     X.cur = 1 - X.cur;
   }
-  float* Rp = c.p.method == FP_METHOD_IBM ? X.R : nullptr;
   if (g.nx % PZ_TX == 0 && g.ny % PZ_TY == 0) {
     // streamed along z, wall rows included
     MTimer t(c, "predictor");
     const long nbk = (g.nz + PZ_KC - 1) / PZ_KC;
     const long grid = (long)(g.nx / PZ_TX) * (g.ny / PZ_TY) * nbk;
     hipLaunchKernelGGL(g.walls ? m_predictor_z<true> : m_predictor_z<false>, dim3(grid), dim3(PZ_THREADS), 0, s, g, PZ_KC,
-                       c.p.method, c.has_history ? 0 : 1, polymer, X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp,
+                       c.p.method, c.has_history ? 0 : 1, polymer, X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0,
                        X.F, (float)c.p.body_force[0], (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   } else {
     MTimer t(c, "predictor");
     hipLaunchKernelGGL(m_predictor<false>, dim3(nblk(n, 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
-                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
+                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   }
   if (g.walls && !(g.nx % PZ_TX == 0 && g.ny % PZ_TY == 0)) {
     MTimer t(c, "predictor_walls");
     hipLaunchKernelGGL(m_predictor<true>, dim3(nblk(wall_rows(g, 1), 256)), dim3(256), 0, s, g, c.p.method, c.has_history ? 0 : 1, polymer,
-                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, Rp, X.F, (float)c.p.body_force[0],
+                       X.U, X.V, X.W, X.P, X.C[X.cur], X.f, X.A0, X.T0, X.F, (float)c.p.body_force[0],
                        (float)c.p.body_force[1], (float)c.p.body_force[2], X.Q);
   }
   // The next step's selection (it reads only the new C) forks onto the second
@@ -3218,22 +3255,22 @@ void mixed_step(Ctx& c) {
       MTimer t(c, "ibm");
       if (X.zinv) {
         hipLaunchKernelGGL(g.walls ? m_ibm_interp_z<true> : m_ibm_interp_z<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0,
-                           s, g, c.nmk, X.zP, X.zi_w, X.zi_org, X.uc, X.Q, X.mf);
+                           s, g, c.nmk, X.zP, X.zi_w, X.zi_org, X.uc, X.U, X.V, X.W, X.Q, X.mf);
         if (X.zs_rows)
           hipLaunchKernelGGL(m_ibm_spread_z, dim3(nblk((long)g.nz * X.zs_rows64, 256)), dim3(256), 0, s, X.zs_rows,
                              X.zs_rows64, g.nz, X.zP, (long)g.nx * g.ny, X.zs_off, X.zs_len, X.zs_tgt, X.zs_ent, X.zs_w,
-                             X.mf, (float)g.dt, X.R);
+                             X.mf, (float)g.dt, X.Q);
       } else {
         hipLaunchKernelGGL(g.walls ? m_ibm_interp<true> : m_ibm_interp<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0, s,
-                           g, c.nmk, X.wi, X.org, X.uc, X.Q, X.mf);
+                           g, c.nmk, X.wi, X.org, X.uc, X.U, X.V, X.W, X.Q, X.mf);
         if (X.fr_rows)
           hipLaunchKernelGGL(m_ibm_spread_folded, dim3(nblk(X.fr_rows, 256)), dim3(256), 0, s, X.fr_rows, X.fr_row,
-                             X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.R);
+                             X.fr_len, X.fr_tgt, X.fr_ent, X.fr_w, X.mf, (float)g.dt, X.Q);
       }
     }
     if (polymer) select_ahead(c);
     MTimer t(c, "helmholtz");
-    helmholtz32(c, X.R, X.Up);
+    helmholtz32(c, X.Q, X.Up);
   }
   if (c.p.poisson_variant == FP_POISSON_SOR) {
     MTimer t(c, "poisson");
@@ -3248,7 +3285,7 @@ void mixed_step(Ctx& c) {
     const int tr = X.slab;
     const size_t lds_inv = tr ? lds_x32_tr(g) : lds_x32(g);
     hipLaunchKernelGGL(g.walls ? m_div_fft_x_fwd<true> : m_div_fft_x_fwd<false>, dim3(nbx), dim3(256), lds_x32(g), s, g,
-                       X.Up, X.spec, X.twx, lx, X.b, X.partb, check ? 1 : 0, tr);
+                       X.U, X.V, X.W, X.Up, X.spec, X.twx, lx, X.b, X.partb, check ? 1 : 0, tr);
     solve32_yz(c, 3, nullptr, tr);
     hipLaunchKernelGGL(m_fft_x_inv_accum, dim3(nbx), dim3(256), lds_inv, s, g, X.spec, X.phi, X.twx, lx, 1, nullptr, tr);
     if (check) {

@@ -243,10 +243,12 @@ def test_cfg3_mixed_2000_steps_against_fp64_and_the_storage_floor(variant):
     report(res)
     m, fl = res["mixed"], res["floor_storage"]
     assert m["sel_differ"] == 0
-    for k in ("u", "drag", "ke", "P_dyn"):
+    for k in ("u", "P_dyn"):
         assert m[k] < 1e-4, k
+    # measured: drag 7.8e-7 (FFT) and 7.9e-6 (SOR), kinetic energy 2.7e-7 and 4.0e-7
+    assert m["drag"] < 2e-5 and m["ke"] < 1e-6
     # the cylinder's wake amplifies the FP32 rounding of the stored state:
-    # each field within 2.5 times the FP32-storage floor
+    # each field within 1.5 times the FP32-storage floor (measured 0.67-1.18)
     for k in ("U", "V", "W", "P", "C"):
         print("%s: mixed / FP32-storage floor = %.2f" % (k, m[k] / fl[k]), flush=True)
-        assert m[k] <= 2.5 * fl[k], k
+        assert m[k] <= 1.5 * fl[k], k
