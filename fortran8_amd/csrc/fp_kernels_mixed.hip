@@ -731,6 +731,14 @@ __device__ __forceinline__ T dpeterlin_t(T L2, T s) {
 // for one cell, in the arithmetic type T (storage stays FP32; the step uses
 // FP32: an FP64 evaluation of the strongly stretched cells was measured to
 // leave the error against the FP64 step unchanged).
+//
+// Increment form: C* = C^n + D* and C^{n+1} = C^n + delta, with D* and the
+// Newton residual summed from O(dt) terms only, so that C is rounded to FP32
+// once per step.  Cstar's weights hold 1/6 on the own cell's two faces
+// (fenep.f90:484-497); both faces of every candidate sum to 2 C^n, so those
+// parts add up to C^n exactly and D* keeps only the velocity-dependent
+// parts.  (Summed as written in FP32, 6 fl(1/6) = 1 + 3e-8 multiplies C by
+// that factor every step: a drift of 3e-5 in 1000 steps where C ~ I.)
 template <typename T>
 __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __restrict__ C,
                                                   const uint8_t* __restrict__ sel, const float* __restrict__ U,
@@ -763,9 +771,10 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     else if (d == 1) { v1 = wlo ? T(0) : (T)ldu(V, g.n, mu + o.y[0]); v2 = whi ? T(0) : (T)ldu(V, g.n, mu); }   // V_kabe
     else { v1 = ldu(W, g.n, mu + o.z[0]); v2 = ldu(W, g.n, mu); }
     const T r2h = dt * (T(0.5) * rcp_t(h));
+    // the weights less their 1/6 parts (s2 and s3 of the reference)
     const T s1 = (-v2 + fabs(v2)) * r2h;
-    const T s2 = T(1) / T(6) + (-v2 - fabs(v2)) * r2h;
-    const T s3 = T(1) / T(6) + (v1 - fabs(v1)) * r2h;
+    const T s2 = (-v2 - fabs(v2)) * r2h;
+    const T s3 = (v1 - fabs(v1)) * r2h;
     const T s4 = (v1 + fabs(v1)) * r2h;
     const int om = d == 0 ? o.x[0] : (d == 1 ? o.y[0] : o.z[0]);
     const int op = d == 0 ? o.x[2] : (d == 1 ? o.y[2] : o.z[2]);
@@ -775,7 +784,8 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     //   pl = Cp_l0(c1,c2,c3)  own left face      nr = Cn_l0(c1,c2,c3) own right face
     //   pr = Cp_lp(c2,c3,c4)  left face of a+1   nl = Cn_lm(c0,c1,c2) right face of a-1
     // so s3 pl + s1 pr + s4 nl + s2 nr = sum_p w_p c_p with five weights per
-    // axis shared by the six components.
+    // axis shared by the six components (the 1/6 parts, 1/6 (pl + nr) =
+    // c2 / 3 per axis, make the C^n of C* = C^n + D*).
     T ap, bp, gp_, an, bn, gn, aq, bq, gq, am, bm, gm;
     cand_coef_p<T>(l0, ap, bp, gp_);
     cand_coef_n<T>(l0, an, bn, gn);
@@ -793,7 +803,8 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
       cx[q] += w0 * (T)v[0][q] + w1 * (T)v[1][q] + w2 * (T)v[2][q] + w3 * (T)v[3][q] + w4 * (T)v[4][q];
   }
   const T dtw = dt * rcp_t((T)g.wi);
-  T b[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
+  // right-hand side less C^n: D* + (dt/Wi) I
+  const T rhs[6] = {cx[0] + dtw, cx[1], cx[2], cx[3] + dtw, cx[4], cx[5] + dtw};
   const T dx = (T)g.dx, dy = (T)g.dy, dz = (T)g.dz;
   // A = transpose(A0) = -dt L with L_ij = du_i/dx_j (fenep.f90:541-550)
   T a11, a12, a13, a21, a22, a23, a31, a32, a33;
@@ -826,9 +837,13 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
 #undef WR
   }
   const T L2 = (T)(g.lp * g.lp);
-  T x[6];
+  T c0[6], x[6], dl[6];
 #pragma unroll
-  for (int q = 0; q < 6; q++) x[q] = ldu(C + q * g.n, g.n, mu);
+  for (int q = 0; q < 6; q++) {
+    c0[q] = ldu(C + q * g.n, g.n, mu);
+    x[q] = c0[q];
+    dl[q] = T(0);
+  }
   int it;
   // Chord iterations: the Jacobian of the first iterate (the linear operator
   // plus the rank-one Peterlin term) is factored once, its multipliers kept
@@ -852,17 +867,19 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
   for (it = 1; it <= itrmax; it++) {
     const T s = x[0] + x[3] + x[5];
     const T f = peterlin_t<T>(L2, s);
-    const T dg = T(0.5) * (T(1) + f * dtw);
-    const T A11 = a11 + dg, A22 = a22 + dg, A33 = a33 + dg;
-    // residual l = M x - b (fenep.f90:598-603)
+    const T dh = T(0.5) * (f * dtw);   // the diagonal less its 1/2
+    const T B11 = a11 + dh, B22 = a22 + dh, B33 = a33 + dh;
+    // residual l = M x - b (fenep.f90:598-603) from O(dt) terms: M = I + M'
+    // and b = C^n + rhs, so l = M' x + (delta - rhs)
     T l[6];
-    l[0] = T(2) * (A11 * x[0] + a12 * x[1] + a13 * x[2]) - b[0];
-    l[1] = a21 * x[0] + (A22 + A11) * x[1] + a23 * x[2] + a12 * x[3] + a13 * x[4] - b[1];
-    l[2] = a31 * x[0] + a32 * x[1] + (A33 + A11) * x[2] + a12 * x[4] + a13 * x[5] - b[2];
-    l[3] = T(2) * (a21 * x[1] + A22 * x[3] + a23 * x[4]) - b[3];
-    l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (A33 + A22) * x[4] + a23 * x[5] - b[4];
-    l[5] = T(2) * (a31 * x[2] + a32 * x[4] + A33 * x[5]) - b[5];
+    l[0] = T(2) * (B11 * x[0] + a12 * x[1] + a13 * x[2]) + (dl[0] - rhs[0]);
+    l[1] = a21 * x[0] + (B22 + B11) * x[1] + a23 * x[2] + a12 * x[3] + a13 * x[4] + (dl[1] - rhs[1]);
+    l[2] = a31 * x[0] + a32 * x[1] + (B33 + B11) * x[2] + a12 * x[4] + a13 * x[5] + (dl[2] - rhs[2]);
+    l[3] = T(2) * (a21 * x[1] + B22 * x[3] + a23 * x[4]) + (dl[3] - rhs[3]);
+    l[4] = a31 * x[1] + a21 * x[2] + a32 * x[3] + (B33 + B22) * x[4] + a23 * x[5] + (dl[4] - rhs[4]);
+    l[5] = T(2) * (a31 * x[2] + a32 * x[4] + B33 * x[5]) + (dl[5] - rhs[5]);
     if (it == 1 || it >= 3) {
+      const T A11 = B11 + T(0.5), A22 = B22 + T(0.5), A33 = B33 + T(0.5);
       // Jacobian: linear operator + rank one Peterlin term
       const T gp = dtw * dpeterlin_t<T>(L2, s);
       const T J0[6][6] = {
@@ -907,7 +924,12 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     }
     T sr = T(0), sx = T(0);
 #pragma unroll
-    for (int q = 0; q < 6; q++) { x[q] -= rr[q]; sr += rr[q] * rr[q]; sx += x[q] * x[q]; }
+    for (int q = 0; q < 6; q++) {
+      dl[q] -= rr[q];
+      x[q] = c0[q] + dl[q];
+      sr += rr[q] * rr[q];
+      sx += x[q] * x[q];
+    }
     if (sr < eps * eps * sx) break;
   }
   if (it > itrmax) it = -itrmax;   // stopped at the cap without meeting eps
