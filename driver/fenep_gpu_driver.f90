@@ -41,8 +41,10 @@ module fenep_gpu_api
     real(c_double) :: body_force(3)
     integer(c_int32_t) :: n_dc_max
     real(c_double) :: dc_tol
-    integer(c_int32_t) :: helm_sweeps
-    integer(c_int32_t) :: reserved(7)
+    integer(c_int32_t) :: helm_sweeps, reserved0
+    real(c_double) :: ibm_weight
+    integer(c_int64_t) :: seed
+    integer(c_int32_t) :: reserved(2)
   end type fp_params
 
   type, bind(c) :: fp_diag
@@ -192,12 +194,13 @@ program fenep_gpu
   real(8) :: DC = 0.5d0, xcyl(64) = 0.0d0, ycyl(64) = 0.0d0
   integer :: newton_itrmax = 10000, sor_itrmax = 10000, n_dc_max = 2
   real(8) :: newton_eps = 1.0d-6, sor_eps = 1.0d-5
+  real(8) :: ibm_weight = 0.0d0      ! spreading weight: 0 = dX**3 (fenep), > 0 e.g. smac's dV
   integer :: map_text = 0
   character(256) :: dir = './'
   namelist /run/ NX, NY, NZ, Lx, Ly, Lz, dt, Re, beta, Wi, Lp, method, walls_y, poisson_variant, &
                  precision, Nstep, Lstep, Gstep, Ostep, koide_step, Tstep, flow_type, input_type, seed, &
                  force_amp, body_force, ncyl, DC, xcyl, ycyl, newton_itrmax, sor_itrmax, n_dc_max, &
-                 newton_eps, sor_eps, map_text, dir
+                 newton_eps, sor_eps, ibm_weight, map_text, dir
 
   type(fp_params), target :: prm
   type(fp_diag) :: dg
@@ -233,6 +236,8 @@ program fenep_gpu
   prm%newton_itrmax = newton_itrmax; prm%newton_eps = newton_eps
   prm%sor_itrmax = sor_itrmax; prm%sor_eps = sor_eps; prm%n_dc_max = n_dc_max
   prm%body_force = body_force
+  prm%seed = seed
+  prm%ibm_weight = ibm_weight
   call check(fp_create(prm, ctx), ctx, 'fp_create')
 
   allocate(U(0:NX+1, 0:NY+1, 0:NZ+1), V(0:NX+1, 0:NY+1, 0:NZ+1), W(0:NX+1, 0:NY+1, 0:NZ+1))

@@ -146,7 +146,7 @@ def kolmogorov(n=64, nz=None, steps=1000, seed=2, lz=None, precision=0, **kw):
     d = L / n
     lz = nz * d if lz is None else lz
     p = default_params(nx=n, ny=n, nz=nz, lx=L, ly=L, lz=lz, dt=0.002, re=100.0, beta=0.9,
-                       wi=2.0, lp=10.0, method=METHOD_FFT, precision=precision, **kw)
+                       wi=2.0, lp=10.0, method=METHOD_FFT, precision=precision, seed=seed, **kw)
     dz = lz / nz
     u, v, w = _noise_velocity(seed, n, n, nz, 1.0e-3, d, d, dz, False, offset=0)
     c = _spd_perturbed_c(seed, n, n, nz, offset=3 * n * n * nz + 8)
@@ -162,7 +162,7 @@ def _channel(name, nx, ny, nz, lx, ly, lz, re, wi, beta, lp, dt, centres, diamet
     G = 12.0 * nu * 1.0 / ly ** 2       # plane Poiseuille gradient for mean U = 1
     p = default_params(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, dt=dt, re=re, beta=beta, wi=wi,
                        lp=lp, method=METHOD_IBM, walls_y=1, poisson_variant=poisson_variant,
-                       precision=precision, body_force=(G, 0.0, 0.0), **kw)
+                       precision=precision, body_force=(G, 0.0, 0.0), seed=seed, **kw)
     dx, dy, dz = lx / nx, ly / ny, lz / nz
     u, v, w = _noise_velocity(seed, nx, ny, nz, 1.0e-3, dx, dy, dz, True, offset=0)
     u = u + 1.0

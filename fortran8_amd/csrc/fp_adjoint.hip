@@ -116,9 +116,10 @@ __global__ __launch_bounds__(256) void k_adj_helm_beta(Grid g, const double* __r
 
 // ------------------------------------------------------------- IBM^T
 // Forward: Ub = interp(Ua); Fc = (Uc - Ub)/dt; f = spread(Fc);
-// R = Ua + dt f - dt B/2; D = -sum Fcx dX^3.  With Rbar given:
-// Fcbar = dt interp(Rbar) - w dX^3 e_x and Uabar = Rbar + spread(-Fcbar/dt).
-// This kernel writes gm = -Fcbar/dt per marker and component.
+// R = Ua + dt f - dt B/2; D = -sum Fcx W (W the spreading weight g.sw, dX^3
+// by default).  With Rbar given: Fcbar = dt spread^T(Rbar) - w W e_x and
+// Uabar = Rbar + interp^T(-Fcbar/dt).
+// This kernel writes gm = -Fcbar/dt (times dX^3 / W) per marker and component.
 __global__ __launch_bounds__(256) void k_adj_ibm_markers(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
                                                          const double* __restrict__ Rb, double wdrag,
                                                          double* __restrict__ gm) {
@@ -126,7 +127,6 @@ __global__ __launch_bounds__(256) void k_adj_ibm_markers(Grid g, long nmk, int n
   if (id >= nmk) return;
   const int n = (int)(id / ((long)nc * nl)) + 1;
   const double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
-  const double dx3 = (g.dx * g.dx) * g.dx;
   for (int c = 0; c < 3; c++) {
     int i0, j0;
     ibm_origin_d(g, c, xc, yc, i0, j0);
@@ -135,9 +135,10 @@ __global__ __launch_bounds__(256) void k_adj_ibm_markers(Grid g, long nmk, int n
     for (int kk = n - 1; kk <= n + 1; kk++)
       for (int jj = j0; jj <= j0 + 2; jj++)
         for (int ii = i0; ii <= i0 + 2; ii++)
-          acc += X[pidx(g, ii - 1, jj - 1, kk - 1)] * ibm_weight(g, c, ii, jj, kk, xc, yc, zc) * dx3;
-    const double fbar = g.dt * acc - (c == 0 ? dx3 * wdrag : 0.0);
-    gm[c * nmk + id] = -fbar / g.dt;
+          acc += X[pidx(g, ii - 1, jj - 1, kk - 1)] * ibm_weight(g, c, ii, jj, kk, xc, yc, zc) * g.sw;
+    const double fbar = g.dt * acc - (c == 0 ? g.sw * wdrag : 0.0);
+    // interp^T(x) = spread(x) dX^3 / W: the spread kernels apply W
+    gm[c * nmk + id] = (-fbar / g.dt) * (((g.dx * g.dx) * g.dx) / g.sw);
   }
 }
 
@@ -566,7 +567,7 @@ __global__ __launch_bounds__(256) void k_adj_vel_grad(Grid g, const double* __re
 }
 
 // drag of the last step: D = -sum Fxc dX^3 (one block)
-__global__ __launch_bounds__(1024) void k_drag(const double* __restrict__ mf, long nmk, double dx3, double* __restrict__ out) {
+__global__ __launch_bounds__(1024) void k_drag(const double* __restrict__ mf, long nmk, double wd, double* __restrict__ out) {
   __shared__ double sh[1024];
   double v = 0.0;
   for (long t = threadIdx.x; t < nmk; t += 1024) v += mf[t];
@@ -576,7 +577,7 @@ __global__ __launch_bounds__(1024) void k_drag(const double* __restrict__ mf, lo
     if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = -(sh[0] * dx3);
+  if (threadIdx.x == 0) out[0] = -(sh[0] * wd);
 }
 
 // ------------------------------------------------------------ host state
@@ -749,7 +750,7 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
   free0(a.drag);
   alloc0(&a.drag, (size_t)nsteps);
   a.ck_hist.assign(nck + 1, 0);
-  const double dx3 = (c.g.dx * c.g.dx) * c.g.dx;
+  const double wd = c.g.sw;   // the drag sum's weight
   const int step0 = c.step;
   // Mixed precision: the checkpoints and the drag are those of the mixed
   // step (its FP32 state widened to FP64); the reverse sweep recomputes each
@@ -764,7 +765,7 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
     if (c.mixed) {
       mixed_step(c);
       if (c.p.method == FP_METHOD_IBM)
-        hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, mixed_marker_forces(c), c.nmk, dx3, a.drag + s);
+        hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, mixed_marker_forces(c), c.nmk, wd, a.drag + s);
       continue;
     }
     c.sel_replay = (replay && polymer) ? a.selh + (size_t)s * n : nullptr;
@@ -772,7 +773,7 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
     c.sel_replay = nullptr;
     if (!replay && polymer) d2d(c, a.selh + (size_t)s * n, c.sel, n);
     if (c.p.method == FP_METHOD_IBM)
-      hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, c.mf, c.nmk, dx3, a.drag + s);
+      hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, c.mf, c.nmk, wd, a.drag + s);
   }
   if (c.mixed) {
     mixed_join(c);

@@ -214,6 +214,7 @@ int fp_create(const fp_params* p, void** out) {
     g.n = (long)p->nx * p->ny * p->nz;
     g.dx = p->lx / p->nx; g.dy = p->ly / p->ny; g.dz = p->lz / p->nz;
     g.dt = p->dt; g.re = p->re; g.beta = p->beta; g.wi = p->wi; g.lp = p->lp;
+    g.sw = p->ibm_weight > 0.0 ? p->ibm_weight : (g.dx * g.dx) * g.dx;
     g.sy = g.walls ? p->ny + 2 : p->ny;
     g.jo = g.walls ? 1 : 0;
     FP_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -589,7 +590,7 @@ int fp_get_state_device(void* vc, double* U, double* V, double* W, double* P, do
 int fp_enable_timing(void* vc, int on) {
   Ctx* c = (Ctx*)vc;
   if (!c) return FP_E_ARG;
-  if (on < 0 || on > 2) return fail(c, FP_E_ARG, "timing level is 0, 1 or 2");
+  if (on < 0 || on > 3) return fail(c, FP_E_ARG, "timing level is 0, 1, 2 or 3");
   c->timing = on;
   c->ktime.clear();
   return FP_OK;

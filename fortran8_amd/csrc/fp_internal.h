@@ -22,6 +22,7 @@ struct Grid {
   long n;           // nx*ny*nz
   double dx, dy, dz;
   double dt, re, beta, wi, lp;
+  double sw;        // IBM spreading and drag weight per marker (dX^3 or ibm_weight)
   long sy;          // spectral rows in y (ny or ny+2)
   int jo;           // spectral row of interior j=0 (0 or 1)
 };
@@ -119,9 +120,13 @@ void adj_free(Ctx& c);
 // an event from the context's pool (created on first use)
 // Phase timing: level 1 records every phase, level 2 only the predictor (the
 // kernel whose roofline the benchmark reports), so that a timed run carries
-// two events per step.
+// two events per step, level 3 the predictor of every 8th step (c.step a
+// multiple of 8): each event idles the stream for a few microseconds.
 inline bool phase_timed(const Ctx& c, const char* name) {
-  return name && (c.timing == 1 || (c.timing == 2 && std::strcmp(name, "predictor") == 0));
+  if (!name) return false;
+  if (c.timing == 1) return true;
+  const bool pred = std::strcmp(name, "predictor") == 0;
+  return pred && (c.timing == 2 || (c.timing == 3 && c.step % 8 == 0));
 }
 // Timing events skip the system-scope fence on record (its cache writeback
 // idles the GPU for about 6 us per event); they are read only after the

@@ -613,7 +613,7 @@ __global__ __launch_bounds__(256) void k_ibm_spread(Grid g, int comp, int rows, 
   for (int e = rowp[r]; e < rowp[r + 1]; e++) {
     long id = ent[e];
     double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
-    acc = acc + (mf[comp * nmk + id] * ibm_weight(g, comp, ii, jj, kk, xc, yc, zc)) * dx3;
+    acc = acc + (mf[comp * nmk + id] * ibm_weight(g, comp, ii, jj, kk, xc, yc, zc)) * g.sw;
   }
   ft[t] = acc;
 }
@@ -1440,9 +1440,8 @@ void f64_diag(Ctx& c, fp_diag* d, double* scratch, const double* mf) {
   d->newton_unconverged = h[5];
   d->nan_flag = h[4] > 0 ? 1 : 0;
   if (mf && c.nmk) {
-    const double dx3 = (g.dx * g.dx) * g.dx;
-    d->drag = -(h[12] * dx3);
-    d->lift = -(h[13] * dx3);
+    d->drag = -(h[12] * g.sw);
+    d->lift = -(h[13] * g.sw);
   }
 }
 

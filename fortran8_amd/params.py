@@ -23,8 +23,9 @@ class FpParams(C.Structure):
         ("sor_omega", C.c_double),
         ("body_force", C.c_double * 3),
         ("n_dc_max", C.c_int32), ("dc_tol", C.c_double),
-        ("helm_sweeps", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("helm_sweeps", C.c_int32), ("reserved0", C.c_int32),
+        ("ibm_weight", C.c_double), ("seed", C.c_uint64),
+        ("reserved", C.c_int32 * 2),
     ]
 
     def copy(self):

@@ -71,7 +71,16 @@ typedef struct fp_params {
   int32_t helm_sweeps;         /* mixed: fixed-point sweeps of the CN Helmholtz
                                 * solve; 0 = as many as bring the error below
                                 * FP32 rounding, -1 = the spectral solve       */
-  int32_t reserved[7];
+  int32_t reserved0;           /* 0                                           */
+  double  ibm_weight;          /* IBM spreading weight per marker and the
+                                * weight of the drag sum: <= 0 = dX^3
+                                * (fenep.f90:2125-2162, the default); > 0 =
+                                * that volume, e.g. smac.f90's dV = pi DC/NL
+                                * dX^2 (smac.f90:1129, 1428)                   */
+  uint64_t seed;               /* seed of the run's synthetic inputs
+                                * (fp_seeded_uniform); carried with the
+                                * parameters, the step itself draws nothing   */
+  int32_t reserved[2];
 } fp_params;
 
 /* Diagnostics (log_progress, fenep.f90:2266-2342, plus drag and divergence). */
@@ -172,7 +181,8 @@ int fp_get_state_device(void* ctx, double* U, double* V, double* W, double* P,
  * timing enabled (milliseconds, summed over steps). names is filled with a
  * comma-separated list; returns the number of entries written to ms.
  * on: 0 off, 1 every phase, 2 only the predictor phase (two events per
- * step, for timing runs). */
+ * step, for timing runs), 3 the predictor phase of the steps whose index
+ * (fp_diag.step before the step) is a multiple of 8. */
 int fp_enable_timing(void* ctx, int on);
 int fp_kernel_times(void* ctx, double* ms, int max_n, char* names,
                     int names_len);

@@ -772,9 +772,17 @@ static inline void ibm_pos(orc_t* o, int c, int i, int j, int k, double* x, doub
   else *z = *z + 0.5 * o->dz;
 }
 
+/* The spreading weight per marker, also the weight of the drag sum:
+ * dX^3 (fenep.f90:2125-2162) unless params.ibm_weight > 0 (smac.f90:1428
+ * spreads with its dV). */
+static double spread_weight(const orc_t* o) {
+  return o->p.ibm_weight > 0.0 ? o->p.ibm_weight : (o->dx * o->dx) * o->dx;
+}
+
 /* ibm_Helmholtz, fenep.f90:2057-2193, with the marker forces kept as
  * computed (fenep_mpi.f90:1845-1853; fenep.f90:2121-2123 overwrites them
- * with 1 for debugging). */
+ * with 1 for debugging).  Interpolation weight dX^3, spreading weight
+ * spread_weight(). */
 static void ibm_helmholtz(orc_t* o) {
   const int nc = o->nc, nl = o->nl, nz = o->nz;
   const double dx3 = (o->dx * o->dx) * o->dx, dt = o->p.dt;
@@ -801,6 +809,7 @@ static void ibm_helmholtz(orc_t* o) {
   for (long id = 0; id < o->nmk; id++)
     for (int c = 0; c < 3; c++) Fm[c][id] = Fm[c][id] + (Cm[c][id] - Bm[c][id]) / dt;
   const int nx = o->nx, ny = o->ny;
+  const double sw = spread_weight(o);
   for (int c = 0; c < 3; c++) {
     real_t* ft = o->ftmp;
     memset((void*)ft, 0, sizeof(real_t) * o->gn);
@@ -815,7 +824,7 @@ static void ibm_helmholtz(orc_t* o) {
               for (int i = i0; i <= i0 + 2; i++) {
                 double x, y, z; ibm_pos(o, c, i, j, k, &x, &y, &z);
                 long g = GI(o, i, j, k);
-                ft[g] = ft[g] + (Fm[c][id] * delta3(o, x, y, z, xc, yc, zc)) * dx3;
+                ft[g] = ft[g] + (Fm[c][id] * delta3(o, x, y, z, xc, yc, zc)) * sw;
               }
         }
     /* ibm_PBM, fenep.f90:2195-2205: fold ghost layers back additively */
@@ -1177,9 +1186,10 @@ int orc_diagnostics(void* vo, fp_diag* d) {
       }
   d->trc_std = sqrt(ss / N);
   d->div_max = dmax;
-  real_t dx3 = (o->dx * o->dx) * o->dx, dr = 0, lf = 0;
+  const double wgt = spread_weight(o);
+  real_t dr = 0, lf = 0;
   for (long id = 0; id < o->nmk; id++) { dr += o->fxc[id]; lf += o->fyc[id]; }
-  d->drag = -(dr * dx3); d->lift = -(lf * dx3);
+  d->drag = rv(-(dr * wgt)); d->lift = rv(-(lf * wgt));
   d->newton_itr_mean = o->newton_itr_total / N;
   d->sor_itr = o->sor_itr;
   d->n_dc = 0;
@@ -1353,14 +1363,14 @@ int orc_cs_seed_force(void* vo, const double* dF, double h) {
   return 0;
 }
 
-/* Drag of the last step, D = -sum_markers Fxc dX^3: out[0] = D, out[1] = its
- * imaginary part. */
+/* Drag of the last step, D = -sum_markers Fxc W (W = spread_weight()):
+ * out[0] = D, out[1] = its imaginary part. */
 int orc_cs_drag(void* vo, double* out2) {
   orc_t* o = (orc_t*)vo;
-  const double dx3 = (o->dx * o->dx) * o->dx;
+  const double sw = spread_weight(o);
   real_t dr = 0.0;
   for (long id = 0; id < o->nmk; id++) dr += o->fxc[id];
-  const real_t d = -(dr * dx3);
+  const real_t d = -(dr * sw);
   out2[0] = d.re;
   out2[1] = d.im;
   return 0;

@@ -55,8 +55,8 @@ def small_case(**kw):
     return case, g, st
 
 
-def check_vjp():
-    case, g, st = small_case()
+def check_vjp(**kw):
+    case, g, st = small_case(**kw)
     x0 = flat_state(st)
     b0 = st.hist[6:9].copy()
     rng = np.random.default_rng(7)
@@ -78,7 +78,7 @@ def check_vjp():
         d = g.adj_record(1, 1, replay=True)
         return float(np.sum(vbar * flat_state(g.get_state()))) + wd * d[0]
 
-    out = {"check": "vjp"}
+    out = {"check": "vjp", "params": {k: float(v) for k, v in kw.items()}}
     for h in (1e-5, 1e-6):
         fd = (J(x0 + h * w) - J(x0 - h * w)) / (2 * h)
         out["fd_h%g" % h] = fd
@@ -93,6 +93,15 @@ def check_vjp():
         fd = (J(x0 + h * wm) - J(x0 - h * wm)) / (2 * h)
         out["rel_" + name] = abs(fd - a) / max(abs(a), 1e-300)
     print(json.dumps(out), flush=True)
+
+
+def check_vjp_smac_weight():
+    """The one-step dot-product test with smac.f90's spreading weight
+    dV = pi DC/NL dX^2 in place of dX^3 (smac.f90:1129, 1428)."""
+    from fortran8_amd.fields import cylinder_markers
+    d = 2 * math.pi / 32
+    nl = cylinder_markers([(math.pi, math.pi / 2)], 0.5, d, d, 16).nl
+    check_vjp(ibm_weight=math.pi * 0.5 / nl * d * d)
 
 
 def central(J, h):
@@ -248,4 +257,5 @@ def check_cfg5():
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "vjp"
     print("adjoint check:", which, flush=True)
-    {"vjp": check_vjp, "params": check_params, "cfg5": check_cfg5}[which]()
+    {"vjp": check_vjp, "vjp_smac_weight": check_vjp_smac_weight, "params": check_params,
+     "cfg5": check_cfg5}[which]()

@@ -74,3 +74,26 @@ def test_library_matches_the_sources_of_this_tree():
     from fortran8_amd import build
     assert api.build_id() == build.source_hash()
     assert len(api.build_id()) == 16
+
+
+def test_params_and_diag_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of fp_params and fp_diag have the header's field
+    offsets and sizes (gcc on the plain-C header)."""
+    from fortran8_amd.params import FpDiag
+    header = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fenep_gpu.h"', "int main(void) {"]
+    for st, cls in (("fp_params", FpParams), ("fp_diag", FpDiag)):
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (st, st))
+        for name, _ in cls._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (st, name, st, name))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", header, str(src), "-o", str(exe)])
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for st, cls in (("fp_params", FpParams), ("fp_diag", FpDiag)):
+        assert got[(st, "size")] == C.sizeof(cls), st
+        for name, _ in cls._fields_:
+            assert got[(st, name)] == getattr(cls, name).offset, (st, name)

@@ -28,6 +28,13 @@ def test_adjoint_one_step_vjp_matches_finite_differences():
     assert out["rel_uvwp"] < 1e-6 and out["rel_c"] < 1e-6 and out["rel_hist"] < 1e-6
 
 
+def test_adjoint_one_step_vjp_with_smac_spreading_weight():
+    out = run_check("vjp_smac_weight")
+    assert out["params"]["ibm_weight"] > 0
+    assert out["rel_h1e-05"] < 1e-8
+    assert out["rel_uvwp"] < 1e-6 and out["rel_c"] < 1e-6 and out["rel_hist"] < 1e-6
+
+
 def test_adjoint_parameter_and_force_gradients_multi_segment():
     # stretched conformation (tr C ~ 30 of L^2 = 100), Newton converged to
     # round-off; reference derivative: Richardson-extrapolated central

@@ -4,6 +4,8 @@ The plain FP64 path evaluates every expression in the oracle's order without
 fused multiply-add, so the bar is bit-exact equality of U, V, W, P and C
 after several steps; the SOR variant (red-black on the GPU, lexicographic in
 the reference) is compared at the solver tolerance."""
+import math
+
 import numpy as np
 import pytest
 
@@ -78,6 +80,35 @@ def test_channel_cylinder_fft_line_solve_bitwise():
     fo, fg = o.marker_forces(), g.marker_forces()
     for a, b in zip(fo, fg):
         assert np.array_equal(a, b)
+
+
+def test_channel_cylinder_smac_spreading_weight_bitwise():
+    """ibm_weight = smac.f90's dV = pi DC/NL dX^2 (smac.f90:1129, 1428) as
+    the spreading and drag weight: FP64 bit-identical to the oracle, marker
+    forces and drag included; the mixed step within 1e-5 of it."""
+    from fortran8_amd.fields import cylinder_markers
+    base = configs.channel_cylinder(nx=64, ny=32, nz=16)
+    d = base.params.lx / base.params.nx
+    nl = cylinder_markers([(1.0, 1.0)], 0.5, d, d, 16).nl
+    dv = math.pi * 0.5 / nl * d * d
+    case = configs.channel_cylinder(nx=64, ny=32, nz=16, ibm_weight=dv)
+    o, g = run_both(case, 5)
+    assert_bitwise(o, g)
+    for a, b in zip(o.marker_forces(), g.marker_forces()):
+        assert np.array_equal(a, b)
+    do, dg = o.diagnostics(), g.diagnostics()
+    # the same marker forces, summed in another order
+    assert abs(dg["drag"] - do["drag"]) <= 1e-12 * abs(do["drag"]) and dg["drag"] != 0.0
+    h = api.load_case(base)
+    h.step(5)
+    assert dg["drag"] != h.diagnostics()["drag"]   # the weight is applied
+    mc = configs.channel_cylinder(nx=64, ny=32, nz=16, ibm_weight=dv, precision=1)
+    m = api.load_case(mc)
+    m.step(5)
+    sm, sg = m.get_state(), g.get_state()
+    for name in ("U", "V", "W"):
+        assert rel_l2(interior(getattr(sm, name)), interior(getattr(sg, name))) < 1e-5, name
+    assert abs(m.diagnostics()["drag"] - dg["drag"]) < 1e-5 * abs(dg["drag"])
 
 
 def test_channel_cylinder_sor_variant_at_solver_tolerance():

@@ -196,10 +196,11 @@ def test_wall_line_solve_residuals(kind):
 
 
 # ------------------------------------------------------------- IBM (A17)
-def ibm_oracle(n=32, walls=0):
+def ibm_oracle(n=32, walls=0, ibm_weight=0.0):
     L = 2.0
     d = L / n
-    p = default_params(nx=n, ny=n, nz=8, lx=L, ly=L, lz=8 * d, dt=0.01, method=METHOD_IBM, walls_y=walls)
+    p = default_params(nx=n, ny=n, nz=8, lx=L, ly=L, lz=8 * d, dt=0.01, method=METHOD_IBM, walls_y=walls,
+                       ibm_weight=ibm_weight)
     o = po.Oracle(p)
     z = np.zeros((8, n, n))
     o.set_state(state_from(z, z, z))
@@ -223,10 +224,20 @@ def test_ibm_interp_partition_of_unity_and_linear_exactness():
     assert np.allclose(fm[0], (0.0 - 1.0) / p.dt)
 
 
-def test_ibm_spread_is_transpose_of_interp_and_conserves_momentum():
-    o, mk, d, p = ibm_oracle()
-    rng = np.random.default_rng(3)
+@pytest.mark.parametrize("smac_weight", [False, True])
+def test_ibm_spread_is_transpose_of_interp_and_conserves_momentum(smac_weight):
+    """With the spreading weight W per marker (dX^3, fenep.f90:2125-2162, or
+    smac.f90's dV = pi DC/NL dX^2, smac.f90:1129, 1428) and the interpolation
+    weight dX^3: spread = (W / dX^3) interp^T, the spread force sums to
+    (W / dX^3) sum F_c (sum_cells delta dX^3 = 1), and the drag is
+    -sum F_c,x W."""
     n, nz = 32, 8
+    d = 2.0 / n
+    nl = cylinder_markers([(0.83, 1.07)], 0.5, d, d, 8).nl
+    W = math.pi * 0.5 / nl * d * d if smac_weight else 0.0
+    o, mk, d, p = ibm_oracle(ibm_weight=W)
+    ratio = W / d ** 3 if smac_weight else 1.0
+    rng = np.random.default_rng(3)
     up = np.zeros((3, nz + 2, n + 2, n + 2))
     up[:, 1:-1, 1:-1, 1:-1] = rng.standard_normal((3, nz, n, n))
     for c in range(3):
@@ -235,8 +246,9 @@ def test_ibm_spread_is_transpose_of_interp_and_conserves_momentum():
     for c in range(3):
         lhs = np.sum(up[c, 1:-1, 1:-1, 1:-1] * f[c])
         rhs = np.sum(fm[c] * ub[c])
-        assert lhs == pytest.approx(rhs, rel=1e-12)
-        assert np.sum(f[c]) == pytest.approx(np.sum(fm[c]), rel=1e-12)
+        assert lhs == pytest.approx(ratio * rhs, rel=1e-12)
+        assert np.sum(f[c]) == pytest.approx(ratio * np.sum(fm[c]), rel=1e-12)
+    assert o.diagnostics()["drag"] == pytest.approx(-np.sum(fm[0]) * (W if smac_weight else d ** 3), rel=1e-12)
 
 
 # ---------------------------------------------------------- whole steps
