@@ -2581,34 +2581,32 @@ __global__ __launch_bounds__(256) void m_ibm_spread_folded(int rows, const int* 
 
 // The same two passes for markers repeated along z.  Interpolation: the
 // weights and the in-plane origin of the layer-0 marker, the layer from the
-// marker index (markers are numbered layer by layer, zP to a layer).  Four
-// lanes per (component, marker): lanes 0-2 take the stencil's planes k-1, k,
-// k+1 (18 loads each, u^n and the increment), lane 3 idles, and two lane
-// exchanges sum the planes, so that four times as many waves keep the
-// gathers in flight.  Groups of four lanes are whole (4 divides the grid's
-// thread count), and every lane takes part in the exchanges.
+// marker index (markers are numbered layer by layer, zP to a layer).  One
+// lane per (component, marker), consecutive lanes on consecutive markers:
+// spreading the 27 points over three lanes or three waves per marker measured
+// 14 us slower per step (more cache lines per load instruction).
 template <bool WL>
 __global__ __launch_bounds__(256) void m_ibm_interp_z(Grid g, long nmk, long zP, const float* __restrict__ wz,
                                                       const int2* __restrict__ orgz, const float* __restrict__ uc,
                                                       const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, const float* __restrict__ dq,
                                                       float* __restrict__ mf) {
-  const long t4 = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long t = t4 >> 2;
-  const int kk = (int)(t4 & 3);
-  const bool valid = t < 3 * nmk;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 3 * nmk) return;
+  const int c = (int)(t / nmk);
+  const long id = t - c * nmk;
+  const int L = (int)(id / zP);
+  const long p = id - (long)L * zP;
+  const float idt = (float)(1.0 / g.dt);
+  const int2 o = orgz[c * zP + p];
+  const float* X = c == 0 ? U : (c == 1 ? V : W);   // three separate planes of the state
+  const float* D = dq + c * g.n;
+  const int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
+  const float* w = wz + (long)c * 27 * zP + p;
   float acc = 0.0f, accd = 0.0f;
-  if (valid && kk < 3) {
-    const int c = (int)(t / nmk);
-    const long id = t - c * nmk;
-    const int L = (int)(id / zP);
-    const long p = id - (long)L * zP;
-    const int2 o = orgz[c * zP + p];
-    const float* X = c == 0 ? U : (c == 1 ? V : W);   // three separate planes of the state
-    const float* D = dq + c * g.n;
-    const int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
-    const float* w = wz + (long)c * 27 * zP + p + (long)(9 * kk) * zP;
-    int s = 0;
+  int s = 0;
+#pragma unroll
+  for (int kk = 0; kk < 3; kk++)
 #pragma unroll
     for (int jj = 0; jj < 3; jj++)
 #pragma unroll
@@ -2617,12 +2615,7 @@ __global__ __launch_bounds__(256) void m_ibm_interp_z(Grid g, long nmk, long zP,
         acc += wv * ldbw<WL>(X, g, o.x + ii - 1, o.y + jj - 1, L + kk - 1, kind);
         accd += wv * ldbw<WL>(D, g, o.x + ii - 1, o.y + jj - 1, L + kk - 1, kind);
       }
-  }
-  acc += __shfl_xor(acc, 1);
-  accd += __shfl_xor(accd, 1);
-  acc += __shfl_xor(acc, 2);
-  accd += __shfl_xor(accd, 2);
-  if (valid && kk == 0) mf[t] = ((uc[t] - acc) - accd) * (float)(1.0 / g.dt);
+  mf[t] = ((uc[t] - acc) - accd) * idt;
 }
 
 // Spreading: thread (plane k, row r of plane 0); an entry names the layer-0
@@ -3288,7 +3281,7 @@ void mixed_step(Ctx& c) {
     {
       MTimer t(c, "ibm");
       if (X.zinv) {
-        hipLaunchKernelGGL(g.walls ? m_ibm_interp_z<true> : m_ibm_interp_z<false>, dim3(nblk(12 * c.nmk, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(g.walls ? m_ibm_interp_z<true> : m_ibm_interp_z<false>, dim3(nblk(3 * c.nmk, 256)), dim3(256), 0,
                            s, g, c.nmk, X.zP, X.zi_w, X.zi_org, X.uc, X.U, X.V, X.W, X.Q, X.mf);
         if (X.zs_rows)
           hipLaunchKernelGGL(m_ibm_spread_z, dim3(nblk((long)g.nz * X.zs_rows64, 256)), dim3(256), 0, s, X.zs_rows,
