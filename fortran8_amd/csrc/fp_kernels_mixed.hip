@@ -1635,57 +1635,82 @@ __global__ __launch_bounds__(256) void m_fft_yz(Grid g, float2* __restrict__ spe
   }
 }
 
-// In-place radix-2 transforms (decimation in time: bit-reversed input,
-// natural-order output) of R rows of n points with row pitch P, all rows in
-// one pass per stage; tws[t] = exp(-2 pi i t / n), t < n/2; inv: conjugate
-// twiddles (unnormalised).  Called by every thread of the block.
-__device__ void fft_rows_inplace(float2* S, int R, int P, int n, int logn, const float2* tws, bool inv) {
-  for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
-    const int row = e >> logn, i = e & (n - 1);
-    const int j = (int)(__brev((unsigned)i) >> (32 - logn));
-    if (i < j) {
-      float2* x = S + (size_t)row * P;
-      const float2 t = x[i];
-      x[i] = x[j];
-      x[j] = t;
-    }
+// In-place radix-2 transforms along the slow index of an LDS array held
+// [point][row] (rows contiguous, point pitch P), for nr rows (a power of two,
+// lnr = log2 nr) of n points.  Consecutive threads take consecutive rows of
+// one butterfly: each LDS access of a wavefront is one contiguous run and its
+// twiddles are broadcasts.  Two stages per pass: the four elements of two
+// consecutive stages stay in registers.  dif = false: decimation in time
+// (bit-reversed input, natural-order output); dif = true: decimation in
+// frequency (natural input, bit-reversed output).  tws[t] = exp(-2 pi i t /
+// n), t < n/2; inv: conjugate twiddles (unnormalised).  Called by every
+// thread of the block; ends with a barrier.
+__device__ __forceinline__ void bfly2(float2& a, float2& b, float2 w, bool dif) {
+  if (dif) {
+    const float2 d = make_float2(a.x - b.x, a.y - b.y);
+    a = make_float2(a.x + b.x, a.y + b.y);
+    b = cmul_f(d, w);
+  } else {
+    const float2 t = cmul_f(b, w);
+    b = make_float2(a.x - t.x, a.y - t.y);
+    a = make_float2(a.x + t.x, a.y + t.y);
   }
-  __syncthreads();
-  // two stages a pass: the four elements i, i+h, i+2h, i+3h of stages st and
-  // st+1 (h = 2^(st-1)) go through both butterflies in registers
-  int st = 1;
-  for (; st + 1 <= logn; st += 2) {
-    const int h = 1 << (st - 1), str = n >> st, str2 = n >> (st + 1);
-    for (int b = threadIdx.x; b < R * (n >> 2); b += blockDim.x) {
-      const int row = b >> (logn - 2), t = b & ((n >> 2) - 1);
+}
+__device__ void fft_cols_inplace(float2* S, int P, int nr, int lnr, int n, int logn, const float2* tws, bool inv,
+                                 bool dif) {
+  const int half = n >> 1;
+  // stage spans in order: DIT 1, 2, ..., n/2; DIF n/2, ..., 2, 1.  A lone
+  // radix-2 stage (log2 n odd) is the last DIT stage or the first DIF stage.
+  int done = 0;
+  if (dif && (logn & 1)) {
+    for (int b = threadIdx.x; b < nr * half; b += blockDim.x) {
+      const int r = b & (nr - 1), i = b >> lnr;
+      float2 x0 = S[(size_t)i * P + r], x1 = S[(size_t)(i + half) * P + r];
+      bfly2(x0, x1, tw_at(tws, half, i, inv), true);
+      S[(size_t)i * P + r] = x0;
+      S[(size_t)(i + half) * P + r] = x1;
+    }
+    __syncthreads();
+    done = 1;
+  }
+  for (; done + 2 <= logn; done += 2) {
+    // spans h (first) and 2h (DIT) or 2h (first) and h (DIF)
+    const int lh = dif ? logn - done - 2 : done, h = 1 << lh;
+    for (int b = threadIdx.x; b < nr * (n >> 2); b += blockDim.x) {
+      const int r = b & (nr - 1), t = b >> lnr;
       const int k = t & (h - 1);
-      const int i0 = ((t >> (st - 1)) << (st + 1)) + k;
-      float2* x = S + (size_t)row * P;
-      const float2 w = tw_at(tws, n >> 1, k * str, inv), wa = tw_at(tws, n >> 1, k * str2, inv),
-                   wb = tw_at(tws, n >> 1, (k + h) * str2, inv);
-      const float2 a0 = x[i0], a1 = x[i0 + h], a2 = x[i0 + 2 * h], a3 = x[i0 + 3 * h];
-      const float2 t1 = cmul_f(a1, w), t3 = cmul_f(a3, w);
-      const float2 c0 = make_float2(a0.x + t1.x, a0.y + t1.y), c1 = make_float2(a0.x - t1.x, a0.y - t1.y);
-      const float2 c2 = make_float2(a2.x + t3.x, a2.y + t3.y), c3 = make_float2(a2.x - t3.x, a2.y - t3.y);
-      const float2 u = cmul_f(c2, wa), v = cmul_f(c3, wb);
-      x[i0] = make_float2(c0.x + u.x, c0.y + u.y);
-      x[i0 + 2 * h] = make_float2(c0.x - u.x, c0.y - u.y);
-      x[i0 + h] = make_float2(c1.x + v.x, c1.y + v.y);
-      x[i0 + 3 * h] = make_float2(c1.x - v.x, c1.y - v.y);
+      const int i0 = ((t >> lh) << (lh + 2)) + k;
+      float2* x = S + (size_t)i0 * P + r;
+      float2 a0 = x[0], a1 = x[(size_t)h * P], a2 = x[(size_t)2 * h * P], a3 = x[(size_t)3 * h * P];
+      // twiddles of span h (index k n/(2h)) and span 2h (k n/(4h), (k+h) n/(4h))
+      const float2 w1 = tw_at(tws, half, k << (logn - lh - 1), inv);
+      const float2 wa = tw_at(tws, half, k << (logn - lh - 2), inv), wb = tw_at(tws, half, (k + h) << (logn - lh - 2), inv);
+      if (dif) {
+        bfly2(a0, a2, wa, true);
+        bfly2(a1, a3, wb, true);
+        bfly2(a0, a1, w1, true);
+        bfly2(a2, a3, w1, true);
+      } else {
+        bfly2(a0, a1, w1, false);
+        bfly2(a2, a3, w1, false);
+        bfly2(a0, a2, wa, false);
+        bfly2(a1, a3, wb, false);
+      }
+      x[0] = a0;
+      x[(size_t)h * P] = a1;
+      x[(size_t)2 * h * P] = a2;
+      x[(size_t)3 * h * P] = a3;
     }
     __syncthreads();
   }
-  if (st == logn) {
-    const int half = 1 << (st - 1), str = n >> st;
-    for (int b = threadIdx.x; b < R * (n >> 1); b += blockDim.x) {
-      const int row = b >> (logn - 1), t = b & ((n >> 1) - 1);
-      const int k = t & (half - 1);
-      const int i = ((t >> (st - 1)) << st) + k, j = i + half;
-      float2* x = S + (size_t)row * P;
-      const float2 tv = cmul_f(x[j], tw_at(tws, n >> 1, k * str, inv));
-      const float2 a = x[i];
-      x[i] = make_float2(a.x + tv.x, a.y + tv.y);
-      x[j] = make_float2(a.x - tv.x, a.y - tv.y);
+  if (done < logn) {
+    // the last DIT stage, span n/2
+    for (int b = threadIdx.x; b < nr * half; b += blockDim.x) {
+      const int r = b & (nr - 1), i = b >> lnr;
+      float2 x0 = S[(size_t)i * P + r], x1 = S[(size_t)(i + half) * P + r];
+      bfly2(x0, x1, tw_at(tws, half, i, inv), false);
+      S[(size_t)i * P + r] = x0;
+      S[(size_t)(i + half) * P + r] = x1;
     }
     __syncthreads();
   }
@@ -1694,20 +1719,26 @@ __device__ void fft_rows_inplace(float2* S, int R, int P, int n, int logn, const
 // Pressure Poisson solve of one kx slab with y-walls (fft_poisson_kabe and
 // tdma, fenep_2d.f90:1145-1177, 1179-1342): forward z transforms of the
 // interior rows, the line solve along y of every kz (boundary rows x_0 = x_1,
-// x_{NY+1} = x_NY), inverse z transforms.  The slab, held [row][kz] with pitch
-// nz + 1 in LDS, is read and written once from the transposed half spectrum
-// (row fastest) and transformed in place; the line solve keeps its P
-// coefficients in pbuf (L2).
-static size_t lds_slab(const Grid& g) { return sizeof(float2) * ((size_t)g.sy * (g.nz + 1) + g.nz / 2); }
+// x_{NY+1} = x_NY), inverse z transforms.  The slab is held in LDS as it lies
+// in the transposed half spectrum, [kz][row] (rows contiguous, kz pitch the
+// odd number slab_pitch >= ny + 2), so its loads and stores are straight
+// copies.  The z transforms run across rows (fft_cols_inplace); their bit
+// reversals are folded into the copies: the load puts kz at its bit-reversed
+// place for the decimation-in-time forward transform, and the store reads
+// the decimation-in-frequency inverse's bit-reversed output.  The line solve
+// keeps its P coefficients in pbuf (L2).
+static int slab_pitch(const Grid& g) { return g.sy | 1; }
+static size_t lds_slab(const Grid& g) { return sizeof(float2) * ((size_t)g.nz * slab_pitch(g) + g.nz / 2); }
 __global__ __launch_bounds__(1024) void m_poisson_slab(Grid g, float2* __restrict__ spec, const float2* __restrict__ tw,
                                                       const float* __restrict__ kx2, const float* __restrict__ kz2,
                                                       float* __restrict__ pbuf, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ float2 lds[];
-  const int nz = g.nz, sy = g.sy, ny = g.ny, pitch = nz + 1;
+  const int nz = g.nz, sy = g.sy, ny = g.ny, pitch = g.sy | 1;
   const long kx = blockIdx.x;
-  float2* S = lds;                              // [sy][pitch]
-  float2* tws = lds + (size_t)sy * pitch;
+  const int logn = ilog2f(nz);
+  float2* S = lds;                              // [kz][pitch]
+  float2* tws = lds + (size_t)nz * pitch;
   load_twiddles(tws, tw, nz);
   float2* slab = spec + (size_t)sy * nz * kx;   // [kz][row]
   // one workgroup per CU: issue a thread's loads together, then store them
@@ -1727,95 +1758,92 @@ __global__ __launch_bounds__(1024) void m_poisson_slab(Grid g, float2* __restric
       const int e = e0 + 1024 * u + (int)threadIdx.x;
       if (e < ny * nz) {
         const int kk = e >> g.lny, j = e & (ny - 1);
-        S[(j + 1) * pitch + kk] = v[u];
+        const int rk = (int)(__brev((unsigned)kk) >> (32 - logn));
+        S[(size_t)rk * pitch + j + 1] = v[u];
       }
     }
   }
   __syncthreads();
-  const int logn = ilog2f(nz);
-  for (int pass = 0; pass < 2; pass++) {
-    if (pass == 1) {
-      // line solve along y, one thread per kz; rows go in batches of TB with
-      // the next batch's operands loaded while the current one is computed
-      // (the back substitution reads P from L2)
-      if ((int)threadIdx.x < nz) {
-        constexpr int TB = 8;
-        const int kk = threadIdx.x;
-        float* P = pbuf + (size_t)sy * nz * kx;   // [row][kz]
-        const float inxz = (float)(1.0 / ((double)g.nx * g.nz));
-        const float idy2 = (float)(1.0 / (g.dy * g.dy));
-        const float kq = kx2[kx] + kz2[kk];
-        const float ai = -(kq + 2.0f * idy2);
-        float Pp = 1.0f, Qr = 0.0f, Qi = 0.0f;   // row 0: a = 1, b = -1
-        P[kk] = Pp;
-        S[kk] = make_float2(0.0f, 0.0f);
-        float2 cur[TB];
+  fft_cols_inplace(S + 1, pitch, ny, g.lny, nz, logn, tws, false, false);
+  // line solve along y, one thread per kz; rows go in batches of TB with the
+  // next batch's operands loaded while the current one is computed (the back
+  // substitution reads P from L2)
+  if ((int)threadIdx.x < nz) {
+    constexpr int TB = 8;
+    const int kk = threadIdx.x;
+    float2* L = S + (size_t)kk * pitch;
+    float* P = pbuf + (size_t)sy * nz * kx;   // [row][kz]
+    const float inxz = (float)(1.0 / ((double)g.nx * g.nz));
+    const float idy2 = (float)(1.0 / (g.dy * g.dy));
+    const float kq = kx2[kx] + kz2[kk];
+    const float ai = -(kq + 2.0f * idy2);
+    float Pp = 1.0f, Qr = 0.0f, Qi = 0.0f;   // row 0: a = 1, b = -1
+    P[kk] = Pp;
+    L[0] = make_float2(0.0f, 0.0f);
+    float2 cur[TB];
 #pragma unroll
-        for (int t = 0; t < TB; t++) cur[t] = 1 + t < sy ? S[(1 + t) * pitch + kk] : make_float2(0.0f, 0.0f);
-        for (int r0 = 1; r0 < sy; r0 += TB) {
-          float2 nxt[TB];
+    for (int t = 0; t < TB; t++) cur[t] = 1 + t < sy ? L[1 + t] : make_float2(0.0f, 0.0f);
+    for (int r0 = 1; r0 < sy; r0 += TB) {
+      float2 nxt[TB];
 #pragma unroll
-          for (int t = 0; t < TB; t++) nxt[t] = r0 + TB + t < sy ? S[(r0 + TB + t) * pitch + kk] : make_float2(0.0f, 0.0f);
+      for (int t = 0; t < TB; t++) nxt[t] = r0 + TB + t < sy ? L[r0 + TB + t] : make_float2(0.0f, 0.0f);
 #pragma unroll
-          for (int t = 0; t < TB; t++) {
-            const int r = r0 + t;
-            if (r < sy) {
-              const bool last = r == sy - 1;
-              const float a = last ? 1.0f : ai;
-              const float bb = last ? 0.0f : idy2;
-              const float cc = last ? -1.0f : idy2;
-              const float dr = last ? 0.0f : cur[t].x * inxz, di = last ? 0.0f : cur[t].y * inxz;
-              const float den = a + cc * Pp;
-              if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
-              else { const float id = __builtin_amdgcn_rcpf(den); Pp = -bb * id; Qr = (dr - cc * Qr) * id; Qi = (di - cc * Qi) * id; }
-              P[r * nz + kk] = Pp;
-              S[r * pitch + kk] = make_float2(Qr, Qi);
-            }
-          }
-#pragma unroll
-          for (int t = 0; t < TB; t++) cur[t] = nxt[t];
-        }
-        // back substitution x_r = P_r x_{r+1} + Q_r, r = sy-2 .. 0
-        float xr = Qr, xi = Qi;
-        float pc[TB];
-        float2 qc[TB];
-#pragma unroll
-        for (int t = 0; t < TB; t++) {
-          const int r = sy - 2 - t;
-          pc[t] = r >= 0 ? P[r * nz + kk] : 0.0f;
-          qc[t] = r >= 0 ? S[r * pitch + kk] : make_float2(0.0f, 0.0f);
-        }
-        for (int r0 = sy - 2; r0 >= 0; r0 -= TB) {
-          float pn[TB];
-          float2 qn[TB];
-#pragma unroll
-          for (int t = 0; t < TB; t++) {
-            const int r = r0 - TB - t;
-            pn[t] = r >= 0 ? P[r * nz + kk] : 0.0f;
-            qn[t] = r >= 0 ? S[r * pitch + kk] : make_float2(0.0f, 0.0f);
-          }
-#pragma unroll
-          for (int t = 0; t < TB; t++) {
-            const int r = r0 - t;
-            if (r >= 0) {
-              xr = pc[t] * xr + qc[t].x;
-              xi = pc[t] * xi + qc[t].y;
-              S[r * pitch + kk] = make_float2(xr, xi);
-            }
-          }
-#pragma unroll
-          for (int t = 0; t < TB; t++) { pc[t] = pn[t]; qc[t] = qn[t]; }
+      for (int t = 0; t < TB; t++) {
+        const int r = r0 + t;
+        if (r < sy) {
+          const bool last = r == sy - 1;
+          const float a = last ? 1.0f : ai;
+          const float bb = last ? 0.0f : idy2;
+          const float cc = last ? -1.0f : idy2;
+          const float dr = last ? 0.0f : cur[t].x * inxz, di = last ? 0.0f : cur[t].y * inxz;
+          const float den = a + cc * Pp;
+          if (fabsf(den) < 1.0e-16f * fabsf(a) + 1.0e-30f) { Pp = 0.0f; Qr = 0.0f; Qi = 0.0f; }
+          else { const float id = __builtin_amdgcn_rcpf(den); Pp = -bb * id; Qr = (dr - cc * Qr) * id; Qi = (di - cc * Qi) * id; }
+          P[r * nz + kk] = Pp;
+          L[r] = make_float2(Qr, Qi);
         }
       }
-      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < TB; t++) cur[t] = nxt[t];
     }
-    // z transforms of the interior rows
-    fft_rows_inplace(S + pitch, ny, pitch, nz, logn, tws, pass == 1);
-    if (pass == 1) break;
+    // back substitution x_r = P_r x_{r+1} + Q_r, r = sy-2 .. 0
+    float xr = Qr, xi = Qi;
+    float pc[TB];
+    float2 qc[TB];
+#pragma unroll
+    for (int t = 0; t < TB; t++) {
+      const int r = sy - 2 - t;
+      pc[t] = r >= 0 ? P[r * nz + kk] : 0.0f;
+      qc[t] = r >= 0 ? L[r] : make_float2(0.0f, 0.0f);
+    }
+    for (int r0 = sy - 2; r0 >= 0; r0 -= TB) {
+      float pn[TB];
+      float2 qn[TB];
+#pragma unroll
+      for (int t = 0; t < TB; t++) {
+        const int r = r0 - TB - t;
+        pn[t] = r >= 0 ? P[r * nz + kk] : 0.0f;
+        qn[t] = r >= 0 ? L[r] : make_float2(0.0f, 0.0f);
+      }
+#pragma unroll
+      for (int t = 0; t < TB; t++) {
+        const int r = r0 - t;
+        if (r >= 0) {
+          xr = pc[t] * xr + qc[t].x;
+          xi = pc[t] * xi + qc[t].y;
+          L[r] = make_float2(xr, xi);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TB; t++) { pc[t] = pn[t]; qc[t] = qn[t]; }
+    }
   }
+  __syncthreads();
+  fft_cols_inplace(S + 1, pitch, ny, g.lny, nz, logn, tws, true, true);
   for (int e = threadIdx.x; e < ny * nz; e += blockDim.x) {
     const int kk = e >> g.lny, j = e & (ny - 1);
-    slab[(size_t)sy * kk + j + 1] = S[(j + 1) * pitch + kk];
+    const int rk = (int)(__brev((unsigned)kk) >> (32 - logn));
+    slab[(size_t)sy * kk + j + 1] = S[(size_t)rk * pitch + j + 1];
   }
 }
 
