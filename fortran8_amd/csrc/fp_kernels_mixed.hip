@@ -207,18 +207,22 @@ __device__ __forceinline__ int woff32(int a, int d, int n, int s) {
 #define SEL_KME 4.76837158e-07f    /* 8 eps32 */
 #define SEL_KCE 4.76837158e-07f    /* 8 eps32 */
 
+// lambda_min of one candidate and the half-width of its interval without
+// the term of the cell's entries (ein, added by the caller): neighbouring
+// cells that share a candidate share its evaluation.
 struct Lmin {
-  float lam, err;
+  float lam, e0;
   bool amb;   // near the sentinel boundary (or degenerate): decide in FP64
 };
 
-__device__ __forceinline__ Lmin lmin32(const float* a, float ein) {
+__device__ __forceinline__ Lmin lmin32(const float* a) {
   const float third = 1.0f / 3.0f;
   const float m = ((a[0] + a[3]) + a[5]) * third;
   const float b11 = a[0] - m, b22 = a[3] - m, b33 = a[5] - m;
   const float o2 = fmaf(a[1], a[1], fmaf(a[2], a[2], a[4] * a[4]));
   const float hp = fmaf(b11, b11, fmaf(b22, b22, b33 * b33));
-  const float r2 = fmaxf(fmaf(hp, 1.0f / 6.0f, o2 * third), 1.0e-30f);   // -p/3
+  // -p/3 >= 0; 0 for an isotropic A, whose arg is then NaN (ambiguous)
+  const float r2 = fmaf(hp, 1.0f / 6.0f, o2 * third);
   const float t1 = fmaf(b22, b33, -(a[4] * a[4]));
   const float t2 = fmaf(a[1], b33, -(a[4] * a[2]));
   const float t3 = fmaf(a[1], a[4], -(b22 * a[2]));
@@ -226,31 +230,32 @@ __device__ __forceinline__ Lmin lmin32(const float* a, float ein) {
   // two reciprocal square roots (quarter-rate) per eigenvalue
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * ir;
-  const float ir3 = (ir * ir) * ir;     // inf for an isotropic A: arg NaN
+  const float ir3 = (ir * ir) * ir;
   const float arg = (0.5f * det) * ir3;
   const float s2 = 1.0f - arg;
   const float S = fmaf(2.0f, r, fabsf(m));
   const float rho = S * ir;
-  const float rho3 = (rho * rho) * rho;
-  const float d = fminf(fmaf(SEL_K3E, rho3, fmaf(SEL_K2E, rho, SEL_K1E)), 1.0f);
+  const float d = fminf(fmaf(rho, fmaf(SEL_K3E, rho * rho, SEL_K2E), SEL_K1E), 1.0f);
   // 1/sqrt(max(1 - arg, delta)); where the decision is certified, 1 - arg >=
-  // 2 delta and s = (1 - arg) v is sqrt(1 - arg)
+  // 2 delta and s = (1 - arg) v is sqrt(1 - arg) (s beyond sqrt 2, arg < -1,
+  // only where the candidate is ambiguous)
   const float v = __builtin_amdgcn_rsqf(fmaxf(s2, d));
-  const float s = fminf(s2, 2.0f) * v;
-  float ps = PSI8;
-  ps = fmaf(ps, s, PSI7);
-  ps = fmaf(ps, s, PSI6);
-  ps = fmaf(ps, s, PSI5);
-  ps = fmaf(ps, s, PSI4);
-  ps = fmaf(ps, s, PSI3);
-  ps = fmaf(ps, s, PSI2);
-  ps = fmaf(ps, s, PSI1);
-  ps = fmaf(ps, s, PSI0);
+  const float s = s2 * v;
+  // 2 psi(s): the coefficients doubled (exactly), lambda = m + r (2 psi)
+  float ps = 2.0f * PSI8;
+  ps = fmaf(ps, s, 2.0f * PSI7);
+  ps = fmaf(ps, s, 2.0f * PSI6);
+  ps = fmaf(ps, s, 2.0f * PSI5);
+  ps = fmaf(ps, s, 2.0f * PSI4);
+  ps = fmaf(ps, s, 2.0f * PSI3);
+  ps = fmaf(ps, s, 2.0f * PSI2);
+  ps = fmaf(ps, s, 2.0f * PSI1);
+  ps = fmaf(ps, s, 2.0f * PSI0);
   Lmin o;
-  o.lam = fmaf(2.0f * r, ps, m);
+  o.lam = fmaf(r, ps, m);
   // written so that a NaN arg counts as ambiguous
   o.amb = !(fabsf(arg) <= fmaf(-2.0f, d, 1.0f));
-  o.err = fmaf(SEL_KME, S, ein) + (r * d) * v;
+  o.e0 = fmaf(r * d, v, SEL_KME * S);
   return o;
 }
 
@@ -260,16 +265,50 @@ __device__ __forceinline__ float entry_bound(const float* c) {
 }
 
 // candidate face values (fenep.f90:179-184) in FP32, each with at most two
-// roundings: left face Cp_l and right face Cn_l, l = 0..2
+// roundings: left face Cp_l and right face Cn_l, l = 0..2.  The halves and
+// quarters are exact, so the midpoint c/2 + cp/2 of one cell's right face is
+// bit for bit the midpoint cm/2 + c/2 of the next cell's left face.
 __device__ __forceinline__ float cand_pf(int l, float cm, float c, float cp) {
-  if (l == 0) return fmaf(c, 1.5f, -(cp * 0.5f));
-  if (l == 1) return (cm + c) * 0.5f;
-  return fmaf(cm, 0.25f, c) - cp * 0.25f;
+  if (l == 0) return fmaf(c, 1.5f, -(0.5f * cp));
+  if (l == 1) return fmaf(c, 0.5f, 0.5f * cm);
+  return fmaf(cp, -0.25f, fmaf(cm, 0.25f, c));
 }
 __device__ __forceinline__ float cand_nf(int l, float cm, float c, float cp) {
-  if (l == 0) return (c + cp) * 0.5f;
-  if (l == 1) return fmaf(c, 1.5f, -(cm * 0.5f));
-  return fmaf(cp, 0.25f, c) - cm * 0.25f;
+  if (l == 0) return fmaf(c, 0.5f, 0.5f * cp);
+  if (l == 1) return fmaf(c, 1.5f, -(0.5f * cm));
+  return fmaf(cm, -0.25f, fmaf(cp, 0.25f, c));
+}
+// the six candidates of one axis line (cm, c, cp) evaluated
+__device__ __forceinline__ Lmin cand_lmin(bool right, int l, const float (&v)[3][6]) {
+  float a[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) a[q] = right ? cand_nf(l, v[0][q], v[1][q], v[2][q]) : cand_pf(l, v[0][q], v[1][q], v[2][q]);
+  return lmin32(a);
+}
+
+// maxloc over {v_0, v_1, v_2, 0} of v_l = min(lambda(Cp_l), lambda(Cn_l))
+// (the first maximum); unsure unless the winner's interval clears every
+// other option's and no candidate is ambiguous
+__device__ __forceinline__ int sel_decide(const Lmin (&P)[3], const Lmin (&N)[3], float ein, bool& unsure) {
+  float vals[4], errs[4];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    vals[l] = fminf(P[l].lam, N[l].lam);
+    errs[l] = fmaxf(P[l].e0, N[l].e0) + ein;
+    unsure = unsure || P[l].amb || N[l].amb;
+  }
+  vals[3] = 0.0f;
+  errs[3] = 0.0f;
+  int idx = 0;
+  float best = vals[0], eb = errs[0];
+  if (vals[1] > best) { best = vals[1]; eb = errs[1]; idx = 1; }
+  if (vals[2] > best) { best = vals[2]; eb = errs[2]; idx = 2; }
+  if (0.0f > best) { best = 0.0f; eb = 0.0f; idx = 3; }
+  // certified only if the winner's interval clears every other interval
+#pragma unroll
+  for (int l = 0; l < 4; l++)
+    if (l != idx) unsure = unsure || !(best - vals[l] > eb + errs[l]);
+  return idx;
 }
 
 // C at cells a-r..a+r along axis d (r = 1 or 2) for the 6 components,
@@ -342,17 +381,43 @@ __device__ __forceinline__ bool row_cell(const Grid& g, long& m, int& i, int& j,
   return true;
 }
 
-// Uncertified (cell, axis) pairs go to a list in per-block regions of 3 * 256
-// entries (block b owns entries [768 b, 768 b + cnt[b])): the slots are taken
-// with LDS atomics, so the kernel issues no device-scope atomic.  The first
-// SEL_STAGE entries of a block also keep the 18 values the selection read
-// (the three cells' six components, plane-major in the block's staging area),
-// so that their re-decision reads them contiguously instead of gathering
-// them again; later entries gather.
+// Uncertified (cell, axis) pairs go to a list in per-block regions of 3
+// entries per cell of the block (block b owns entries [region b, region b +
+// cnt[b]))): the slots are taken with LDS atomics, so the kernel issues no
+// device-scope atomic.  The first `cap` entries of a block also keep the 18
+// values the selection read (the three cells' six components, plane-major in
+// the block's staging area of 18 cap values), so that their re-decision
+// reads them contiguously instead of gathering them again; later entries
+// gather.
 #define SEL_BLOCK 256
 #define SEL_REGION (3 * SEL_BLOCK)
 #define SEL_STAGE 64
 
+// one LDS atomic per wavefront: the first unsure lane reserves the slots.
+// Called by every lane of the wavefront.
+__device__ __forceinline__ void sel_record(bool unsure, unsigned entry, const float (&v)[3][6], unsigned* s_cnt,
+                                           unsigned* __restrict__ region, float* __restrict__ stage, unsigned cap) {
+  const unsigned long long ball = __ballot(unsure);
+  if (ball) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)ball) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(s_cnt, (unsigned)__popcll(ball));
+    base = __shfl(base, leader);
+    if (unsure) {
+      const unsigned slot = base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
+      region[slot] = entry;
+      if (slot < cap) {
+#pragma unroll
+        for (int o = 0; o < 3; o++)
+#pragma unroll
+          for (int q = 0; q < 6; q++) stage[(o * 6 + q) * cap + slot] = v[o][q];
+      }
+    }
+  }
+}
+
+// One cell per thread, any grid shape.
 __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                       unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list,
                                                       float* __restrict__ amb_data) {
@@ -373,53 +438,154 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
     if (valid) {
       load_line<1>(C, g, m, i, j, k, d, v);
       const float ein = SEL_KCE * fmaxf(fmaxf(entry_bound(v[0]), entry_bound(v[1])), entry_bound(v[2]));
-      float vals[4], errs[4];
+      Lmin P[3], N[3];
 #pragma unroll
-      for (int l = 0; l < 3; l++) {
-        float ap[6], an[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, v[0][q], v[1][q], v[2][q]); an[q] = cand_nf(l, v[0][q], v[1][q], v[2][q]); }
-        const Lmin P = lmin32(ap, ein), N = lmin32(an, ein);
-        vals[l] = fminf(P.lam, N.lam);
-        errs[l] = fmaxf(P.err, N.err);
-        unsure = unsure || P.amb || N.amb;
-      }
-      vals[3] = 0.0f;
-      errs[3] = 0.0f;
-      // maxloc over {v0, v1, v2, 0}: first maximum
-      float best = vals[0], eb = errs[0];
-      if (vals[1] > best) { best = vals[1]; eb = errs[1]; idx = 1; }
-      if (vals[2] > best) { best = vals[2]; eb = errs[2]; idx = 2; }
-      if (0.0f > best) { best = 0.0f; eb = 0.0f; idx = 3; }
-      // certified only if the winner's interval clears every other interval
-#pragma unroll
-      for (int l = 0; l < 4; l++)
-        if (l != idx) unsure = unsure || !(best - vals[l] > eb + errs[l]);
+      for (int l = 0; l < 3; l++) { P[l] = cand_lmin(false, l, v); N[l] = cand_lmin(true, l, v); }
+      idx = sel_decide(P, N, ein, unsure);
     }
-    // one LDS atomic per wavefront: the first unsure lane reserves the slots
-    const unsigned long long ball = __ballot(unsure);
-    if (ball) {
-      const int lane = __lane_id();
-      const int leader = __ffsll((long long)ball) - 1;
-      unsigned base = 0;
-      if (lane == leader) base = atomicAdd(&s_cnt, (unsigned)__popcll(ball));
-      base = __shfl(base, leader);
-      if (unsure) {
-        const unsigned slot = base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
-        region[slot] = (unsigned)(m * 3 + d);
-        if (slot < SEL_STAGE) {
-#pragma unroll
-          for (int o = 0; o < 3; o++)
-#pragma unroll
-            for (int q = 0; q < 6; q++) stage[(o * 6 + q) * SEL_STAGE + slot] = v[o][q];
-        }
-      }
-    }
+    sel_record(unsure, (unsigned)(m * 3 + d), v, &s_cnt, region, stage, SEL_STAGE);
     s |= (uint8_t)(idx << (2 * d));
   }
   if (valid) sel[m] = s;
   __syncthreads();
   if (threadIdx.x == 0) cnt[blockIdx.x] = s_cnt;
+}
+
+// The selection streamed along z, for nx = 64, 128 or 256 and nz a multiple
+// of SZ_K: a block of 256 threads holds 256/nx whole x-rows and walks SZ_K
+// planes.  The midpoint candidate of a face, C_a/2 + C_{a+1}/2, is the right
+// face's candidate of level 0 for cell a and the left face's of level 1 for
+// cell a+1 (fenep.f90:179-184), bit for bit.  Along x a thread evaluates it
+// for its right face and hands it to its neighbour through LDS (the rows are
+// whole, so the periodic neighbour is in the block); along z it carries it
+// to the next plane in registers.  16 of the 18 eigenvalue evaluations of a
+// cell remain, plus one per column for its first plane, and the z line is
+// loaded one plane per step.
+#define SZ_K 8
+__global__ __launch_bounds__(SEL_BLOCK) void m_select_z(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
+                                                        unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list,
+                                                        float* __restrict__ amb_data) {
+  __shared__ unsigned s_cnt;
+  __shared__ float s_lam[2][SEL_BLOCK], s_e0[2][SEL_BLOCK];
+  __shared__ int s_amb[2][SEL_BLOCK];
+  const int t = threadIdx.x;
+  if (t == 0) s_cnt = 0u;
+  const long xb = xcd_block();
+  const int nkg = g.nz / SZ_K;
+  const int kg = (int)(xb % nkg), jg = (int)(xb / nkg);
+  const int i = t & (g.nx - 1);
+  const int j = (jg << (8 - g.lnx)) + (t >> g.lnx);
+  const int tl = (t & ~(g.nx - 1)) | ((i - 1) & (g.nx - 1));   // the x neighbour on the left
+  const int k0 = kg * SZ_K;
+  const unsigned cap = SEL_STAGE * SZ_K;
+  unsigned* region = amb_list + (size_t)blockIdx.x * (SEL_REGION * SZ_K);
+  float* stage = amb_data + (size_t)blockIdx.x * 18 * cap;
+  const long n = g.n;
+  const int sz = g.nx * g.ny;
+  unsigned m = (unsigned)cidx(g, i, j, k0);
+  const int exm = woff32(i, -1, g.nx, 1), exp = woff32(i, 1, g.nx, 1);
+  const int eym = woff32(j, -1, g.ny, g.nx), eyp = woff32(j, 1, g.ny, g.nx);
+  const bool ylo = g.walls && j == 0, yhi = g.walls && j == g.ny - 1;
+  // the z line (k-1, k, k+1) and the entry bounds of its cells
+  float z[3][6];
+  {
+    const unsigned e = m + (unsigned)woff32(k0, -1, g.nz, sz);
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      z[0][q] = ldu(C + q * n, n, e);
+      z[1][q] = ldu(C + q * n, n, m);
+    }
+  }
+  float ebm = entry_bound(z[0]), ebc = entry_bound(z[1]);
+  // Cp_1 along z of the first plane; later planes take the previous Cn_0
+  Lmin zl = cand_lmin(false, 1, z);
+  __syncthreads();
+  for (int kk = 0; kk < SZ_K; kk++) {
+    const int par = kk & 1;
+    {
+      const unsigned e = m + (unsigned)woff32(k0 + kk, 1, g.nz, sz);
+#pragma unroll
+      for (int q = 0; q < 6; q++) z[2][q] = ldu(C + q * n, n, e);
+    }
+    const float ebp = entry_bound(z[2]);
+    // x: the right-face midpoint first, for the neighbour
+    float x[3][6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      x[0][q] = ldu(C + q * n, n, m + (unsigned)exm);
+      x[1][q] = z[1][q];
+      x[2][q] = ldu(C + q * n, n, m + (unsigned)exp);
+    }
+    Lmin XP[3], XN[3];
+    XN[0] = cand_lmin(true, 0, x);
+    s_lam[par][t] = XN[0].lam;
+    s_e0[par][t] = XN[0].e0;
+    s_amb[par][t] = XN[0].amb;
+    XP[0] = cand_lmin(false, 0, x);
+    XN[1] = cand_lmin(true, 1, x);
+    XP[2] = cand_lmin(false, 2, x);
+    XN[2] = cand_lmin(true, 2, x);
+    const float einx = SEL_KCE * fmaxf(fmaxf(entry_bound(x[0]), ebc), entry_bound(x[2]));
+    uint8_t sb = 0;
+    // z: Cp_1 carried in
+    {
+      Lmin P[3], N[3];
+      P[1] = zl;
+      P[0] = cand_lmin(false, 0, z);
+      N[0] = cand_lmin(true, 0, z);
+      N[1] = cand_lmin(true, 1, z);
+      P[2] = cand_lmin(false, 2, z);
+      N[2] = cand_lmin(true, 2, z);
+      bool unsure = false;
+      const int idx = sel_decide(P, N, SEL_KCE * fmaxf(fmaxf(ebm, ebc), ebp), unsure);
+      sel_record(unsure, m * 3u + 2u, z, &s_cnt, region, stage, cap);
+      sb |= (uint8_t)(idx << 4);
+      zl = N[0];
+    }
+    // y: C_kabe ghosts at the walls (as load_line)
+    {
+      float y[3][6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        y[0][q] = ldu(C + q * n, n, m + (unsigned)eym);
+        y[1][q] = z[1][q];
+        y[2][q] = ldu(C + q * n, n, m + (unsigned)eyp);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        y[0][q] = ylo ? 2.0f * y[1][q] - y[2][q] : y[0][q];
+        y[2][q] = yhi ? 2.0f * y[1][q] - y[0][q] : y[2][q];
+      }
+      Lmin P[3], N[3];
+#pragma unroll
+      for (int l = 0; l < 3; l++) { P[l] = cand_lmin(false, l, y); N[l] = cand_lmin(true, l, y); }
+      bool unsure = false;
+      const int idx = sel_decide(P, N, SEL_KCE * fmaxf(fmaxf(entry_bound(y[0]), ebc), entry_bound(y[2])), unsure);
+      sel_record(unsure, m * 3u + 1u, y, &s_cnt, region, stage, cap);
+      sb |= (uint8_t)(idx << 2);
+    }
+    __syncthreads();
+    XP[1].lam = s_lam[par][tl];
+    XP[1].e0 = s_e0[par][tl];
+    XP[1].amb = s_amb[par][tl] != 0;
+    {
+      bool unsure = false;
+      const int idx = sel_decide(XP, XN, einx, unsure);
+      sel_record(unsure, m * 3u, x, &s_cnt, region, stage, cap);
+      sb |= (uint8_t)idx;
+    }
+    sel[m] = sb;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      z[0][q] = z[1][q];
+      z[1][q] = z[2][q];
+    }
+    ebm = ebc;
+    ebc = ebp;
+    m += (unsigned)sz;
+  }
+  __syncthreads();
+  if (t == 0) cnt[blockIdx.x] = s_cnt;
 }
 
 // Exclusive offsets of the per-block counts and their total: one block of
@@ -526,12 +692,12 @@ __device__ __forceinline__ int select_pruned(const float (&fm)[6], const float (
     float ap[6], an[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) { ap[q] = cand_pf(l, fm[q], fc[q], fp[q]); an[q] = cand_nf(l, fm[q], fc[q], fp[q]); }
-    const Lmin P = lmin32(ap, ein), N = lmin32(an, ein);
+    const Lmin P = lmin32(ap), N = lmin32(an);
 #pragma unroll
     for (int u = 0; u < 3; u++)
       if (u == l) {
-        lam[2 * u] = P.lam; err[2 * u] = P.err;
-        lam[2 * u + 1] = N.lam; err[2 * u + 1] = N.err;
+        lam[2 * u] = P.lam; err[2 * u] = P.e0 + ein;
+        lam[2 * u + 1] = N.lam; err[2 * u + 1] = N.e0 + ein;
         amb[u] = P.amb || N.amb;
       }
   }
@@ -605,7 +771,8 @@ __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restri
                                                     const unsigned* __restrict__ off, long nb,
                                                     const unsigned* __restrict__ amb_list,
                                                     const float* __restrict__ amb_data,
-                                                    const unsigned* __restrict__ wstart) {
+                                                    const unsigned* __restrict__ wstart, unsigned region,
+                                                    unsigned cap) {
   const long cnt = (long)*total;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < cnt; t += (long)gridDim.x * blockDim.x) {
     // largest b with off[b] <= t (that block holds entry t), searched between
@@ -618,17 +785,17 @@ __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restri
       else hi = mid - 1;
     }
     const unsigned slot = (unsigned)(t - (long)off[lo]);
-    const unsigned e = amb_list[(size_t)lo * SEL_REGION + slot];
+    const unsigned e = amb_list[(size_t)lo * region + slot];
     const long m = (long)(e / 3u);
     const int d = (int)(e % 3u);
     float fm[6], fc[6], fp[6];
-    if (slot < SEL_STAGE) {
-      const float* src = amb_data + (size_t)lo * 18 * SEL_STAGE + slot;
+    if (slot < cap) {
+      const float* src = amb_data + (size_t)lo * 18 * cap + slot;
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        fm[q] = src[q * SEL_STAGE];
-        fc[q] = src[(6 + q) * SEL_STAGE];
-        fp[q] = src[(12 + q) * SEL_STAGE];
+        fm[q] = src[q * cap];
+        fc[q] = src[(6 + q) * cap];
+        fp[q] = src[(12 + q) * cap];
       }
     } else {
       sel_cells<WL>(g, C, m, d, fm, fc, fp);
@@ -3180,22 +3347,36 @@ static void dc_record(Ctx& c) {
   X.dc_pending = true;
 }
 
+// the z-streamed selection's shapes: whole x-rows in a block of 256 threads
+static bool select_streamed(const Grid& g) {
+  return (g.nx == 64 || g.nx == 128 || g.nx == 256) && g.nz % SZ_K == 0;
+}
+
 static void select_fenep(Ctx& c, int tm, hipStream_t s) {
   Mixed& X = MX(c);
   const Grid& g = c.g;
   const long nb = nblk(g.n, SEL_BLOCK);
+  // select blocks, and each block's region and staging entries (the same
+  // totals either way: 3 n entries, 18 n / 4 staged values)
+  const bool zs = select_streamed(g);
+  const long nbs = zs ? g.n / (SEL_BLOCK * SZ_K) : nb;
+  const unsigned region = zs ? SEL_REGION * SZ_K : SEL_REGION, cap = zs ? SEL_STAGE * SZ_K : SEL_STAGE;
   {
     MTimer t(c, tm ? "fenep_select" : nullptr, s);
-    hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list,
-                       X.amb_data);
+    if (zs)
+      hipLaunchKernelGGL(m_select_z, dim3(nbs), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list,
+                         X.amb_data);
+    else
+      hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list,
+                         X.amb_data);
   }
   {
     MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr, s);
-    hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nb, X.amb_off, X.amb, X.amb_wstart);
-    // one wavefront per select block: 16 per SIMD at 256x128x128, enough to
-    // hide the search and FP64 latencies; n / 4 entries per pass
+    hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nbs, X.amb_off, X.amb, X.amb_wstart);
+    // n / 4 threads, one wavefront per block: 16 per SIMD at 256x128x128,
+    // enough to hide the search and FP64 latencies
     hipLaunchKernelGGL(g.walls ? m_select_fix<true> : m_select_fix<false>, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel,
-                       X.amb, X.amb_off, nb, X.amb_list, X.amb_data, X.amb_wstart);
+                       X.amb, X.amb_off, nbs, X.amb_list, X.amb_data, X.amb_wstart, region, cap);
   }
 }
 

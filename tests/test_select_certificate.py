@@ -75,7 +75,7 @@ class Model:
         b11, b22, b33 = a[0] - m, a[3] - m, a[5] - m
         o2 = fma(a[1], a[1], fma(a[2], a[2], a[4] * a[4]))
         hp = fma(b11, b11, fma(b22, b22, b33 * b33))
-        r2 = np.maximum(fma(hp, f32(1 / 6), o2 * third), f32(1e-30))
+        r2 = fma(hp, f32(1 / 6), o2 * third)
         t1 = fma(b22, b33, -(a[4] * a[4]))
         t2 = fma(a[1], b33, -(a[4] * a[2]))
         t3 = fma(a[1], a[4], -(b22 * a[2]))
@@ -88,16 +88,17 @@ class Model:
             s2 = f32(1) - arg
             S = fma(f32(2), r, np.abs(m))
             rho = S * ir
-            rho3 = (rho * rho) * rho
-            d = np.fmin(fma(f32(self.k3), rho3, fma(f32(self.k2), rho, f32(self.k1))), f32(1))
-            v = self.noise((1 / np.sqrt(np.maximum(s2, d))).astype(f32))
-            s = np.minimum(s2, f32(2)) * v
-            ps = np.full_like(s, f32(self.psi[8]))
+            d = np.fmin(fma(rho, fma(f32(self.k3), rho * rho, f32(self.k2)), f32(self.k1)), f32(1))
+            v = self.noise((1 / np.sqrt(np.fmax(s2, d))).astype(f32))
+            s = s2 * v
+            # 2 psi with the coefficients doubled; lambda = m + r (2 psi)
+            ps = np.full_like(s, f32(2 * self.psi[8]))
             for c in self.psi[7::-1]:
-                ps = fma(ps, s, f32(c))
-            lam = fma(f32(2) * r, ps, m)
+                ps = fma(ps, s, f32(2 * c))
+            lam = fma(r, ps, m)
             amb = ~(np.abs(arg) <= fma(f32(-2), d, f32(1)))
-            err = fma(f32(self.km), S, f32(0)) + (r * d) * v
+            # the caller adds the cells' entry term (0 here)
+            err = fma(r * d, v, f32(self.km) * S)
         return lam.astype(f64), err.astype(f64), amb
 
 
