@@ -121,10 +121,10 @@ void dfree(T*& p) {
 
 // ================================================================ FENE-P ===
 // Wrapped offset of cell a+d along an axis of n cells and stride s (|d| <= 2).
+// Every axis and stride is a power of two: the wrap is a mask and the stride
+// a shift (no compare, select or 32-bit multiply per offset).
 __device__ __forceinline__ long woff(int a, int d, int n, long s) {
-  int t = a + d;
-  t = t < 0 ? t + n : (t >= n ? t - n : t);
-  return (long)(t - a) * s;
+  return (long)((unsigned long)(long)(((a + d) & (n - 1)) - a) << __builtin_ctzl((unsigned long)s));
 }
 // Element e of an array of n elements through a raw buffer resource: the
 // base and the size sit in scalar registers and each load or store takes one
@@ -150,9 +150,7 @@ __device__ __forceinline__ void stu(V* base, long n, unsigned e, V v) {
 }
 // wrapped neighbour offset along an axis in 32 bits
 __device__ __forceinline__ int woff32(int a, int d, int n, int s) {
-  int t = a + d;
-  t = t < 0 ? t + n : (t >= n ? t - n : t);
-  return (t - a) * s;
+  return (int)((unsigned)(((a + d) & (n - 1)) - a) << __builtin_ctz((unsigned)s));
 }
 
 // ---- C+- selection in FP32 with certified decisions ----------------------
@@ -333,7 +331,9 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
   for (int o = 0; o < 2 * R + 1; o++)
 #pragma unroll
     for (int q = 0; q < 6; q++) v[o][q] = ldu(C + q * g.n, g.n, e[o]);
-  if (R == 1 && d == 1 && g.walls) {
+  // (the lanes of a wavefront share their row in the callers' cell orders,
+  // so the interior rows branch past the patches)
+  if (R == 1 && d == 1 && g.walls && (j == 0 || j == g.ny - 1)) {
     const bool lo = j == 0, hi = j == g.ny - 1;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
@@ -341,7 +341,7 @@ __device__ __forceinline__ void load_line(const float* __restrict__ C, const Gri
       v[2][q] = hi ? 2.0f * v[1][q] - v[0][q] : v[2][q];
     }
   }
-  if (R == 2 && d == 1 && g.walls) {
+  if (R == 2 && d == 1 && g.walls && (j < 2 || j >= g.ny - 2)) {
     // rows 0, 1 at the low wall and ny-2, ny-1 at the high wall; each
     // replacement reads only entries that no earlier one changes
     const bool l0 = j == 0, l1 = j == 1, h1 = j == g.ny - 2, h0 = j == g.ny - 1;
@@ -987,15 +987,24 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
     // the grid spacings through reciprocals (no division per term)
     const T idx = rcp_t(dx), idy = rcp_t(dy), idz = rcp_t(dz);
     const T qdx = T(0.25) * idx, qdy = T(0.25) * idy, qdz = T(0.25) * idz;
-    a11 = -dt * (UF(1, 1, 1) - UF(0, 1, 1)) * idx;
-    a21 = -dt * ((VF(2, 0, 1) - VF(0, 0, 1)) + (VF(2, 1, 1) - VF(0, 1, 1))) * qdx;
-    a31 = -dt * ((WF(2, 1, 0) - WF(0, 1, 0)) + (WF(2, 1, 1) - WF(0, 1, 1))) * qdx;
-    a12 = -dt * ((UF(0, 2, 1) + UF(1, 2, 1)) - (UF(0, 0, 1) + UF(1, 0, 1))) * qdy;
-    a22 = -dt * (VF(1, 1, 1) - VF(1, 0, 1)) * idy;
-    a32 = -dt * ((WF(1, 2, 0) - WF(1, 0, 0)) + (WF(1, 2, 1) - WF(1, 0, 1))) * qdy;
-    a13 = -dt * ((UF(0, 1, 2) + UF(1, 1, 2)) - (UF(0, 1, 0) + UF(1, 1, 0))) * qdz;
-    a23 = -dt * ((VF(1, 0, 2) + VF(1, 1, 2)) - (VF(1, 0, 0) + VF(1, 1, 0))) * qdz;
-    a33 = -dt * (WF(1, 1, 1) - WF(1, 1, 0)) * idz;
+#define GRAD(UU, VV, WW)                                                                        \
+  a11 = -dt * (UU(1, 1, 1) - UU(0, 1, 1)) * idx;                                                \
+  a21 = -dt * ((VV(2, 0, 1) - VV(0, 0, 1)) + (VV(2, 1, 1) - VV(0, 1, 1))) * qdx;                \
+  a31 = -dt * ((WW(2, 1, 0) - WW(0, 1, 0)) + (WW(2, 1, 1) - WW(0, 1, 1))) * qdx;                \
+  a12 = -dt * ((UU(0, 2, 1) + UU(1, 2, 1)) - (UU(0, 0, 1) + UU(1, 0, 1))) * qdy;                \
+  a22 = -dt * (VV(1, 1, 1) - VV(1, 0, 1)) * idy;                                                \
+  a32 = -dt * ((WW(1, 2, 0) - WW(1, 0, 0)) + (WW(1, 2, 1) - WW(1, 0, 1))) * qdy;                \
+  a13 = -dt * ((UU(0, 1, 2) + UU(1, 1, 2)) - (UU(0, 1, 0) + UU(1, 1, 0))) * qdz;                \
+  a23 = -dt * ((VV(1, 0, 2) + VV(1, 1, 2)) - (VV(1, 0, 0) + VV(1, 1, 0))) * qdz;                \
+  a33 = -dt * (WW(1, 1, 1) - WW(1, 1, 0)) * idz;
+    // the interior rows read plainly; the two wall rows (a wavefront shares
+    // its row) again with the boundary rules
+    if (wlo || whi) {
+      GRAD(UF, VF, WF)
+    } else {
+      GRAD(UR, VR, WR)
+    }
+#undef GRAD
 #undef UF
 #undef VF
 #undef WF
