@@ -461,7 +461,7 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
 // to the next plane in registers.  16 of the 18 eigenvalue evaluations of a
 // cell remain, plus one per column for its first plane, and the z line is
 // loaded one plane per step.
-#define SZ_K 8
+#define SZ_K 4
 __global__ __launch_bounds__(SEL_BLOCK) void m_select_z(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                         unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list,
                                                         float* __restrict__ amb_data) {
@@ -551,10 +551,12 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select_z(Grid g, const float* __r
         y[1][q] = z[1][q];
         y[2][q] = ldu(C + q * n, n, m + (unsigned)eyp);
       }
+      if (ylo || yhi) {   // a wavefront holds one row: interior rows skip this
 #pragma unroll
-      for (int q = 0; q < 6; q++) {
-        y[0][q] = ylo ? 2.0f * y[1][q] - y[2][q] : y[0][q];
-        y[2][q] = yhi ? 2.0f * y[1][q] - y[0][q] : y[2][q];
+        for (int q = 0; q < 6; q++) {
+          y[0][q] = ylo ? 2.0f * y[1][q] - y[2][q] : y[0][q];
+          y[2][q] = yhi ? 2.0f * y[1][q] - y[0][q] : y[2][q];
+        }
       }
       Lmin P[3], N[3];
 #pragma unroll
@@ -1116,11 +1118,11 @@ __device__ __forceinline__ void cstar_newton_cell(const Grid& g, const float* __
 }
 
 // one instance over every cell (the wall rows through the boundary-rule
-// patches in cstar_newton_cell and load_line), in band order; held to at
-// least four waves per SIMD (128 registers): with 64-bit addresses the
-// patches took it to 130 registers and three waves (180 us instead of 163 at
-// 256x128x128); the buffer-resource loads leave it at 98
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
+// patches in cstar_newton_cell and load_line), in band order.  The kernel
+// waits on its loads more than it issues: it is held to five waves per SIMD
+// (96 registers, no spills; 145 -> 140 us at 256x128x128 against four waves;
+// six waves spill)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5, 8))) void m_cstar_newton(Grid g, const float* __restrict__ C, const uint8_t* __restrict__ sel,
                                                       const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, float* __restrict__ Cn,
                                                       float* __restrict__ fC, int* __restrict__ nitr, float eps,
