@@ -62,9 +62,7 @@ struct Mixed {
   int helm_iters = 0;             // fixed-point sweeps (0: spectral solve)
   unsigned* amb = nullptr;        // uncertified selections, re-decided in FP64
   unsigned* amb_list = nullptr;   // per-block regions of m * 3 + axis entries
-  unsigned* amb_cnt = nullptr;    // entries per select block
-  unsigned* amb_off = nullptr;    // exclusive offsets of amb_cnt
-  unsigned* amb_wstart = nullptr; // per 64 entries: the block holding the first
+  unsigned* amb_idx = nullptr;    // the entries' places in amb_list, packed (3 n)
   float* amb_data = nullptr;      // staged cell values, 18 * SEL_STAGE per select block
   unsigned* chk = nullptr;        // selection check: differing decisions
   // IBM
@@ -417,11 +415,26 @@ __device__ __forceinline__ void sel_record(bool unsure, unsigned entry, const fl
   }
 }
 
+// The block's entries join the packed list of all uncertified entries: one
+// device-scope atomic per block reserves [base, base + count), and the
+// places of the block's entries in amb_list (block * region + slot) go there,
+// so that the FP64 re-decision walks one dense list.  Called by every thread
+// of the block once its entries are recorded.
+__device__ __forceinline__ void sel_publish(unsigned* s_cnt, unsigned* s_base, long blk, unsigned region,
+                                            unsigned* __restrict__ total, unsigned* __restrict__ idx) {
+  __syncthreads();
+  const unsigned c = *s_cnt;
+  if (threadIdx.x == 0) *s_base = c ? atomicAdd(total, c) : 0u;
+  __syncthreads();
+  const unsigned base = *s_base;
+  for (unsigned sl = threadIdx.x; sl < c; sl += blockDim.x) idx[base + sl] = (unsigned)blk * region + sl;
+}
+
 // One cell per thread, any grid shape.
 __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
-                                                      unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list,
-                                                      float* __restrict__ amb_data) {
-  __shared__ unsigned s_cnt;
+                                                      unsigned* __restrict__ total, unsigned* __restrict__ amb_idx,
+                                                      unsigned* __restrict__ amb_list, float* __restrict__ amb_data) {
+  __shared__ unsigned s_cnt, s_base;
   if (threadIdx.x == 0) s_cnt = 0u;
   __syncthreads();
   long m;
@@ -447,8 +460,7 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
     s |= (uint8_t)(idx << (2 * d));
   }
   if (valid) sel[m] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) cnt[blockIdx.x] = s_cnt;
+  sel_publish(&s_cnt, &s_base, blockIdx.x, SEL_REGION, total, amb_idx);
 }
 
 // The selection streamed along z, for nx = 64, 128 or 256 and nz a multiple
@@ -463,9 +475,9 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select(Grid g, const float* __res
 // loaded one plane per step.
 #define SZ_K 4
 __global__ __launch_bounds__(SEL_BLOCK) void m_select_z(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
-                                                        unsigned* __restrict__ cnt, unsigned* __restrict__ amb_list,
-                                                        float* __restrict__ amb_data) {
-  __shared__ unsigned s_cnt;
+                                                        unsigned* __restrict__ total, unsigned* __restrict__ amb_idx,
+                                                        unsigned* __restrict__ amb_list, float* __restrict__ amb_data) {
+  __shared__ unsigned s_cnt, s_base;
   __shared__ float s_lam[2][SEL_BLOCK], s_e0[2][SEL_BLOCK];
   __shared__ int s_amb[2][SEL_BLOCK];
   const int t = threadIdx.x;
@@ -586,57 +598,7 @@ __global__ __launch_bounds__(SEL_BLOCK) void m_select_z(Grid g, const float* __r
     ebc = ebp;
     m += (unsigned)sz;
   }
-  __syncthreads();
-  if (t == 0) cnt[blockIdx.x] = s_cnt;
-}
-
-// Exclusive offsets of the per-block counts and their total: one block of
-// 1024 threads over rows of 1024 counts (16 rows per pass, all loads of a
-// pass in flight together); each row is scanned with wavefront prefix sums
-// and a 16-entry LDS combine, so every access is coalesced or a broadcast.
-#define SCAN_ROWS 16
-__global__ __launch_bounds__(1024) void m_sel_scan(const unsigned* __restrict__ cnt, long nb,
-                                                   unsigned* __restrict__ off, unsigned* __restrict__ total,
-                                                   unsigned* __restrict__ wstart) {
-  __shared__ unsigned wsum[16];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  unsigned carry = 0;
-  for (long p0 = 0; p0 < nb; p0 += 1024L * SCAN_ROWS) {
-    unsigned x[SCAN_ROWS];
-#pragma unroll
-    for (int r = 0; r < SCAN_ROWS; r++) {
-      const long b = p0 + 1024L * r + t;
-      x[r] = b < nb ? cnt[b] : 0u;
-    }
-#pragma unroll
-    for (int r = 0; r < SCAN_ROWS; r++) {
-      unsigned v = x[r];
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const unsigned y = __shfl_up(v, o);
-        if (lane >= o) v += y;
-      }
-      if (lane == 63) wsum[w] = v;
-      __syncthreads();
-      unsigned pre = 0, tot = 0;
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const unsigned sq = wsum[q];
-        pre += q < w ? sq : 0u;
-        tot += sq;
-      }
-      const long b = p0 + 1024L * r + t;
-      if (b < nb) {
-        const unsigned o0 = carry + pre + v - x[r], o1 = o0 + x[r];
-        off[b] = o0;
-        // the groups of 64 entries whose first entry this block holds
-        for (unsigned long long w = (o0 + 63ull) >> 6; (w << 6) < o1; w++) wstart[w] = (unsigned)b;
-      }
-      carry += tot;
-      __syncthreads();
-    }
-  }
-  if (t == 0) *total = carry;
+  sel_publish(&s_cnt, &s_base, blockIdx.x, SEL_REGION * SZ_K, total, amb_idx);
 }
 
 // the reference's selection (select_d, evaluated as written) of one (cell,
@@ -763,36 +725,25 @@ __device__ __forceinline__ int select_pruned(const float (&fm)[6], const float (
 }
 
 // FP64 re-decision of the uncertified selections: grid-stride over the
-// total, each entry found in its block's region by a binary search of the
-// exclusive offsets (between the blocks the scan recorded for its group of
-// 64 entries); its 18 input values come from the block's staging area for
-// the first SEL_STAGE entries of a block and are gathered for the rest
+// packed list; an entry's 18 input values come from its select block's
+// staging area for the block's first `cap` entries and are gathered for the
+// rest
 template <bool WL>
 __global__ __launch_bounds__(64) void m_select_fix(Grid g, const float* __restrict__ C, uint8_t* __restrict__ sel,
                                                     const unsigned* __restrict__ total,
-                                                    const unsigned* __restrict__ off, long nb,
+                                                    const unsigned* __restrict__ amb_idx,
                                                     const unsigned* __restrict__ amb_list,
-                                                    const float* __restrict__ amb_data,
-                                                    const unsigned* __restrict__ wstart, unsigned region,
-                                                    unsigned cap) {
+                                                    const float* __restrict__ amb_data, unsigned region, unsigned cap) {
   const long cnt = (long)*total;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < cnt; t += (long)gridDim.x * blockDim.x) {
-    // largest b with off[b] <= t (that block holds entry t), searched between
-    // the blocks holding the first entries of t's group of 64 and of the next
-    const long w = t >> 6;
-    long lo = (long)wstart[w], hi = ((w + 1) << 6) < cnt ? (long)wstart[w + 1] : nb - 1;
-    while (lo < hi) {
-      const long mid = (lo + hi + 1) >> 1;
-      if ((long)off[mid] <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const unsigned slot = (unsigned)(t - (long)off[lo]);
-    const unsigned e = amb_list[(size_t)lo * region + slot];
+    const unsigned at = amb_idx[t];
+    const unsigned blk = at / region, slot = at - blk * region;
+    const unsigned e = amb_list[at];
     const long m = (long)(e / 3u);
     const int d = (int)(e % 3u);
     float fm[6], fc[6], fp[6];
     if (slot < cap) {
-      const float* src = amb_data + (size_t)lo * 18 * cap + slot;
+      const float* src = amb_data + (size_t)blk * 18 * cap + slot;
 #pragma unroll
       for (int q = 0; q < 6; q++) {
         fm[q] = src[q * cap];
@@ -2935,8 +2886,7 @@ void mixed_create(Ctx& c) {
   {
     const long nb = nblk(n, SEL_BLOCK);
     dalloc(&X->amb, 1); dalloc(&X->amb_list, nb * SEL_REGION); dalloc(&X->chk, 1);
-    dalloc(&X->amb_cnt, nb); dalloc(&X->amb_off, nb);
-    dalloc(&X->amb_wstart, nb * (SEL_REGION / 64) + 2);
+    dalloc(&X->amb_idx, nb * SEL_REGION);
     dalloc(&X->amb_data, nb * 18 * SEL_STAGE);
   }
   // FP32 twiddles and wavenumbers from the FP64 host tables
@@ -2979,7 +2929,7 @@ void mixed_destroy(Ctx& c) {
   if (X->dc_ev) (void)hipEventDestroy(X->dc_ev);
   if (X->dc_host) (void)hipHostFree(X->dc_host);
   dfree(X->X2); dfree(X->amb); dfree(X->amb_list); dfree(X->chk);
-  dfree(X->amb_cnt); dfree(X->amb_off); dfree(X->amb_wstart); dfree(X->amb_data);
+  dfree(X->amb_idx); dfree(X->amb_data);
   dfree(X->wi); dfree(X->org); dfree(X->uc); dfree(X->mf); dfree(X->mf64); dfree(X->ftmp);
   for (int k = 0; k < 3; k++) dfree(X->sw[k]);
   dfree(X->fr_row); dfree(X->fr_len); dfree(X->fr_tgt); dfree(X->fr_ent); dfree(X->fr_w);
@@ -3374,20 +3324,20 @@ static void select_fenep(Ctx& c, int tm, hipStream_t s) {
   const unsigned region = zs ? SEL_REGION * SZ_K : SEL_REGION, cap = zs ? SEL_STAGE * SZ_K : SEL_STAGE;
   {
     MTimer t(c, tm ? "fenep_select" : nullptr, s);
+    zero_words(X.amb, 1, s);
     if (zs)
-      hipLaunchKernelGGL(m_select_z, dim3(nbs), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list,
-                         X.amb_data);
+      hipLaunchKernelGGL(m_select_z, dim3(nbs), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_idx,
+                         X.amb_list, X.amb_data);
     else
-      hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb_cnt, X.amb_list,
-                         X.amb_data);
+      hipLaunchKernelGGL(m_select, dim3(nb), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_idx,
+                         X.amb_list, X.amb_data);
   }
   {
     MTimer t(c, tm ? "fenep_select_fp64_fix" : nullptr, s);
-    hipLaunchKernelGGL(m_sel_scan, dim3(1), dim3(1024), 0, s, X.amb_cnt, nbs, X.amb_off, X.amb, X.amb_wstart);
     // n / 4 threads, one wavefront per block: 16 per SIMD at 256x128x128,
-    // enough to hide the search and FP64 latencies
+    // enough to hide the FP64 latencies
     hipLaunchKernelGGL(g.walls ? m_select_fix<true> : m_select_fix<false>, dim3(nb), dim3(64), 0, s, g, X.C[X.cur], c.sel,
-                       X.amb, X.amb_off, nbs, X.amb_list, X.amb_data, X.amb_wstart, region, cap);
+                       X.amb, X.amb_idx, X.amb_list, X.amb_data, region, cap);
   }
 }
 
