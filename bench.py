@@ -240,7 +240,9 @@ def main():
     step_bytes = STEP_WORDS * 8 * cells if prec == 0 else mixed_step_bytes(d.get("n_dc", 0)) * cells
     walls = "true" if case.params.walls_y else "false"
     mixed_pred = "fp::m_predictor<false>" if "predictor_walls" in kt else "fp::m_predictor_z<%s>" % walls
-    kernel_names = {"fenep_select": "fp::m_select" if prec else "fp::k_select",
+    # the selection streamed along z where a block holds whole x-rows
+    streamed = case.params.nx in (64, 128, 256) and case.params.nz % 4 == 0
+    kernel_names = {"fenep_select": ("fp::m_select_z" if streamed else "fp::m_select") if prec else "fp::k_select",
                     "fenep_cstar_newton": "fp::m_cstar_newton" if prec else "fp::k_cstar_newton<%s>" % walls,
                     "predictor": mixed_pred if prec else "fp::k_predictor<%s>" % walls}
     out = {

@@ -3322,9 +3322,9 @@ static void select_fenep(Ctx& c, int tm, hipStream_t s) {
   const bool zs = select_streamed(g);
   const long nbs = zs ? g.n / (SEL_BLOCK * SZ_K) : nb;
   const unsigned region = zs ? SEL_REGION * SZ_K : SEL_REGION, cap = zs ? SEL_STAGE * SZ_K : SEL_STAGE;
+  zero_words(X.amb, 1, s);
   {
     MTimer t(c, tm ? "fenep_select" : nullptr, s);
-    zero_words(X.amb, 1, s);
     if (zs)
       hipLaunchKernelGGL(m_select_z, dim3(nbs), dim3(SEL_BLOCK), 0, s, g, X.C[X.cur], c.sel, X.amb, X.amb_idx,
                          X.amb_list, X.amb_data);
