@@ -1270,19 +1270,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 // the tile with a one-cell halo for the ten stencil fields (u, v, w, p and
 // c0 of the six components); every neighbour the predictor reads lies there.
 // Plane k+2 is loaded into registers while plane k is computed, then takes
-// the place of plane k-1.  Each value is fetched from L2 about 1.6 times at
-// 64 x 4 tiles and 32 planes (halo rows and chunk ends), in whole rows,
-// where the per-cell stencil of m_predictor fetches it about 6 times.  The
+// the place of plane k-1.  Each value is fetched from L2 about 1.3 times at
+// 64 x 16 tiles and 16 planes (halo rows and chunk ends), in whole rows,
+// where the per-cell stencil of m_predictor fetches it about 6 times.  At
+// 256x128x128 that is one 1024-thread block per CU (142 KB of LDS); 64 x 4
+// tiles over 32 planes (1.6 fetches, two 256-thread blocks per CU) took
+// 11 us more per step, 64 x 8 over 16 planes 4 us more.  The
 // values in LDS are those of the boundary-rule accessor (wall ghosts and the
 // wall face of V included), so one kernel covers the wall rows too.
 #define PZ_TX 64
-#define PZ_TY 4
+#define PZ_TY 16
 #define PZ_HX (PZ_TX + 2)
 #define PZ_PTS (PZ_HX * (PZ_TY + 2))
 #define PZ_NF 10
 #define PZ_THREADS (PZ_TX * PZ_TY)
 #define PZ_PER ((PZ_PTS + PZ_THREADS - 1) / PZ_THREADS)
-#define PZ_KC 32                  // planes per block (16 and 64 measured slower)
+#define PZ_KC 16                  // planes per block
 
 struct PredLds {
   const float *pm, *p0, *pp;   // planes k-1, k, k+1: PZ_NF fields of PZ_PTS
@@ -2523,17 +2526,17 @@ __global__ __launch_bounds__(256) void m_march(Grid g, const float* __restrict__
   P[m] = (float)(((double)P[m] + ph) - coef * lap);
 }
 
-// The same correction streamed along z: a 64 x 4 tile walks 8 planes with
+// The same correction streamed along z: a 64 x 16 tile walks 8 planes with
 // three planes of Phi (FP64, one-cell halo, the boundary rule of P applied)
 // in LDS and the next plane prefetched; u^n, the increment d, P and the
 // outputs are the cell's own, read and written once.
 #define MZ_TX 64
-#define MZ_TY 4
+#define MZ_TY 16
 #define MZ_THREADS (MZ_TX * MZ_TY)
 #define MZ_HX (MZ_TX + 2)
 #define MZ_PTS (MZ_HX * (MZ_TY + 2))
 #define MZ_PER ((MZ_PTS + MZ_THREADS - 1) / MZ_THREADS)
-#define MZ_KC 8                   // planes per block: a light kernel wants many blocks (32 measured slower)
+#define MZ_KC 8                   // planes per block (64 x 4 tiles: +3 us per step; 16 planes: +3)
 
 template <bool WL>
 __device__ __forceinline__ void mz_fetch(const Grid& g, int i0, int j0, int kk, const double* __restrict__ phi,
