@@ -2381,14 +2381,27 @@ __device__ __forceinline__ void hz_put(const Grid& g, int j0, const HzRegs& r, f
   }
 }
 
-// one sweep at the x1-halo point (hx, hy) of plane p from the q planes
+// one sweep at the x1-halo points (hx, hy) of plane p from the q planes
 // p-1, p, p+1: q + eps L q, the wall face of V set to zero, ghost rows by the
-// rules from the nearest row
+// rules from the nearest row.  A thread's points are pt = threadIdx.x +
+// e HZ_THREADS, e = 0, 1; qce holds their q-tile offsets, taken where no point
+// is a ghost (WL = false).
+static_assert(HZ_XP <= 2 * HZ_THREADS, "two x1 points per thread");
+__device__ __forceinline__ void hz_x1_offsets(int (&qce)[2]) {
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    const int pt = threadIdx.x + HZ_THREADS * e;
+    qce[e] = (pt / HZ_XX + 1) * HZ_QX + pt % HZ_XX + 1;
+  }
+}
 template <bool WL>
 __device__ __forceinline__ void hz_x1(const Grid& g, int j0, float eps, float ix2, float iy2, float iz2,
                                       const float* __restrict__ qm, const float* __restrict__ q0,
-                                      const float* __restrict__ qp, float* __restrict__ xs) {
-  for (int pt = threadIdx.x; pt < HZ_XP; pt += HZ_THREADS) {
+                                      const float* __restrict__ qp, float* __restrict__ xs, const int (&qce)[2]) {
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    const int pt = threadIdx.x + HZ_THREADS * e;
+    if (pt >= HZ_XP) break;
     const int hx = pt % HZ_XX, hy = pt / HZ_XX;
     const int jj = j0 + hy - 1;
     const bool lo = WL && jj < 0, hi = WL && jj >= g.ny;
@@ -2396,7 +2409,7 @@ __device__ __forceinline__ void hz_x1(const Grid& g, int j0, float eps, float ix
     // interior row of a ghost)
     const int hyc = lo ? hy + 1 : (hi ? hy - 1 : hy);
     const int jc = j0 + hyc - 1;
-    const int qc = (hyc + 1) * HZ_QX + hx + 1;
+    const int qc = WL ? (hyc + 1) * HZ_QX + hx + 1 : qce[e];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const float* Qm = qm + c * HZ_QP;
@@ -2413,14 +2426,9 @@ __device__ __forceinline__ void hz_x1(const Grid& g, int j0, float eps, float ix
 }
 
 template <bool WL>
-__global__ __launch_bounds__(HZ_THREADS) void m_helm_2sweeps_z(Grid g, float eps, const float* __restrict__ q,
-                                                              float* __restrict__ x) {
-  __shared__ float qs[4][3 * HZ_QP];
-  __shared__ float xs[3][3 * HZ_XP];
-  const int ntx = g.nx / HZ_TX, nty = g.ny / HZ_TY;
-  const long b = xcd_block();
-  const int bx = (int)(b % ntx), by = (int)((b / ntx) % nty), bz = (int)(b / ((long)ntx * nty));
-  const int i0 = bx * HZ_TX, j0 = by * HZ_TY, k0 = bz * HZ_KC;
+__device__ __forceinline__ void helm_2sweeps_tile(const Grid& g, float eps, const float* __restrict__ q,
+                                                  float* __restrict__ x, int i0, int j0, int k0, float (*qs)[3 * HZ_QP],
+                                                  float (*xs)[3 * HZ_XP]) {
   const int k1 = min(k0 + HZ_KC, g.nz);
   const int tx = threadIdx.x % HZ_TX, ty = threadIdx.x / HZ_TX;
   const float ix2 = (float)(1.0 / (g.dx * g.dx)), iy2 = (float)(1.0 / (g.dy * g.dy)), iz2 = (float)(1.0 / (g.dz * g.dz));
@@ -2433,8 +2441,10 @@ __global__ __launch_bounds__(HZ_THREADS) void m_helm_2sweeps_z(Grid g, float eps
   }
   if (k0 + 2 <= k1 + 1) hz_fetch<WL>(g, i0, j0, k0 + 2, q, r);
   __syncthreads();
-  hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[0], qs[1], qs[2], xs[0]);   // x1 at k0-1
-  hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[1], qs[2], qs[3], xs[1]);   // x1 at k0
+  int qce[2];
+  hz_x1_offsets(qce);
+  hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[0], qs[1], qs[2], xs[0], qce);   // x1 at k0-1
+  hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[1], qs[2], qs[3], xs[1], qce);   // x1 at k0
   __syncthreads();
   if (k0 + 2 <= k1 + 1) hz_put<WL>(g, j0, r, qs[0]);                  // q at k0+2
   if (k0 + 3 <= k1 + 1) hz_fetch<WL>(g, i0, j0, k0 + 3, q, r);
@@ -2443,7 +2453,7 @@ __global__ __launch_bounds__(HZ_THREADS) void m_helm_2sweeps_z(Grid g, float eps
   for (int kk = k0; kk < k1; kk++) {
     const int sq = kk - k0 + 2, sx = kk - k0 + 1;
     // x1 at kk+1 from q at kk, kk+1, kk+2 (into the slot of x1 at kk-2)
-    hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[sq & 3], qs[(sq + 1) & 3], qs[(sq + 2) & 3], xs[(sx + 1) % 3]);
+    hz_x1<WL>(g, j0, eps, ix2, iy2, iz2, qs[sq & 3], qs[(sq + 1) & 3], qs[(sq + 2) & 3], xs[(sx + 1) % 3], qce);
     __syncthreads();
     {
       // x2 at kk from x1 at kk-1, kk, kk+1 and q at kk
@@ -2470,6 +2480,22 @@ __global__ __launch_bounds__(HZ_THREADS) void m_helm_2sweeps_z(Grid g, float eps
     if (kk + 4 <= k1 + 1) hz_fetch<WL>(g, i0, j0, kk + 4, q, r);
     __syncthreads();
   }
+}
+
+// A tile whose q rows (j0-2 .. j0+HZ_TY+1) all lie inside the walls and
+// short of the wall face of V (row ny-1) takes no boundary rule: it runs the
+// periodic path, whose values are then the same.
+template <bool WL>
+__global__ __launch_bounds__(HZ_THREADS) void m_helm_2sweeps_z(Grid g, float eps, const float* __restrict__ q,
+                                                              float* __restrict__ x) {
+  __shared__ float qs[4][3 * HZ_QP];
+  __shared__ float xs[3][3 * HZ_XP];
+  const int ntx = g.nx / HZ_TX, nty = g.ny / HZ_TY;
+  const long b = xcd_block();
+  const int bx = (int)(b % ntx), by = (int)((b / ntx) % nty), bz = (int)(b / ((long)ntx * nty));
+  const int i0 = bx * HZ_TX, j0 = by * HZ_TY, k0 = bz * HZ_KC;
+  if (WL && (j0 < 2 || j0 + HZ_TY + 1 >= g.ny - 1)) helm_2sweeps_tile<true>(g, eps, q, x, i0, j0, k0, qs, xs);
+  else helm_2sweeps_tile<false>(g, eps, q, x, i0, j0, k0, qs, xs);
 }
 
 // Helmholtz solves of the three predictor components: fixed-point sweeps when
