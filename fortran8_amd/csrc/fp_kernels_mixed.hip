@@ -229,9 +229,9 @@ __device__ __forceinline__ Lmin lmin32(const float* a) {
   const float ir3 = (ir * ir) * ir;
   const float arg = (0.5f * det) * ir3;
   const float s2 = 1.0f - arg;
-  const float S = fmaf(2.0f, r, fabsf(m));
-  const float rho = S * ir;
-  const float d = fminf(fmaf(rho, fmaf(SEL_K3E, rho * rho, SEL_K2E), SEL_K1E), 1.0f);
+  // rho = (|m| + 2 r)/r; d > 1 only where the candidate is flagged below
+  const float rho = fmaf(fabsf(m), ir, 2.0f);
+  const float d = fmaf(rho, fmaf(SEL_K3E, rho * rho, SEL_K2E), SEL_K1E);
   // 1/sqrt(max(1 - arg, delta)); where the decision is certified, 1 - arg >=
   // 2 delta and s = (1 - arg) v is sqrt(1 - arg) (s beyond sqrt 2, arg < -1,
   // only where the candidate is ambiguous)
@@ -251,7 +251,7 @@ __device__ __forceinline__ Lmin lmin32(const float* a) {
   o.lam = fmaf(r, ps, m);
   // written so that a NaN arg counts as ambiguous
   o.amb = !(fabsf(arg) <= fmaf(-2.0f, d, 1.0f));
-  o.e0 = fmaf(r * d, v, SEL_KME * S);
+  o.e0 = r * fmaf(d, v, SEL_KME * rho);
   return o;
 }
 

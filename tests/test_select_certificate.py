@@ -86,9 +86,8 @@ class Model:
             ir3 = (ir * ir) * ir
             arg = (f32(0.5) * det) * ir3
             s2 = f32(1) - arg
-            S = fma(f32(2), r, np.abs(m))
-            rho = S * ir
-            d = np.fmin(fma(rho, fma(f32(self.k3), rho * rho, f32(self.k2)), f32(self.k1)), f32(1))
+            rho = fma(np.abs(m), ir, f32(2))
+            d = fma(rho, fma(f32(self.k3), rho * rho, f32(self.k2)), f32(self.k1))
             v = self.noise((1 / np.sqrt(np.fmax(s2, d))).astype(f32))
             s = s2 * v
             # 2 psi with the coefficients doubled; lambda = m + r (2 psi)
@@ -98,7 +97,7 @@ class Model:
             lam = fma(r, ps, m)
             amb = ~(np.abs(arg) <= fma(f32(-2), d, f32(1)))
             # the caller adds the cells' entry term (0 here)
-            err = fma(r * d, v, f32(self.km) * S)
+            err = r * fma(d, v, f32(self.km) * rho)
         return lam.astype(f64), err.astype(f64), amb
 
 
