@@ -3,11 +3,13 @@
 //
 // Forward (fp_adj_record): the step runs unchanged; every `every` steps the
 // full state is checkpointed, the C+- selections of every step and the drag
-// D = -sum_markers Fxc dX^3 of every step are kept.
+// D = -sum_markers Fxc dX^3 of every step are kept, and (FP64 step) the last
+// segment is taped as it runs.
 //
 // Reverse (fp_adj_backward): segments are recomputed from their checkpoint
 // with the recorded selections (a tape of the state, u* and Phi of each step)
-// and traversed backwards.  Each forward kernel has an adjoint kernel here:
+// and traversed backwards; the segment the tape already holds is not
+// recomputed.  Each forward kernel has an adjoint kernel here:
 //   fft_march          -> k_adj_march            (transposed gradient/Laplacian)
 //   fft_poisson solve  -> the same spectral solve (symmetric operator)
 //   divergence         -> k_adj_div
@@ -27,7 +29,10 @@
 // the body-force field and the initial state.
 #include "fp_device.h"
 #include "fp_mixed.h"
+#include <algorithm>
 #include <cmath>
+#include <initializer_list>
+#include <utility>
 #include <vector>
 
 namespace fp {
@@ -71,7 +76,7 @@ __global__ __launch_bounds__(256) void k_adj_march(Grid g, const double* __restr
   }
   __shared__ double sh[256];
   const double s = block_sum256(sb, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[blockIdx.x] += s;
 }
 
 // ------------------------------------------------------ divergence^T
@@ -111,7 +116,7 @@ __global__ __launch_bounds__(256) void k_adj_helm_beta(Grid g, const double* __r
   }
   __shared__ double sh[256];
   const double s = block_sum256(sb, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[blockIdx.x] += s;
 }
 
 // ------------------------------------------------------------- IBM^T
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(256) void k_adj_pred_u(Grid g, const double* __rest
   }
   __shared__ double sh[256];
   const double s = block_sum256(sb, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[blockIdx.x] += s;
 }
 
 // Polymer stress T = coef div(tau), tau = f(C) C - I (C the new
@@ -322,8 +327,8 @@ __global__ __launch_bounds__(256) void k_adj_pred_tau(Grid g, const double* __re
   __syncthreads();
   const double b = block_sum256(s_l2, sh);
   if (threadIdx.x == 0) {
-    part[ACC_COEF * nb + blockIdx.x] = a;
-    part[ACC_L2 * nb + blockIdx.x] = b;
+    part[ACC_COEF * nb + blockIdx.x] += a;
+    part[ACC_L2 * nb + blockIdx.x] += b;
   }
 }
 
@@ -448,8 +453,8 @@ __global__ __launch_bounds__(256) void k_adj_newton(Grid g, const double* __rest
   __syncthreads();
   const double b = block_sum256(s_l2, sh);
   if (threadIdx.x == 0) {
-    part[ACC_WI * nb + blockIdx.x] = a;
-    part[ACC_L2 * nb + blockIdx.x] = b;
+    part[ACC_WI * nb + blockIdx.x] += a;
+    part[ACC_L2 * nb + blockIdx.x] += b;
   }
 }
 
@@ -596,6 +601,7 @@ struct Adj {
   double* drag = nullptr;          // nsteps
   double* tape = nullptr;          // every * TAPE_WORDS * n + 6n (C after the segment)
   std::vector<int> tape_first;
+  int tape_seg = -1;               // the segment the tape holds (-1: none)
   double *bar = nullptr, *ubs = nullptr, *phib = nullptr, *bb = nullptr, *Rb = nullptr, *Qb = nullptr;
   double *ABT = nullptr, *Xb = nullptr, *lam = nullptr, *Mb = nullptr, *gm = nullptr, *Fb = nullptr;
   double *acc = nullptr, *part = nullptr;
@@ -621,6 +627,55 @@ Adj& adj_of(Ctx& c) {
 
 void d2d(Ctx& c, void* dst, const void* src, size_t bytes) {
   FP_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+}
+
+// up to eight arrays copied one after another into one range, in one launch
+struct Gather {
+  const double* src[8];
+  long len[8];
+  int count;
+};
+__global__ __launch_bounds__(256) void k_gather(double* __restrict__ dst, Gather gt, long total) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    long off = e;
+    int q = 0;
+    while (q < gt.count - 1 && off >= gt.len[q]) {
+      off -= gt.len[q];
+      q++;
+    }
+    dst[e] = gt.src[q][off];
+  }
+}
+void gather(Ctx& c, double* dst, std::initializer_list<std::pair<const double*, long>> parts) {
+  Gather gt{};
+  long total = 0;
+  for (const auto& pr : parts) {
+    gt.src[gt.count] = pr.first;
+    gt.len[gt.count] = pr.second;
+    gt.count++;
+    total += pr.second;
+  }
+  const long nb = std::min((long)nblk(total, 256), 2048L);
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)nb), dim3(256), 0, c.stream, dst, gt, total);
+}
+// tape entry of one step: the state before it, then (after the step) u* and Phi
+void tape_before(Ctx& c, Adj& a, double* E, int t) {
+  const long n = c.g.n;
+  gather(c, E, {{c.U, n}, {c.V, n}, {c.W, n}, {c.P, n}, {c.C[c.cur], 6 * n}});
+  a.tape_first[t] = c.has_history ? 0 : 1;
+}
+void tape_after(Ctx& c, double* E) {
+  const long n = c.g.n;
+  gather(c, E + 10 * n, {{c.Up, 3 * n}, {c.Phi, n}});
+}
+void ensure_tape(Adj& a, int every, long n) {
+  if (!a.tape || a.tape_every != every) {
+    free0(a.tape);
+    alloc0(&a.tape, (size_t)every * TAPE_WORDS * n + 6 * n);
+    a.tape_every = every;
+    a.tape_seg = -1;
+  }
+  if ((int)a.tape_first.size() != every) a.tape_first.assign(every, 0);
 }
 
 void save_state(Ctx& c, double* dst) {
@@ -677,13 +732,11 @@ void adjoint_step(Ctx& c, Adj& a, int t, int seg_len, int s, double wdrag) {
   const int polymer = c.p.beta != 1.0;
   // projection
   hipLaunchKernelGGL(k_adj_march, dim3(nb), dim3(256), 0, st, g, aU, aP, Phi, a.ubs, a.phib, a.part);
-  sum_part(c, a, ACC_BETA, nb);
   f64_solve(c, 3, a.phib, a.bb);
   hipLaunchKernelGGL(k_adj_div, dim3(nb), dim3(256), 0, st, g, a.bb, a.ubs);
   // Helmholtz
   for (int comp = 0; comp < 3; comp++) f64_solve(c, comp, a.ubs + comp * n, a.Rb + comp * n);
   hipLaunchKernelGGL(k_adj_helm_beta, dim3(nb), dim3(256), 0, st, g, a.Rb, Up, a.part);
-  sum_part(c, a, ACC_BETA, nb);
   // immersed boundary
   const double* Qb = a.Rb;
   if (c.p.method == FP_METHOD_IBM) {
@@ -697,14 +750,9 @@ void adjoint_step(Ctx& c, Adj& a, int t, int seg_len, int s, double wdrag) {
   hipLaunchKernelGGL(k_adj_pred_local, dim3(nblk(3 * n, 256)), dim3(256), 0, st, 3 * n, g.dt, c.p.method,
                      a.tape_first[t], Qb, a.Rb, aA0, aT0, a.ABT, a.Fb);
   hipLaunchKernelGGL(k_adj_pred_u, dim3(nb), dim3(256), 0, st, g, Qb, a.ABT, U, V, W, aU, aP, a.part);
-  sum_part(c, a, ACC_BETA, nb);
   if (polymer) {
     hipLaunchKernelGGL(k_adj_pred_tau, dim3(nb), dim3(256), 0, st, g, a.ABT + 6 * n, C1, aC, a.Xb, a.part, nb);
-    sum_part(c, a, ACC_COEF, nb);
-    sum_part(c, a, ACC_L2, nb);
     hipLaunchKernelGGL(k_adj_newton, dim3(nb), dim3(256), 0, st, g, a.Xb, C1, U, V, W, a.lam, a.Mb, a.part, nb);
-    sum_part(c, a, ACC_WI, nb);
-    sum_part(c, a, ACC_L2, nb);
     hipLaunchKernelGGL(k_adj_cstar, dim3(nb), dim3(256), 0, st, g, a.lam, C0, sel, U, V, W, aC, aU);
     hipLaunchKernelGGL(k_adj_vel_grad, dim3(nb), dim3(256), 0, st, g, a.Mb, aU);
   } else {
@@ -750,6 +798,12 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
   free0(a.drag);
   alloc0(&a.drag, (size_t)nsteps);
   a.ck_hist.assign(nck + 1, 0);
+  // the FP64 recording tapes its last segment, which the reverse sweep then
+  // need not recompute (the recomputation replays the recorded selections,
+  // so it is the same computation)
+  const int tape_from = c.mixed ? nsteps : (nck - 1) * every;
+  a.tape_seg = -1;
+  if (!c.mixed) ensure_tape(a, every, n);
   const double wd = c.g.sw;   // the drag sum's weight
   const int step0 = c.step;
   // Mixed precision: the checkpoints and the drag are those of the mixed
@@ -768,12 +822,19 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out) {
         hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, mixed_marker_forces(c), c.nmk, wd, a.drag + s);
       continue;
     }
+    double* E = s >= tape_from ? a.tape + (size_t)(s - tape_from) * TAPE_WORDS * n : nullptr;
+    if (E) tape_before(c, a, E, s - tape_from);
     c.sel_replay = (replay && polymer) ? a.selh + (size_t)s * n : nullptr;
     step_f64(c);
     c.sel_replay = nullptr;
+    if (E) tape_after(c, E);
     if (!replay && polymer) d2d(c, a.selh + (size_t)s * n, c.sel, n);
     if (c.p.method == FP_METHOD_IBM)
       hipLaunchKernelGGL(k_drag, dim3(1), dim3(1024), 0, c.stream, c.mf, c.nmk, wd, a.drag + s);
+  }
+  if (!c.mixed) {
+    d2d(c, a.tape + (size_t)every * TAPE_WORDS * n, c.C[c.cur], sizeof(double) * 6 * n);
+    a.tape_seg = nck - 1;
   }
   if (c.mixed) {
     mixed_join(c);
@@ -798,11 +859,7 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
   const Grid& g = c.g;
   const long n = g.n;
   const int every = a.every;
-  if (!a.tape || a.tape_every != every) {
-    free0(a.tape);
-    alloc0(&a.tape, (size_t)every * TAPE_WORDS * n + 6 * n);
-    a.tape_every = every;
-  }
+  ensure_tape(a, every, n);
   if (!a.bar) {
     alloc0(&a.bar, (size_t)BAR_WORDS * n);
     alloc0(&a.ubs, 3 * n); alloc0(&a.phib, n); alloc0(&a.bb, n); alloc0(&a.Rb, 3 * n); alloc0(&a.Qb, 3 * n);
@@ -816,28 +873,28 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
   else zero_words(a.bar, 2 * BAR_WORDS * n, st);
   zero_words(a.Fb, 6 * n, st);
   zero_words(a.acc, 2 * NACC, st);
-  a.tape_first.assign(every, 0);
+  // the kernels add their per-block partials of the parameter gradients over
+  // all steps; they are summed once, after the sweep
+  const int nbp = nblk(n, 256);
+  zero_words(a.part, 2 * NACC * nbp, st);
   const int step0 = c.step;
   for (int seg = a.nck - 1; seg >= 0; seg--) {
     const int s0 = seg * every, s1 = std::min(s0 + every, a.nsteps);
-    load_state(c, a.ck + (size_t)seg * CK_WORDS * n);
-    c.has_history = a.ck_hist[seg];
-    for (int s = s0; s < s1; s++) {
-      double* E = a.tape + (size_t)(s - s0) * TAPE_WORDS * n;
-      d2d(c, E, c.U, sizeof(double) * n);
-      d2d(c, E + n, c.V, sizeof(double) * n);
-      d2d(c, E + 2 * n, c.W, sizeof(double) * n);
-      d2d(c, E + 3 * n, c.P, sizeof(double) * n);
-      d2d(c, E + 4 * n, c.C[c.cur], sizeof(double) * 6 * n);
-      a.tape_first[s - s0] = c.has_history ? 0 : 1;
-      c.sel_replay = (c.p.beta != 1.0 && !a.mixed_rec) ? a.selh + (size_t)s * n : nullptr;
-      step_f64(c);
-      c.sel_replay = nullptr;
-      if (a.mixed_rec && c.p.beta != 1.0) d2d(c, a.selh + (size_t)s * n, c.sel, n);
-      d2d(c, E + 10 * n, c.Up, sizeof(double) * 3 * n);
-      d2d(c, E + 13 * n, c.Phi, sizeof(double) * n);
+    if (seg != a.tape_seg) {
+      load_state(c, a.ck + (size_t)seg * CK_WORDS * n);
+      c.has_history = a.ck_hist[seg];
+      for (int s = s0; s < s1; s++) {
+        double* E = a.tape + (size_t)(s - s0) * TAPE_WORDS * n;
+        tape_before(c, a, E, s - s0);
+        c.sel_replay = (c.p.beta != 1.0 && !a.mixed_rec) ? a.selh + (size_t)s * n : nullptr;
+        step_f64(c);
+        c.sel_replay = nullptr;
+        if (a.mixed_rec && c.p.beta != 1.0) d2d(c, a.selh + (size_t)s * n, c.sel, n);
+        tape_after(c, E);
+      }
+      d2d(c, a.tape + (size_t)every * TAPE_WORDS * n, c.C[c.cur], sizeof(double) * 6 * n);
+      a.tape_seg = seg;
     }
-    d2d(c, a.tape + (size_t)every * TAPE_WORDS * n, c.C[c.cur], sizeof(double) * 6 * n);
     // tape entry t holds the state before step s0 + t; the C after the
     // segment follows the last entry
     const int len = s1 - s0;
@@ -850,6 +907,7 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
   if (c.mixed) mixed_load_from_f64(c);
   if (bar_initial) d2d(c, bar_initial, a.bar, sizeof(double) * BAR_WORDS * n);
   if (bar_force) d2d(c, bar_force, a.Fb, sizeof(double) * 3 * n);
+  for (int q = 0; q < NACC; q++) sum_part(c, a, q, nbp);
   double acc[NACC];
   FP_HIP_CHECK(hipMemcpyAsync(acc, a.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
   FP_HIP_CHECK(hipStreamSynchronize(st));
