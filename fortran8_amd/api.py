@@ -76,6 +76,7 @@ def lib():
         L.fp_time_stats.argtypes = [_vp, _dp]
         L.fp_energy_spectrum.argtypes = [_vp, _dp, C.c_int]
         L.fp_check_selection.argtypes = [_vp, _dp]
+        L.fp_get_selection.argtypes = [_vp, C.c_void_p]
         L.fp_time_stats_map.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp, _dp]
         L.fp_energy_spectrum_z.argtypes = [_vp, C.c_int, _dp, C.c_double, _dp, _dp]
         L.fp_mean_accumulate.argtypes = [_vp]
@@ -229,6 +230,14 @@ class FenepGPU:
         out = np.zeros(3)
         self._check(lib().fp_check_selection(self.h, _ptr(out)), "fp_check_selection")
         return {"fp64": int(out[0]), "differ": int(out[1]), "total": int(out[2])}
+
+    def get_selection(self):
+        """Mixed precision: the C+- selection of the current stored field,
+        (nz, ny, nx) bytes; bits 2d, 2d+1 the choice along axis d (x, y, z),
+        0-2 the candidate levels, 3 the zero slope."""
+        out = np.zeros((self.nz, self.ny, self.nx), dtype=np.uint8)
+        self._check(lib().fp_get_selection(self.h, out.ctypes.data_as(C.c_void_p)), "fp_get_selection")
+        return out
 
     # ---- sensitivities (FP64 path, periodic grids)
     def set_physics(self, wi, beta, lp):

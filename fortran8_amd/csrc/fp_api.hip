@@ -723,6 +723,19 @@ int fp_check_selection(void* vc, double* out) {
   return FP_OK;
 }
 
+int fp_get_selection(void* vc, unsigned char* out) {
+  Ctx* c = (Ctx*)vc;
+  if (!c || !out) return FP_E_ARG;
+  if (!c->mixed) return fail(c, FP_E_ARG, "the selection bytes are those of the mixed-precision path");
+  if (!c->state_set) return fail(c, FP_E_STATE, "state not set");
+  try {
+    mixed_get_selection(*c, out);
+  } catch (const std::string& e) {
+    return fail(c, FP_E_HIP, e);
+  }
+  return FP_OK;
+}
+
 // Errors of the recording/adjoint layer arrive as "E_ARG:..." or
 // "E_STATE:..." (argument / call order) or as HIP errors.
 static int adj_fail(Ctx* c, const std::string& e) {

@@ -3323,6 +3323,24 @@ int mixed_check_selection(Ctx& c, double* out3) {
   return 0;
 }
 
+// the selection of the current stored field (as mixed_check_selection makes
+// it), copied to host memory
+int mixed_get_selection(Ctx& c, uint8_t* out) {
+  Mixed& X = MX(c);
+  const Grid& g = c.g;
+  hipStream_t s = c.stream;
+  if (c.p.beta == 1.0) {
+    std::memset(out, 0, (size_t)g.n);
+    return 0;
+  }
+  if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(s, X.sel_done, 0));
+  select_fenep(c, 0, s);
+  X.sel_ready = 1;
+  FP_HIP_CHECK(hipMemcpyAsync(out, c.sel, (size_t)g.n, hipMemcpyDeviceToHost, s));
+  FP_HIP_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
 // statistics of the last step: [0] |b|^2, [1] |r|^2 before the last
 // correction, [2] guarded selections re-decided in FP64
 // The context's stream waits for work made ahead on the side stream, so that

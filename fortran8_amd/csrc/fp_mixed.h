@@ -16,6 +16,7 @@ const double* mixed_marker_forces(Ctx& c);
 double mixed_last_ndc(Ctx& c);
 int mixed_stats(Ctx& c, double* out3);
 int mixed_check_selection(Ctx& c, double* out3);
+int mixed_get_selection(Ctx& c, uint8_t* out);
 void mixed_step(Ctx& c);
 void mixed_join(Ctx& c);
 void mixed_physics(Ctx& c);   // after Wi, beta or Lp changed

@@ -248,6 +248,16 @@ int fp_mean_apply(void* ctx, int mode);
  * in total (3 NX NY NZ).  The state is not changed. */
 int fp_check_selection(void* ctx, double* out);
 
+/* Mixed precision: the C+- selection that fp_check_selection checks, as
+ * bytes: out[i + NX (j + NY k)] over the interior cells, bits 2d and 2d+1 the
+ * choice along axis d (x, y, z): 0, 1, 2 the candidate levels of
+ * fenep.f90:179-184 (maxloc - 1 of fenep.f90:186-221), 3 the zero slope.
+ * With beta = 1 (no polymer stress, no selection) out is set to zero.  The
+ * state is not changed.  No counterpart in the reference, whose selection
+ * stays inside Cstar: it lets a caller compare the selection with the
+ * reference formula on the host. */
+int fp_get_selection(void* ctx, unsigned char* out);
+
 /* ---- Sensitivities (periodic grids, FFT Poisson, methods 1 and 2).  The
  * reference has no derivative code: these entry points serve the discrete
  * adjoint of the step it advances (fenep.f90:2477-2527) with respect to Wi,

@@ -135,6 +135,49 @@ def test_mixed_selection_is_the_reference_selection():
         g.close()
 
 
+def oracle_selection(Cst, walls):
+    """The oracle's selection (select_r, fenep.f90:179-230 with the Vieta test
+    of fenep.f90:425-444, in FP64) of a stored FP32 conformation field at
+    every interior cell and axis, as bytes in the layout of get_selection.
+    Periodic neighbours in x and z (and y without walls); with walls the
+    ghost rows are the ones C_kabe stores (fenep_2d.f90:1098-1143), 2 C_1 -
+    C_2 in the stored precision."""
+    C = interior_c(Cst)                      # (nz, ny, nx, 6), FP32 values
+    out = np.zeros(C.shape[:3], dtype=np.uint8)
+    f32 = np.float32
+    for d, axis in ((0, 2), (1, 1), (2, 0)):
+        cm, cp = np.roll(C, 1, axis=axis), np.roll(C, -1, axis=axis)
+        if d == 1 and walls:
+            cm[:, 0] = (f32(2) * C[:, 0].astype(f32) - C[:, 1].astype(f32)).astype(np.float64)
+            cp[:, -1] = (f32(2) * C[:, -1].astype(f32) - C[:, -2].astype(f32)).astype(np.float64)
+        for idx in np.ndindex(*C.shape[:3]):
+            lvl = po.select(cm[idx], C[idx], cp[idx])[0] - 1    # 1-based, 4 = zero slope
+            out[idx] |= np.uint8(lvl << (2 * d))
+    return out
+
+
+def test_mixed_selection_bytes_equal_the_oracle_selection():
+    """The mixed step's C+- selection bytes (FP32 decisions with certificates
+    plus the FP64 re-decisions) equal, at every cell and axis, the oracle's
+    selection computed on the host from the same stored field."""
+    cases = [("kolmogorov 32^3", mixed(configs.kolmogorov(n=32)), 40, False),
+             ("throughput 64x64x16", mixed(configs.throughput(nx=64, ny=64, nz=16, ncyl=2)), 20, True)]
+    for label, case, steps, walls in cases:
+        g = api.load_case(case)
+        g.step(steps)
+        sel = g.get_selection()
+        chk = g.check_selection()
+        ref = oracle_selection(g.get_state().C, walls)
+        nd = int(np.count_nonzero(sel != ref))
+        levels = [np.bincount(((sel >> (2 * d)) & 3).ravel(), minlength=4).tolist() for d in range(3)]
+        print(label, "steps", steps, "differing cells", nd, "levels per axis", levels, chk, flush=True)
+        assert nd == 0, label
+        # the comparison covers both kinds of decision: certified in FP32 and
+        # re-decided in FP64, over more than one choice
+        assert chk["fp64"] > 0 and sum(1 for c in levels[0] if c) > 1, label
+        g.close()
+
+
 def test_mixed_rejects_unsupported_combinations():
     p = default_params(nx=8, ny=8, nz=8, precision=PREC_MIXED, method=METHOD_EXPLICIT)
     with pytest.raises(api.FenepError, match="FP_E_ARG"):
