@@ -53,6 +53,10 @@ def test_null_handles_are_safe():
     assert L.fp_destroy(None) == 0
     assert L.fp_last_error(None) == b"null context"
     assert L.fp_step(None, 1, 1) == 1
+    buf = (C.c_ubyte * 8)()
+    assert L.fp_get_selection(None, buf) == 1
+    out3 = (C.c_double * 3)()
+    assert L.fp_check_selection(None, out3) == 1
 
 
 def test_seeded_uniform_stream_matches_host_implementation():
