@@ -126,21 +126,23 @@ __global__ __launch_bounds__(256) void k_adj_helm_beta(Grid g, const double* __r
 // Uabar = Rbar + interp^T(-Fcbar/dt).
 // This kernel writes gm = -Fcbar/dt (times dX^3 / W) per marker and component.
 __global__ __launch_bounds__(256) void k_adj_ibm_markers(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
-                                                         const double* __restrict__ Rb, double wdrag,
-                                                         double* __restrict__ gm) {
+                                                         const double* __restrict__ w, const double* __restrict__ Rb,
+                                                         double wdrag, double* __restrict__ gm) {
   const long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (id >= nmk) return;
   const int n = (int)(id / ((long)nc * nl)) + 1;
-  const double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
+  const double xc = mk[id], yc = mk[nmk + id];
   for (int c = 0; c < 3; c++) {
     int i0, j0;
     ibm_origin_d(g, c, xc, yc, i0, j0);
     const double* X = Rb + c * g.n;
+    const double* wm = w + (c * nmk + id) * 27;   // the interpolation weights
     double acc = 0.0;
+    int slot = 0;
     for (int kk = n - 1; kk <= n + 1; kk++)
       for (int jj = j0; jj <= j0 + 2; jj++)
-        for (int ii = i0; ii <= i0 + 2; ii++)
-          acc += X[pidx(g, ii - 1, jj - 1, kk - 1)] * ibm_weight(g, c, ii, jj, kk, xc, yc, zc) * g.sw;
+        for (int ii = i0; ii <= i0 + 2; ii++, slot++)
+          acc += X[pidx(g, ii - 1, jj - 1, kk - 1)] * wm[slot] * g.sw;
     const double fbar = g.dt * acc - (c == 0 ? g.sw * wdrag : 0.0);
     // interp^T(x) = spread(x) dX^3 / W: the spread kernels apply W
     gm[c * nmk + id] = (-fbar / g.dt) * (((g.dx * g.dx) * g.dx) / g.sw);
@@ -740,8 +742,8 @@ void adjoint_step(Ctx& c, Adj& a, int t, int seg_len, int s, double wdrag) {
   // immersed boundary
   const double* Qb = a.Rb;
   if (c.p.method == FP_METHOD_IBM) {
-    hipLaunchKernelGGL(k_adj_ibm_markers, dim3(nblk(c.nmk, 256)), dim3(256), 0, st, g, c.nmk, c.nc, c.nl, c.mk, a.Rb,
-                       wdrag, a.gm);
+    hipLaunchKernelGGL(k_adj_ibm_markers, dim3(nblk(c.nmk, 256)), dim3(256), 0, st, g, c.nmk, c.nc, c.nl, c.mk, c.ib_w,
+                       a.Rb, wdrag, a.gm);
     f64_spread_markers(c, a.gm, c.ftmp);
     hipLaunchKernelGGL(k_adj_ibm_q, dim3(nblk(3 * n, 256)), dim3(256), 0, st, g, a.Rb, c.ftmp, a.Qb);
     Qb = a.Qb;

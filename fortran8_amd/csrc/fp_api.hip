@@ -267,7 +267,8 @@ int fp_destroy(void* vc) {
   dfree(c->twx); dfree(c->twy); dfree(c->twz); dfree(c->kx2); dfree(c->ky2); dfree(c->kz2);
   dfree(c->mk); dfree(c->mf); dfree(c->mb); dfree(c->ftmp);
   dfree(c->mean_acc); dfree(c->mean_u);
-  for (int k = 0; k < 3; k++) { dfree(c->sp_row[k]); dfree(c->sp_tgt[k]); dfree(c->sp_ent[k]); }
+  for (int k = 0; k < 3; k++) { dfree(c->sp_row[k]); dfree(c->sp_tgt[k]); dfree(c->sp_ent[k]); dfree(c->sp_w[k]); }
+  dfree(c->ib_w);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return FP_OK;

@@ -64,8 +64,10 @@ struct Ctx {
   long gn = 0;
   int* sp_row[3] = {nullptr, nullptr, nullptr};   // CSR row pointers
   long* sp_tgt[3] = {nullptr, nullptr, nullptr};  // ghosted target index
-  int* sp_ent[3] = {nullptr, nullptr, nullptr};   // marker*27 + slot
+  int* sp_ent[3] = {nullptr, nullptr, nullptr};   // marker of each entry
   int sp_rows[3] = {0, 0, 0};
+  double* sp_w[3] = {nullptr, nullptr, nullptr};  // weight of each entry (ibm_weight)
+  double* ib_w = nullptr;             // interpolation weights, (3 nmk) x 27 stencil points
   // SOR
   double sor_itr = -1.0;
   // timing: recorded (name, event) pairs and a pool of created events, so

@@ -572,24 +572,27 @@ __global__ __launch_bounds__(256) void k_march(Grid g, int explicit_p, const dou
 // ============================================================ IBM ==========
 // interpolation and marker forces of ibm_Helmholtz (fenep.f90:2076-2119,
 // forces kept as in fenep_mpi.f90:1845-1853)
+// w: the 27 weights ibm_weight of each marker and component (k_ibm_interp_w).
 __global__ __launch_bounds__(256) void k_ibm_interp(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
-                                                    const double* __restrict__ ua, double* __restrict__ mb,
-                                                    double* __restrict__ mf) {
+                                                    const double* __restrict__ w, const double* __restrict__ ua,
+                                                    double* __restrict__ mb, double* __restrict__ mf) {
   long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (id >= nmk) return;
   int n = (int)(id / ((long)nc * nl)) + 1;
-  double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
+  double xc = mk[id], yc = mk[nmk + id];
   const double dx3 = (g.dx * g.dx) * g.dx;
   for (int c = 0; c < 3; c++) {
     int i0, j0; ibm_origin_d(g, c, xc, yc, i0, j0);
     const double* X = ua + c * g.n;
+    const double* wm = w + (c * nmk + id) * 27;
     int kind = c == 0 ? BC_U : (c == 1 ? BC_V : BC_W);
     double acc = 0.0;
+    int slot = 0;
     for (int kk = n - 1; kk <= n + 1; kk++)
       for (int jj = j0; jj <= j0 + 2; jj++)
-        for (int ii = i0; ii <= i0 + 2; ii++) {
+        for (int ii = i0; ii <= i0 + 2; ii++, slot++) {
           double v = ldb(X, g, ii - 1, jj - 1, kk - 1, kind);
-          acc = acc + (v * ibm_weight(g, c, ii, jj, kk, xc, yc, zc)) * dx3;
+          acc = acc + (v * wm[slot]) * dx3;
         }
     mb[c * nmk + id] = acc;
     double uc = mk[(3 + c) * nmk + id];
@@ -599,23 +602,49 @@ __global__ __launch_bounds__(256) void k_ibm_interp(Grid g, long nmk, int nc, in
 
 // spreading (fenep.f90:2125-2162) as a gather over the CSR of contributions
 // of each ghosted target, in the reference's marker loop order.
+// (w: each entry's weight, k_ibm_spread_w)
 __global__ __launch_bounds__(256) void k_ibm_spread(Grid g, int comp, int rows, const int* __restrict__ rowp,
                                                     const long* __restrict__ tgt, const int* __restrict__ ent,
-                                                    long nmk, int nc, int nl, const double* __restrict__ mk,
+                                                    const double* __restrict__ w, long nmk,
                                                     const double* __restrict__ mf, double* __restrict__ ft) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  long t = tgt[r];
+  double acc = 0.0;
+  for (int e = rowp[r]; e < rowp[r + 1]; e++) acc = acc + (mf[comp * nmk + ent[e]] * w[e]) * g.sw;
+  ft[t] = acc;
+}
+
+// The weights depend only on the markers and the grid: ibm_weight of every
+// spreading entry (its marker at its row's target) and of every marker's 27
+// interpolation points per component, evaluated once per marker set.
+__global__ __launch_bounds__(256) void k_ibm_spread_w(Grid g, int comp, int rows, const int* __restrict__ rowp,
+                                                      const long* __restrict__ tgt, const int* __restrict__ ent,
+                                                      long nmk, const double* __restrict__ mk, double* __restrict__ w) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   const long gx = g.nx + 2, gy = g.ny + 2;
   long t = tgt[r];
   int ii = (int)(t % gx), jj = (int)((t / gx) % gy), kk = (int)(t / (gx * gy));
-  const double dx3 = (g.dx * g.dx) * g.dx;
-  double acc = 0.0;
   for (int e = rowp[r]; e < rowp[r + 1]; e++) {
     long id = ent[e];
-    double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
-    acc = acc + (mf[comp * nmk + id] * ibm_weight(g, comp, ii, jj, kk, xc, yc, zc)) * g.sw;
+    w[e] = ibm_weight(g, comp, ii, jj, kk, mk[id], mk[nmk + id], mk[2 * nmk + id]);
   }
-  ft[t] = acc;
+}
+__global__ __launch_bounds__(256) void k_ibm_interp_w(Grid g, long nmk, int nc, int nl, const double* __restrict__ mk,
+                                                      double* __restrict__ w) {
+  long id = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (id >= nmk) return;
+  int n = (int)(id / ((long)nc * nl)) + 1;
+  double xc = mk[id], yc = mk[nmk + id], zc = mk[2 * nmk + id];
+  for (int c = 0; c < 3; c++) {
+    int i0, j0; ibm_origin_d(g, c, xc, yc, i0, j0);
+    double* wm = w + (c * nmk + id) * 27;
+    int slot = 0;
+    for (int kk = n - 1; kk <= n + 1; kk++)
+      for (int jj = j0; jj <= j0 + 2; jj++)
+        for (int ii = i0; ii <= i0 + 2; ii++, slot++) wm[slot] = ibm_weight(g, c, ii, jj, kk, xc, yc, zc);
+  }
 }
 
 // ibm_PBM fold (fenep.f90:2195-2205), one axis per launch
@@ -668,7 +697,7 @@ void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp) {
     double* ft = ftmp + comp * c.gn;
     if (c.sp_rows[comp] > 0)
       hipLaunchKernelGGL(k_ibm_spread, dim3(nblk(c.sp_rows[comp], 256)), dim3(256), 0, s, g, comp, c.sp_rows[comp],
-                         c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.nmk, c.nc, c.nl, c.mk, mfield, ft);
+                         c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.sp_w[comp], c.nmk, mfield, ft);
     hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gy * gz, 256)), dim3(256), 0, s, g, 0, ft);
     hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
     hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
@@ -678,7 +707,8 @@ void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp) {
 void f64_ibm(Ctx& c) {
   const Grid& g = c.g;
   hipStream_t s = c.stream;
-  hipLaunchKernelGGL(k_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, c.nc, c.nl, c.mk, c.Q, c.mb, c.mf);
+  hipLaunchKernelGGL(k_ibm_interp, dim3(nblk(c.nmk, 256)), dim3(256), 0, s, g, c.nmk, c.nc, c.nl, c.mk, c.ib_w, c.Q, c.mb,
+                     c.mf);
   f64_spread_markers(c, c.mf, c.ftmp);
   hipLaunchKernelGGL(k_ibm_rhs, dim3(nblk(3 * g.n, 256)), dim3(256), 0, s, g, c.Q, c.ftmp, c.B, c.Up);
 }
@@ -732,15 +762,27 @@ void f64_ibm_setup(Ctx& c) {
       tgt.push_back(t);
       rowp.push_back((int)ent.size());
     }
-    if (c.sp_row[comp]) { (void)hipFree(c.sp_row[comp]); (void)hipFree(c.sp_tgt[comp]); (void)hipFree(c.sp_ent[comp]); }
+    if (c.sp_row[comp]) {
+      (void)hipFree(c.sp_row[comp]); (void)hipFree(c.sp_tgt[comp]); (void)hipFree(c.sp_ent[comp]); (void)hipFree(c.sp_w[comp]);
+    }
     c.sp_rows[comp] = (int)tgt.size();
     FP_HIP_CHECK(hipMalloc(&c.sp_row[comp], sizeof(int) * rowp.size()));
     FP_HIP_CHECK(hipMalloc(&c.sp_tgt[comp], sizeof(long) * std::max<size_t>(1, tgt.size())));
     FP_HIP_CHECK(hipMalloc(&c.sp_ent[comp], sizeof(int) * std::max<size_t>(1, ent.size())));
+    FP_HIP_CHECK(hipMalloc(&c.sp_w[comp], sizeof(double) * std::max<size_t>(1, ent.size())));
     FP_HIP_CHECK(hipMemcpy(c.sp_row[comp], rowp.data(), sizeof(int) * rowp.size(), hipMemcpyHostToDevice));
     if (!tgt.empty()) FP_HIP_CHECK(hipMemcpy(c.sp_tgt[comp], tgt.data(), sizeof(long) * tgt.size(), hipMemcpyHostToDevice));
     if (!ent.empty()) FP_HIP_CHECK(hipMemcpy(c.sp_ent[comp], ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice));
+    if (!tgt.empty())
+      hipLaunchKernelGGL(k_ibm_spread_w, dim3(nblk((long)tgt.size(), 256)), dim3(256), 0, c.stream, g, comp,
+                         (int)tgt.size(), c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.nmk, c.mk, c.sp_w[comp]);
   }
+  if (c.ib_w) (void)hipFree(c.ib_w);
+  FP_HIP_CHECK(hipMalloc(&c.ib_w, sizeof(double) * std::max<long>(1, 3 * 27 * c.nmk)));
+  if (c.nmk > 0)
+    hipLaunchKernelGGL(k_ibm_interp_w, dim3(nblk(c.nmk, 256)), dim3(256), 0, c.stream, g, c.nmk, c.nc, c.nl, c.mk, c.ib_w);
+  FP_HIP_CHECK(hipGetLastError());
+  FP_HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 // ============================================================ SOR ==========
