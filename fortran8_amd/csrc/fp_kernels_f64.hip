@@ -647,9 +647,11 @@ __global__ __launch_bounds__(256) void k_ibm_interp_w(Grid g, long nmk, int nc, 
   }
 }
 
-// ibm_PBM fold (fenep.f90:2195-2205), one axis per launch
-__global__ __launch_bounds__(256) void k_ibm_fold(Grid g, int axis, double* __restrict__ ft) {
+// ibm_PBM fold (fenep.f90:2195-2205), one axis per launch, the component
+// (ghosted array ft + blockIdx.y gn) per grid row
+__global__ __launch_bounds__(256) void k_ibm_fold(Grid g, int axis, double* __restrict__ ft0, long gn) {
   const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
+  double* __restrict__ ft = ft0 + blockIdx.y * gn;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   auto G = [&](long i, long j, long k) { return i + gx * (j + gy * k); };
   if (axis == 0) {
@@ -698,10 +700,11 @@ void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp) {
     if (c.sp_rows[comp] > 0)
       hipLaunchKernelGGL(k_ibm_spread, dim3(nblk(c.sp_rows[comp], 256)), dim3(256), 0, s, g, comp, c.sp_rows[comp],
                          c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.sp_w[comp], c.nmk, mfield, ft);
-    hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gy * gz, 256)), dim3(256), 0, s, g, 0, ft);
-    hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gz, 256)), dim3(256), 0, s, g, 1, ft);
-    hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gy, 256)), dim3(256), 0, s, g, 2, ft);
   }
+  // the folds of the three components, x, then y, then z
+  hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gy * gz, 256), 3), dim3(256), 0, s, g, 0, ftmp, c.gn);
+  hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gz, 256), 3), dim3(256), 0, s, g, 1, ftmp, c.gn);
+  hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gy, 256), 3), dim3(256), 0, s, g, 2, ftmp, c.gn);
 }
 
 void f64_ibm(Ctx& c) {
