@@ -764,6 +764,13 @@ void adjoint_step(Ctx& c, Adj& a, int t, int seg_len, int s, double wdrag) {
 
 }  // namespace
 
+// A change of the step's physics, force or markers after a recording: the
+// reverse sweep recomputes every segment (with the settings then current), as
+// it would without the taped segment.
+void adj_drop_tape(Ctx& c) {
+  if (c.adj) ((Adj*)c.adj)->tape_seg = -1;
+}
+
 void adj_free(Ctx& c) {
   if (!c.adj) return;
   Adj& a = *(Adj*)c.adj;

@@ -378,6 +378,7 @@ int fp_set_body_force(void* vc, const double* fx, const double* fy, const double
   if (!c) return FP_E_ARG;
   try {
     drain(*c);
+    adj_drop_tape(*c);
     const long n = c->g.n;
     if (!fx && !fy && !fz) { dfree(c->F); if (c->mixed) mixed_set_force(*c); return FP_OK; }
     if (!c->F) dalloc(&c->F, 3 * n);
@@ -399,6 +400,7 @@ int fp_set_markers(void* vc, int nc, int nl, const double* xc, const double* yc,
   if (!c || nc <= 0 || nl <= 0 || !xc || !yc || !zc || !uc || !vc_ || !wc) return fail(c, FP_E_ARG, "bad markers");
   try {
     drain(*c);
+    adj_drop_tape(*c);
     const Grid& g = c->g;
     long nmk = (long)nc * nl * g.nz;
     // every stencil must stay within one ghost layer (and off the walls)
@@ -755,6 +757,7 @@ int fp_set_physics(void* vc, double wi, double beta, double lp) {
   } catch (const std::string& e) {
     return fail(c, FP_E_HIP, e);
   }
+  adj_drop_tape(*c);
   c->p.wi = wi; c->p.beta = beta; c->p.lp = lp;
   c->g.wi = wi; c->g.beta = beta; c->g.lp = lp;
   if (c->mixed) {

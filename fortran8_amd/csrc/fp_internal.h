@@ -118,6 +118,7 @@ void adj_record(Ctx& c, int nsteps, int every, int replay, double* drag_out);
 void adj_backward(Ctx& c, const double* weights, const double* bar_final, double* bar_initial, double* bar_params,
                   double* bar_force);
 void adj_free(Ctx& c);
+void adj_drop_tape(Ctx& c);
 
 // an event from the context's pool (created on first use)
 // Phase timing: level 1 records every phase, level 2 only the predictor (the

@@ -75,3 +75,29 @@ def test_adjoint_cfg5_drag_gradients_and_cost():
     for k in ("mixed_rel_wi_vs_fp64", "mixed_rel_beta_vs_fp64", "mixed_rel_F_vs_fp64"):
         print(k, out[k])
         assert out[k] < 1e-2, k
+
+
+def test_adjoint_taped_last_segment_equals_its_recomputation():
+    """The FP64 recording tapes its last checkpoint segment, which the reverse
+    sweep then takes instead of recomputing it; a setter call after the
+    recording (here the physics, set to the same values) drops that tape, so
+    the sweep recomputes every segment.  Both give the same gradients."""
+    import numpy as np
+    from fortran8_amd import api, configs
+    case = configs.adjoint_case()
+    p = case.params
+    nsteps, every = 60, 20
+    w = np.zeros(nsteps)
+    w[nsteps // 2:] = 1.0 / (nsteps - nsteps // 2)
+    g = api.load_case(case)
+    out = []
+    for drop in (False, True):
+        g.set_state(case.state)
+        g.adj_record(nsteps, every)
+        if drop:
+            g.set_physics(p.wi, p.beta, p.lp)
+        out.append(g.adj_backward(weights=w))
+    g.close()
+    print("taped", out[0], "recomputed", out[1], flush=True)
+    assert np.all(np.isfinite(out[0])) and np.any(out[0] != 0.0)
+    assert np.allclose(out[0], out[1], rtol=1e-12, atol=0.0)
