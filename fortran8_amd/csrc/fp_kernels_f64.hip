@@ -602,17 +602,27 @@ __global__ __launch_bounds__(256) void k_ibm_interp(Grid g, long nmk, int nc, in
 
 // spreading (fenep.f90:2125-2162) as a gather over the CSR of contributions
 // of each ghosted target, in the reference's marker loop order.
-// (w: each entry's weight, k_ibm_spread_w)
-__global__ __launch_bounds__(256) void k_ibm_spread(Grid g, int comp, int rows, const int* __restrict__ rowp,
-                                                    const long* __restrict__ tgt, const int* __restrict__ ent,
-                                                    const double* __restrict__ w, long nmk,
-                                                    const double* __restrict__ mf, double* __restrict__ ft) {
+// (w: each entry's weight, k_ibm_spread_w).  The three components in one
+// launch: component blockIdx.y, its CSR and its ghosted array ft0 + comp gn.
+struct SpreadCsr {
+  const int* rowp[3];
+  const long* tgt[3];
+  const int* ent[3];
+  const double* w[3];
+  int rows[3];
+};
+__global__ __launch_bounds__(256) void k_ibm_spread(Grid g, SpreadCsr cs, long nmk, const double* __restrict__ mf,
+                                                    double* __restrict__ ft0, long gn) {
+  const int comp = blockIdx.y;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  long t = tgt[r];
+  if (r >= cs.rows[comp]) return;
+  const int* __restrict__ rowp = cs.rowp[comp];
+  const int* __restrict__ ent = cs.ent[comp];
+  const double* __restrict__ w = cs.w[comp];
+  long t = cs.tgt[comp][r];
   double acc = 0.0;
   for (int e = rowp[r]; e < rowp[r + 1]; e++) acc = acc + (mf[comp * nmk + ent[e]] * w[e]) * g.sw;
-  ft[t] = acc;
+  ft0[comp * gn + t] = acc;
 }
 
 // The weights depend only on the markers and the grid: ibm_weight of every
@@ -695,12 +705,18 @@ void f64_spread_markers(Ctx& c, const double* mfield, double* ftmp) {
   const Grid& g = c.g;
   hipStream_t s = c.stream;
   const long gx = g.nx + 2, gy = g.ny + 2, gz = g.nz + 2;
+  SpreadCsr cs{};
+  int rmax = 0;
   for (int comp = 0; comp < 3; comp++) {
-    double* ft = ftmp + comp * c.gn;
-    if (c.sp_rows[comp] > 0)
-      hipLaunchKernelGGL(k_ibm_spread, dim3(nblk(c.sp_rows[comp], 256)), dim3(256), 0, s, g, comp, c.sp_rows[comp],
-                         c.sp_row[comp], c.sp_tgt[comp], c.sp_ent[comp], c.sp_w[comp], c.nmk, mfield, ft);
+    cs.rowp[comp] = c.sp_row[comp];
+    cs.tgt[comp] = c.sp_tgt[comp];
+    cs.ent[comp] = c.sp_ent[comp];
+    cs.w[comp] = c.sp_w[comp];
+    cs.rows[comp] = c.sp_rows[comp];
+    rmax = std::max(rmax, c.sp_rows[comp]);
   }
+  if (rmax > 0)
+    hipLaunchKernelGGL(k_ibm_spread, dim3(nblk(rmax, 256), 3), dim3(256), 0, s, g, cs, c.nmk, mfield, ftmp, c.gn);
   // the folds of the three components, x, then y, then z
   hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gy * gz, 256), 3), dim3(256), 0, s, g, 0, ftmp, c.gn);
   hipLaunchKernelGGL(k_ibm_fold, dim3(nblk(gx * gz, 256), 3), dim3(256), 0, s, g, 1, ftmp, c.gn);
