@@ -892,16 +892,31 @@ void adj_backward(Ctx& c, const double* weights, const double* bar_final, double
     if (seg != a.tape_seg) {
       load_state(c, a.ck + (size_t)seg * CK_WORDS * n);
       c.has_history = a.ck_hist[seg];
-      for (int s = s0; s < s1; s++) {
-        double* E = a.tape + (size_t)(s - s0) * TAPE_WORDS * n;
-        tape_before(c, a, E, s - s0);
-        c.sel_replay = (c.p.beta != 1.0 && !a.mixed_rec) ? a.selh + (size_t)s * n : nullptr;
-        step_f64(c);
-        c.sel_replay = nullptr;
-        if (a.mixed_rec && c.p.beta != 1.0) d2d(c, a.selh + (size_t)s * n, c.sel, n);
-        tape_after(c, E);
+      if (a.mixed_rec) {
+        // a mixed recording: the segment is recomputed with the mixed step
+        // from its checkpoint (FP32 values, so exactly), and the tape holds
+        // its states widened; the FP64 adjoint steps are linearised there
+        mixed_load_from_f64(c);
+        for (int s = s0; s < s1; s++) {
+          double* E = a.tape + (size_t)(s - s0) * TAPE_WORDS * n;
+          a.tape_first[s - s0] = c.has_history ? 0 : 1;
+          mixed_tape_in(c, E, c.p.beta != 1.0 ? a.selh + (size_t)s * n : nullptr);
+          mixed_step(c);
+          mixed_tape_out(c, E);
+        }
+        mixed_join(c);
+        mixed_widen_c(c, a.tape + (size_t)every * TAPE_WORDS * n);
+      } else {
+        for (int s = s0; s < s1; s++) {
+          double* E = a.tape + (size_t)(s - s0) * TAPE_WORDS * n;
+          tape_before(c, a, E, s - s0);
+          c.sel_replay = c.p.beta != 1.0 ? a.selh + (size_t)s * n : nullptr;
+          step_f64(c);
+          c.sel_replay = nullptr;
+          tape_after(c, E);
+        }
+        d2d(c, a.tape + (size_t)every * TAPE_WORDS * n, c.C[c.cur], sizeof(double) * 6 * n);
       }
-      d2d(c, a.tape + (size_t)every * TAPE_WORDS * n, c.C[c.cur], sizeof(double) * 6 * n);
       a.tape_seg = seg;
     }
     // tape entry t holds the state before step s0 + t; the C after the

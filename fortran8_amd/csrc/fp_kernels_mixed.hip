@@ -3323,6 +3323,44 @@ int mixed_check_selection(Ctx& c, double* out3) {
   return 0;
 }
 
+// Tape entry of one mixed step for the adjoint's reverse sweep (FP64 words
+// U V W P C6 u*(3) Phi, as the FP64 step's): before the step, the stored
+// state widened and the selection the step will take (made now, or the one
+// made ahead); after it, u* = u^n + d from the step's FP32 increment d and
+// the FP64 potential.
+__global__ __launch_bounds__(256) void m_ustar64(long n3, const double* __restrict__ un, const float* __restrict__ d,
+                                                 double* __restrict__ us) {
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e < n3) us[e] = un[e] + (double)d[e];
+}
+void mixed_tape_in(Ctx& c, double* E, uint8_t* sel_out) {
+  Mixed& X = MX(c);
+  const long n = c.g.n;
+  hipStream_t s = c.stream;
+  if (sel_out) {
+    if (X.sel_ready) FP_HIP_CHECK(hipStreamWaitEvent(s, X.sel_done, 0));
+    else select_fenep(c, 0, s);
+    X.sel_ready = 1;
+    FP_HIP_CHECK(hipMemcpyAsync(sel_out, c.sel, (size_t)n, hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.U, E);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.V, E + n);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.W, E + 2 * n);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(n, 256)), dim3(256), 0, s, n, X.P, E + 3 * n);
+  hipLaunchKernelGGL(m_to64, dim3(nblk(6 * n, 256)), dim3(256), 0, s, 6 * n, X.C[X.cur], E + 4 * n);
+}
+void mixed_tape_out(Ctx& c, double* E) {
+  Mixed& X = MX(c);
+  const long n = c.g.n;
+  hipLaunchKernelGGL(m_ustar64, dim3(nblk(3 * n, 256)), dim3(256), 0, c.stream, 3 * n, E, X.Up, E + 10 * n);
+  FP_HIP_CHECK(hipMemcpyAsync(E + 13 * n, X.phi, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+}
+void mixed_widen_c(Ctx& c, double* dst) {
+  Mixed& X = MX(c);
+  const long n = c.g.n;
+  hipLaunchKernelGGL(m_to64, dim3(nblk(6 * n, 256)), dim3(256), 0, c.stream, 6 * n, X.C[X.cur], dst);
+}
+
 // the selection of the current stored field (as mixed_check_selection makes
 // it), copied to host memory
 int mixed_get_selection(Ctx& c, uint8_t* out) {

@@ -17,6 +17,10 @@ double mixed_last_ndc(Ctx& c);
 int mixed_stats(Ctx& c, double* out3);
 int mixed_check_selection(Ctx& c, double* out3);
 int mixed_get_selection(Ctx& c, uint8_t* out);
+// tape entries of the adjoint's reverse sweep from the mixed step
+void mixed_tape_in(Ctx& c, double* E, uint8_t* sel_out);
+void mixed_tape_out(Ctx& c, double* E);
+void mixed_widen_c(Ctx& c, double* dst);
 void mixed_step(Ctx& c);
 void mixed_join(Ctx& c);
 void mixed_physics(Ctx& c);   // after Wi, beta or Lp changed
