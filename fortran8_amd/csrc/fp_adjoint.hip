@@ -9,7 +9,8 @@
 // Reverse (fp_adj_backward): segments are recomputed from their checkpoint
 // with the recorded selections (a tape of the state, u* and Phi of each step)
 // and traversed backwards; the segment the tape already holds is not
-// recomputed.  Each forward kernel has an adjoint kernel here:
+// recomputed.  After a mixed-precision recording the segments are recomputed
+// with the mixed step, and the tape holds its states widened to FP64.  Each forward kernel has an adjoint kernel here:
 //   fft_march          -> k_adj_march            (transposed gradient/Laplacian)
 //   fft_poisson solve  -> the same spectral solve (symmetric operator)
 //   divergence         -> k_adj_div

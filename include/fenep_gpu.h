@@ -265,8 +265,9 @@ int fp_get_selection(void* ctx, unsigned char* out);
  * F of init (fenep.f90:96-120).  On an FP64 context the adjoint is that of the
  * recorded FP64 steps.  On a mixed-precision context the recording runs the
  * mixed step (its drag and its FP32 state, widened, as checkpoints) and the
- * reverse sweep is the FP64 adjoint along FP64 recomputations of each
- * checkpoint segment, with their own C+- selections. */
+ * reverse sweep is the FP64 adjoint along mixed-step recomputations of each
+ * checkpoint segment (their states widened to FP64), with the C+- selections
+ * those steps make. */
 
 /* Change Wi, beta and Lp of a context between runs. */
 int fp_set_physics(void* ctx, double wi, double beta, double lp);
